@@ -1,0 +1,739 @@
+// ORACLE — test infrastructure only.  Never linked into, loaded by, or called
+// from the product library or CLI.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load this file's build (_ref/liboracle.so).
+//
+// A plain sequential CPU restatement of the ASZ compression core, following the
+// reference's block-major loop order (compress.tpp:179-208) and the exact
+// floating-point lowering that GCC 11 -Ofast -mfma gives the reference sources
+// (derived from the optimized GIMPLE of a local build of those sources):
+//
+//   * cubic/quintic/edge predictors           compress.tpp:9-34
+//   * quantizer, axis 0 (inlined in the hot loop):  q = lround((0.5/eb) * (p - x))
+//     quantizer, axes 1/2 (out-of-line call):       q = lround(((p - x) * 0.5) / eb)
+//                                                   compress.tpp:35, block_compress_impl :67-87
+//   * reconstruction  x' = (float) fma(-(2 eb), q, p)  compress.tpp:37, decompress.tpp:34
+//   * per-block t selection with the libstdc++ unordered_map iteration order
+//     (the only way to reproduce the reference's float summation order),
+//     get_AB_map.tpp:5-24, get_h_psnr.tpp:5-31, compress.tpp:90-144
+//   * Huffman with the reference tree (min (freq, node index) merging),
+//     huffman.hpp:189-246, 263-381, 650-783
+//   * the pre-Zstd stream layout, compress.tpp:243-291, decompress.tpp:88-143
+//
+// The process runs with MXCSR FTZ|DAZ while inside these functions, as the
+// reference's -Ofast link (crtfastmath.o) does for its whole process.
+//
+// Parity status of each function is pinned by tests/test_oracle_golden.py
+// against streams and reconstructions written by the reference CLI built from
+// its own sources (oracle/Makefile, target "ref").
+
+#include <xmmintrin.h>
+#include <pmmintrin.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cfloat>
+#include <map>
+#include <queue>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- fp helpers
+
+struct FtzDaz {
+    unsigned saved;
+    FtzDaz() : saved(_mm_getcsr()) { _mm_setcsr(saved | 0x8040u); }
+    ~FtzDaz() { _mm_setcsr(saved); }
+};
+
+// GCC -Ofast expands lround/llround inline as trunc(y + copysign(pred(0.5), y))
+// followed by cvttsd2si (out-of-range -> INT64_MIN).
+inline int64_t lround_d(double y) {
+    double s = y + std::copysign(0.49999999999999994, y);
+    if (!(s > -9223372036854775808.0 && s < 9223372036854775808.0)) return INT64_MIN;
+    return (int64_t)s;
+}
+inline int64_t lround_f(float y) {
+    float s = y + std::copysign(0.49999997f, y);
+    if (!(s > -9223372036854775808.0f && s < 9223372036854775808.0f)) return INT64_MIN;
+    return (int64_t)s;
+}
+
+// ---------------------------------------------------------------- geometry
+
+struct Geometry {
+    uint64_t D[3];
+    int L;
+    uint8_t axis[64], order[64];
+    uint64_t pivot_stride[3], block_shape[3], block_num[3];
+};
+
+// generate_config.tpp:51-68
+Geometry make_geometry(const uint64_t dims[3], int L, const uint8_t* axis, const uint8_t* order, uint64_t bpr) {
+    Geometry g;
+    for (int a = 0; a < 3; a++) { g.D[a] = dims[a]; g.pivot_stride[a] = 1; }
+    g.L = L;
+    for (int r = 0; r < L; r++) {
+        g.axis[r] = axis[r];
+        g.order[r] = order[r];
+        g.pivot_stride[axis[r]] *= 2;
+    }
+    for (int a = 0; a < 3; a++) {
+        g.block_shape[a] = g.pivot_stride[a] * bpr;
+        g.block_num[a] = (g.D[a] + g.block_shape[a] - 1) / g.block_shape[a];
+    }
+    return g;
+}
+
+// asz_param.hpp:105-115
+void layer_qb_shape(const uint64_t lo[3], const uint64_t hi[3], const uint64_t stride[3], int d, uint64_t out[3]) {
+    for (int a = 0; a < 3; a++) {
+        uint64_t start = a == d ? lo[a] + stride[a] / 2 : (lo[a] == 0 ? 0 : lo[a] + stride[a]);
+        out[a] = start >= hi[a] ? 0 : (hi[a] - start + stride[a] - 1) / stride[a];
+    }
+}
+
+// asz_param.hpp:122-129  (order 0 = "120", 1 = "021", 2 = "012")
+inline uint64_t qb_index(uint64_t j0, uint64_t j1, uint64_t j2, const uint64_t n[3], int order) {
+    if (order == 0) return (j1 * n[2] + j2) * n[0] + j0;
+    if (order == 1) return (j0 * n[2] + j2) * n[1] + j1;
+    return (j0 * n[1] + j1) * n[2] + j2;
+}
+
+// ---------------------------------------------------------------- predictor
+
+// compress.tpp:9-34 / decompress.tpp:8-33 with GCC's contraction (see header).
+inline float predict(const float* r, uint64_t i, uint64_t h, uint64_t right, uint64_t pos, uint64_t dist,
+                     float t, bool quintic) {
+    if (quintic && i >= 5 * h && i + 5 * h < right) {
+        float s1 = r[pos - dist] + r[pos + dist];
+        float s3 = r[pos - 3 * dist] + r[pos + 3 * dist];
+        float s5 = r[pos - 5 * dist] + r[pos + 5 * dist];
+        double acc = (double)s3 * (-25.0 / 256);
+        acc = std::fma((double)s1, 150.0 / 256, acc);
+        acc = std::fma((double)s5, 3.0 / 256, acc);
+        return (float)acc;
+    }
+    float a = r[pos - dist];
+    if (i <= h) {
+        if (i + h >= right) return a;
+        float b = r[pos + dist];
+        if (i + 3 * h >= right) return (a + b) * 0.5f;
+        float e = r[pos + 3 * dist];
+        return (std::fmaf(a, 3.0f, b * 6.0f) - e) * 0.125f;
+    }
+    float c = r[pos - 3 * dist];
+    if (i + h >= right) return std::fmaf(a, 3.0f, -c) * 0.5f;
+    float b = r[pos + dist];
+    if (i + 3 * h >= right) return std::fmaf(b, 3.0f, std::fmaf(a, 6.0f, -c)) * 0.125f;
+    float e = r[pos + 3 * dist];
+    double td = (double)t;
+    float p0 = (float)((0.25 - td) * (double)(e + c));
+    return (float)std::fma(td + 0.25, (double)(a + b), (double)p0);
+}
+
+// compress.tpp:35 (both GCC forms, see header)
+inline int32_t quantize(float p, float x, float eb, int d) {
+    double diff = (double)p - (double)x;
+    double y = d == 0 ? (0.5 / (double)eb) * diff : (diff * 0.5) / (double)eb;
+    return (int32_t)lround_d(y);
+}
+inline float reconstruct(float p, int32_t q, float eb) {
+    return (float)std::fma(-(2.0 * (double)eb), (double)q, (double)p);
+}
+
+// ---------------------------------------------------------------- t selection
+
+struct KeyHash {  // asz_param.hpp:22-40
+    size_t operator()(const std::tuple<int64_t, int64_t>& k) const {
+        int64_t h = 0;
+        h = (int64_t)((uint64_t)h * 1145141919810ull + (uint64_t)std::get<0>(k));
+        h = (int64_t)((uint64_t)h * 1145141919810ull + (uint64_t)std::get<1>(k));
+        return (size_t)h;
+    }
+};
+using ABMap = std::unordered_map<std::tuple<int64_t, int64_t>, uint64_t, KeyHash>;
+
+struct Ctx {
+    Geometry g;
+    const float* data;
+    float* recon;
+    bool quintic;
+    float min_val, max_val, abs_eb, stat_eb_ratio;
+    uint64_t h1_num;
+};
+
+// get_AB_map.tpp:5-24; values at the target come from the original field
+// (compressed_data still holds data there, compress.tpp:151).
+void get_ab_map(const Ctx& c, const float* nb, ABMap& m, const uint64_t lo[3], const uint64_t hi[3],
+                const uint64_t stride[3], uint64_t dist, float neg_2_stat_eb) {
+    uint64_t vol = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    uint64_t den = stride[2] * (stride[0] * stride[1] * c.h1_num);
+    uint64_t ss = (uint64_t)std::cbrt((double)((float)vol / (float)den));
+    if (ss == 0) ss = 1;
+    const uint64_t D1 = c.g.D[1], D2 = c.g.D[2];
+    for (uint64_t i0 = lo[0]; i0 < hi[0]; i0 += stride[0] * ss)
+        for (uint64_t i1 = lo[1]; i1 < hi[1]; i1 += stride[1] * ss)
+            for (uint64_t i2 = lo[2]; i2 < hi[2]; i2 += stride[2] * ss) {
+                uint64_t pos = (i0 * D1 + i1) * D2 + i2;
+                float a = nb[pos - dist], cc = nb[pos - 3 * dist];
+                float b = nb[pos + dist], e = nb[pos + 3 * dist];
+                float A = (a + b) - (cc + e);
+                float S = (b + e) + (a + cc);
+                float B = (float)std::fma((double)S, 0.25, -(double)c.data[pos]);
+                int64_t ak = lround_f(neg_2_stat_eb * A);
+                int64_t bk = lround_f(neg_2_stat_eb * B);
+                m[std::make_tuple(ak, bk)]++;
+            }
+}
+
+// get_h_psnr.tpp:5-31 in GCC's reassociated form
+void get_h1_psnr(const Ctx& c, const ABMap& m, float t, float abs_eb, float stat_eb, float& h1, float& psnr) {
+    float neg_2_eb = 0.5f / abs_eb;
+    float two_t = t * 2.0f;
+    float two_eb = abs_eb * 2.0f;
+    std::unordered_map<int64_t, uint64_t> qb_map;
+    float mse = 0.0f;
+    uint64_t n = 0;
+    for (const auto& kv : m) {
+        float af = (float)std::get<0>(kv.first), bf = (float)std::get<1>(kv.first);
+        float err = std::fmaf(bf, 2.0f, af * two_t) * stat_eb;
+        int64_t q = lround_f(err * neg_2_eb);
+        qb_map[q] += kv.second;
+        float de = std::fmaf(-(float)q, two_eb, err);
+        double ded = (double)de;
+        mse = (float)std::fma((double)kv.second, ded * ded, (double)mse);
+        n += kv.second;
+    }
+    float dn = (float)n;
+    float rec = 1.0f / dn;
+    h1 = 0.0f;
+    for (const auto& kv : qb_map) {
+        float p = (float)kv.second * rec;
+        h1 = std::fmaf(-p, log2f(p), h1);
+    }
+    float range = c.max_val - c.min_val;
+    float r2 = range * range;
+    psnr = log10f((r2 / mse) * dn) * 10.0f;
+}
+
+// compress.tpp:90-144
+float select_block_t(const Ctx& c, int d, const uint64_t blo[3], const uint64_t bhi[3], const uint64_t stride[3],
+                     float layer_eb, float layer_slope, bool& fallback) {
+    static const float cand[5] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
+    fallback = false;
+    float best_t = 4.0f / 16;
+    if (bhi[d] <= blo[d] + 3 * stride[d]) { fallback = true; return best_t; }
+    uint64_t lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = blo[a];
+        hi[a] = bhi[a];
+        if (a != d && blo[a] != 0) lo[a] += stride[a];
+    }
+    lo[d] += 3 * stride[d] / 2;
+    hi[d] -= 3 * stride[d] / 2;
+    uint64_t dist = stride[d] / 2;
+    for (int a = d + 1; a < 3; a++) dist *= c.g.D[a];
+    float stat_eb = c.stat_eb_ratio * layer_eb;
+    float neg2 = 0.5f / stat_eb;
+    ABMap m;
+    get_ab_map(c, c.recon, m, lo, hi, stride, dist, neg2);
+    if (m.empty()) { fallback = true; return best_t; }
+    float best_h1 = 0, best_psnr = 0;
+    get_h1_psnr(c, m, best_t, layer_eb, stat_eb, best_h1, best_psnr);
+    for (float t : cand) {
+        if (t == best_t || best_psnr > FLT_MAX) continue;
+        float h1, psnr;
+        get_h1_psnr(c, m, t, layer_eb, stat_eb, h1, psnr);
+        float score = std::fmaf(best_h1 - h1, layer_slope, psnr - best_psnr);
+        if (score > 0.0f || psnr > FLT_MAX) { best_t = t; best_h1 = h1; best_psnr = psnr; }
+    }
+    return best_t;
+}
+
+// ---------------------------------------------------------------- bit I/O
+
+struct BitWriter {
+    std::vector<uint8_t>& out;
+    uint8_t cur = 0;
+    int fill = 0;
+    explicit BitWriter(std::vector<uint8_t>& o) : out(o) {}
+    void put(uint64_t v, int len) {  // LSB first
+        for (int k = 0; k < len; k++) {
+            cur |= (uint8_t)(((v >> k) & 1u) << fill);
+            if (++fill == 8) { out.push_back(cur); cur = 0; fill = 0; }
+        }
+    }
+    void flush() { if (fill) { out.push_back(cur); cur = 0; fill = 0; } }
+};
+
+void put_be64(std::vector<uint8_t>& o, uint64_t v) { for (int k = 7; k >= 0; k--) o.push_back((uint8_t)(v >> (8 * k))); }
+void put_le64(std::vector<uint8_t>& o, uint64_t v) { for (int k = 0; k < 8; k++) o.push_back((uint8_t)(v >> (8 * k))); }
+uint64_t get_be64(const uint8_t* p) { uint64_t v = 0; for (int k = 0; k < 8; k++) v = (v << 8) | p[k]; return v; }
+uint64_t get_le64(const uint8_t* p) { uint64_t v = 0; for (int k = 0; k < 8; k++) v |= (uint64_t)p[k] << (8 * k); return v; }
+
+// ---------------------------------------------------------------- Huffman
+
+struct HNode { uint64_t c; int64_t l, r; };
+
+// huffman.hpp:263-381 + save :650-691.  T is int32_t (QB arrays) or int64_t (t array).
+template <typename T>
+void huffman_encode(const T* v, uint64_t n, std::vector<uint8_t>& out) {
+    T mn = v[0], mx = v[0];
+    for (uint64_t i = 1; i < n; i++) { mn = std::min(mn, v[i]); mx = std::max(mx, v[i]); }
+    using U = typename std::make_unsigned<T>::type;
+    T maxval = (T)((U)mx - (U)mn + (U)1);
+    bool usemp = ((int64_t)maxval >= (1 << 12) && n < 2 * (uint64_t)(int64_t)mx) || (int64_t)maxval >= (1 << 28);
+    std::map<uint64_t, uint64_t> freq;
+    for (uint64_t i = 0; i < n; i++) freq[(uint64_t)(int64_t)(T)((U)v[i] - (U)mn)]++;
+    std::vector<HNode> ht;
+    std::vector<uint64_t> f;
+    for (auto& kv : freq) { ht.push_back({kv.first, -1, -1}); f.push_back(kv.second); }
+    uint64_t nleaf = ht.size();
+    T offset = mn;
+    if (nleaf == 1 || maxval == 1) {
+        offset = (T)((U)offset + (U)ht[0].c);
+        out.push_back((uint8_t)((usemp << 7) | (1 << 6) | 1));
+        put_le64(out, (uint64_t)(int64_t)offset);
+        put_be64(out, nleaf);
+        put_be64(out, 1);
+        put_be64(out, n ^ 0x1234abcdull);
+        return;
+    }
+    int mbft = 1;
+    while ((1ull << mbft) < (uint64_t)(int64_t)maxval) ++mbft;
+    // min-(freq, index) merging; internal nodes get increasing indices
+    using P = std::pair<uint64_t, uint64_t>;  // (freq, index)
+    std::priority_queue<P, std::vector<P>, std::greater<P>> q;
+    for (uint64_t i = 0; i < nleaf; i++) q.push({f[i], i});
+    while (q.size() > 1) {
+        P u = q.top(); q.pop();
+        P w = q.top(); q.pop();
+        ht.push_back({0, (int64_t)u.second, (int64_t)w.second});
+        q.push({u.first + w.first, ht.size() - 1});
+    }
+    uint64_t root = ht.size() - 1;
+    // codes: bit j = branch taken at depth j (huffman.hpp:80-95)
+    std::vector<uint64_t> code(nleaf, 0);
+    std::vector<int> len(nleaf, 0);
+    {
+        std::vector<std::tuple<uint64_t, uint64_t, int>> st{{root, 0, 0}};
+        while (!st.empty()) {
+            auto [u, cd, ln] = st.back();
+            st.pop_back();
+            if (ht[u].l < 0) { code[u] = cd; len[u] = ln; continue; }
+            if (ln >= 64) throw std::runtime_error("Huffman code longer than 64 bits");
+            st.push_back({(uint64_t)ht[u].r, cd | (1ull << ln), ln + 1});
+            st.push_back({(uint64_t)ht[u].l, cd, ln + 1});
+        }
+    }
+    out.push_back((uint8_t)((usemp << 7) | mbft));
+    put_le64(out, (uint64_t)(int64_t)offset);
+    put_be64(out, nleaf);
+    put_be64(out, (uint64_t)(int64_t)maxval);
+    {  // DFS preorder tree (huffman.hpp:669-690)
+        BitWriter bw(out);
+        std::vector<uint64_t> st{root};
+        while (!st.empty()) {
+            uint64_t u = st.back();
+            st.pop_back();
+            if (ht[u].l < 0) { bw.put(1, 1); bw.put(ht[u].c, mbft); }
+            else { bw.put(0, 1); st.push_back((uint64_t)ht[u].r); st.push_back((uint64_t)ht[u].l); }
+        }
+        bw.flush();
+    }
+    // symbol index -> leaf id
+    std::unordered_map<uint64_t, uint64_t> leaf_of;
+    for (uint64_t i = 0; i < nleaf; i++) leaf_of[ht[i].c] = i;
+    uint64_t bits = 0;
+    for (uint64_t i = 0; i < nleaf; i++) bits += f[i] * (uint64_t)len[i];
+    put_be64(out, bits ^ 0x1234abcdull);
+    BitWriter bw(out);
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t li = leaf_of[(uint64_t)(int64_t)(T)((U)v[i] - (U)mn)];
+        bw.put(code[li], len[li]);
+    }
+    bw.flush();
+}
+
+// huffman.hpp:383-526 + load :693-783.  Returns bytes consumed.
+template <typename T>
+uint64_t huffman_decode(const uint8_t* p, uint64_t avail, T* out, uint64_t count) {
+    const uint8_t* p0 = p;
+    if (avail < 25) throw std::runtime_error("truncated Huffman header");
+    uint8_t hb = *p++;
+    int mbft = hb & 0x3f;
+    int64_t offset = (int64_t)get_le64(p); p += 8;
+    uint64_t nleaf = get_be64(p); p += 8;
+    int64_t maxval = (int64_t)get_be64(p); p += 8;
+    if (nleaf == 0) return (uint64_t)(p - p0);
+    if (maxval == 1 || nleaf == 1) {
+        uint64_t cnt = get_be64(p) ^ 0x1234abcdull; p += 8;
+        if (count && cnt != count) throw std::runtime_error("Huffman payload length does not match its value count");
+        for (uint64_t i = 0; i < cnt; i++) out[i] = (T)offset;
+        return (uint64_t)(p - p0);
+    }
+    auto bit = [&](uint64_t i) { return (p[i >> 3] >> (i & 7)) & 1; };
+    std::vector<HNode> nodes;
+    nodes.reserve(2 * nleaf);
+    nodes.push_back({0, -1, -1});
+    std::vector<uint64_t> st{0};
+    uint64_t i = 1;
+    while (!st.empty()) {
+        uint64_t u = st.back();
+        int64_t v;
+        if (bit(i++) == 0) {
+            nodes.push_back({0, -1, -1});
+            v = (int64_t)nodes.size() - 1;
+            if (nodes[u].l < 0) nodes[u].l = v; else nodes[u].r = v;
+            st.push_back((uint64_t)v);
+        } else {
+            uint64_t c = 0;
+            for (int j = 0; j < mbft; j++) c |= (uint64_t)bit(i++) << j;
+            nodes.push_back({c, -1, -1});
+            v = (int64_t)nodes.size() - 1;
+            if (nodes[u].l < 0) nodes[u].l = v; else nodes[u].r = v;
+            while (!st.empty() && nodes[st.back()].r >= 0) st.pop_back();
+        }
+    }
+    p += (i + 7) >> 3;
+    uint64_t bits = get_be64(p) ^ 0x1234abcdull; p += 8;
+    uint64_t outn = 0, u = 0;
+    for (uint64_t b = 0; b < bits; b++) {
+        u = bit(b) ? (uint64_t)nodes[u].r : (uint64_t)nodes[u].l;
+        if (nodes[u].l < 0) {
+            if (count && outn >= count) throw std::runtime_error("Huffman payload contains too many values");
+            out[outn++] = (T)(offset + (int64_t)nodes[u].c);
+            u = 0;
+        }
+    }
+    if (u != 0) throw std::runtime_error("Huffman payload ends inside a code");
+    if (count && outn != count) throw std::runtime_error("Huffman payload contains too few values");
+    p += (bits + 7) >> 3;
+    return (uint64_t)(p - p0);
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// =============================================================== C ABI
+
+extern "C" {
+
+// param_c<float>::get_default_slope (get_default_slope.tpp:5-83).  info[0] =
+// max_invalid_ratio, info[1] = 1 if the fallback was used, info[2] = reason
+// (0 none, 1 invalid_ratio, 2 invalid_difference).  Returns the slope.
+float oracle_default_slope(const float* data, const uint64_t dims[3], float abs_eb, float stat_eb_ratio,
+                           float eb_span, float invalid_threshold, float fallback_slope, float min_val,
+                           float max_val, float t_min, float t_max, int t_precision, uint64_t h1_num, float* info) {
+    FtzDaz mode;
+    uint8_t ax[1] = {0}, od[1] = {0};
+    Geometry g = make_geometry(dims, 0, ax, od, 1);
+    Ctx c{g, data, const_cast<float*>(data), false, min_val, max_val, abs_eb, stat_eb_ratio, h1_num};
+    float ebs[2] = {abs_eb * eb_span, abs_eb / eb_span};
+    float base = abs_eb * stat_eb_ratio;
+    float stat_eb = base * 0.5f;
+    float neg2 = 1.0f / base;
+    std::vector<float> cand;
+    int64_t tlo = lround_f(t_min * (float)t_precision), thi = lround_f(t_max * (float)t_precision);
+    for (int64_t ti = tlo; ti <= thi; ti++) cand.push_back((float)ti / (float)t_precision);
+    float avg_h1[2] = {0, 0}, avg_psnr[2] = {0, 0};
+    uint64_t valid[2] = {0, 0};
+    for (int d = 0; d < 3; d++) {
+        uint64_t lo[3] = {0, 0, 0}, hi[3] = {g.D[0], g.D[1], g.D[2]}, st[3] = {1, 1, 1};
+        lo[d] = 3;
+        hi[d] = g.D[d] - 3;
+        st[d] = 2;
+        uint64_t dist = 1;
+        for (int a = d + 1; a < 3; a++) dist *= g.D[a];
+        ABMap m;
+        get_ab_map(c, data, m, lo, hi, st, dist, neg2);
+        for (int i = 0; i < 2; i++)
+            for (float t : cand) {
+                float h1, psnr;
+                get_h1_psnr(c, m, t, ebs[i], stat_eb, h1, psnr);
+                if (psnr > FLT_MAX) continue;
+                avg_h1[i] += h1;
+                avg_psnr[i] += psnr;
+                valid[i]++;
+            }
+    }
+    uint64_t total = 3 * cand.size();
+    float rec = 1.0f / (float)total;
+    float max_invalid = 0.0f;
+    for (int i = 0; i < 2; i++) max_invalid = std::max(max_invalid, std::fmaf(-(float)valid[i], rec, 1.0f));
+    info[0] = max_invalid;
+    info[1] = 0;
+    info[2] = 0;
+    if (max_invalid >= invalid_threshold) { info[1] = 1; info[2] = 1; return fallback_slope; }
+    for (int i = 0; i < 2; i++) { avg_h1[i] /= (float)valid[i]; avg_psnr[i] /= (float)valid[i]; }
+    float dh = avg_h1[1] - avg_h1[0];
+    float dp = avg_psnr[1] - avg_psnr[0];
+    float scale = std::max(std::fabs(avg_h1[1]), std::fabs(avg_h1[0]));
+    float tol = scale > 1.0f ? scale * 7.62939453125e-6f : 7.62939453125e-6f;
+    if (dh <= tol || dp <= 0.0f) { info[1] = 1; info[2] = 2; return fallback_slope; }
+    return dp / dh;
+}
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+// Sizes needed by the caller: total QB count (= N) and #t = L * #blocks.
+int oracle_geometry(const uint64_t dims[3], int L, const uint8_t* axis, uint64_t bpr, uint64_t* pivot_stride,
+                    uint64_t* block_shape, uint64_t* block_num) {
+    uint8_t ord[64] = {0};
+    Geometry g = make_geometry(dims, L, axis, ord, bpr);
+    for (int a = 0; a < 3; a++) {
+        pivot_stride[a] = g.pivot_stride[a];
+        block_shape[a] = g.block_shape[a];
+        block_num[a] = g.block_num[a];
+    }
+    return 0;
+}
+
+// param_c<float>::compress() core (compress.tpp:151-231): pivots, then block-major
+// rounds.  t_inject (nullable) holds t_int per (block, round) block-major,
+// round-minor; otherwise t is selected per block (compress.tpp:198).
+// Outputs: t_out[#blocks*L], qb_out[N] (pivot array then round arrays,
+// concatenated), qb_counts[L+1], recon[N].  Returns the t fallback count or -1.
+int64_t oracle_compress(const float* data, const uint64_t dims[3], int L, const uint8_t* axis, const uint8_t* order,
+                        int quintic, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
+                        float default_slope, uint64_t bpr, int t_precision, float stat_eb_ratio, uint64_t h1_num,
+                        float min_val, float max_val, const int64_t* t_inject, int64_t* t_out, int32_t* qb_out,
+                        uint64_t* qb_counts, float* recon) {
+    try {
+        FtzDaz mode;
+        Geometry g = make_geometry(dims, L, axis, order, bpr);
+        Ctx c{g, data, recon, quintic != 0, min_val, max_val, abs_eb, stat_eb_ratio, h1_num};
+        const uint64_t D0 = g.D[0], D1 = g.D[1], D2 = g.D[2];
+        std::vector<std::vector<int32_t>> qbs(L + 1);
+        // pivots (compress.tpp:161-175)
+        float pe = abs_eb * pivot_eb_ratio;
+        double inv = 0.5 / (double)pe, two = (double)pe * 2.0;
+        for (uint64_t i0 = 0; i0 < D0; i0 += g.pivot_stride[0])
+            for (uint64_t i1 = 0; i1 < D1; i1 += g.pivot_stride[1])
+                for (uint64_t i2 = 0; i2 < D2; i2 += g.pivot_stride[2]) {
+                    uint64_t pos = (i0 * D1 + i1) * D2 + i2;
+                    int32_t q = (int32_t)lround_d((double)data[pos] * inv);
+                    qbs[0].push_back(q);
+                    recon[pos] = (float)((double)q * two);
+                }
+        std::vector<float> layer_eb(L), layer_slope(L);
+        for (int r = 0; r < L; r++) {
+            layer_eb[r] = (float)((double)abs_eb * std::pow((double)eb_ratio, (double)(L - r - 1)));
+            layer_slope[r] = (float)((double)default_slope * std::pow((double)slope_ratio, (double)(L - r - 1)));
+        }
+        int64_t fallbacks = 0;
+        uint64_t tpos = 0;
+        for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
+            for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)
+                for (uint64_t b2 = 0; b2 < g.block_num[2]; b2++) {
+                    uint64_t blo[3] = {b0 * g.block_shape[0], b1 * g.block_shape[1], b2 * g.block_shape[2]};
+                    uint64_t bhi[3];
+                    for (int a = 0; a < 3; a++) bhi[a] = std::min(blo[a] + g.block_shape[a] + 1, g.D[a]);
+                    uint64_t stride[3] = {g.pivot_stride[0], g.pivot_stride[1], g.pivot_stride[2]};
+                    for (int r = 0; r < L; r++) {
+                        int d = g.axis[r];
+                        float t;
+                        if (t_inject) {
+                            t = (float)t_inject[tpos] / (float)t_precision;
+                            t_out[tpos] = t_inject[tpos];
+                        } else {
+                            bool fb = false;
+                            t = select_block_t(c, d, blo, bhi, stride, layer_eb[r], layer_slope[r], fb);
+                            fallbacks += fb;
+                            t_out[tpos] = lround_f(t * (float)t_precision);
+                        }
+                        tpos++;
+                        // block_compress_impl (compress.tpp:51-88)
+                        uint64_t n[3];
+                        layer_qb_shape(blo, bhi, stride, d, n);
+                        uint64_t first[3];
+                        for (int a = 0; a < 3; a++)
+                            first[a] = a == d ? blo[a] + stride[a] / 2 : (blo[a] == 0 ? 0 : blo[a] + stride[a]);
+                        std::vector<int32_t>& out = qbs[r + 1];
+                        uint64_t base = out.size();
+                        out.resize(base + n[0] * n[1] * n[2]);
+                        uint64_t h = stride[d] / 2;
+                        uint64_t dist = h;
+                        for (int a = d + 1; a < 3; a++) dist *= g.D[a];
+                        float eb = layer_eb[r];
+                        uint64_t j0 = 0;
+                        for (uint64_t i0 = first[0]; i0 < bhi[0]; i0 += stride[0], j0++) {
+                            uint64_t j1 = 0;
+                            for (uint64_t i1 = first[1]; i1 < bhi[1]; i1 += stride[1], j1++) {
+                                uint64_t j2 = 0;
+                                for (uint64_t i2 = first[2]; i2 < bhi[2]; i2 += stride[2], j2++) {
+                                    uint64_t pos = (i0 * D1 + i1) * D2 + i2;
+                                    uint64_t i = d == 0 ? i0 : d == 1 ? i1 : i2;
+                                    float p = predict(recon, i, h, bhi[d], pos, dist, t, c.quintic);
+                                    int32_t q = quantize(p, data[pos], eb, d);
+                                    out[base + qb_index(j0, j1, j2, n, g.order[r])] = q;
+                                    recon[pos] = reconstruct(p, q, eb);
+                                }
+                            }
+                        }
+                        stride[d] /= 2;
+                    }
+                }
+        uint64_t off = 0;
+        for (int k = 0; k <= L; k++) {
+            qb_counts[k] = qbs[k].size();
+            std::memcpy(qb_out + off, qbs[k].data(), qbs[k].size() * sizeof(int32_t));
+            off += qbs[k].size();
+        }
+        return fallbacks;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// param_c<float>::decompress() reconstruction (decompress.tpp:145-196).
+int oracle_decompress(const int32_t* qb_in, const uint64_t* qb_counts, const int64_t* t_int, const uint64_t dims[3],
+                      int L, const uint8_t* axis, const uint8_t* order, int quintic, float abs_eb,
+                      float pivot_eb_ratio, float eb_ratio, uint64_t bpr, int t_precision, float* recon) {
+    try {
+        FtzDaz mode;
+        Geometry g = make_geometry(dims, L, axis, order, bpr);
+        const uint64_t D0 = g.D[0], D1 = g.D[1], D2 = g.D[2];
+        std::vector<uint64_t> base(L + 1, 0), pos_k(L + 1, 0);
+        for (int k = 1; k <= L; k++) base[k] = base[k - 1] + qb_counts[k - 1];
+        float pe = abs_eb * pivot_eb_ratio;
+        double two = (double)pe * 2.0;
+        for (uint64_t i0 = 0; i0 < D0; i0 += g.pivot_stride[0])
+            for (uint64_t i1 = 0; i1 < D1; i1 += g.pivot_stride[1])
+                for (uint64_t i2 = 0; i2 < D2; i2 += g.pivot_stride[2]) {
+                    if (pos_k[0] >= qb_counts[0]) throw std::runtime_error("too few pivot QBs");
+                    recon[(i0 * D1 + i1) * D2 + i2] = (float)((double)qb_in[pos_k[0]++] * two);
+                }
+        std::vector<float> layer_eb(L);
+        for (int r = 0; r < L; r++)
+            layer_eb[r] = (float)((double)abs_eb * std::pow((double)eb_ratio, (double)(L - r - 1)));
+        uint64_t tpos = 0;
+        for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
+            for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)
+                for (uint64_t b2 = 0; b2 < g.block_num[2]; b2++) {
+                    uint64_t blo[3] = {b0 * g.block_shape[0], b1 * g.block_shape[1], b2 * g.block_shape[2]};
+                    uint64_t bhi[3];
+                    for (int a = 0; a < 3; a++) bhi[a] = std::min(blo[a] + g.block_shape[a] + 1, g.D[a]);
+                    uint64_t stride[3] = {g.pivot_stride[0], g.pivot_stride[1], g.pivot_stride[2]};
+                    for (int r = 0; r < L; r++) {
+                        int d = g.axis[r];
+                        float t = (float)t_int[tpos++] / (float)t_precision;
+                        uint64_t n[3];
+                        layer_qb_shape(blo, bhi, stride, d, n);
+                        uint64_t cnt = n[0] * n[1] * n[2];
+                        if (pos_k[r + 1] + cnt > qb_counts[r + 1]) throw std::runtime_error("compressed stream has too few QBs");
+                        const int32_t* q = qb_in + base[r + 1] + pos_k[r + 1];
+                        uint64_t first[3];
+                        for (int a = 0; a < 3; a++)
+                            first[a] = a == d ? blo[a] + stride[a] / 2 : (blo[a] == 0 ? 0 : blo[a] + stride[a]);
+                        uint64_t h = stride[d] / 2;
+                        uint64_t dist = h;
+                        for (int a = d + 1; a < 3; a++) dist *= g.D[a];
+                        uint64_t j0 = 0;
+                        for (uint64_t i0 = first[0]; i0 < bhi[0]; i0 += stride[0], j0++) {
+                            uint64_t j1 = 0;
+                            for (uint64_t i1 = first[1]; i1 < bhi[1]; i1 += stride[1], j1++) {
+                                uint64_t j2 = 0;
+                                for (uint64_t i2 = first[2]; i2 < bhi[2]; i2 += stride[2], j2++) {
+                                    uint64_t pos = (i0 * D1 + i1) * D2 + i2;
+                                    uint64_t i = d == 0 ? i0 : d == 1 ? i1 : i2;
+                                    float p = predict(recon, i, h, bhi[d], pos, dist, t, quintic != 0);
+                                    recon[pos] = reconstruct(p, q[qb_index(j0, j1, j2, n, g.order[r])], layer_eb[r]);
+                                }
+                            }
+                        }
+                        pos_k[r + 1] += cnt;
+                        stride[d] /= 2;
+                    }
+                }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Huffman-encode one array into out (caller supplies capacity). Returns bytes or -1.
+int64_t oracle_huffman_encode_i32(const int32_t* v, uint64_t n, uint8_t* out, uint64_t cap) {
+    try {
+        std::vector<uint8_t> o;
+        huffman_encode<int32_t>(v, n, o);
+        if (o.size() > cap) throw std::runtime_error("output buffer too small");
+        std::memcpy(out, o.data(), o.size());
+        return (int64_t)o.size();
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+int64_t oracle_huffman_decode_i32(const uint8_t* p, uint64_t avail, int32_t* out, uint64_t count) {
+    try { return (int64_t)huffman_decode<int32_t>(p, avail, out, count); }
+    catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Pre-Zstd stream (compress.tpp:243-291).  qb arrays concatenated, counts[L+1].
+int64_t oracle_build_payload(float min_val, float max_val, int L, const uint8_t* axis, const uint8_t* order,
+                             int quintic, const int64_t* t_int, uint64_t nt, const int32_t* qb,
+                             const uint64_t* counts, uint8_t* out, uint64_t cap) {
+    try {
+        std::vector<uint8_t> o;
+        uint8_t b4[4];
+        std::memcpy(b4, &min_val, 4); o.insert(o.end(), b4, b4 + 4);
+        std::memcpy(b4, &max_val, 4); o.insert(o.end(), b4, b4 + 4);
+        o.push_back((uint8_t)(L | 0x40 | (quintic ? 0x80 : 0)));
+        for (int r = 0; r < L; r++) { o.push_back((uint8_t)('0' + axis[r])); o.push_back((uint8_t)('0' + order[r])); }
+        put_le64(o, nt);
+        if (nt) huffman_encode<int64_t>(t_int, nt, o);
+        uint64_t off = 0;
+        for (int k = 0; k <= L; k++) {
+            put_le64(o, counts[k]);
+            if (counts[k]) huffman_encode<int32_t>(qb + off, counts[k], o);
+            off += counts[k];
+        }
+        if (o.size() > cap) throw std::runtime_error("output buffer too small");
+        std::memcpy(out, o.data(), o.size());
+        return (int64_t)o.size();
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Parse a pre-Zstd stream (decompress.tpp:88-143).  Call first with qb == NULL
+// to get L, quintic, axis/order, nt and counts; then with buffers to decode.
+int64_t oracle_parse_payload(const uint8_t* p, uint64_t n, float* min_max, int* L_out, uint8_t* axis, uint8_t* order,
+                             int* quintic, uint64_t* nt, int64_t* t_int, uint64_t* counts, int32_t* qb) {
+    try {
+        const uint8_t* p0 = p;
+        const uint8_t* end = p + n;
+        std::memcpy(min_max, p, 8); p += 8;
+        uint8_t f = *p++;
+        int L = (f & 0x40) ? (f & 0x3f) : f;
+        *quintic = (f & 0x40) ? ((f & 0x80) != 0) : 0;
+        *L_out = L;
+        for (int r = 0; r < L; r++) {
+            if (p[0] < '0' || p[0] > '2' || p[1] < '0' || p[1] > '2') throw std::runtime_error("invalid predictor");
+            axis[r] = (uint8_t)(p[0] - '0'); order[r] = (uint8_t)(p[1] - '0'); p += 2;
+        }
+        *nt = get_le64(p); p += 8;
+        std::vector<int64_t> tmp_t(*nt);
+        p += huffman_decode<int64_t>(p, (uint64_t)(end - p), t_int ? t_int : tmp_t.data(), *nt);
+        uint64_t off = 0;
+        for (int k = 0; k <= L; k++) {
+            counts[k] = get_le64(p); p += 8;
+            if (!counts[k]) continue;
+            if (qb) p += huffman_decode<int32_t>(p, (uint64_t)(end - p), qb + off, counts[k]);
+            else {
+                std::vector<int32_t> tmp(counts[k]);
+                p += huffman_decode<int32_t>(p, (uint64_t)(end - p), tmp.data(), counts[k]);
+            }
+            off += counts[k];
+        }
+        return (int64_t)(p - p0);
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+}  // extern "C"
