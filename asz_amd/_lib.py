@@ -1,0 +1,95 @@
+"""ctypes binding of lib/libasz_gpu.so (declarations in include/asz_gpu.h).
+
+The library is the product: every compute call goes to its HIP kernels.  If it
+is not built, importing this module raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libasz_gpu.so")
+BIN_DIR = os.path.join(HERE, "bin")
+
+ASZ_OK = 0
+ASZ_TSEL_GPU, ASZ_TSEL_EXACT, ASZ_TSEL_INJECT = 0, 1, 2
+MAX_LAYERS = 63
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("rel_eb", ctypes.c_float), ("pivot_eb_ratio", ctypes.c_float), ("eb_ratio", ctypes.c_float),
+                ("slope_ratio", ctypes.c_float), ("block_pivot_ratio", ctypes.c_uint64),
+                ("t_precision", ctypes.c_int32), ("stat_eb_ratio", ctypes.c_float),
+                ("h1_block_sampling_num", ctypes.c_uint64), ("t_min", ctypes.c_float), ("t_max", ctypes.c_float),
+                ("default_slope_eb_span", ctypes.c_float), ("default_slope_invalid_ratio_threshold", ctypes.c_float),
+                ("default_slope_fallback", ctypes.c_float), ("default_slope", ctypes.c_float),
+                ("quintic", ctypes.c_int32)]
+
+
+class Plan(ctypes.Structure):
+    _fields_ = [("layers", ctypes.c_int32), ("axis", ctypes.c_uint8 * MAX_LAYERS),
+                ("order", ctypes.c_uint8 * MAX_LAYERS)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("range_ms", ctypes.c_double), ("preprocess_ms", ctypes.c_double), ("tsel_ms", ctypes.c_double),
+                ("rounds_ms", ctypes.c_double), ("huffman_ms", ctypes.c_double),
+                ("huffman_host_ms", ctypes.c_double), ("total_gpu_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double), ("max_err", ctypes.c_double), ("sq_err_sum", ctypes.c_double),
+                ("stream_bytes", ctypes.c_uint64), ("t_count", ctypes.c_uint64), ("t_fallbacks", ctypes.c_uint64),
+                ("layers", ctypes.c_uint64), ("qb_counts", ctypes.c_uint64 * (MAX_LAYERS + 1))]
+
+
+# exported entry points (must match include/asz_gpu.h)
+EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_default", "asz_get_stats",
+           "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
+           "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
+           "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
+           "asz_round_layout")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.asz_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_P)]
+    L.asz_ctx_destroy.argtypes = [_P]
+    L.asz_ctx_destroy.restype = None
+    L.asz_last_error.argtypes = [_P]
+    L.asz_last_error.restype = ctypes.c_char_p
+    L.asz_params_default.argtypes = [ctypes.POINTER(Params)]
+    L.asz_params_default.restype = None
+    L.asz_get_stats.argtypes = [_P, ctypes.POINTER(Stats)]
+    L.asz_device_alloc.argtypes = [_P, _U64, ctypes.POINTER(_P)]
+    L.asz_device_free.argtypes = [_P, _P]
+    L.asz_copy_to_device.argtypes = [_P, _P, _P, _U64]
+    L.asz_copy_to_host.argtypes = [_P, _P, _P, _U64]
+    L.asz_geometry.argtypes = [_P, ctypes.POINTER(Plan), _U64, _P, _P, _P]
+    L.asz_range_f32.argtypes = [_P, _P, _U64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.asz_default_slope.argtypes = [_P, _P, _P, ctypes.POINTER(Params), ctypes.c_float, ctypes.c_float,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.asz_compress.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_float,
+                               ctypes.c_float, ctypes.c_int, _P, ctypes.POINTER(_P), _U64P, _P, _P]
+    L.asz_compress_host.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_int, _P,
+                                    ctypes.POINTER(_P), _U64P, _P, _P, _P]
+    L.asz_decompress.argtypes = [_P, _P, _U64, _P, ctypes.POINTER(Params), _P, _P, _P]
+    L.asz_decompress_host.argtypes = [_P, _P, _U64, _P, ctypes.POINTER(Params), _P]
+    L.asz_huffman_codebook.argtypes = [_P, _U64, ctypes.c_int64, _U64, ctypes.c_int, _P, _U64, _U64P, _P, _P,
+                                       _U64P]
+    L.asz_zstd_compress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
+    L.asz_zstd_decompress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
+    L.asz_free.argtypes = [_P]
+    L.asz_free.restype = None
+    _lib = L
+    return L

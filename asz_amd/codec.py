@@ -1,0 +1,232 @@
+"""Python front end of the MI355X ASZ core (over the C ABI in include/asz_gpu.h).
+
+Mirrors asz::param_c<float>'s interface for this path: a parameter block with
+the reference's defaults (asz_param.hpp:147-183), a predictor plan of
+"cubic<axis>_<order>" strings, compress() producing the pre-Zstd stream of
+compress.tpp:243-291, decompress() consuming it (decompress.tpp:88-196), and
+errors raised with the reference's messages.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import ASZ_TSEL_EXACT, ASZ_TSEL_GPU, ASZ_TSEL_INJECT, Params, Plan, Stats
+
+ORDERS = ("120", "021", "012")
+TSEL = {"gpu": ASZ_TSEL_GPU, "exact": ASZ_TSEL_EXACT, "inject": ASZ_TSEL_INJECT}
+
+
+class AszError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def make_params(**kw) -> Params:
+    p = Params()
+    _lib.lib().asz_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "family":
+            p.quintic = 1 if v == "quintic" else 0
+            continue
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def make_plan(predictors) -> Plan:
+    """["cubic0_012", ...] (or (axis, order) pairs) -> Plan."""
+    pl = Plan()
+    pl.layers = len(predictors)
+    for r, pr in enumerate(predictors):
+        if isinstance(pr, str):
+            if len(pr) != 10 or not pr.startswith("cubic") or pr[5] not in "012" or pr[7:] not in ORDERS:
+                raise ValueError(f"invalid predictor in plan: {pr}")
+            pl.axis[r] = int(pr[5])
+            pl.order[r] = ORDERS.index(pr[7:])
+        else:
+            pl.axis[r], pl.order[r] = int(pr[0]), int(pr[1])
+    return pl
+
+
+def plan_strings(axis, order):
+    return [f"cubic{a}_{ORDERS[o]}" for a, o in zip(axis, order)]
+
+
+@dataclass
+class CompressResult:
+    stream: bytes                       # pre-Zstd Huffman stream
+    min_val: float
+    max_val: float
+    stats: Stats
+    qb: np.ndarray | None = None        # pivot array then round arrays
+    t: np.ndarray | None = None         # t_int per (block, round), block-major
+    recon: np.ndarray | None = None
+    counts: list = field(default_factory=list)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One HIP device and stream; calls are serialised on it."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.lib()
+        self._c = ctypes.c_void_p()
+        rc = self._L.asz_ctx_create(device, ctypes.byref(self._c))
+        if rc != 0:
+            raise AszError(rc, "cannot create an ASZ context on HIP device %d" % device)
+
+    def close(self):
+        if self._c:
+            self._L.asz_ctx_destroy(self._c)
+            self._c = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AszError(rc, self._L.asz_last_error(self._c).decode())
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._L.asz_get_stats(self._c, ctypes.byref(s))
+        return s
+
+    # ------------------------------------------------------------ device helpers
+    def range(self, d_ptr: int, n: int):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        self._check(self._L.asz_range_f32(self._c, ctypes.c_void_p(d_ptr), n, ctypes.byref(mn), ctypes.byref(mx)))
+        return mn.value, mx.value
+
+    def default_slope(self, d_ptr: int, dims, params: Params, mn: float, mx: float):
+        d = np.asarray(dims, np.uint64)
+        s, inv = ctypes.c_float(), ctypes.c_float()
+        fb, reason = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.asz_default_slope(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(params), mn, mx,
+                                              ctypes.byref(s), ctypes.byref(inv), ctypes.byref(fb),
+                                              ctypes.byref(reason)))
+        return s.value, inv.value, bool(fb.value), ("none", "invalid_ratio", "invalid_difference")[reason.value]
+
+    def geometry(self, dims, plan: Plan, bpr: int):
+        d = np.asarray(dims, np.uint64)
+        ps, bs, bn = (np.zeros(3, np.uint64) for _ in range(3))
+        self._check(self._L.asz_geometry(_ptr(d), ctypes.byref(plan), bpr, _ptr(ps), _ptr(bs), _ptr(bn)))
+        return ps, bs, bn
+
+    # ------------------------------------------------------------ compression
+    def compress_device(self, d_ptr: int, dims, plan: Plan, params: Params, mn: float, mx: float, tsel="gpu",
+                        t_inject=None, want_qb=False, want_t=False) -> CompressResult:
+        """d_ptr: device float32 field, overwritten with the reconstruction."""
+        d = np.asarray(dims, np.uint64)
+        N = int(np.prod(d))
+        _, _, bn = self.geometry(dims, plan, params.block_pivot_ratio)
+        nt = int(np.prod(bn)) * plan.layers
+        ti = None if t_inject is None else np.ascontiguousarray(t_inject, np.int64)
+        qb = np.zeros(N, np.int32) if want_qb else None
+        t = np.zeros(nt, np.int64) if want_t else None
+        sp, sn = ctypes.c_void_p(), ctypes.c_uint64()
+        self._check(self._L.asz_compress(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
+                                         ctypes.byref(params), mn, mx, TSEL[tsel], _ptr(ti), ctypes.byref(sp),
+                                         ctypes.byref(sn), _ptr(qb), _ptr(t)))
+        stream = ctypes.string_at(sp, sn.value)
+        st = self.stats()
+        return CompressResult(stream, mn, mx, st, qb, t, None, [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
+
+    def compress(self, field_host: np.ndarray, plan: Plan, params: Params, tsel="gpu", t_inject=None,
+                 want_qb=False, want_t=False, want_recon=False) -> CompressResult:
+        f = np.ascontiguousarray(field_host, np.float32)
+        d = np.asarray(f.shape, np.uint64)
+        _, _, bn = self.geometry(f.shape, plan, params.block_pivot_ratio)
+        nt = int(np.prod(bn)) * plan.layers
+        ti = None if t_inject is None else np.ascontiguousarray(t_inject, np.int64)
+        qb = np.zeros(f.size, np.int32) if want_qb else None
+        t = np.zeros(nt, np.int64) if want_t else None
+        recon = np.zeros_like(f) if want_recon else None
+        sp, sn = ctypes.c_void_p(), ctypes.c_uint64()
+        self._check(self._L.asz_compress_host(self._c, _ptr(f), _ptr(d), ctypes.byref(plan), ctypes.byref(params),
+                                              TSEL[tsel], _ptr(ti), ctypes.byref(sp), ctypes.byref(sn), _ptr(recon),
+                                              _ptr(qb), _ptr(t)))
+        stream = ctypes.string_at(sp, sn.value)
+        st = self.stats()
+        mn, mx = np.frombuffer(stream[:8], np.float32)
+        return CompressResult(stream, float(mn), float(mx), st, qb, t, recon,
+                              [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
+
+    # ------------------------------------------------------------ decompression
+    def decompress(self, stream: bytes, dims, params: Params, want_qb=False, want_t=False, nt=None):
+        d = np.asarray(dims, np.uint64)
+        N = int(np.prod(d))
+        buf = np.frombuffer(stream, np.uint8)
+        out = np.zeros(tuple(int(x) for x in dims), np.float32)
+        if not want_qb and not want_t:
+            self._check(self._L.asz_decompress_host(self._c, _ptr(buf), buf.size, _ptr(d), ctypes.byref(params),
+                                                    _ptr(out)))
+            return out
+        import torch  # device buffer for the reconstruction
+        dev = torch.empty(N, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        qb = np.zeros(N, np.int32) if want_qb else None
+        t = np.zeros(nt, np.int64) if want_t else None
+        self._check(self._L.asz_decompress(self._c, _ptr(buf), buf.size, _ptr(d), ctypes.byref(params),
+                                           ctypes.c_void_p(dev.data_ptr()), _ptr(qb), _ptr(t)))
+        out[...] = dev.cpu().numpy().reshape(out.shape)
+        return out, qb, t
+
+
+def huffman_codebook(hist, offset: int, count: int, is_int64=False):
+    """Host codebook + header bytes of one array (no GPU needed)."""
+    L = _lib.lib()
+    h = np.ascontiguousarray(hist, np.uint64)
+    rng = h.size
+    hdr = np.zeros(64 + 4 * rng * 8, np.uint8)
+    hn = ctypes.c_uint64()
+    codes = np.zeros(rng, np.uint64)
+    lens = np.zeros(rng, np.uint8)
+    bits = ctypes.c_uint64()
+    rc = L.asz_huffman_codebook(_ptr(h), rng, offset, count, int(is_int64), _ptr(hdr), hdr.size, ctypes.byref(hn),
+                                _ptr(codes), _ptr(lens), ctypes.byref(bits))
+    if rc != 0:
+        raise AszError(rc, "asz_huffman_codebook failed")
+    return hdr[:hn.value].tobytes(), codes, lens, bits.value
+
+
+def zstd_compress(payload: bytes) -> bytes:
+    L = _lib.lib()
+    buf = np.frombuffer(payload, np.uint8)
+    out, n = ctypes.c_void_p(), ctypes.c_uint64()
+    if L.asz_zstd_compress(_ptr(buf), buf.size, ctypes.byref(out), ctypes.byref(n)) != 0:
+        raise AszError(-2, "ZSTD_compress failed")
+    try:
+        return ctypes.string_at(out, n.value)
+    finally:
+        L.asz_free(out)
+
+
+def zstd_decompress(frame: bytes) -> bytes:
+    L = _lib.lib()
+    buf = np.frombuffer(frame, np.uint8)
+    out, n = ctypes.c_void_p(), ctypes.c_uint64()
+    if L.asz_zstd_decompress(_ptr(buf), buf.size, ctypes.byref(out), ctypes.byref(n)) != 0:
+        raise AszError(-2, "ZSTD_decompress error in decompress()")
+    try:
+        return ctypes.string_at(out, n.value)
+    finally:
+        L.asz_free(out)

@@ -1,0 +1,818 @@
+// Host orchestration of the GPU compression core and the C ABI (asz_gpu.h).
+//
+// compress (param_c::compress, compress.tpp:146-299 without Zstd):
+//   pivots -> for each round: t (GPU selection / exact host scoring / injected)
+//   -> round kernel -> per-array QB range (fused into the round kernels)
+//   -> one histogram launch over all arrays -> host reference-tree codebooks
+//   -> chunk bit counts -> per-array scan -> encode into a stream buffer that
+//   already holds every header -> one D2H of the whole pre-Zstd stream.
+// decompress (param_c::decompress, decompress.tpp:88-196): host parses the
+//   headers and builds decode tables; the GPU decodes every array and runs
+//   pivots and rounds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "internal.hpp"
+#include "launch.hpp"
+#include "launch_misc.hpp"
+#include "scoring.hpp"
+
+#define HIPCHECK(x)                                                                               \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            ::asz::fail(ASZ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+namespace asz {
+
+// ------------------------------------------------------------------ device buffers
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <typename T>
+    T* get(size_t n) {
+        size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+        if (bytes > cap) {
+            if (p) HIPCHECK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) fail(ASZ_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+            cap = bytes;
+        }
+        return static_cast<T*>(p);
+    }
+};
+
+struct Timer {
+    hipEvent_t a, b;
+    explicit Timer() { (void)hipEventCreate(&a); (void)hipEventCreate(&b); }
+    ~Timer() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+};
+
+}  // namespace asz
+
+struct asz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    asz_stats stats{};
+    asz::DevBuf work, sym, tabs, ss, tblk, acc, hist, book, arrays, chunk_arr, hchunk_arr, chunk_bits, chunk_off,
+        words, keys, key_off, misc, dec_tab, dec_arrays, sub_arr, start, exit0, exit1, count, out_pos;
+    std::vector<uint8_t> host_stream;
+    hipEvent_t ev[8];
+};
+
+namespace asz {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+void put_le64(std::vector<uint8_t>& o, uint64_t v) {
+    for (int k = 0; k < 8; k++) o.push_back((uint8_t)(v >> (8 * k)));
+}
+uint64_t get_le64(const uint8_t* p) {
+    uint64_t v = 0;
+    for (int k = 0; k < 8; k++) v |= (uint64_t)p[k] << (8 * k);
+    return v;
+}
+
+void validate_params(const asz_params& p) {
+    if (!(p.rel_eb > 0 && std::isfinite(p.rel_eb))) fail(ASZ_ERR_ARG, "relative error bound must be positive and finite");
+    if (!(p.pivot_eb_ratio > 0 && p.pivot_eb_ratio <= 1)) fail(ASZ_ERR_ARG, "pivot_eb_ratio must be in (0,1]");
+    if (!(p.eb_ratio > 0 && p.eb_ratio <= 1)) fail(ASZ_ERR_ARG, "eb_ratio must be in (0,1]");
+    if (!(p.slope_ratio > 0 && p.slope_ratio <= 1)) fail(ASZ_ERR_ARG, "slope_ratio must be in (0,1]");
+    if (p.block_pivot_ratio == 0) fail(ASZ_ERR_ARG, "block_pivot_ratio must be positive");
+    if (p.t_precision < 16 || p.t_precision % 16 != 0) fail(ASZ_ERR_ARG, "t_precision must be a positive multiple of 16");
+}
+
+PivotDesc pivot_desc(const Plan& P) {
+    PivotDesc D{};
+    for (int a = 0; a < 3; a++) {
+        D.D[a] = (uint32_t)P.g.D[a];
+        D.ps[a] = (uint32_t)P.g.pivot_stride[a];
+        D.pn[a] = P.pn[a];
+    }
+    D.inv_half_pe = 0.5 / (double)P.pivot_eb;
+    D.two_pe = (double)P.pivot_eb * 2.0;
+    return D;
+}
+
+void check_grid(const RoundDesc& R) {
+    if (R.n[0] > 65535 || R.n[1] > 65535) fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
+}
+
+float abs_eb_of(float rel_eb, float mn, float mx) {
+    float range = daz(daz(mx) - daz(mn));
+    return daz(daz(rel_eb) * range);
+}
+
+// Layout of the Huffman-coded arrays: array 0 = t array (int64 semantics),
+// arrays 1..L+1 = pivot array and round arrays.
+struct Layout {
+    std::vector<HuffArray> arrays;
+    std::vector<HuffArrayDev> dev;
+    std::vector<uint64_t> block_off;  // byte offset of each array's HUF block (after its u64 count)
+    uint64_t total_bytes = 0;
+    int max_len = 0;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ range
+
+static void range_scan(asz_ctx* c, const float* d_field, uint64_t n, float* mn, float* mx) {
+    uint32_t* r = c->misc.get<uint32_t>(8);
+    uint32_t init[3] = {0xFFFFFFFFu, 0u, 0u};
+    HIPCHECK(hipMemcpyAsync(r, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    launch_range(d_field, n, r, c->stream);
+    uint32_t res[3];
+    HIPCHECK(hipMemcpyAsync(res, r, sizeof(res), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (res[2]) fail(ASZ_ERR_DATA, "input contains a nonfinite value");
+    auto key_to_float = [](uint32_t k) {
+        uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        float f;
+        std::memcpy(&f, &b, 4);
+        return f;
+    };
+    float vmin = key_to_float(res[0]), vmax = key_to_float(res[1]);
+    if (vmin == 0.0f || vmax == 0.0f) {
+        // ties among zeros / denormals under DAZ: minmax_element keeps the first
+        // smallest and the last largest element, with their own bits
+        unsigned long long* zi = reinterpret_cast<unsigned long long*>(r + 4);
+        unsigned long long zinit[2] = {~0ull, 0ull};
+        HIPCHECK(hipMemcpyAsync(zi, zinit, sizeof(zinit), hipMemcpyHostToDevice, c->stream));
+        launch_zero_index(d_field, n, zi, c->stream);
+        unsigned long long idx[2];
+        HIPCHECK(hipMemcpyAsync(idx, zi, sizeof(idx), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (vmin == 0.0f) HIPCHECK(hipMemcpy(&vmin, d_field + idx[0], 4, hipMemcpyDeviceToHost));
+        if (vmax == 0.0f) HIPCHECK(hipMemcpy(&vmax, d_field + idx[1], 4, hipMemcpyDeviceToHost));
+    }
+    if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, "max_val==min_val");
+    *mn = vmin;
+    *mx = vmax;
+}
+
+// ------------------------------------------------------------------ default slope
+
+static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_t dims[3], const asz_params& p,
+                                 float mn, float mx) {
+    const float aeb = abs_eb_of(p.rel_eb, mn, mx);
+    const float base = aeb * p.stat_eb_ratio;
+    std::vector<std::vector<int64_t>> keys(3);
+    for (int d = 0; d < 3; d++) {
+        SlopeSampleDesc S{};
+        uint64_t lo[3] = {0, 0, 0}, hi[3] = {dims[0], dims[1], dims[2]}, st[3] = {1, 1, 1};
+        lo[d] = 3;
+        hi[d] = dims[d] - 3;
+        st[d] = 2;
+        uint64_t dist = 1;
+        for (int a = d + 1; a < 3; a++) dist *= dims[a];
+        uint64_t vol = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+        uint64_t den = st[2] * (st[0] * st[1] * p.h1_block_sampling_num);
+        uint64_t ss = (uint64_t)std::cbrt((double)((float)vol / (float)den));
+        if (ss == 0) ss = 1;
+        for (int a = 0; a < 3; a++) {
+            S.D[a] = (uint32_t)dims[a];
+            S.lo[a] = (uint32_t)lo[a];
+            S.hi[a] = (uint32_t)hi[a];
+            S.step[a] = (uint32_t)(st[a] * ss);
+            S.cnt[a] = (uint32_t)((hi[a] - lo[a] + S.step[a] - 1) / S.step[a]);
+        }
+        S.dist = dist;
+        S.neg_2_stat_eb = 1.0f / base;
+        uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
+        int64_t* dk = c->keys.get<int64_t>(2 * ns + 2);
+        launch_slope_sample(S, d_field, dk, c->stream);
+        keys[d].resize(2 * ns);
+        HIPCHECK(hipMemcpyAsync(keys[d].data(), dk, 2 * ns * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    return default_slope_from_keys(keys, aeb, p.stat_eb_ratio, p.default_slope_eb_span,
+                                   p.default_slope_invalid_ratio_threshold, p.default_slope_fallback, p.t_min, p.t_max,
+                                   p.t_precision, mn, mx);
+}
+
+// ------------------------------------------------------------------ compress
+
+static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
+                          const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
+                          int32_t* qb_out, int64_t* t_out) {
+    auto wall0 = Clock::now();
+    validate_params(p);
+    hipStream_t st = c->stream;
+    Geometry g = make_geometry(dims, plan, p.block_pivot_ratio);
+    const float aeb = abs_eb_of(p.rel_eb, mn, mx);
+    Plan P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope, p.h1_block_sampling_num);
+    const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
+    asz_stats& S = c->stats;
+    S = asz_stats{};
+    S.layers = L;
+    S.t_count = nt;
+
+    // symbol buffer: t array (nt) then pivot + round QB arrays (N)
+    int32_t* sym = c->sym.get<int32_t>(nt + N + 16);
+    int32_t* qb = sym + nt;
+    uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
+    HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
+    uint32_t* ss = c->ss.get<uint32_t>(P.ss.size() + 1);
+    HIPCHECK(hipMemcpyAsync(ss, P.ss.data(), P.ss.size() * 4, hipMemcpyHostToDevice, st));
+    float* tblk = c->tblk.get<float>(nt + 1);
+
+    // accumulators: error stats, fallback counter, QB range per array (t array + L+1)
+    const uint64_t narr = L + 2;
+    DevAcc init{};
+    for (uint64_t k = 0; k < narr; k++) { init.arr[k].qmin = INT32_MAX; init.arr[k].qmax = INT32_MIN; }
+    DevAcc* acc = c->acc.get<DevAcc>(1);
+    HIPCHECK(hipMemcpyAsync(acc, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    auto aacc = [&](uint64_t k) { return (void*)&acc->arr[k]; };
+    void* eacc = (void*)&acc->err;
+
+    if (tsel_mode == ASZ_TSEL_INJECT) {
+        if (!t_inject) fail(ASZ_ERR_ARG, "t_inject is required with ASZ_TSEL_INJECT");
+        std::vector<float> tf(nt);
+        std::vector<int32_t> ti(nt);
+        for (uint64_t b = 0; b < nbl; b++)
+            for (uint64_t r = 0; r < L; r++) {
+                int64_t v = t_inject[b * L + r];
+                tf[r * nbl + b] = (float)v / (float)p.t_precision;
+                ti[b * L + r] = (int32_t)v;
+            }
+        HIPCHECK(hipMemcpyAsync(tblk, tf.data(), nt * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(sym, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
+    }
+
+    HIPCHECK(hipEventRecord(c->ev[0], st));
+    PivotDesc PD = pivot_desc(P);
+    launch_pivots(true, d_work, qb, PD, eacc, aacc(1), st);
+    double tsel_host_ms = 0;
+    std::vector<int64_t> hkeys;
+    std::vector<uint64_t> koff;
+    for (uint64_t r = 0; r < L; r++) {
+        const RoundDesc& R = P.rounds[r];
+        check_grid(R);
+        float* tb = tblk + r * nbl;
+        if (tsel_mode == ASZ_TSEL_GPU) {
+            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
+            launch_tsel(T, d_work, ss + r * nbl, tb, sym, &acc->fallbacks, nbl, st);
+        } else if (tsel_mode == ASZ_TSEL_EXACT) {
+            auto h0 = Clock::now();
+            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
+            koff.assign(nbl + 1, 0);
+            for (uint64_t b = 0; b < nbl; b++) koff[b + 1] = koff[b] + P.ns[r * nbl + b];
+            uint64_t* dko = c->key_off.get<uint64_t>(nbl + 1);
+            int64_t* dk = c->keys.get<int64_t>(2 * koff[nbl] + 2);
+            HIPCHECK(hipMemcpyAsync(dko, koff.data(), (nbl + 1) * 8, hipMemcpyHostToDevice, st));
+            launch_tsel_keys(T, d_work, ss + r * nbl, dko, dk, nbl, st);
+            hkeys.resize(2 * koff[nbl] + 2);
+            HIPCHECK(hipMemcpyAsync(hkeys.data(), dk, 2 * koff[nbl] * 8, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            std::vector<float> tf(nbl);
+            std::vector<int32_t> ti(nbl);
+            const float stat_eb = p.stat_eb_ratio * R.eb;
+            for (uint64_t b = 0; b < nbl; b++) {
+                uint64_t n = koff[b + 1] - koff[b];
+                float t = 0.25f;
+                if (P.ss[r * nbl + b] == 0 || n == 0) S.t_fallbacks++;
+                else t = select_t_exact(hkeys.data() + 2 * koff[b], n, R.eb, stat_eb, R.layer_slope, mn, mx);
+                tf[b] = t;
+                float prod = t * (float)p.t_precision;
+                ti[b] = (int32_t)(prod + std::copysign(0.49999997f, prod));
+            }
+            HIPCHECK(hipMemcpyAsync(tb, tf.data(), nbl * 4, hipMemcpyHostToDevice, st));
+            // scatter into the block-major t array
+            std::vector<int32_t> cur(nt);
+            if (r > 0) {
+                HIPCHECK(hipMemcpyAsync(cur.data(), sym, nt * 4, hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipStreamSynchronize(st));
+            }
+            for (uint64_t b = 0; b < nbl; b++) cur[b * L + r] = ti[b];
+            HIPCHECK(hipMemcpyAsync(sym, cur.data(), nt * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            tsel_host_ms += ms_since(h0);
+        }
+        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, eacc, aacc(2 + r), 1, st);
+    }
+    HIPCHECK(hipGetLastError());
+    // t array range
+    launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
+    HIPCHECK(hipEventRecord(c->ev[1], st));
+
+    DevAcc hacc;
+    HIPCHECK(hipMemcpyAsync(&hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    S.max_err = hacc.err.max_err;
+    S.sq_err_sum = hacc.err.sq_sum;
+    if (tsel_mode == ASZ_TSEL_GPU) S.t_fallbacks = hacc.fallbacks;
+    if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
+
+    // ---------------------------------------------------------------- Huffman
+    auto hh0 = Clock::now();
+    Layout lay;
+    lay.arrays.resize(narr);
+    lay.dev.resize(narr);
+    uint64_t hist_total = 0, hchunks = 0, chunks = 0;
+    for (uint64_t k = 0; k < narr; k++) {
+        HuffArray& A = lay.arrays[k];
+        HuffArrayDev& D = lay.dev[k];
+        std::memset(&D, 0, sizeof(D));
+        const int32_t q[2] = {hacc.arr[k].qmin, hacc.arr[k].qmax};
+        A.count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
+        A.is_int64 = k == 0;
+        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+        D.count = A.count;
+        if (A.count == 0) continue;
+        A.offset = q[0];
+        A.range = (uint64_t)((int64_t)q[1] - (int64_t)q[0] + 1);
+        if (A.range > (1ull << 24)) fail(ASZ_ERR_DATA, "QB alphabet wider than 2^24 symbols");
+        D.offset = A.offset;
+        D.range = A.range;
+        D.hist_off = hist_total;
+        hist_total += A.range;
+        D.hchunk0 = hchunks;
+        hchunks += (A.count + kHistChunk - 1) / kHistChunk;
+        D.chunk0 = chunks;
+        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
+        chunks += D.nchunk;
+    }
+    std::vector<uint32_t> harr(hchunks), carr(chunks);
+    for (uint64_t k = 0; k < narr; k++) {
+        const HuffArrayDev& D = lay.dev[k];
+        if (!D.count) continue;
+        for (uint64_t i = 0; i < (D.count + kHistChunk - 1) / kHistChunk; i++) harr[D.hchunk0 + i] = (uint32_t)k;
+        for (uint64_t i = 0; i < D.nchunk; i++) carr[D.chunk0 + i] = (uint32_t)k;
+    }
+    uint32_t* dhist = c->hist.get<uint32_t>(hist_total + 1);
+    HuffArrayDev* darr = c->arrays.get<HuffArrayDev>(narr);
+    uint32_t* dharr = c->hchunk_arr.get<uint32_t>(hchunks + 1);
+    uint32_t* dcarr = c->chunk_arr.get<uint32_t>(chunks + 1);
+    HIPCHECK(hipMemsetAsync(dhist, 0, hist_total * 4, st));
+    HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(dharr, harr.data(), hchunks * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), chunks * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipEventRecord(c->ev[2], st));
+    launch_hist(sym, darr, dharr, hchunks, dhist, st);
+    HIPCHECK(hipEventRecord(c->ev[3], st));
+    std::vector<uint32_t> hh(hist_total);
+    HIPCHECK(hipMemcpyAsync(hh.data(), dhist, hist_total * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+
+    // codebooks and stream layout (compress.tpp:243-291)
+    std::vector<uint8_t> skel;
+    skel.resize(8);
+    std::memcpy(skel.data(), &mn, 4);
+    std::memcpy(skel.data() + 4, &mx, 4);
+    skel.push_back((uint8_t)(L | 0x40 | (p.quintic ? 0x80 : 0)));
+    for (uint64_t r = 0; r < L; r++) {
+        skel.push_back((uint8_t)('0' + g.axis[r]));
+        skel.push_back((uint8_t)('0' + g.order[r]));
+    }
+    std::vector<uint64_t> book(hist_total);
+    for (uint64_t k = 0; k < narr; k++) {
+        HuffArray& A = lay.arrays[k];
+        HuffArrayDev& D = lay.dev[k];
+        put_le64(skel, A.count);
+        if (!A.count) continue;
+        build_codebook(A, hh.data() + D.hist_off);
+        skel.insert(skel.end(), A.header.begin(), A.header.end());
+        D.single = A.single ? 1 : 0;
+        if (!A.single) {
+            D.payload_bit = (uint64_t)skel.size() * 8;
+            skel.resize(skel.size() + ((A.payload_bits + 7) >> 3), 0);
+            for (uint64_t s = 0; s < A.range; s++) {
+                if (A.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
+                book[D.hist_off + s] = A.code[s] | ((uint64_t)A.len[s] << 56);
+                lay.max_len = std::max<int>(lay.max_len, A.len[s]);
+            }
+        }
+    }
+    lay.total_bytes = skel.size();
+    S.stream_bytes = lay.total_bytes;
+    for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lay.arrays[k + 1].count;
+    const uint64_t nwords = (lay.total_bytes + 3) / 4 + 4;
+    uint32_t* dwords = c->words.get<uint32_t>(nwords);
+    uint64_t* dbook = c->book.get<uint64_t>(hist_total + 1);
+    uint32_t* dcbits = c->chunk_bits.get<uint32_t>(chunks + 1);
+    uint64_t* dcoff = c->chunk_off.get<uint64_t>(chunks + 1);
+    HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
+    HIPCHECK(hipMemcpyAsync(dwords, skel.data(), skel.size(), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(dbook, book.data(), hist_total * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
+    double hhost_ms = ms_since(hh0);
+    HIPCHECK(hipEventRecord(c->ev[4], st));
+    launch_chunk_bits(sym, darr, dcarr, chunks, dbook, dcbits, st);
+    launch_chunk_scan(darr, (uint32_t)narr, dcbits, dcoff, st);
+    launch_encode(sym, darr, dcarr, chunks, dbook, dcoff, dwords, lay.max_len, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev[5], st));
+    c->host_stream.resize(lay.total_bytes);
+    HIPCHECK(hipMemcpyAsync(c->host_stream.data(), dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
+    if (qb_out) HIPCHECK(hipMemcpyAsync(qb_out, qb, N * 4, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> t32;
+    if (t_out) {
+        t32.resize(nt);
+        HIPCHECK(hipMemcpyAsync(t32.data(), sym, nt * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    if (t_out)
+        for (uint64_t i = 0; i < nt; i++) t_out[i] = t32[i];
+    S.rounds_ms = ev_ms(c->ev[0], c->ev[1]);
+    S.huffman_ms = ev_ms(c->ev[2], c->ev[3]) + ev_ms(c->ev[4], c->ev[5]);
+    S.huffman_host_ms = hhost_ms;
+    S.tsel_ms = tsel_host_ms;
+    S.total_gpu_ms = S.rounds_ms + S.huffman_ms;
+    S.wall_ms = ms_since(wall0);
+}
+
+// ------------------------------------------------------------------ decompress
+
+struct Parsed {
+    float mn, mx;
+    asz_plan plan{};
+    bool quintic;
+    uint64_t nt;
+    std::vector<DecodeInfo> arrays;   // t array then L+1 QB arrays
+    std::vector<uint64_t> counts;     // symbols per array
+};
+
+static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p) {
+    Parsed P;
+    if (n < 9) fail(ASZ_ERR_DATA, "compressed stream is truncated");
+    std::memcpy(&P.mn, s, 4);
+    std::memcpy(&P.mx, s + 4, 4);
+    uint8_t f = s[8];
+    uint64_t L;
+    if (f & 0x40) {
+        P.quintic = (f & 0x80) != 0;
+        L = f & 0x3F;
+    } else {
+        P.quintic = p.quintic != 0;
+        L = f;
+    }
+    uint64_t off = 9;
+    if (L == 0 || L > ASZ_MAX_LAYERS || off + 2 * L + 8 > n) fail(ASZ_ERR_DATA, "compressed stream is truncated");
+    P.plan.layers = (int32_t)L;
+    for (uint64_t r = 0; r < L; r++) {
+        uint8_t a = s[off], o = s[off + 1];
+        if (a < '0' || a > '2' || o < '0' || o > '2') fail(ASZ_ERR_DATA, "compressed stream contains an invalid predictor");
+        P.plan.axis[r] = (uint8_t)(a - '0');
+        P.plan.order[r] = (uint8_t)(o - '0');
+        off += 2;
+    }
+    for (uint64_t k = 0; k < L + 2; k++) {
+        if (off + 8 > n) fail(ASZ_ERR_DATA, "compressed stream is truncated");
+        uint64_t cnt = get_le64(s + off);
+        off += 8;
+        P.counts.push_back(cnt);
+        if (k == 0) P.nt = cnt;
+        if (cnt == 0 && k > 0) { P.arrays.push_back(DecodeInfo{}); continue; }
+        DecodeInfo di = parse_huffman(s, n, off, cnt);
+        off = di.end_off;
+        P.arrays.push_back(std::move(di));
+    }
+    return P;
+}
+
+static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint64_t dims[3], const asz_params& p,
+                            float* d_out, int32_t* qb_out, int64_t* t_out) {
+    auto wall0 = Clock::now();
+    validate_params(p);
+    hipStream_t st = c->stream;
+    asz_stats& S = c->stats;
+    S = asz_stats{};
+    Parsed Q = parse_stream(s, n, p);
+    Geometry g = make_geometry(dims, Q.plan, p.block_pivot_ratio);
+    const float aeb = abs_eb_of(p.rel_eb, Q.mn, Q.mx);
+    Plan P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, 1.0f, p.h1_block_sampling_num);
+    const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
+    if (Q.nt < nt) fail(ASZ_ERR_DATA, "compressed stream contains too few t values");
+    if (Q.counts[1] != P.pivot_count) fail(ASZ_ERR_DATA, "compressed stream has a wrong pivot count");
+    for (uint64_t r = 0; r < L; r++)
+        if (Q.counts[r + 2] < P.rounds[r].count) fail(ASZ_ERR_DATA, "compressed stream has too few QBs");
+    S.layers = L;
+    S.t_count = Q.nt;
+    S.stream_bytes = n;
+
+    // symbols: t array (Q.nt) then the L+1 QB arrays as stored
+    uint64_t total_sym = 0;
+    std::vector<uint64_t> out_off(L + 2);
+    for (uint64_t k = 0; k < L + 2; k++) { out_off[k] = total_sym; total_sym += Q.counts[k]; }
+    int32_t* sym = c->sym.get<int32_t>(total_sym + 16);
+    // decode tables and descriptors
+    std::vector<uint32_t> tables;
+    std::vector<DecArrayDev> da;
+    std::vector<uint32_t> sub_arr;
+    for (uint64_t k = 0; k < L + 2; k++) {
+        const DecodeInfo& di = Q.arrays[k];
+        if (Q.counts[k] == 0) continue;
+        if (di.single) {
+            if (di.single_count != Q.counts[k])
+                fail(ASZ_ERR_DATA, "Huffman payload length does not match its value count");
+            launch_fill_i32(sym + out_off[k], Q.counts[k], (int32_t)di.offset, st);
+            continue;
+        }
+        if (di.max_len > 64) fail(ASZ_ERR_DATA, "Huffman code longer than 64 bits");
+        DecArrayDev D{};
+        D.payload_bit = di.payload_off * 8;
+        D.bits = di.payload_bits;
+        D.count = Q.counts[k];
+        D.out_off = out_off[k];
+        D.table_off = tables.size();
+        D.offset = di.offset;
+        D.root_bits = (uint32_t)std::min(di.max_len, kRootBits);
+        D.max_len = (uint32_t)di.max_len;
+        D.sub0 = sub_arr.size();
+        D.nsub = (di.payload_bits + kSubBits - 1) / kSubBits;
+        tables.insert(tables.end(), di.table.begin(), di.table.end());
+        for (uint64_t i = 0; i < D.nsub; i++) sub_arr.push_back((uint32_t)da.size());
+        da.push_back(D);
+    }
+    const uint64_t nsub = sub_arr.size();
+    const uint64_t nwords = (n + 3) / 4 + 4;
+    uint32_t* dwords = c->words.get<uint32_t>(nwords);
+    HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
+    HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
+    uint32_t* dtab = c->dec_tab.get<uint32_t>(tables.size() + 1);
+    HIPCHECK(hipMemcpyAsync(dtab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
+    DecArrayDev* dda = c->dec_arrays.get<DecArrayDev>(da.size() + 1);
+    HIPCHECK(hipMemcpyAsync(dda, da.data(), da.size() * sizeof(DecArrayDev), hipMemcpyHostToDevice, st));
+    uint32_t* dsub = c->sub_arr.get<uint32_t>(nsub + 1);
+    HIPCHECK(hipMemcpyAsync(dsub, sub_arr.data(), nsub * 4, hipMemcpyHostToDevice, st));
+    uint64_t* dstart = c->start.get<uint64_t>(nsub + 1);
+    uint64_t* dex0 = c->exit0.get<uint64_t>(nsub + 1);
+    uint64_t* dex1 = c->exit1.get<uint64_t>(nsub + 1);
+    uint32_t* dcnt = c->count.get<uint32_t>(nsub + 1);
+    uint64_t* dpos = c->out_pos.get<uint64_t>(nsub + 1);
+    unsigned int* flags = c->misc.get<unsigned int>(8);
+    HIPCHECK(hipMemsetAsync(flags, 0, 8 * 4, st));
+    HIPCHECK(hipEventRecord(c->ev[0], st));
+    launch_dec_sync(dwords, nwords, dda, dsub, nsub, dtab, dstart, dex0, dcnt, st);
+    uint64_t iters = 0;
+    for (;;) {
+        HIPCHECK(hipMemsetAsync(flags, 0, 4, st));
+        launch_dec_fix(dwords, nwords, dda, dsub, nsub, dtab, dstart, dex0, dex1, dcnt, flags, st);
+        std::swap(dex0, dex1);
+        unsigned int changed = 0;
+        HIPCHECK(hipMemcpyAsync(&changed, flags, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        iters++;
+        if (!changed) break;
+        if (iters > nsub + 2) fail(ASZ_ERR_DATA, "Huffman decoder did not synchronise");
+    }
+    launch_dec_scan(dda, (uint32_t)da.size(), dcnt, dpos, flags + 1, st);
+    launch_dec_write(dwords, nwords, dda, dsub, nsub, dtab, dstart, dpos, sym, flags + 1, st);
+    HIPCHECK(hipEventRecord(c->ev[1], st));
+    unsigned int derr = 0;
+    HIPCHECK(hipMemcpyAsync(&derr, flags + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (derr & 1) fail(ASZ_ERR_DATA, "Huffman payload length does not match its value count");
+    if (derr & 2) fail(ASZ_ERR_DATA, "Huffman payload contains too many values");
+    if (derr & 4) fail(ASZ_ERR_DATA, "Huffman payload ends inside a code");
+
+    // reconstruction: t floats, pivots, rounds (block-major QB order within each round array)
+    uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
+    HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
+    float* tblk = c->tblk.get<float>(nt + 1);
+    HIPCHECK(hipEventRecord(c->ev[2], st));
+    launch_t_float(sym, nbl, (uint32_t)L, (float)p.t_precision, tblk, st);
+    PivotDesc PD = pivot_desc(P);
+    launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, nullptr, st);
+    for (uint64_t r = 0; r < L; r++) {
+        RoundDesc R = P.rounds[r];
+        check_grid(R);
+        R.qb_off = 0;
+        launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, st);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev[3], st));
+    if (qb_out)
+        HIPCHECK(hipMemcpyAsync(qb_out, sym + out_off[1], (total_sym - out_off[1]) * 4, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> t32;
+    if (t_out) {
+        t32.resize(Q.nt);
+        HIPCHECK(hipMemcpyAsync(t32.data(), sym, Q.nt * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    if (t_out)
+        for (uint64_t i = 0; i < Q.nt; i++) t_out[i] = t32[i];
+    for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = Q.counts[k + 1];
+    S.huffman_ms = ev_ms(c->ev[0], c->ev[1]);
+    S.rounds_ms = ev_ms(c->ev[2], c->ev[3]);
+    S.total_gpu_ms = S.huffman_ms + S.rounds_ms;
+    S.wall_ms = ms_since(wall0);
+    (void)N;
+}
+
+}  // namespace asz
+
+// ================================================================== C ABI
+
+using asz::fail;
+
+template <typename F>
+static int guarded(asz_ctx* c, F&& f) {
+    try {
+        f();
+        return ASZ_OK;
+    } catch (const asz::Error& e) {
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host allocation failed";
+        return ASZ_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return ASZ_ERR_DATA;
+    }
+}
+
+extern "C" {
+
+void asz_params_default(asz_params* p) {
+    p->rel_eb = 1e-3f;
+    p->pivot_eb_ratio = 0.05f;
+    p->eb_ratio = 0.76f;
+    p->slope_ratio = 0.54f;
+    p->block_pivot_ratio = 4;
+    p->t_precision = 32;
+    p->stat_eb_ratio = 0.2f;
+    p->h1_block_sampling_num = 256;
+    p->t_min = 1.0f / 16;
+    p->t_max = 6.0f / 16;
+    p->default_slope_eb_span = 1.5f;
+    p->default_slope_invalid_ratio_threshold = 1.0f;
+    p->default_slope_fallback = 1000.0f;
+    p->default_slope = 1000.0f;
+    p->quintic = 0;
+}
+
+int asz_ctx_create(int device, asz_ctx** out) {
+    *out = nullptr;
+    asz_ctx* c = new asz_ctx();
+    int rc = guarded(c, [&] {
+        int n = 0;
+        HIPCHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(ASZ_ERR_ARG, "no such HIP device");
+        HIPCHECK(hipSetDevice(device));
+        c->device = device;
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+    });
+    if (rc != ASZ_OK) {
+        std::string msg = c->err;
+        delete c;
+        fprintf(stderr, "asz_ctx_create: %s\n", msg.c_str());
+        return rc;
+    }
+    *out = c;
+    return ASZ_OK;
+}
+
+void asz_ctx_destroy(asz_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* asz_last_error(asz_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int asz_get_stats(asz_ctx* c, asz_stats* out) {
+    *out = c->stats;
+    return ASZ_OK;
+}
+
+int asz_device_alloc(asz_ctx* c, uint64_t bytes, void** d_ptr) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess)
+            fail(ASZ_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    });
+}
+
+int asz_device_free(asz_ctx* c, void* d_ptr) {
+    return guarded(c, [&] { HIPCHECK(hipFree(d_ptr)); });
+}
+
+int asz_copy_to_device(asz_ctx* c, void* d_dst, const void* src, uint64_t bytes) {
+    return guarded(c, [&] {
+        HIPCHECK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int asz_copy_to_host(asz_ctx* c, void* dst, const void* d_src, uint64_t bytes) {
+    return guarded(c, [&] {
+        HIPCHECK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int asz_geometry(const uint64_t dims[3], const asz_plan* plan, uint64_t bpr, uint64_t ps[3], uint64_t bs[3],
+                 uint64_t bn[3]) {
+    return guarded(nullptr, [&] {
+        asz::Geometry g = asz::make_geometry(dims, *plan, bpr);
+        for (int a = 0; a < 3; a++) { ps[a] = g.pivot_stride[a]; bs[a] = g.block_shape[a]; bn[a] = g.block_num[a]; }
+    });
+}
+
+int asz_range_f32(asz_ctx* c, const float* d_field, uint64_t n, float* mn, float* mx) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (n == 0) fail(ASZ_ERR_ARG, "empty field");
+        auto t0 = asz::Clock::now();
+        asz::range_scan(c, d_field, n, mn, mx);
+        c->stats.range_ms = asz::ms_since(t0);
+    });
+}
+
+int asz_default_slope(asz_ctx* c, const float* d_field, const uint64_t dims[3], const asz_params* p, float mn,
+                      float mx, float* slope, float* inv, int* fb, int* reason) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        auto t0 = asz::Clock::now();
+        asz::SlopeResult r = asz::default_slope(c, d_field, dims, *p, mn, mx);
+        for (const auto& w : r.warnings)
+            printf("Warning: for rel_eb %f, only %zu/%zu valid default-slope cases were found\n", (double)w.rel,
+                   (size_t)w.valid, (size_t)w.total);
+        *slope = r.slope;
+        *inv = r.max_invalid_ratio;
+        *fb = r.fallback ? 1 : 0;
+        *reason = r.reason;
+        c->stats.preprocess_ms = asz::ms_since(t0);
+    });
+}
+
+int asz_compress(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan* plan, const asz_params* p,
+                 float mn, float mx, int tsel_mode, const int64_t* t_inject, const uint8_t** stream,
+                 uint64_t* stream_bytes, int32_t* qb_out, int64_t* t_out) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        asz::compress_impl(c, d_work, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
+        *stream = c->host_stream.data();
+        *stream_bytes = c->host_stream.size();
+    });
+}
+
+int asz_compress_host(asz_ctx* c, const float* field, const uint64_t dims[3], const asz_plan* plan,
+                      const asz_params* p, int tsel_mode, const int64_t* t_inject, const uint8_t** stream,
+                      uint64_t* stream_bytes, float* recon, int32_t* qb_out, int64_t* t_out) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        const uint64_t N = dims[0] * dims[1] * dims[2];
+        float* w = c->work.get<float>(N);
+        HIPCHECK(hipMemcpyAsync(w, field, N * 4, hipMemcpyHostToDevice, c->stream));
+        float mn, mx;
+        asz::range_scan(c, w, N, &mn, &mx);
+        asz::compress_impl(c, w, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
+        if (recon) {
+            HIPCHECK(hipMemcpyAsync(recon, w, N * 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+        }
+        *stream = c->host_stream.data();
+        *stream_bytes = c->host_stream.size();
+    });
+}
+
+int asz_decompress(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,
+                   float* d_out, int32_t* qb_out, int64_t* t_out) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        asz::decompress_impl(c, stream, n, dims, *p, d_out, qb_out, t_out);
+    });
+}
+
+int asz_decompress_host(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,
+                        float* out) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        const uint64_t N = dims[0] * dims[1] * dims[2];
+        float* w = c->work.get<float>(N);
+        asz::decompress_impl(c, stream, n, dims, *p, w, nullptr, nullptr);
+        HIPCHECK(hipMemcpyAsync(out, w, N * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

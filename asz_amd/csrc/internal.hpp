@@ -1,0 +1,130 @@
+// Internal declarations shared by the host orchestration (codec.cpp) and the
+// HIP kernels.  Field layout in HBM: one float32 array, C order (DIM2 fastest),
+// the same as the reference's data / compressed_data vectors.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/asz_gpu.h"
+
+namespace asz {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+// ---------------------------------------------------------------- geometry
+
+// Plan-derived geometry (generate_config.tpp:51-68).
+struct Geometry {
+    uint64_t D[3];
+    int L;
+    uint8_t axis[ASZ_MAX_LAYERS], order[ASZ_MAX_LAYERS];
+    uint64_t pivot_stride[3], block_shape[3], block_num[3];
+    uint64_t nblocks() const { return block_num[0] * block_num[1] * block_num[2]; }
+    uint64_t N() const { return D[0] * D[1] * D[2]; }
+};
+Geometry make_geometry(const uint64_t dims[3], const asz_plan& plan, uint64_t bpr);
+
+// Per-axis block tables of one round.  cnt[b] = targets of block b along the
+// axis, pre[b] = exclusive prefix of cnt (asz_param.hpp:105-115 per axis).
+// The QB base of block (b0,b1,b2) in the round's array is
+// pre0[b0]*C1*C2 + cnt0[b0]*(pre1[b1]*C2 + cnt1[b1]*pre2[b2]), which is the
+// block-major concatenation of compress.tpp:61-62.
+struct RoundDesc {
+    uint32_t D[3];
+    uint32_t s[3];      // strides of this round, before stride[d] is halved
+    uint32_t h;         // s[d] / 2
+    uint32_t n[3];      // global target counts per axis
+    uint32_t B[3];      // block shape
+    uint32_t nb[3];     // block counts
+    uint32_t C[3];      // per-axis target totals (sum of cnt)
+    uint32_t tab_off[3];// offsets of the cnt/pre tables of each axis in the table buffer
+    uint32_t d, order;
+    uint64_t dist;      // h * prod_{a > d} D[a]
+    uint64_t qb_off;    // offset of this round's array in the concatenated QB buffer
+    uint64_t count;     // targets of this round (= C0*C1*C2)
+    float eb;           // layer error bound
+    float layer_slope;  // layer default slope (compress.tpp:195)
+    double inv_half_eb; // 0.5 / eb           (axis-0 quantizer form)
+    double eb_d;        // (double) eb         (axes-1/2 quantizer form)
+    double neg_two_eb;  // -(2.0 * (double) eb) (reconstruction)
+};
+
+struct Plan {
+    Geometry g;
+    std::vector<RoundDesc> rounds;
+    std::vector<uint32_t> tables;  // per round, per axis: cnt[nb] then pre[nb]
+    std::vector<uint32_t> ss;      // t-selection sampling stride per (round, block): r * nblocks + b
+    std::vector<uint32_t> ns;      // t-selection sample count per (round, block), 0 for fallback blocks
+    uint64_t pivot_count;
+    uint32_t pn[3];                // pivot lattice counts per axis
+    float pivot_eb;
+};
+Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
+               float default_slope, uint64_t h1_num);
+
+// Accumulators written by the pivot / round kernels (device memory).
+struct ArrayAcc {
+    int32_t qmin, qmax;  // QB range of one array, feeds the Huffman histogram
+    uint32_t pad[2];
+};
+struct ErrAcc {
+    double max_err;      // max |x - x'|
+    double sq_sum;       // sum (x - x')^2
+};
+struct DevAcc {
+    ErrAcc err;
+    uint32_t fallbacks, pad;
+    ArrayAcc arr[ASZ_MAX_LAYERS + 2];  // [0] t array, [1] pivots, [2 + r] round r
+};
+
+// ---------------------------------------------------------------- fp helpers (host)
+
+// Reference process mode is FTZ/DAZ (crtfastmath.o from the -Ofast link).
+inline float daz(float x) {
+    return (x != 0.0f && x > -1.17549435e-38f && x < 1.17549435e-38f) ? (x < 0 ? -0.0f : 0.0f) : x;
+}
+
+// ---------------------------------------------------------------- Huffman (host)
+
+struct HuffArray {
+    // inputs
+    uint64_t count = 0;          // symbols in the array
+    int64_t offset = 0;          // min symbol
+    uint64_t range = 0;          // max - min + 1
+    bool is_int64 = false;
+    // outputs of build_codebook
+    std::vector<uint8_t> header; // type byte, offset, n, maxval, tree bits (or the single-symbol form)
+    std::vector<uint64_t> code;  // per symbol index: code (LSB = first bit)
+    std::vector<uint8_t> len;    // per symbol index: length, 0 = absent
+    uint64_t payload_bits = 0;
+    bool single = false;         // maxval == 1 path (huffman.hpp:193-210, 342-347)
+};
+// Build the reference tree codebook from a dense histogram (huffman.hpp:263-320, 650-691).
+void build_codebook(HuffArray& a, const uint32_t* hist);
+
+// Decode tables built from a serialized tree (huffman.hpp:693-783).
+struct DecodeInfo {
+    int64_t offset = 0;
+    uint64_t nleaf = 0;
+    bool single = false;
+    uint64_t single_count = 0;
+    uint64_t payload_bits = 0;
+    uint64_t payload_off = 0;    // byte offset of the payload in the stream
+    uint64_t end_off = 0;        // byte offset after this array
+    int max_len = 0;
+    std::vector<uint32_t> table; // multi-level lookup, see huffman_host.cpp
+};
+// Parse one HUF<T> block at stream offset off. Throws on malformed input.
+DecodeInfo parse_huffman(const uint8_t* s, uint64_t n, uint64_t off, uint64_t count);
+
+// Lookup-table geometry for the GPU decoder.
+constexpr int kRootBits = 12;
+
+}  // namespace asz

@@ -1,0 +1,105 @@
+// Host-callable launchers of the HIP kernels (defined in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace asz {
+
+struct PivotDesc {
+    uint32_t D[3];
+    uint32_t ps[3];   // pivot stride
+    uint32_t pn[3];   // pivot lattice counts
+    double inv_half_pe;  // 0.5 / (double) pivot_eb
+    double two_pe;       // (double) pivot_eb * 2
+};
+
+// t-selection inputs for one round (compress.tpp:90-144).
+struct TselDesc {
+    RoundDesc R;
+    uint32_t L, r;
+    float stat_eb_ratio;
+    float range;            // max_val - min_val (float)
+    float t_precision;
+};
+
+// AB-key sampling over a whole axis for the default slope (get_default_slope.tpp:15-25).
+struct SlopeSampleDesc {
+    uint32_t D[3];
+    uint32_t lo[3], hi[3], step[3];  // step = stride * sampling stride
+    uint32_t cnt[3];                 // samples per axis
+    uint64_t dist;
+    float neg_2_stat_eb;
+};
+
+void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st);
+void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st);
+void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, void* eacc, void* aacc,
+                   hipStream_t st);
+void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
+                           const float* tblk, void* eacc, void* aacc, int write_recon, hipStream_t st);
+void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
+                             const float* tblk, hipStream_t st);
+
+void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
+                 unsigned int* fallbacks, uint64_t nblocks, hipStream_t st);
+void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st);
+void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,
+                      int64_t* keys, uint64_t nblocks, hipStream_t st);
+
+// Huffman kernels (huffman.hip)
+struct HuffArrayDev {
+    uint64_t src_off;     // first symbol in the concatenated symbol buffer
+    uint64_t count;       // symbols
+    int64_t offset;       // min symbol
+    uint64_t range;       // histogram bins
+    uint64_t hist_off;    // offset of the histogram / codebook in their buffers
+    uint64_t hchunk0;     // first histogram chunk of this array
+    uint64_t chunk0;      // first encode chunk of this array
+    uint64_t nchunk;      // encode chunks of this array
+    uint64_t payload_bit; // absolute bit position of the payload in the stream
+    uint32_t single;      // single-symbol array: no payload
+    uint32_t pad;
+};
+constexpr uint32_t kHistChunk = 16384;   // symbols per histogram workgroup
+constexpr uint32_t kEncChunk = 4096;     // symbols per encode workgroup (256 threads x 16)
+
+void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
+                 uint32_t* hist, hipStream_t st);
+void launch_chunk_bits(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
+                       const uint64_t* book, uint32_t* chunk_bits, hipStream_t st);
+void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint32_t* chunk_bits,
+                       uint64_t* chunk_off, hipStream_t st);
+void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
+                   const uint64_t* book, const uint64_t* chunk_off, uint32_t* stream_words, int max_len,
+                   hipStream_t st);
+
+// Decoder (self-synchronising parallel decode of one bit stream per array).
+struct DecArrayDev {
+    uint64_t payload_bit;  // absolute bit position of the payload in the stream buffer
+    uint64_t bits;         // payload bits
+    uint64_t count;        // symbols to produce
+    uint64_t out_off;      // first output slot
+    uint64_t table_off;    // decode table offset (words)
+    uint64_t sub0;         // first subsequence index of this array
+    uint64_t nsub;         // subsequences
+    int64_t offset;        // symbol offset
+    uint32_t root_bits;
+    uint32_t max_len;
+};
+constexpr uint32_t kSubBits = 1024;  // bits per subsequence
+
+void launch_dec_sync(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
+                     uint64_t nsub, const uint32_t* tables, uint64_t* start, uint64_t* exit_pos, uint32_t* count,
+                     hipStream_t st);
+void launch_dec_fix(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
+                    uint64_t nsub, const uint32_t* tables, uint64_t* start, const uint64_t* exit_in,
+                    uint64_t* exit_out, uint32_t* count, unsigned int* changed, hipStream_t st);
+void launch_dec_scan(const DecArrayDev* arrays, uint32_t narrays, const uint32_t* count, uint64_t* out_pos,
+                     unsigned int* err, hipStream_t st);
+void launch_dec_write(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
+                      uint64_t nsub, const uint32_t* tables, const uint64_t* start, const uint64_t* out_pos,
+                      int32_t* out, unsigned int* err, hipStream_t st);
+
+}  // namespace asz

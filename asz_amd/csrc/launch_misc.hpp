@@ -1,0 +1,13 @@
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace asz {
+
+void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st);
+void launch_fill_i32(int32_t* v, uint64_t n, int32_t val, hipStream_t st);
+void launch_t_float(const int32_t* t_int, uint64_t nblocks, uint32_t L, float prec, float* tblk, hipStream_t st);
+
+}  // namespace asz

@@ -1,0 +1,66 @@
+// Target -> (block, local index) -> QB slot mapping of one round, shared by
+// the round kernels and the host-side layout query (asz_round_slots).
+//
+// Along the round axis d, targets are i = h + 2h k; block b = i / B holds the
+// targets in [bB, (b+1)B).  Along another axis, targets are i = s k; block
+// b > 0 owns (bB, (b+1)B] (its lower face belongs to block b-1), block 0 owns
+// [0, B].  This is the ownership of compress.tpp:57-59 with the blocks of
+// compress.tpp:183-188.  The slot is the block-major concatenation
+// (compress.tpp:61-62) of each block's order-permuted array
+// (asz_param.hpp:122-129).
+#pragma once
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+#if defined(__HIP__)
+#define ASZ_HD __host__ __device__ __forceinline__
+#else
+#define ASZ_HD inline
+#endif
+
+namespace asz {
+
+struct TargetLoc {
+    uint32_t b[3], j[3];
+};
+
+ASZ_HD void locate(const RoundDesc& R, int ax, const uint32_t k[3], const uint32_t i[3], TargetLoc& L) {
+    for (int a = 0; a < 3; a++) {
+        if (a == ax) {
+            uint32_t b = i[a] / R.B[a];
+            L.b[a] = b;
+            L.j[a] = k[a] - b * (R.B[a] / (2 * R.h));
+        } else if (i[a] == 0) {
+            L.b[a] = 0;
+            L.j[a] = 0;
+        } else {
+            uint32_t b = (i[a] - 1) / R.B[a];
+            L.b[a] = b;
+            L.j[a] = b == 0 ? k[a] : k[a] - b * (R.B[a] / R.s[a]) - 1;
+        }
+    }
+}
+
+ASZ_HD uint64_t qb_slot(const RoundDesc& R, const uint32_t* tabs, const TargetLoc& L) {
+    uint32_t cnt[3], pre[3];
+    for (int a = 0; a < 3; a++) {
+        cnt[a] = tabs[R.tab_off[a] + L.b[a]];
+        pre[a] = tabs[R.tab_off[a] + R.nb[a] + L.b[a]];
+    }
+    uint64_t base = (uint64_t)pre[0] * R.C[1] * R.C[2] +
+                    (uint64_t)cnt[0] * ((uint64_t)pre[1] * R.C[2] + (uint64_t)cnt[1] * pre[2]);
+    uint64_t local;
+    if (R.order == 0) local = ((uint64_t)L.j[1] * cnt[2] + L.j[2]) * cnt[0] + L.j[0];
+    else if (R.order == 1) local = ((uint64_t)L.j[0] * cnt[2] + L.j[2]) * cnt[1] + L.j[1];
+    else local = ((uint64_t)L.j[0] * cnt[1] + L.j[1]) * cnt[2] + L.j[2];
+    return base + local;
+}
+
+// Coordinates of target k of round R.
+ASZ_HD void target_coords(const RoundDesc& R, int ax, const uint32_t k[3], uint32_t i[3]) {
+    for (int a = 0; a < 3; a++) i[a] = a == ax ? R.h + 2 * R.h * k[a] : R.s[a] * k[a];
+}
+
+}  // namespace asz

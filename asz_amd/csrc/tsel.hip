@@ -1,0 +1,255 @@
+// Per-block t selection (compress.tpp:90-144) and AB-key sampling for the
+// default slope (get_default_slope.tpp:15-25).
+//
+// One 256-thread workgroup scores one block.  The per-sample arithmetic is the
+// reference's, operation for operation (get_AB_map.tpp:18-21 and
+// get_h_psnr.tpp:12-20 as GCC lowers them): A = (a+b)-(c+e),
+// B = (float)fma(S, 0.25, -x0) with S = (b+e)+(a+c), keys = lroundf(A or B / 2 sigma),
+// err = fmaf(Bk, 2, Ak*2t)*sigma, q = lroundf(err/2eb), de = err - q*2eb.
+// The reference then sums over distinct keys in unordered_map order in float;
+// here the sums run over samples in double (the same terms, count-weighted),
+// so H1 and PSNR agree with the reference to float rounding and the chosen t
+// agrees except where two candidates score within that rounding of each other.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace asz {
+namespace dev {
+
+constexpr int kTselThreads = 256;
+constexpr int kHashSlots = 2048;
+constexpr int kMaxKeep = 4096;  // samples whose keys stay in LDS; more are recomputed
+
+struct TselShared {
+    float af[kMaxKeep], bf[kMaxKeep];
+    long long hkey[kHashSlots];
+    unsigned int hcnt[kHashSlots];
+    double red[kTselThreads / kWave];
+    double red2[kTselThreads / kWave];
+    unsigned int overflow;
+};
+
+__device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
+                                            const uint32_t step[3], const uint32_t cnt[3], uint32_t idx,
+                                            float neg2, float& af, float& bf) {
+    const RoundDesc& R = T.R;
+    uint32_t q2 = idx % cnt[2];
+    uint32_t q01 = idx / cnt[2];
+    uint32_t q1 = q01 % cnt[1];
+    uint32_t q0 = q01 / cnt[1];
+    uint64_t pos = ((uint64_t)(lo[0] + q0 * step[0]) * R.D[1] + (lo[1] + q1 * step[1])) * R.D[2] + (lo[2] + q2 * step[2]);
+    const int64_t dist = (int64_t)R.dist;
+    const float* p = work + pos;
+    float a = p[-dist], c = p[-3 * dist], b = p[dist], e = p[3 * dist];
+    float x0 = daz(p[0]);
+    float A = (a + b) - (c + e);
+    float S = (b + e) + (a + c);
+    float B = (float)__fma_rn((double)S, 0.25, -(double)x0);
+    af = (float)lround_f64(neg2 * A);
+    bf = (float)lround_f64(neg2 * B);
+}
+
+__device__ __forceinline__ void hash_insert(TselShared& sh, long long key) {
+    uint32_t hsh = (uint32_t)((unsigned long long)key * 0x9E3779B97F4A7C15ull >> 53) & (kHashSlots - 1);
+    for (int probe = 0; probe < kHashSlots; probe++) {
+        uint32_t s = (hsh + probe) & (kHashSlots - 1);
+        long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&sh.hkey[s]), (unsigned long long)LLONG_MIN,
+                                   (unsigned long long)key);
+        if (prev == LLONG_MIN || prev == key) {
+            atomicAdd(&sh.hcnt[s], 1u);
+            return;
+        }
+    }
+    atomicOr(&sh.overflow, 1u);
+}
+
+__device__ double block_sum(TselShared& sh, double v, double* buf) {
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) buf[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int k = 0; k < kTselThreads / kWave; k++) s += buf[k];
+    __syncthreads();
+    return s;
+}
+
+// One candidate t: returns (h1, psnr) as the reference's float results would
+// be, up to summation order.
+__device__ void score(TselShared& sh, const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
+                      const uint32_t step[3], const uint32_t cnt[3], uint32_t ns, float neg2, float stat_eb,
+                      float t, float& h1_out, float& psnr_out) {
+    const float eb = T.R.eb;
+    const float neg_2_eb = 0.5f / eb;
+    const float two_t = t * 2.0f;
+    const float two_eb = eb * 2.0f;
+    for (int s = threadIdx.x; s < kHashSlots; s += kTselThreads) { sh.hkey[s] = LLONG_MIN; sh.hcnt[s] = 0; }
+    __syncthreads();
+    double mse = 0;
+    for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
+        float af, bf;
+        if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
+        else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf);
+        float err = __fmaf_rn(bf, 2.0f, af * two_t) * stat_eb;
+        long long q = lround_f64(err * neg_2_eb);
+        float de = __fmaf_rn(-(float)q, two_eb, err);
+        mse += (double)de * (double)de;
+        hash_insert(sh, q);
+    }
+    __syncthreads();
+    mse = block_sum(sh, mse, sh.red);
+    const float dn = (float)ns;
+    const float rec = 1.0f / dn;
+    double h = 0;
+    for (int s = threadIdx.x; s < kHashSlots; s += kTselThreads) {
+        if (sh.hcnt[s]) {
+            float p = (float)sh.hcnt[s] * rec;
+            h -= (double)p * (double)log2f(p);
+        }
+    }
+    h = block_sum(sh, h, sh.red2);
+    const float msef = (float)mse;
+    const float r2 = T.range * T.range;
+    h1_out = (float)h;
+    psnr_out = log10f((r2 / msef) * dn) * 10.0f;
+}
+
+__global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
+                                                       const uint32_t* __restrict__ ss_tab, float* tblk,
+                                                       int32_t* t_int, unsigned int* fallbacks) {
+    __shared__ TselShared sh;
+    const RoundDesc& R = T.R;
+    const uint32_t bid = blockIdx.x;
+    const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
+    const uint32_t bb[3] = {b0, b1, b2};
+    const uint32_t ss = ss_tab[bid];
+    float best_t = 0.25f;  // insufficient_sample_t (asz_param.hpp:167)
+    bool fallback = ss == 0;
+    if (!fallback) {
+        const int d = (int)R.d;
+        uint32_t lo[3], hi[3], step[3], cnt[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = bb[a] * R.B[a];
+            hi[a] = min(lo[a] + R.B[a] + 1, R.D[a]);
+            if (a != d && lo[a] != 0) lo[a] += R.s[a];
+            step[a] = R.s[a] * ss;
+        }
+        lo[d] += 3 * R.s[d] / 2;
+        hi[d] -= 3 * R.s[d] / 2;
+        uint32_t ns = 1;
+        for (int a = 0; a < 3; a++) {
+            cnt[a] = (hi[a] - lo[a] + step[a] - 1) / step[a];
+            ns *= cnt[a];
+        }
+        const float stat_eb = T.stat_eb_ratio * R.eb;
+        const float neg2 = 0.5f / stat_eb;
+        if (threadIdx.x == 0) sh.overflow = 0;
+        for (uint32_t idx = threadIdx.x; idx < ns && idx < kMaxKeep; idx += kTselThreads)
+            sample_keys(T, work, lo, step, cnt, idx, neg2, sh.af[idx], sh.bf[idx]);
+        __syncthreads();
+        const float cand[5] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
+        float best_h1, best_psnr;
+        score(sh, T, work, lo, step, cnt, ns, neg2, stat_eb, best_t, best_h1, best_psnr);
+        for (int k = 0; k < 5; k++) {
+            const float t = cand[k];
+            if (t == best_t || best_psnr > FLT_MAX) continue;
+            float h1, psnr;
+            score(sh, T, work, lo, step, cnt, ns, neg2, stat_eb, t, h1, psnr);
+            const float sc = __fmaf_rn(best_h1 - h1, R.layer_slope, psnr - best_psnr);
+            if (sc > 0.0f || psnr > FLT_MAX) { best_t = t; best_h1 = h1; best_psnr = psnr; }
+        }
+    }
+    if (threadIdx.x == 0) {
+        tblk[bid] = best_t;
+        t_int[(uint64_t)bid * T.L + T.r] = (int32_t)lround_f64(best_t * T.t_precision);
+        if (fallback) atomicAdd(fallbacks, 1u);
+    }
+}
+
+// Keys of every sample of every block, in the reference's loop order, for
+// the host scoring of ASZ_TSEL_EXACT (compress.tpp:102-120).
+__global__ __launch_bounds__(256) void k_tsel_keys(TselDesc T, const float* __restrict__ work,
+                                                   const uint32_t* __restrict__ ss_tab,
+                                                   const uint64_t* __restrict__ key_off, long long* keys) {
+    const RoundDesc& R = T.R;
+    const uint32_t bid = blockIdx.x;
+    const uint32_t ss = ss_tab[bid];
+    if (ss == 0) return;
+    const uint32_t bb[3] = {bid / (R.nb[2] * R.nb[1]), (bid / R.nb[2]) % R.nb[1], bid % R.nb[2]};
+    const int d = (int)R.d;
+    uint32_t lo[3], hi[3], step[3], cnt[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = bb[a] * R.B[a];
+        hi[a] = min(lo[a] + R.B[a] + 1, R.D[a]);
+        if (a != d && lo[a] != 0) lo[a] += R.s[a];
+        step[a] = R.s[a] * ss;
+    }
+    lo[d] += 3 * R.s[d] / 2;
+    hi[d] -= 3 * R.s[d] / 2;
+    uint32_t ns = 1;
+    for (int a = 0; a < 3; a++) {
+        cnt[a] = (hi[a] - lo[a] + step[a] - 1) / step[a];
+        ns *= cnt[a];
+    }
+    const float neg2 = 0.5f / (T.stat_eb_ratio * R.eb);
+    long long* out = keys + 2 * key_off[bid];
+    for (uint32_t idx = threadIdx.x; idx < ns; idx += blockDim.x) {
+        uint32_t q2 = idx % cnt[2], q01 = idx / cnt[2], q1 = q01 % cnt[1], q0 = q01 / cnt[1];
+        uint64_t pos = ((uint64_t)(lo[0] + q0 * step[0]) * R.D[1] + (lo[1] + q1 * step[1])) * R.D[2] +
+                       (lo[2] + q2 * step[2]);
+        const int64_t dist = (int64_t)R.dist;
+        const float* p = work + pos;
+        float a = p[-dist], c = p[-3 * dist], b = p[dist], e = p[3 * dist];
+        float x0 = daz(p[0]);
+        float A = (a + b) - (c + e);
+        float S = (b + e) + (a + c);
+        float B = (float)__fma_rn((double)S, 0.25, -(double)x0);
+        out[2 * idx] = lround_f64(neg2 * A);
+        out[2 * idx + 1] = lround_f64(neg2 * B);
+    }
+}
+
+// Keys (A_key, B_key) of every sample, in the reference's loop order (i0, i1, i2).
+__global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const float* __restrict__ f,
+                                                      long long* keys) {
+    const uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ns) return;
+    uint32_t q2 = idx % S.cnt[2];
+    uint64_t q01 = idx / S.cnt[2];
+    uint32_t q1 = q01 % S.cnt[1];
+    uint32_t q0 = q01 / S.cnt[1];
+    uint64_t pos = ((uint64_t)(S.lo[0] + q0 * S.step[0]) * S.D[1] + (S.lo[1] + q1 * S.step[1])) * S.D[2] +
+                   (S.lo[2] + q2 * S.step[2]);
+    const int64_t dist = (int64_t)S.dist;
+    const float* p = f + pos;
+    float a = daz(p[-dist]), c = daz(p[-3 * dist]), b = daz(p[dist]), e = daz(p[3 * dist]);
+    float x0 = daz(p[0]);
+    float A = (a + b) - (c + e);
+    float Ssum = (b + e) + (a + c);
+    float B = (float)__fma_rn((double)Ssum, 0.25, -(double)x0);
+    keys[2 * idx] = lround_f64(S.neg_2_stat_eb * A);
+    keys[2 * idx + 1] = lround_f64(S.neg_2_stat_eb * B);
+}
+
+}  // namespace dev
+
+void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
+                 unsigned int* fallbacks, uint64_t nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_tsel, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss, tblk,
+                       t_int, fallbacks);
+}
+
+void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,
+                      int64_t* keys, uint64_t nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_tsel_keys, dim3((uint32_t)nblocks), dim3(256), 0, st, T, work, ss, key_off,
+                       (long long*)keys);
+}
+
+void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st) {
+    uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
+    if (!ns) return;
+    hipLaunchKernelGGL(dev::k_slope_sample, dim3((uint32_t)((ns + 255) / 256)), dim3(256), 0, st, S, field,
+                       (long long*)keys);
+}
+
+}  // namespace asz

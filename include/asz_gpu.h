@@ -1,0 +1,176 @@
+/*
+ * asz_gpu.h — C ABI of the MI355X ASZ compression core (libasz_gpu.so).
+ *
+ * The reference has no library ABI: its core is header-only C++ behind two
+ * executables (c_codes/s1_compress.cpp, c_codes/s1_decompress.cpp).  Each
+ * entry point below replaces one member of asz::param_c<float> or one step of
+ * those mains; the reference location is given with it.  Plain pointers and
+ * sizes only.  Every call returns ASZ_OK (0) or a negative error code;
+ * asz_last_error(ctx) then holds the message (the text the reference's
+ * exceptions carry where one exists).  Pointers named d_* are HIP device
+ * pointers on the context's device; all others are host pointers.
+ *
+ * One context per host thread; calls on one context are serialised on its
+ * HIP stream.  Contexts are not shared between threads.
+ */
+#ifndef ASZ_GPU_H
+#define ASZ_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASZ_OK 0
+#define ASZ_ERR_ARG (-1)     /* invalid argument (std::invalid_argument in the reference) */
+#define ASZ_ERR_DATA (-2)    /* data / stream error (std::runtime_error in the reference)  */
+#define ASZ_ERR_HIP (-3)     /* HIP runtime failure                                         */
+#define ASZ_ERR_NOMEM (-4)   /* device or host allocation failure                           */
+
+#define ASZ_MAX_LAYERS 63
+
+typedef struct asz_ctx asz_ctx;
+
+/* The numeric knobs of param_c<float> that the core reads (asz_param.hpp:147-183).
+ * asz_params_default fills the reference defaults (block_pivot_ratio 4). */
+typedef struct asz_params {
+    float rel_eb;                   /* -M REL x                          (s1_compress.cpp:191-196) */
+    float pivot_eb_ratio;           /* --pivot-eb-ratio      default 0.05 */
+    float eb_ratio;                 /* --eb-ratio            default 0.76 */
+    float slope_ratio;              /* --slope-ratio         default 0.54 */
+    uint64_t block_pivot_ratio;     /* --block-pivot-ratio   default 4    */
+    int32_t t_precision;            /* --t-precision         default 32   */
+    float stat_eb_ratio;            /* --stat-eb-ratio       default 0.2  */
+    uint64_t h1_block_sampling_num; /* --h1-block-sampling-num default 256 */
+    float t_min, t_max;             /* --t-min / --t-max     default 1/16, 6/16 */
+    float default_slope_eb_span;    /* --default-slope-eb-span            default 1.5  */
+    float default_slope_invalid_ratio_threshold; /* default 1 */
+    float default_slope_fallback;   /* --default-slope-fallback           default 1000 */
+    float default_slope;            /* lambda of the RD proxy, from asz_default_slope */
+    int32_t quintic;                /* predictor family: 0 cubic, 1 quintic (force_quintic) */
+} asz_params;
+
+/* A predictor plan: one (axis, QB order) pair per round, coarsest first.
+ * order 0 = "120", 1 = "021", 2 = "012" (asz_param.hpp:122-129). */
+typedef struct asz_plan {
+    int32_t layers;
+    uint8_t axis[ASZ_MAX_LAYERS];
+    uint8_t order[ASZ_MAX_LAYERS];
+} asz_plan;
+
+/* Per-block t source for asz_compress. */
+#define ASZ_TSEL_GPU 0      /* GPU H1-PSNR selection (compress.tpp:90-144 on the GPU) */
+#define ASZ_TSEL_EXACT 1    /* GPU sampling + host scoring in the reference's map order   */
+#define ASZ_TSEL_INJECT 2   /* caller's t_int values (block-major, round-minor)           */
+
+/* Timings (milliseconds) and diagnostics of the last asz_compress / asz_decompress. */
+typedef struct asz_stats {
+    double range_ms, preprocess_ms, tsel_ms, rounds_ms, huffman_ms, huffman_host_ms, total_gpu_ms, wall_ms;
+    double max_err, sq_err_sum;    /* over the reconstruction, in double */
+    uint64_t stream_bytes;         /* pre-Zstd Huffman stream bytes (S_h) */
+    uint64_t t_count, t_fallbacks; /* #t = #blocks * layers, and fallbacks to t = 4/16 */
+    uint64_t layers;
+    uint64_t qb_counts[ASZ_MAX_LAYERS + 1];
+} asz_stats;
+
+/* ------------------------------------------------------------------ context */
+int asz_ctx_create(int device, asz_ctx **out);
+void asz_ctx_destroy(asz_ctx *ctx);
+const char *asz_last_error(asz_ctx *ctx);
+void asz_params_default(asz_params *p);
+/* Last call's stats (valid until the next call). */
+int asz_get_stats(asz_ctx *ctx, asz_stats *out);
+
+/* Device memory helpers for callers without HIP headers (the CLI drivers).
+ * No reference counterpart: the reference keeps everything in std::vector. */
+int asz_device_alloc(asz_ctx *ctx, uint64_t bytes, void **d_ptr);
+int asz_device_free(asz_ctx *ctx, void *d_ptr);
+int asz_copy_to_device(asz_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
+int asz_copy_to_host(asz_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
+
+/* ------------------------------------------------------------------ geometry
+ * generate_pivot_stride_and_block_shape_and_block_num (generate_config.tpp:51-68).
+ * Host only; writes pivot_stride[3], block_shape[3], block_num[3]. */
+int asz_geometry(const uint64_t dims[3], const asz_plan *plan, uint64_t block_pivot_ratio, uint64_t pivot_stride[3],
+                 uint64_t block_shape[3], uint64_t block_num[3]);
+
+/* Layout query (host only).  For round `round` of the plan, writes for every
+ * target, in the round kernels' thread order, its linear field position
+ * (pos) and its slot in the round's QB array (slot); with pos == NULL only
+ * *count is set.  It exposes the index arithmetic the GPU kernels use for
+ * block_compress_impl's loops and get_qb_index (compress.tpp:51-88,
+ * asz_param.hpp:105-129) so that it can be checked without a GPU. */
+int asz_round_layout(const uint64_t dims[3], const asz_plan *plan, uint64_t block_pivot_ratio, int round,
+                     uint64_t *count, uint64_t *pos, uint64_t *slot);
+
+/* ------------------------------------------------------------------ range scan
+ * load_input's finite check and minmax_element (s1_compress.cpp:128-137), on the
+ * GPU over d_field[n].  min/max are the first-smallest / last-largest elements
+ * under the reference's FTZ/DAZ comparison, returned bit-exactly.
+ * Fails with ASZ_ERR_DATA on a non-finite value or max == min. */
+int asz_range_f32(asz_ctx *ctx, const float *d_field, uint64_t n, float *min_val, float *max_val);
+
+/* ------------------------------------------------------------------ default slope
+ * param_c::get_default_slope (get_default_slope.tpp:5-83): AB keys sampled on
+ * the GPU, scored on the host in the reference's order.  Writes the slope and
+ * the statistics the reference prints (max_invalid_ratio, fallback 0/1,
+ * reason 0 none / 1 invalid_ratio / 2 invalid_difference). */
+int asz_default_slope(asz_ctx *ctx, const float *d_field, const uint64_t dims[3], const asz_params *p, float min_val,
+                      float max_val, float *slope, float *max_invalid_ratio, int *fallback, int *reason);
+
+/* ------------------------------------------------------------------ compression
+ * param_c::compress (compress.tpp:146-299) up to, not including, ZSTD_compress:
+ * pivots, every round (per-block t, prediction, quantization, reconstruction,
+ * QB serialization) and Huffman coding of the t array and the L+1 QB arrays.
+ *
+ * d_work holds the field (dims[0]*dims[1]*dims[2] floats, C order) and is
+ * overwritten with the reconstruction (the reference's compressed_data).
+ * min_val/max_val come from asz_range_f32.  tsel_mode picks the t source; with
+ * ASZ_TSEL_INJECT, t_inject holds #blocks*layers t_int values.
+ * On success *stream / *stream_bytes hold the pre-Zstd stream in a buffer the
+ * context owns (valid until the next call on ctx).
+ * If qb_out is not NULL it receives the N QB values (pivot array then rounds);
+ * if t_out is not NULL it receives the #t t_int values. */
+int asz_compress(asz_ctx *ctx, float *d_work, const uint64_t dims[3], const asz_plan *plan, const asz_params *p,
+                 float min_val, float max_val, int tsel_mode, const int64_t *t_inject, const uint8_t **stream,
+                 uint64_t *stream_bytes, int32_t *qb_out, int64_t *t_out);
+
+/* Host-buffer convenience: copies field to the device, runs asz_range_f32 +
+ * asz_compress, and optionally copies the reconstruction back (recon may be NULL). */
+int asz_compress_host(asz_ctx *ctx, const float *field, const uint64_t dims[3], const asz_plan *plan,
+                      const asz_params *p, int tsel_mode, const int64_t *t_inject, const uint8_t **stream,
+                      uint64_t *stream_bytes, float *recon, int32_t *qb_out, int64_t *t_out);
+
+/* ------------------------------------------------------------------ decompression
+ * param_c::decompress (decompress.tpp:78-208) after ZSTD_decompress: parses the
+ * pre-Zstd stream, Huffman-decodes every array on the GPU, and reconstructs the
+ * field into d_out (dims[0]*dims[1]*dims[2] floats).  Trailing bytes after the
+ * last QB array are ignored, as in the reference. */
+int asz_decompress(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, const uint64_t dims[3],
+                   const asz_params *p, float *d_out, int32_t *qb_out, int64_t *t_out);
+int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, const uint64_t dims[3],
+                        const asz_params *p, float *out);
+
+/* ------------------------------------------------------------------ Huffman (host side)
+ * The codebook and header of one array as huffman_encoder_class<T>::preprocess_encode
+ * + save build them (huffman.hpp:263-320, 650-691), from a dense histogram of
+ * symbols offset..offset+range-1.  is_int64 selects the T = int64_t header
+ * semantics (the t array).  Writes the header+tree bytes to hdr (capacity
+ * hdr_cap), the code of each symbol to codes[range] and its length to
+ * lens[range] (0 for absent symbols), and the payload bit count.  Host only. */
+int asz_huffman_codebook(const uint64_t *hist, uint64_t range, int64_t offset, uint64_t count, int is_int64,
+                         uint8_t *hdr, uint64_t hdr_cap, uint64_t *hdr_bytes, uint64_t *codes, uint8_t *lens,
+                         uint64_t *payload_bits);
+
+/* Compression / decompression of a whole .asz frame with host Zstd (level 3),
+ * compress.tpp:294-299 / decompress.tpp:83-86.  Host only. */
+int asz_zstd_compress(const uint8_t *src, uint64_t n, uint8_t **dst, uint64_t *dst_bytes);
+int asz_zstd_decompress(const uint8_t *src, uint64_t n, uint8_t **dst, uint64_t *dst_bytes);
+void asz_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASZ_GPU_H */
