@@ -42,7 +42,12 @@ class Stats(ctypes.Structure):
                 ("huffman_host_ms", ctypes.c_double), ("total_gpu_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("max_err", ctypes.c_double), ("sq_err_sum", ctypes.c_double),
                 ("stream_bytes", ctypes.c_uint64), ("t_count", ctypes.c_uint64), ("t_fallbacks", ctypes.c_uint64),
-                ("layers", ctypes.c_uint64), ("qb_counts", ctypes.c_uint64 * (MAX_LAYERS + 1))]
+                ("layers", ctypes.c_uint64), ("qb_counts", ctypes.c_uint64 * (MAX_LAYERS + 1)),
+                ("pivot_ms", ctypes.c_double), ("tsel_round_ms", ctypes.c_double * MAX_LAYERS),
+                ("round_ms", ctypes.c_double * MAX_LAYERS), ("hist_ms", ctypes.c_double),
+                ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double)]
+
+ASZ_OPT_KERNEL_EVENTS = 1
 
 
 # exported entry points (must match include/asz_gpu.h)
@@ -50,7 +55,7 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
-           "asz_round_layout")
+           "asz_round_layout", "asz_set_option")
 
 _lib = None
 
@@ -70,6 +75,7 @@ def lib():
     L.asz_params_default.argtypes = [ctypes.POINTER(Params)]
     L.asz_params_default.restype = None
     L.asz_get_stats.argtypes = [_P, ctypes.POINTER(Stats)]
+    L.asz_set_option.argtypes = [_P, ctypes.c_int, ctypes.c_int64]
     L.asz_device_alloc.argtypes = [_P, _U64, ctypes.POINTER(_P)]
     L.asz_device_free.argtypes = [_P, _P]
     L.asz_copy_to_device.argtypes = [_P, _P, _P, _U64]

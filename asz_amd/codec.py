@@ -105,6 +105,9 @@ class Context:
         if rc != 0:
             raise AszError(rc, self._L.asz_last_error(self._c).decode())
 
+    def set_kernel_events(self, on: bool = True):
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_KERNEL_EVENTS, 1 if on else 0))
+
     def stats(self) -> Stats:
         s = Stats()
         self._L.asz_get_stats(self._c, ctypes.byref(s))

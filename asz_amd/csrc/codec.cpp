@@ -66,9 +66,15 @@ struct asz_ctx {
     std::string err;
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, ss, tblk, acc, hist, book, arrays, chunk_arr, hchunk_arr, chunk_bits, chunk_off,
-        words, keys, key_off, misc, dec_tab, dec_arrays, sub_arr, start, exit0, exit1, count, out_pos;
+        words, keys, key_off, misc, dec_tab, dec_arrays, sub_arr, start, exit0, exit1, count, out_pos, segs,
+        partials, pranges;
     std::vector<uint8_t> host_stream;
     hipEvent_t ev[8];
+    // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
+    bool kev_on = false;
+    std::vector<hipEvent_t> kev;
+    std::vector<std::pair<int, int>> kev_tag;
+    int kev_n = 0;
 };
 
 namespace asz {
@@ -83,6 +89,38 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, a, b);
     return ms;
+}
+
+enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_CHUNK, KEV_ENCODE, KEV_DECODE };
+
+int kev_begin(asz_ctx* c, int kind, int r) {
+    if (!c->kev_on || c->kev_n * 2 + 2 > (int)c->kev.size()) return -1;
+    int i = c->kev_n++;
+    c->kev_tag[i] = {kind, r};
+    (void)hipEventRecord(c->kev[2 * i], c->stream);
+    return i;
+}
+void kev_end(asz_ctx* c, int i) {
+    if (i >= 0) (void)hipEventRecord(c->kev[2 * i + 1], c->stream);
+}
+// After the stream is synchronised: fold the recorded pairs into the stats.
+void kev_collect(asz_ctx* c) {
+    asz_stats& S = c->stats;
+    for (int i = 0; i < c->kev_n; i++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->kev[2 * i], c->kev[2 * i + 1]);
+        int r = c->kev_tag[i].second;
+        switch (c->kev_tag[i].first) {
+            case KEV_PIVOT: S.pivot_ms += ms; break;
+            case KEV_TSEL: S.tsel_round_ms[r] += ms; break;
+            case KEV_ROUND: S.round_ms[r] += ms; break;
+            case KEV_HIST: S.hist_ms += ms; break;
+            case KEV_CHUNK: S.chunk_ms += ms; break;
+            case KEV_ENCODE: S.encode_ms += ms; break;
+            default: S.decode_ms += ms; break;
+        }
+    }
+    c->kev_n = 0;
 }
 
 void put_le64(std::vector<uint8_t>& o, uint64_t v) {
@@ -237,6 +275,19 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     uint32_t* ss = c->ss.get<uint32_t>(P.ss.size() + 1);
     HIPCHECK(hipMemcpyAsync(ss, P.ss.data(), P.ss.size() * 4, hipMemcpyHostToDevice, st));
     float* tblk = c->tblk.get<float>(nt + 1);
+    SweepSeg* segs = c->segs.get<SweepSeg>(P.segs.size() + 1);
+    if (!P.segs.empty())
+        HIPCHECK(hipMemcpyAsync(segs, P.segs.data(), P.segs.size() * sizeof(SweepSeg), hipMemcpyHostToDevice, st));
+    Partial* part = c->partials.get<Partial>(P.total_partials + 1);
+    std::vector<uint64_t> pranges(2 * L);
+    for (uint64_t r = 0; r < L; r++) {
+        pranges[2 * r] = P.rounds[r].part_off;
+        pranges[2 * r + 1] = P.rounds[r].count ? (r + 1 < L ? P.rounds[r + 1].part_off : P.total_partials) -
+                                                     P.rounds[r].part_off
+                                               : 0;
+    }
+    uint64_t* dranges = c->pranges.get<uint64_t>(2 * L);
+    HIPCHECK(hipMemcpyAsync(dranges, pranges.data(), 2 * L * 8, hipMemcpyHostToDevice, st));
 
     // accumulators: error stats, fallback counter, QB range per array (t array + L+1)
     const uint64_t narr = L + 2;
@@ -261,9 +312,12 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         HIPCHECK(hipMemcpyAsync(sym, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
     }
 
+    c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
     PivotDesc PD = pivot_desc(P);
+    int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(true, d_work, qb, PD, eacc, aacc(1), st);
+    kev_end(c, kp);
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
     std::vector<uint64_t> koff;
@@ -273,7 +327,9 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         float* tb = tblk + r * nbl;
         if (tsel_mode == ASZ_TSEL_GPU) {
             TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
+            int kt = kev_begin(c, KEV_TSEL, (int)r);
             launch_tsel(T, d_work, ss + r * nbl, tb, sym, &acc->fallbacks, nbl, st);
+            kev_end(c, kt);
         } else if (tsel_mode == ASZ_TSEL_EXACT) {
             auto h0 = Clock::now();
             TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
@@ -310,9 +366,12 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             HIPCHECK(hipStreamSynchronize(st));
             tsel_host_ms += ms_since(h0);
         }
-        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, eacc, aacc(2 + r), 1, st);
+        int kr = kev_begin(c, KEV_ROUND, (int)r);
+        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, 1, st);
+        kev_end(c, kr);
     }
     HIPCHECK(hipGetLastError());
+    launch_reduce_partials(part, dranges, (uint32_t)L, acc, st);
     // t array range
     launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
     HIPCHECK(hipEventRecord(c->ev[1], st));
@@ -322,6 +381,11 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipStreamSynchronize(st));
     S.max_err = hacc.err.max_err;
     S.sq_err_sum = hacc.err.sq_sum;
+    for (uint64_t r = 0; r < L; r++) {
+        if (!P.rounds[r].count) continue;
+        S.max_err = std::max(S.max_err, hacc.rerr[r].max_err);
+        S.sq_err_sum += hacc.rerr[r].sq_sum;
+    }
     if (tsel_mode == ASZ_TSEL_GPU) S.t_fallbacks = hacc.fallbacks;
     if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
 
@@ -370,7 +434,9 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipMemcpyAsync(dharr, harr.data(), hchunks * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), chunks * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipEventRecord(c->ev[2], st));
+    int kh = kev_begin(c, KEV_HIST, 0);
     launch_hist(sym, darr, dharr, hchunks, dhist, st);
+    kev_end(c, kh);
     HIPCHECK(hipEventRecord(c->ev[3], st));
     std::vector<uint32_t> hh(hist_total);
     HIPCHECK(hipMemcpyAsync(hh.data(), dhist, hist_total * 4, hipMemcpyDeviceToHost, st));
@@ -419,9 +485,13 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
     double hhost_ms = ms_since(hh0);
     HIPCHECK(hipEventRecord(c->ev[4], st));
+    int kc = kev_begin(c, KEV_CHUNK, 0);
     launch_chunk_bits(sym, darr, dcarr, chunks, dbook, dcbits, st);
     launch_chunk_scan(darr, (uint32_t)narr, dcbits, dcoff, st);
+    kev_end(c, kc);
+    int ke = kev_begin(c, KEV_ENCODE, 0);
     launch_encode(sym, darr, dcarr, chunks, dbook, dcoff, dwords, lay.max_len, st);
+    kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
     c->host_stream.resize(lay.total_bytes);
@@ -435,6 +505,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipStreamSynchronize(st));
     if (t_out)
         for (uint64_t i = 0; i < nt; i++) t_out[i] = t32[i];
+    kev_collect(c);
     S.rounds_ms = ev_ms(c->ev[0], c->ev[1]);
     S.huffman_ms = ev_ms(c->ev[2], c->ev[3]) + ev_ms(c->ev[4], c->ev[5]);
     S.huffman_host_ms = hhost_ms;
@@ -564,6 +635,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     uint64_t* dpos = c->out_pos.get<uint64_t>(nsub + 1);
     unsigned int* flags = c->misc.get<unsigned int>(8);
     HIPCHECK(hipMemsetAsync(flags, 0, 8 * 4, st));
+    c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
     launch_dec_sync(dwords, nwords, dda, dsub, nsub, dtab, dstart, dex0, dcnt, st);
     uint64_t iters = 0;
@@ -592,15 +664,22 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
     HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
     float* tblk = c->tblk.get<float>(nt + 1);
+    SweepSeg* dsegs = c->segs.get<SweepSeg>(P.segs.size() + 1);
+    if (!P.segs.empty())
+        HIPCHECK(hipMemcpyAsync(dsegs, P.segs.data(), P.segs.size() * sizeof(SweepSeg), hipMemcpyHostToDevice, st));
     HIPCHECK(hipEventRecord(c->ev[2], st));
     launch_t_float(sym, nbl, (uint32_t)L, (float)p.t_precision, tblk, st);
     PivotDesc PD = pivot_desc(P);
+    int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, nullptr, st);
+    kev_end(c, kp);
     for (uint64_t r = 0; r < L; r++) {
         RoundDesc R = P.rounds[r];
         check_grid(R);
         R.qb_off = 0;
-        launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, st);
+        int kr = kev_begin(c, KEV_ROUND, (int)r);
+        launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, dsegs, st);
+        kev_end(c, kr);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[3], st));
@@ -615,7 +694,9 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     if (t_out)
         for (uint64_t i = 0; i < Q.nt; i++) t_out[i] = t32[i];
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = Q.counts[k + 1];
+    kev_collect(c);
     S.huffman_ms = ev_ms(c->ev[0], c->ev[1]);
+    S.decode_ms = S.huffman_ms;
     S.rounds_ms = ev_ms(c->ev[2], c->ev[3]);
     S.total_gpu_ms = S.huffman_ms + S.rounds_ms;
     S.wall_ms = ms_since(wall0);
@@ -692,11 +773,25 @@ void asz_ctx_destroy(asz_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
+    for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
 const char* asz_last_error(asz_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int asz_set_option(asz_ctx* c, int option, int64_t value) {
+    return guarded(c, [&] {
+        if (option != ASZ_OPT_KERNEL_EVENTS) fail(ASZ_ERR_ARG, "unknown option");
+        HIPCHECK(hipSetDevice(c->device));
+        c->kev_on = value != 0;
+        if (c->kev_on && c->kev.empty()) {
+            c->kev.resize(2 * 256);
+            c->kev_tag.resize(256);
+            for (auto& e : c->kev) HIPCHECK(hipEventCreate(&e));
+        }
+    });
+}
 
 int asz_get_stats(asz_ctx* c, asz_stats* out) {
     *out = c->stats;

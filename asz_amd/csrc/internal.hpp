@@ -54,6 +54,27 @@ struct RoundDesc {
     double inv_half_eb; // 0.5 / eb           (axis-0 quantizer form)
     double eb_d;        // (double) eb         (axes-1/2 quantizer form)
     double neg_two_eb;  // -(2.0 * (double) eb) (reconstruction)
+    uint32_t kind;      // kRoundDirect or kRoundSweep
+    uint32_t seg_off;   // first sweep segment of this round
+    uint32_t nseg;      // sweep segments of this round
+    uint32_t pad;
+    uint64_t part_off;  // first per-workgroup partial of this round
+};
+
+constexpr uint32_t kRoundDirect = 0;  // one thread per target (round axis 2)
+constexpr uint32_t kRoundSweep = 1;   // register-window sweep along the round axis (axis 0 / 1)
+constexpr uint32_t kSweepSeg = 32;    // targets per sweep segment
+
+// A sweep segment: targets k0 .. k0+cnt-1 (global indices along the round
+// axis) inside block b along that axis.
+struct SweepSeg {
+    uint32_t k0, cnt, b, pad;
+};
+
+// Per-workgroup reduction result of a round kernel.
+struct Partial {
+    double max_err, sq;
+    int32_t qmin, qmax;
 };
 
 struct Plan {
@@ -62,6 +83,8 @@ struct Plan {
     std::vector<uint32_t> tables;  // per round, per axis: cnt[nb] then pre[nb]
     std::vector<uint32_t> ss;      // t-selection sampling stride per (round, block): r * nblocks + b
     std::vector<uint32_t> ns;      // t-selection sample count per (round, block), 0 for fallback blocks
+    std::vector<SweepSeg> segs;    // sweep segments of every sweep round
+    uint64_t total_partials = 0;   // per-workgroup partials over all rounds
     uint64_t pivot_count;
     uint32_t pn[3];                // pivot lattice counts per axis
     float pivot_eb;
@@ -79,9 +102,10 @@ struct ErrAcc {
     double sq_sum;       // sum (x - x')^2
 };
 struct DevAcc {
-    ErrAcc err;
+    ErrAcc err;                        // pivots
     uint32_t fallbacks, pad;
     ArrayAcc arr[ASZ_MAX_LAYERS + 2];  // [0] t array, [1] pivots, [2 + r] round r
+    ErrAcc rerr[ASZ_MAX_LAYERS];       // round r
 };
 
 // ---------------------------------------------------------------- fp helpers (host)

@@ -38,9 +38,12 @@ void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, h
 void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, void* eacc, void* aacc,
                    hipStream_t st);
 void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
-                           const float* tblk, void* eacc, void* aacc, int write_recon, hipStream_t st);
+                           const float* tblk, const SweepSeg* segs, Partial* part, int write_recon, hipStream_t st);
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
-                             const float* tblk, hipStream_t st);
+                             const float* tblk, const SweepSeg* segs, hipStream_t st);
+// ranges: (partial offset, partial count) per round; writes DevAcc::arr[2 + r] and rerr[r]
+void launch_reduce_partials(const Partial* part, const uint64_t* ranges, uint32_t rounds, DevAcc* acc,
+                            hipStream_t st);
 
 void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
                  unsigned int* fallbacks, uint64_t nblocks, hipStream_t st);

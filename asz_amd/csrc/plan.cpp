@@ -93,6 +93,22 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.count = count;
         R.qb_off = qb_off;
         qb_off += count;
+        // kernel shape: register-window sweep for rounds along axis 0/1, direct otherwise
+        R.kind = d == 2 ? kRoundDirect : kRoundSweep;
+        if (R.kind == kRoundSweep) {
+            R.seg_off = (uint32_t)P.segs.size();
+            const uint32_t per_block = (uint32_t)(g.block_shape[d] / s[d]);  // targets per full block along d
+            const uint32_t* cntd = P.tables.data() + R.tab_off[d];
+            for (uint32_t b = 0; b < R.nb[d]; b++) {
+                const uint32_t k0 = b * per_block;
+                for (uint32_t c = 0; c < cntd[b]; c += kSweepSeg)
+                    P.segs.push_back(SweepSeg{k0 + c, std::min<uint32_t>(kSweepSeg, cntd[b] - c), b, 0});
+            }
+            R.nseg = (uint32_t)P.segs.size() - R.seg_off;
+        }
+        R.part_off = P.total_partials;
+        const uint64_t gx = (R.n[2] + 255) / 256;
+        P.total_partials += R.kind == kRoundSweep ? gx * R.n[1 - d] * R.nseg : gx * R.n[1] * R.n[0];
         // t-selection sample strides (compress.tpp:96-116, get_AB_map.tpp:9-12); 0 marks a fallback block
         for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
             for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)

@@ -1,14 +1,25 @@
 // Range scan, pivots and the level-synchronous interpolation rounds.
 //
-// One launch per round covers every block: target (k0, k1, k2) of the round's
-// global lattice maps to one thread, k2 (DIM2) fastest so loads and stores
-// coalesce along the fastest axis.  The reference runs block -> round
+// One launch per round covers every block.  The reference runs block -> round
 // (compress.tpp:179-208); every round-r read touches points written in rounds
 // < r or the pivots, and every target is owned by exactly one block, so the
-// round-major order here reads exactly the same values (no block can see
-// another block's round-r writes).  The field is updated in place: targets
-// hold the original value until their own thread overwrites it, neighbours
-// (coarse points) are never written during the round.
+// round-major order here reads exactly the same values.  The field is updated
+// in place: targets hold the original value until their own thread rewrites
+// them, neighbours (coarse points, even multiples of h along the round axis)
+// are never written during the round.
+//
+// Two kernel shapes:
+//  * k_round_sweep (round axis 0 or 1): a wave owns 64 consecutive lattice
+//    columns along DIM2 and walks a segment (<= kSeg targets) of one block
+//    along the round axis, keeping the coarse neighbours in a register window
+//    (each coarse value is loaded once per segment).  QB values go straight
+//    to their slots when the serialization order runs along DIM2 ("012"), are
+//    staged in LDS and written as contiguous runs when it runs along the round
+//    axis ("120" on axis 0, "021" on axis 1), and are scattered otherwise.
+//  * k_round_direct (round axis 2, and every order): one thread per target,
+//    lanes along DIM2.
+// Error statistics and QB ranges are reduced per workgroup into partials and
+// folded by one small kernel after the last round.
 #include "device.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
@@ -27,6 +38,7 @@ __device__ __forceinline__ uint32_t order_key(float x) {
 
 __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint64_t n, uint32_t* out) {
     // out[0] = min key, out[1] = max key, out[2] = non-finite flag
+    __shared__ uint32_t s_mn[4], s_mx[4], s_bad[4];
     uint32_t mn = 0xFFFFFFFFu, mx = 0u, bad = 0u;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
@@ -51,7 +63,11 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
     mn = wave_min(mn);
     mx = wave_max(mx);
     bad = wave_max(bad);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { s_mn[w] = mn; s_mx[w] = mx; s_bad[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) { mn = min(mn, s_mn[k]); mx = max(mx, s_mx[k]); bad |= s_bad[k]; }
         atomicMin(&out[0], mn);
         atomicMax(&out[1], mx);
         if (bad) atomicOr(&out[2], 1u);
@@ -61,28 +77,28 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
 // first index whose DAZ value equals zero (min side) / last such index (max side)
 __global__ __launch_bounds__(256) void k_zero_index(const float* __restrict__ x, uint64_t n, unsigned long long* out) {
     unsigned long long first = ~0ull, last = 0ull;
-    bool any = false;
+    int any = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         if (daz(x[i]) == 0.0f) {
             first = min(first, (unsigned long long)i);
             last = max(last, (unsigned long long)i);
-            any = true;
+            any = 1;
         }
     }
     first = wave_min(first);
     last = wave_max(last);
-    int a = wave_max((int)any);
-    if ((threadIdx.x & (kWave - 1)) == 0 && a) {
+    any = wave_max(any);
+    if ((threadIdx.x & (kWave - 1)) == 0 && any) {
         atomicMin(&out[0], first);
         atomicMax(&out[1], last);
     }
 }
 
-// ------------------------------------------------------------------ accumulators
+// ------------------------------------------------------------------ partial reductions
 
-__device__ __forceinline__ void flush_acc(double err, double sq, int32_t qmin, int32_t qmax, ErrAcc* eacc,
-                                          ArrayAcc* aacc) {
+// One Partial per workgroup: max |x - x'|, sum (x - x')^2, QB range.
+__device__ __forceinline__ void write_partial(double err, double sq, int32_t qmin, int32_t qmax, Partial* out) {
     __shared__ double s_err[4], s_sq[4];
     __shared__ int32_t s_min[4], s_max[4];
     err = wave_max(err);
@@ -93,8 +109,68 @@ __device__ __forceinline__ void flush_acc(double err, double sq, int32_t qmin, i
     if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int nw = blockDim.x / kWave;
-        for (int k = 1; k < nw; k++) {
+        for (int k = 1; k < 4; k++) {
+            err = fmax(err, s_err[k]);
+            sq += s_sq[k];
+            qmin = min(qmin, s_min[k]);
+            qmax = max(qmax, s_max[k]);
+        }
+        const uint64_t wg = blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
+        out[wg] = Partial{err, sq, qmin, qmax};
+    }
+}
+
+// One workgroup per round: fold that round's partials into arr[2 + r] / rerr[r].
+__global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restrict__ part,
+                                                         const uint64_t* __restrict__ ranges, DevAcc* acc) {
+    const uint32_t r = blockIdx.x;
+    const uint64_t off = ranges[2 * r], n = ranges[2 * r + 1];
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        Partial p = part[off + i];
+        err = fmax(err, p.max_err);
+        sq += p.sq;
+        qmin = min(qmin, p.qmin);
+        qmax = max(qmax, p.qmax);
+    }
+    __shared__ double s_err[4], s_sq[4];
+    __shared__ int32_t s_min[4], s_max[4];
+    err = wave_max(err);
+    sq = wave_sum(sq);
+    qmin = wave_min(qmin);
+    qmax = wave_max(qmax);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            err = fmax(err, s_err[k]);
+            sq += s_sq[k];
+            qmin = min(qmin, s_min[k]);
+            qmax = max(qmax, s_max[k]);
+        }
+        acc->rerr[r] = ErrAcc{err, sq};
+        acc->arr[2 + r].qmin = qmin;
+        acc->arr[2 + r].qmax = qmax;
+    }
+}
+
+// ------------------------------------------------------------------ pivots (compress.tpp:161-175)
+
+__device__ __forceinline__ void flush_atomic(double err, double sq, int32_t qmin, int32_t qmax, ErrAcc* eacc,
+                                             ArrayAcc* aacc) {
+    __shared__ double s_err[4], s_sq[4];
+    __shared__ int32_t s_min[4], s_max[4];
+    err = wave_max(err);
+    sq = wave_sum(sq);
+    qmin = wave_min(qmin);
+    qmax = wave_max(qmax);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
             err = fmax(err, s_err[k]);
             sq += s_sq[k];
             qmin = min(qmin, s_min[k]);
@@ -108,8 +184,6 @@ __device__ __forceinline__ void flush_acc(double err, double sq, int32_t qmin, i
         }
     }
 }
-
-// ------------------------------------------------------------------ pivots (compress.tpp:161-175)
 
 __global__ __launch_bounds__(256) void k_pivots_compress(float* __restrict__ work, int32_t* __restrict__ qb,
                                                          PivotDesc P, ErrAcc* eacc, ArrayAcc* aacc) {
@@ -128,7 +202,7 @@ __global__ __launch_bounds__(256) void k_pivots_compress(float* __restrict__ wor
         sq = e * e;
         qmin = qmax = q;
     }
-    flush_acc(err, sq, qmin, qmax, eacc, aacc);
+    flush_atomic(err, sq, qmin, qmax, eacc, aacc);
 }
 
 __global__ __launch_bounds__(256) void k_pivots_decompress(float* __restrict__ out, const int32_t* __restrict__ qb,
@@ -140,13 +214,12 @@ __global__ __launch_bounds__(256) void k_pivots_decompress(float* __restrict__ o
     out[pos] = (float)((double)q * P.two_pe);
 }
 
-// ------------------------------------------------------------------ rounds
+// ------------------------------------------------------------------ direct rounds
 
-template <int AX, bool QUIN>
-__global__ __launch_bounds__(256) void k_round_compress(RoundDesc R, float* __restrict__ work,
-                                                        int32_t* __restrict__ qb, const uint32_t* __restrict__ tabs,
-                                                        const float* __restrict__ tblk, ErrAcc* eacc,
-                                                        ArrayAcc* aacc, int write_recon) {
+template <int AX, bool QUIN, bool DEC>
+__global__ __launch_bounds__(256) void k_round_direct(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
+                                                      const uint32_t* __restrict__ tabs,
+                                                      const float* __restrict__ tblk, Partial* part, int write_recon) {
     uint32_t k[3] = {blockIdx.z, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x};
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
@@ -160,37 +233,181 @@ __global__ __launch_bounds__(256) void k_round_compress(RoundDesc R, float* __re
         const uint64_t pos = ((uint64_t)i[0] * R.D[1] + i[1]) * R.D[2] + i[2];
         float* rp = work + pos;
         const float p = predict<QUIN>(rp, i[AX], R.h, right, (int64_t)R.dist, t);
-        const float x = *rp;
-        const int32_t q = quantize<AX>(p, x, R);
-        const float xr = reconstruct(p, q, R.neg_two_eb);
-        if (write_recon) *rp = xr;
-        qb[R.qb_off + qb_slot(R, tabs, L)] = q;
-        double e = (double)daz(x) - (double)xr;
-        err = fabs(e);
-        sq = e * e;
-        qmin = qmax = q;
+        const uint64_t slot = R.qb_off + qb_slot(R, tabs, L);
+        if (DEC) {
+            *rp = reconstruct(p, qb[slot], R.neg_two_eb);
+        } else {
+            const float x = *rp;
+            const int32_t q = quantize<AX>(p, x, R);
+            const float xr = reconstruct(p, q, R.neg_two_eb);
+            if (write_recon) *rp = xr;
+            qb[slot] = q;
+            double e = (double)daz(x) - (double)xr;
+            err = fabs(e);
+            sq = e * e;
+            qmin = qmax = q;
+        }
     }
-    flush_acc(err, sq, qmin, qmax, eacc, aacc);
+    if (!DEC) write_partial(err, sq, qmin, qmax, part + R.part_off);
 }
 
-template <int AX, bool QUIN>
-__global__ __launch_bounds__(256) void k_round_decompress(RoundDesc R, float* __restrict__ out,
-                                                          const int32_t* __restrict__ qb,
-                                                          const uint32_t* __restrict__ tabs,
-                                                          const float* __restrict__ tblk) {
-    uint32_t k[3] = {blockIdx.z, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x};
-    if (k[2] >= R.n[2]) return;
-    uint32_t i[3];
-    target_coords(R, AX, k, i);
-    TargetLoc L;
-    locate(R, AX, k, i, L);
-    const uint32_t right = min((L.b[AX] + 1) * R.B[AX] + 1, R.D[AX]);
-    const float t = tblk[((uint64_t)L.b[0] * R.nb[1] + L.b[1]) * R.nb[2] + L.b[2]];
-    const uint64_t pos = ((uint64_t)i[0] * R.D[1] + i[1]) * R.D[2] + i[2];
-    float* rp = out + pos;
-    const float p = predict<QUIN>(rp, i[AX], R.h, right, (int64_t)R.dist, t);
-    const int32_t q = qb[R.qb_off + qb_slot(R, tabs, L)];
-    *rp = reconstruct(p, q, R.neg_two_eb);
+// ------------------------------------------------------------------ sweep rounds (axis 0 / 1)
+
+// The reference predictor over a register window w[m] = coarse value at
+// i + (2m - 5) h, m = 0..5 (w[0] = i-5h ... w[5] = i+5h).  Same branches and
+// arithmetic as predict<QUIN>.
+template <bool QUIN>
+__device__ __forceinline__ float predict_w(uint32_t i, uint32_t h, uint32_t right, const float w[6], float t) {
+    if (QUIN && i >= 5 * h && i + 5 * h < right) {
+        float s1 = w[2] + w[3];
+        float s3 = w[1] + w[4];
+        float s5 = w[0] + w[5];
+        double acc = (double)s3 * (-25.0 / 256);
+        acc = __fma_rn((double)s1, 150.0 / 256, acc);
+        acc = __fma_rn((double)s5, 3.0 / 256, acc);
+        return (float)acc;
+    }
+    const float a = w[2];
+    if (i <= h) {
+        if (i + h >= right) return a;
+        const float b = w[3];
+        if (i + 3 * h >= right) return (a + b) * 0.5f;
+        return (__fmaf_rn(a, 3.0f, b * 6.0f) - w[4]) * 0.125f;
+    }
+    const float c = w[1];
+    if (i + h >= right) return __fmaf_rn(a, 3.0f, -c) * 0.5f;
+    const float b = w[3];
+    if (i + 3 * h >= right) return __fmaf_rn(b, 3.0f, __fmaf_rn(a, 6.0f, -c)) * 0.125f;
+    const float e = w[4];
+    const double td = (double)t;
+    const float p0 = (float)((0.25 - td) * (double)(e + c));
+    return (float)__fma_rn(td + 0.25, (double)(a + b), (double)p0);
+}
+
+// WMODE 0: order "012" (QB runs along DIM2)      -> direct per-lane writes
+// WMODE 1: order runs along the round axis       -> LDS tile, contiguous runs
+// WMODE 2: order runs along the other axis       -> scattered per-lane writes
+template <int AX, bool QUIN, bool DEC, int WMODE>
+__global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
+                                                     const uint32_t* __restrict__ tabs,
+                                                     const float* __restrict__ tblk, const SweepSeg* __restrict__ segs,
+                                                     Partial* part, int write_recon) {
+    constexpr int OA = 1 - AX;  // the non-round axis other than DIM2
+    __shared__ int32_t tile[WMODE == 1 ? 4 : 1][WMODE == 1 ? kSweepSeg : 1][65];
+    __shared__ uint64_t lbase[WMODE == 1 ? 4 : 1][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const SweepSeg S = segs[R.seg_off + blockIdx.z];
+    const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ko = blockIdx.y;
+    const bool valid = k2 < R.n[2];
+    const uint32_t kk2 = valid ? k2 : 0;
+    const uint32_t h = R.h;
+
+    // block coordinates and local indices of this lane's column
+    uint32_t b[3], j[3];
+    b[AX] = S.b;
+    j[AX] = S.k0 - S.b * (R.B[AX] / (2 * h));
+    {
+        const uint32_t i2 = R.s[2] * kk2;
+        b[2] = i2 == 0 ? 0 : (i2 - 1) / R.B[2];
+        j[2] = i2 == 0 ? 0 : (b[2] == 0 ? kk2 : kk2 - b[2] * (R.B[2] / R.s[2]) - 1);
+        const uint32_t io = R.s[OA] * ko;
+        b[OA] = io == 0 ? 0 : (io - 1) / R.B[OA];
+        j[OA] = io == 0 ? 0 : (b[OA] == 0 ? ko : ko - b[OA] * (R.B[OA] / R.s[OA]) - 1);
+    }
+    uint32_t cnt[3], pre[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        cnt[a] = tabs[R.tab_off[a] + b[a]];
+        pre[a] = tabs[R.tab_off[a] + R.nb[a] + b[a]];
+    }
+    const uint64_t base = R.qb_off + (uint64_t)pre[0] * R.C[1] * R.C[2] +
+                          (uint64_t)cnt[0] * ((uint64_t)pre[1] * R.C[2] + (uint64_t)cnt[1] * pre[2]);
+    const uint32_t right = min((S.b + 1) * R.B[AX] + 1, R.D[AX]);
+    const float t = tblk[((uint64_t)b[0] * R.nb[1] + b[1]) * R.nb[2] + b[2]];
+    const uint64_t sd = AX == 0 ? (uint64_t)R.D[1] * R.D[2] : (uint64_t)R.D[2];
+    const uint64_t so = OA == 0 ? (uint64_t)R.D[1] * R.D[2] : (uint64_t)R.D[2];
+    const uint64_t col = (uint64_t)(R.s[OA] * ko) * so + (uint64_t)R.s[2] * kk2;
+    const int64_t Dd = R.D[AX];
+
+    // QB slot of target c of the segment (orders of asz_param.hpp:122-129)
+    auto slot_of = [&](uint32_t c) -> uint64_t {
+        uint32_t jj[3] = {j[0], j[1], j[2]};
+        jj[AX] += c;
+        if (R.order == 0) return base + ((uint64_t)jj[1] * cnt[2] + jj[2]) * cnt[0] + jj[0];
+        if (R.order == 1) return base + ((uint64_t)jj[0] * cnt[2] + jj[2]) * cnt[1] + jj[1];
+        return base + ((uint64_t)jj[0] * cnt[1] + jj[1]) * cnt[2] + jj[2];
+    };
+    if (WMODE == 1) {
+        lbase[wave][lane] = slot_of(0);
+        if (DEC) {
+            __syncthreads();
+            for (uint32_t e = lane; e < 64 * S.cnt; e += 64) {
+                const uint32_t jl = e / S.cnt, c = e - jl * S.cnt;
+                const uint32_t kl = blockIdx.x * blockDim.x + wave * 64 + jl;
+                if (kl < R.n[2]) tile[wave][c][jl] = qb[lbase[wave][jl] + c];
+            }
+        }
+        __syncthreads();
+    }
+
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    if (valid) {
+        auto ld = [&](int64_t c) -> float { return (c >= 0 && c < Dd) ? work[col + (uint64_t)c * sd] : 0.0f; };
+        uint32_t i = h + 2 * h * S.k0;
+        const int64_t ih = (int64_t)i, hh = (int64_t)h;
+        float w[6];
+        w[0] = QUIN ? ld(ih - 5 * hh) : 0.0f;
+        w[1] = ld(ih - 3 * hh);
+        w[2] = ld(ih - hh);
+        w[3] = ld(ih + hh);
+        w[4] = ld(ih + 3 * hh);
+        w[5] = QUIN ? ld(ih + 5 * hh) : 0.0f;
+        for (uint32_t c = 0; c < S.cnt; c++, i += 2 * h) {
+            float* rp = work + col + (uint64_t)i * sd;
+            const float p = predict_w<QUIN>(i, h, right, w, t);
+            if (DEC) {
+                const int32_t q = WMODE == 1 ? tile[wave][c][lane] : qb[slot_of(c)];
+                *rp = reconstruct(p, q, R.neg_two_eb);
+            } else {
+                const float x = *rp;
+                const int32_t q = quantize<AX>(p, x, R);
+                const float xr = reconstruct(p, q, R.neg_two_eb);
+                if (write_recon) *rp = xr;
+                if (WMODE == 1) tile[wave][c][lane] = q;
+                else qb[slot_of(c)] = q;
+                const double e = (double)daz(x) - (double)xr;
+                err = fmax(err, fabs(e));
+                sq += e * e;
+                qmin = min(qmin, q);
+                qmax = max(qmax, q);
+            }
+            // slide the window by 2h
+            w[0] = w[1];
+            w[1] = w[2];
+            w[2] = w[3];
+            w[3] = w[4];
+            const int64_t inext = (int64_t)i + 2 * hh;
+            if (QUIN) {
+                w[4] = w[5];
+                w[5] = ld(inext + 5 * hh);
+            } else {
+                w[4] = ld(inext + 3 * hh);
+            }
+        }
+    }
+    if (!DEC) {
+        if (WMODE == 1) {
+            __syncthreads();
+            for (uint32_t e = lane; e < 64 * S.cnt; e += 64) {
+                const uint32_t jl = e / S.cnt, c = e - jl * S.cnt;
+                const uint32_t kl = blockIdx.x * blockDim.x + wave * 64 + jl;
+                if (kl < R.n[2]) qb[lbase[wave][jl] + c] = tile[wave][c][jl];
+            }
+        }
+        write_partial(err, sq, qmin, qmax, part + R.part_off);
+    }
 }
 
 }  // namespace dev
@@ -198,11 +415,11 @@ __global__ __launch_bounds__(256) void k_round_decompress(RoundDesc R, float* __
 // ------------------------------------------------------------------ launchers
 
 void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st) {
-    uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, 8192);
+    uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, 2048);
     hipLaunchKernelGGL(dev::k_range, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out3);
 }
 void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st) {
-    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
     hipLaunchKernelGGL(dev::k_zero_index, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out2);
 }
 
@@ -210,50 +427,87 @@ void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, 
                    hipStream_t st) {
     dim3 grid((P.pn[2] + 255) / 256, P.pn[1], P.pn[0]);
     if (compress)
-        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, (dev::ErrAcc*)eacc,
-                           (dev::ArrayAcc*)aacc);
+        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, (ErrAcc*)eacc,
+                           (ArrayAcc*)aacc);
     else
         hipLaunchKernelGGL(dev::k_pivots_decompress, grid, dim3(256), 0, st, work, (const int32_t*)qb, P);
 }
 
-template <int AX, bool Q>
-static void round_c(const RoundDesc& R, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk, void* e,
-                    void* a, int wr, hipStream_t st) {
-    dim3 grid((R.n[2] + 255) / 256, R.n[1], R.n[0]);
-    hipLaunchKernelGGL((dev::k_round_compress<AX, Q>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk,
-                       (dev::ErrAcc*)e, (dev::ArrayAcc*)a, wr);
-}
-template <int AX, bool Q>
-static void round_d(const RoundDesc& R, float* w, const int32_t* qb, const uint32_t* tabs, const float* tblk,
-                    hipStream_t st) {
-    dim3 grid((R.n[2] + 255) / 256, R.n[1], R.n[0]);
-    hipLaunchKernelGGL((dev::k_round_decompress<AX, Q>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk);
+void launch_reduce_partials(const Partial* part, const uint64_t* ranges, uint32_t rounds, DevAcc* acc,
+                            hipStream_t st) {
+    if (!rounds) return;
+    hipLaunchKernelGGL(dev::k_reduce_partials, dim3(rounds), dim3(256), 0, st, part, ranges, acc);
 }
 
-void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
-                           const float* tblk, void* eacc, void* aacc, int write_recon, hipStream_t st) {
-    if (R.count == 0) return;
-    switch (R.d * 2 + (quintic ? 1 : 0)) {
-        case 0: round_c<0, false>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
-        case 1: round_c<0, true>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
-        case 2: round_c<1, false>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
-        case 3: round_c<1, true>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
-        case 4: round_c<2, false>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
-        default: round_c<2, true>(R, work, qb, tabs, tblk, eacc, aacc, write_recon, st); break;
+uint64_t round_workgroups(const RoundDesc& R) {
+    const uint64_t gx = (R.n[2] + 255) / 256;
+    if (R.kind == kRoundSweep) return gx * R.n[1 - R.d] * R.nseg;
+    return gx * R.n[1] * R.n[0];
+}
+
+namespace {
+
+template <int AX, bool Q, bool DEC>
+void direct(const RoundDesc& R, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk, Partial* part, int wr,
+            hipStream_t st) {
+    dim3 grid((R.n[2] + 255) / 256, R.n[1], R.n[0]);
+    hipLaunchKernelGGL((dev::k_round_direct<AX, Q, DEC>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk, part, wr);
+}
+
+template <int AX, bool Q, bool DEC, int WM>
+void sweep(const RoundDesc& R, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk, const SweepSeg* segs,
+           Partial* part, int wr, hipStream_t st) {
+    dim3 grid((R.n[2] + 255) / 256, R.n[1 - AX], R.nseg);
+    hipLaunchKernelGGL((dev::k_round_sweep<AX, Q, DEC, WM>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk, segs,
+                       part, wr);
+}
+
+template <bool DEC>
+void dispatch(const RoundDesc& R, bool q, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk,
+              const SweepSeg* segs, Partial* part, int wr, hipStream_t st) {
+    if (R.kind == kRoundDirect) {
+        switch (R.d * 2 + (q ? 1 : 0)) {
+            case 0: direct<0, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+            case 1: direct<0, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+            case 2: direct<1, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+            case 3: direct<1, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+            case 4: direct<2, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+            default: direct<2, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
+        }
+        return;
     }
+    // sweep: write mode from the serialization order
+    const int fastest = R.order == 2 ? 2 : (R.order == 0 ? 0 : 1);
+    const int wm = fastest == 2 ? 0 : (fastest == (int)R.d ? 1 : 2);
+    const int key = ((int)R.d * 2 + (q ? 1 : 0)) * 3 + wm;
+    switch (key) {
+        case 0: sweep<0, false, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 1: sweep<0, false, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 2: sweep<0, false, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 3: sweep<0, true, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 4: sweep<0, true, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 5: sweep<0, true, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 6: sweep<1, false, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 7: sweep<1, false, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 8: sweep<1, false, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 9: sweep<1, true, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 10: sweep<1, true, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        default: sweep<1, true, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+    }
+}
+
+}  // namespace
+
+void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
+                           const float* tblk, const SweepSeg* segs, Partial* part, int write_recon, hipStream_t st) {
+    if (R.count == 0) return;
+    dispatch<false>(R, quintic, work, qb, tabs, tblk, segs, part, write_recon, st);
 }
 
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
-                             const float* tblk, hipStream_t st) {
+                             const float* tblk, const SweepSeg* segs, hipStream_t st) {
     if (R.count == 0) return;
-    switch (R.d * 2 + (quintic ? 1 : 0)) {
-        case 0: round_d<0, false>(R, out, qb, tabs, tblk, st); break;
-        case 1: round_d<0, true>(R, out, qb, tabs, tblk, st); break;
-        case 2: round_d<1, false>(R, out, qb, tabs, tblk, st); break;
-        case 3: round_d<1, true>(R, out, qb, tabs, tblk, st); break;
-        case 4: round_d<2, false>(R, out, qb, tabs, tblk, st); break;
-        default: round_d<2, true>(R, out, qb, tabs, tblk, st); break;
-    }
+    dispatch<true>(R, quintic, out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, 1, st);
 }
 
 }  // namespace asz

@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X ASZ compression core.
+
+Workload (one GPU): a 512^3 float32 field, anisotropic |K|^-11/3 Gaussian random
+field (axis0 stretched 4x, axis2 2x) plus an oblique planar step of 0.3 x range,
+compressed at REL 1e-3 with the cubic family and a fixed plan that the
+reference s1_compress chose for this field.  One step = one full compression
+of the field resident in HBM: range scan, pivots, every round (GPU per-block t
+selection, prediction, quantization, reconstruction, QB serialization) and
+Huffman coding (histogram, codebook, encode, stream D2H).  Zstd runs on the
+host and is timed separately; the default slope (preprocessing) is computed
+once before timing, as the reference excludes it from its compression time.
+
+Prints one JSON line: value = compression throughput in uncompressed GB/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Plan the reference s1_compress picked for this field at REL 1e-3
+# (--block-pivot-ratio 3 --predictor-family cubic, auto plan generation).
+PLAN_REL_1E3 = ["cubic1_012", "cubic2_012", "cubic2_012", "cubic2_012", "cubic1_012", "cubic1_012", "cubic0_012",
+                "cubic2_012", "cubic2_012", "cubic0_012", "cubic1_021", "cubic0_012", "cubic2_012", "cubic0_120"]
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 512])
+    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--tsel", default="gpu", choices=["gpu", "exact"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--json-out", default=None)
+    return ap.parse_args()
+
+
+def cpu_baseline(field_np, plan, rel, bpr):
+    """Reference s1_compress / s1_decompress (built from the reference's sources
+    into oracle/_ref) on the same field, one core, pinned with taskset."""
+    from oracle import api
+    if not api.ref_available():
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        inp = os.path.join(tmp, "field.f32")
+        field_np.tofile(inp)
+        asz = os.path.join(tmp, "field.asz")
+        rec = os.path.join(tmp, "field.rec")
+        dims = [str(x) for x in field_np.shape]
+        common = ["-f", "-3", *dims, "-M", "REL", f"{rel:.17g}", "--block-pivot-ratio", str(bpr),
+                  "--predictor-family", "cubic"]
+        pin = ["taskset", "-c", "0"] if subprocess.run(["which", "taskset"], capture_output=True).returncode == 0 else []
+        t0 = time.time()
+        out = subprocess.run(pin + [api.REF_COMPRESS, "-i", inp, "-z", asz, *common, "--predictor-plan", ",".join(plan)],
+                             capture_output=True, text=True, timeout=600, check=True).stdout
+        wall_c = time.time() - t0
+        dout = subprocess.run(pin + [api.REF_DECOMPRESS, "-z", asz, "-o", rec, *common], capture_output=True,
+                              text=True, timeout=600, check=True).stdout
+        ctime = float(re.search(r"^Compression time: ([0-9.eE+-]+)", out, re.M).group(1))
+        dtime = float(re.search(r"^Decompression time: ([0-9.eE+-]+)", dout, re.M).group(1))
+        cr = float(re.search(r"^CR= ([0-9.eE+-]+)", dout, re.M).group(1))
+        with open(asz, "rb") as fh:
+            ref_frame = fh.read()
+    nbytes = field_np.nbytes
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = re.search(r"^model name\s*:\s*(.*)$", fh.read(), re.M).group(1).strip()
+    except Exception:
+        pass
+    return {"value": nbytes / ctime / 1e9, "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": f"whole {'x'.join(map(str, field_np.shape))} field, same plan/REL/family; reference "
+                      f"'Compression time' (range + core, no preprocessing) {ctime:.3f} s, wall {wall_c:.2f} s",
+            "decompress_GBps": nbytes / dtime / 1e9, "cr": cr, "cpu_model": cpu, "_frame": ref_frame}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (every run here has a single MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from asz_amd import codec, fields
+
+    torch.cuda.set_device(0)
+    shape = tuple(a.shape)
+    N = int(np.prod(shape))
+    nbytes = N * 4
+    bpr = 3 if a.rel > 1e-4 else 96
+    plan = PLAN_REL_1E3
+    field = fields.config2_field(shape, seed=2, device="cuda").contiguous()
+    torch.cuda.synchronize()
+
+    ctx = codec.Context(0)
+    ctx.set_kernel_events(True)
+    params = codec.make_params(rel_eb=float(np.float32(a.rel)), block_pivot_ratio=bpr, family="cubic")
+    pl = codec.make_plan(plan)
+    mn, mx = ctx.range(field.data_ptr(), N)
+    slope, *_ = ctx.default_slope(field.data_ptr(), shape, params, mn, mx)
+    params.default_slope = slope
+
+    # inputs resident in HBM: one copy per step (compression overwrites its input with the reconstruction)
+    nsteps = a.warmup + a.steps
+    works = [field.clone() for _ in range(nsteps)]
+    torch.cuda.synchronize()
+
+    def step(w):
+        m0, m1 = ctx.range(w.data_ptr(), N)
+        return ctx.compress_device(w.data_ptr(), shape, pl, params, m0, m1, tsel=a.tsel)
+
+    for k in range(a.warmup):
+        step(works[k])
+    acc = {"pivot": 0.0, "hist": 0.0, "chunk": 0.0, "encode": 0.0}
+    round_ms = [0.0] * len(plan)
+    tsel_ms = [0.0] * len(plan)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for k in range(a.steps):
+        res = step(works[a.warmup + k])
+        st = res.stats
+        acc["pivot"] += st.pivot_ms
+        acc["hist"] += st.hist_ms
+        acc["chunk"] += st.chunk_ms
+        acc["encode"] += st.encode_ms
+        for r in range(len(plan)):
+            round_ms[r] += st.round_ms[r]
+            tsel_ms[r] += st.tsel_round_ms[r]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    value = nbytes / (ms_per_step / 1e3) / 1e9
+    K = a.steps
+    round_ms = [x / K for x in round_ms]
+    tsel_ms = [x / K for x in tsel_ms]
+    for k in acc:
+        acc[k] /= K
+    st = res.stats
+    counts = res.counts
+    S_h = len(res.stream)
+
+    # Zstd on the host, timed separately
+    z0 = time.perf_counter()
+    frame = codec.zstd_compress(res.stream)
+    zstd_ms = (time.perf_counter() - z0) * 1e3
+    cr = nbytes / len(frame)
+
+    # decompression throughput (GPU Huffman decode + reconstruction, stream already on the host)
+    out = torch.empty(N, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    dec_ms = []
+    dec_round = [0.0] * len(plan)
+    dec_huff = 0.0
+    for k in range(a.warmup + a.steps):
+        d0 = time.perf_counter()
+        ctx._check(ctx._L.asz_decompress(ctx._c, np.frombuffer(res.stream, np.uint8).ctypes.data_as(codec.ctypes.c_void_p),
+                                         S_h, np.asarray(shape, np.uint64).ctypes.data_as(codec.ctypes.c_void_p),
+                                         codec.ctypes.byref(params), codec.ctypes.c_void_p(out.data_ptr()), None, None))
+        d1 = time.perf_counter()
+        if k >= a.warmup:
+            dec_ms.append((d1 - d0) * 1e3)
+            ds = ctx.stats()
+            dec_huff += ds.decode_ms / K
+            for r in range(len(plan)):
+                dec_round[r] += ds.round_ms[r] / K
+    dec_ms_step = sum(dec_ms) / len(dec_ms)
+    recon = out.view(shape)
+    err = (recon.double() - field.double()).abs()
+    max_err = float(err.max())
+    mse = float((err * err).mean())
+    rng = float(mx) - float(mn)
+    psnr = 20 * np.log10(rng) - 10 * np.log10(mse) if mse > 0 else float("inf")
+    abs_eb = float(np.float32(a.rel) * (np.float32(mx) - np.float32(mn)))
+
+    # roofline of the dominant kernel: the largest round kernel; algorithmic bytes
+    # per target = 16 (original read, coarse read, reconstruction write, QB write)
+    dom = max(range(len(plan)), key=lambda r: round_ms[r])
+    T_dom = counts[dom + 1]
+    dom_bytes = 16 * T_dom
+    dom_gbps = dom_bytes / (round_ms[dom] / 1e3) / 1e9
+    P = counts[0]
+    gpu_ms = acc["pivot"] + sum(round_ms) + sum(tsel_ms) + acc["hist"] + acc["chunk"] + acc["encode"]
+    B_c = 28 * N - 4 * P + S_h
+    dec_gpu_ms = dec_huff + sum(dec_round)
+    B_d = 16 * N - 4 * P + S_h
+
+    line = {
+        "metric": "compress/decompress GB/s + % HBM roofline; CR at REL 1e-3, tuned vs plain",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded GRF generated on the GPU)",
+        "config": {"workload": "config2: 512^3 f32 anisotropic |K|^-11/3 GRF + oblique step, REL 1e-3, cubic, "
+                               "fixed reference plan, block_pivot_ratio 3, GPU t selection; one step = range + "
+                               "compress to the pre-Zstd stream",
+                   "shape": list(shape), "rel_eb": a.rel, "plan": plan, "family": "cubic", "block_pivot_ratio": bpr,
+                   "tsel": a.tsel},
+        "roofline": {"bound": "hbm", "achieved": round(dom_gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": f"k_round_compress (round {dom}, axis {plan[dom][5]}, order {plan[dom][7:]})",
+                     "algorithmic_bytes": dom_bytes, "avg_ms": round(round_ms[dom], 5)},
+        "decompress_GBps": round(nbytes / (dec_ms_step / 1e3) / 1e9, 3),
+        "decompress_ms_per_step": round(dec_ms_step, 4),
+        "pipeline": {
+            "compress_gpu_kernel_ms": round(gpu_ms, 4),
+            "compress_algorithmic_bytes": B_c,
+            "compress_pipeline_roofline_frac": round(B_c / (gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "decompress_gpu_kernel_ms": round(dec_gpu_ms, 4),
+            "decompress_algorithmic_bytes": B_d,
+            "decompress_pipeline_roofline_frac": round(B_d / (dec_gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "round_ms": [round(x, 4) for x in round_ms],
+            "tsel_ms": [round(x, 4) for x in tsel_ms],
+            "pivot_ms": round(acc["pivot"], 4), "hist_ms": round(acc["hist"], 4), "chunk_ms": round(acc["chunk"], 4),
+            "encode_ms": round(acc["encode"], 4), "huffman_host_ms": round(st.huffman_host_ms, 4),
+            "decode_ms": round(dec_huff, 4), "dec_round_ms": [round(x, 4) for x in dec_round],
+            "zstd_host_ms": round(zstd_ms, 3),
+        },
+        "cr": round(cr, 4), "huffman_stream_bytes": S_h, "asz_bytes": len(frame),
+        "max_err_over_eb": round(max_err / abs_eb, 7), "psnr": round(psnr, 4),
+        "t_fallbacks": int(st.t_fallbacks),
+    }
+    if not a.no_cpu_baseline:
+        cb = cpu_baseline(field.cpu().numpy(), plan, a.rel, bpr)
+        if cb is not None:
+            ref_frame = cb.pop("_frame")
+            line["cpu_baseline"] = cb
+            line["cr_reference"] = cb.pop("cr")
+            line["cr_vs_reference"] = round(cr / line["cr_reference"], 5)
+            # cross-decode: the reference stream through the GPU decoder
+            ref_out = ctx.decompress(codec.zstd_decompress(ref_frame), shape, params)
+            rerr = float(np.max(np.abs(ref_out.astype(np.float64) - field.cpu().numpy().astype(np.float64))))
+            line["reference_stream_gpu_decoded_max_err_over_eb"] = round(rerr / abs_eb, 7)
+    else:
+        line["cpu_baseline"] = None
+    txt = json.dumps(line)
+    print(txt, flush=True)
+    if a.json_out:
+        with open(a.json_out, "w") as fh:
+            fh.write(txt + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -73,7 +73,18 @@ typedef struct asz_stats {
     uint64_t t_count, t_fallbacks; /* #t = #blocks * layers, and fallbacks to t = 4/16 */
     uint64_t layers;
     uint64_t qb_counts[ASZ_MAX_LAYERS + 1];
+    /* per-kernel device time, filled when ASZ_OPT_KERNEL_EVENTS is on */
+    double pivot_ms;
+    double tsel_round_ms[ASZ_MAX_LAYERS];
+    double round_ms[ASZ_MAX_LAYERS];
+    double hist_ms, chunk_ms, encode_ms, decode_ms;
 } asz_stats;
+
+/* Options (asz_set_option).  ASZ_OPT_KERNEL_EVENTS: record a HIP event pair on
+ * the context's stream around every kernel of asz_compress / asz_decompress
+ * and report each kernel's device time in asz_stats. */
+#define ASZ_OPT_KERNEL_EVENTS 1
+int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 
 /* ------------------------------------------------------------------ context */
 int asz_ctx_create(int device, asz_ctx **out);
