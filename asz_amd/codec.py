@@ -136,8 +136,11 @@ class Context:
 
     # ------------------------------------------------------------ compression
     def compress_device(self, d_ptr: int, dims, plan: Plan, params: Params, mn: float, mx: float, tsel="gpu",
-                        t_inject=None, want_qb=False, want_t=False) -> CompressResult:
-        """d_ptr: device float32 field, overwritten with the reconstruction."""
+                        t_inject=None, want_qb=False, want_t=False, copy_stream=True) -> CompressResult:
+        """d_ptr: device float32 field, overwritten with the reconstruction.
+
+        With copy_stream=False the result's stream is None and stream_view()
+        returns the context's pinned buffer (valid until the next call)."""
         d = np.asarray(dims, np.uint64)
         N = int(np.prod(d))
         _, _, bn = self.geometry(dims, plan, params.block_pivot_ratio)
@@ -149,9 +152,15 @@ class Context:
         self._check(self._L.asz_compress(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
                                          ctypes.byref(params), mn, mx, TSEL[tsel], _ptr(ti), ctypes.byref(sp),
                                          ctypes.byref(sn), _ptr(qb), _ptr(t)))
-        stream = ctypes.string_at(sp, sn.value)
+        self._last_stream = (sp.value, sn.value)
+        stream = ctypes.string_at(sp, sn.value) if copy_stream else None
         st = self.stats()
         return CompressResult(stream, mn, mx, st, qb, t, None, [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
+
+    def stream_view(self) -> np.ndarray:
+        """The last compressed stream in the context's pinned buffer (no copy)."""
+        ptr, n = self._last_stream
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)), shape=(n,))
 
     def compress(self, field_host: np.ndarray, plan: Plan, params: Params, tsel="gpu", t_inject=None,
                  want_qb=False, want_t=False, want_recon=False) -> CompressResult:

@@ -52,10 +52,25 @@ struct DevBuf {
     }
 };
 
-struct Timer {
-    hipEvent_t a, b;
-    explicit Timer() { (void)hipEventCreate(&a); (void)hipEventCreate(&b); }
-    ~Timer() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+// Page-locked host memory for the large host <-> device transfers.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    template <typename T>
+    T* get(size_t n) {
+        size_t bytes = std::max<size_t>(n * sizeof(T), 64);
+        if (bytes > cap) {
+            if (p) HIPCHECK(hipHostFree(p));
+            p = nullptr;
+            cap = 0;
+            bytes = std::max(bytes, (size_t)1 << 20);
+            if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess)
+                fail(ASZ_ERR_NOMEM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+            cap = bytes;
+        }
+        return static_cast<T*>(p);
+    }
 };
 
 }  // namespace asz
@@ -67,8 +82,10 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, ss, tblk, acc, hist, book, arrays, chunk_arr, hchunk_arr, chunk_bits, chunk_off,
         words, keys, key_off, misc, dec_tab, dec_arrays, sub_arr, start, exit0, exit1, count, out_pos, segs,
-        partials, pranges;
-    std::vector<uint8_t> host_stream;
+        partials, pranges, blob;
+    asz::PinnedBuf pin_blob, pin_stream;
+    uint8_t* out_stream = nullptr;  // last compressed stream (in pin_stream)
+    uint64_t out_bytes = 0;
     hipEvent_t ev[8];
     // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
     bool kev_on = false;
@@ -123,9 +140,6 @@ void kev_collect(asz_ctx* c) {
     c->kev_n = 0;
 }
 
-void put_le64(std::vector<uint8_t>& o, uint64_t v) {
-    for (int k = 0; k < 8; k++) o.push_back((uint8_t)(v >> (8 * k)));
-}
 uint64_t get_le64(const uint8_t* p) {
     uint64_t v = 0;
     for (int k = 0; k < 8; k++) v |= (uint64_t)p[k] << (8 * k);
@@ -442,28 +456,43 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipMemcpyAsync(hh.data(), dhist, hist_total * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
 
-    // codebooks and stream layout (compress.tpp:243-291)
-    std::vector<uint8_t> skel;
-    skel.resize(8);
-    std::memcpy(skel.data(), &mn, 4);
-    std::memcpy(skel.data() + 4, &mx, 4);
-    skel.push_back((uint8_t)(L | 0x40 | (p.quintic ? 0x80 : 0)));
-    for (uint64_t r = 0; r < L; r++) {
-        skel.push_back((uint8_t)('0' + g.axis[r]));
-        skel.push_back((uint8_t)('0' + g.order[r]));
+    // codebooks and stream layout (compress.tpp:243-291).  Only the header
+    // bytes travel to the device (a compact blob scattered into the zeroed
+    // stream buffer); payload bits are ORed in by the encoder.
+    std::vector<uint8_t> blob;
+    std::vector<ByteSeg> bsegs;
+    uint64_t spos = 0;
+    auto emit = [&](const uint8_t* q, uint64_t n) {
+        if (!bsegs.empty() && bsegs.back().dst + bsegs.back().len == spos) bsegs.back().len += n;
+        else bsegs.push_back(ByteSeg{spos, blob.size(), n});
+        blob.insert(blob.end(), q, q + n);
+        spos += n;
+    };
+    {
+        uint8_t pre[9 + 2 * ASZ_MAX_LAYERS];
+        std::memcpy(pre, &mn, 4);
+        std::memcpy(pre + 4, &mx, 4);
+        pre[8] = (uint8_t)(L | 0x40 | (p.quintic ? 0x80 : 0));
+        for (uint64_t r = 0; r < L; r++) {
+            pre[9 + 2 * r] = (uint8_t)('0' + g.axis[r]);
+            pre[10 + 2 * r] = (uint8_t)('0' + g.order[r]);
+        }
+        emit(pre, 9 + 2 * L);
     }
     std::vector<uint64_t> book(hist_total);
     for (uint64_t k = 0; k < narr; k++) {
         HuffArray& A = lay.arrays[k];
         HuffArrayDev& D = lay.dev[k];
-        put_le64(skel, A.count);
+        uint8_t cnt8[8];
+        for (int b = 0; b < 8; b++) cnt8[b] = (uint8_t)(A.count >> (8 * b));
+        emit(cnt8, 8);
         if (!A.count) continue;
         build_codebook(A, hh.data() + D.hist_off);
-        skel.insert(skel.end(), A.header.begin(), A.header.end());
+        emit(A.header.data(), A.header.size());
         D.single = A.single ? 1 : 0;
         if (!A.single) {
-            D.payload_bit = (uint64_t)skel.size() * 8;
-            skel.resize(skel.size() + ((A.payload_bits + 7) >> 3), 0);
+            D.payload_bit = spos * 8;
+            spos += (A.payload_bits + 7) >> 3;
             for (uint64_t s = 0; s < A.range; s++) {
                 if (A.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
                 book[D.hist_off + s] = A.code[s] | ((uint64_t)A.len[s] << 56);
@@ -471,7 +500,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             }
         }
     }
-    lay.total_bytes = skel.size();
+    lay.total_bytes = spos;
     S.stream_bytes = lay.total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lay.arrays[k + 1].count;
     const uint64_t nwords = (lay.total_bytes + 3) / 4 + 4;
@@ -479,8 +508,15 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     uint64_t* dbook = c->book.get<uint64_t>(hist_total + 1);
     uint32_t* dcbits = c->chunk_bits.get<uint32_t>(chunks + 1);
     uint64_t* dcoff = c->chunk_off.get<uint64_t>(chunks + 1);
+    uint8_t* pblob = c->pin_blob.get<uint8_t>(blob.size() + bsegs.size() * sizeof(ByteSeg) + 64);
+    std::memcpy(pblob, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
+    std::memcpy(pblob + bsegs.size() * sizeof(ByteSeg), blob.data(), blob.size());
+    const uint64_t blob_bytes = bsegs.size() * sizeof(ByteSeg) + blob.size();
+    uint8_t* dblob = c->blob.get<uint8_t>(blob_bytes + 64);
     HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
-    HIPCHECK(hipMemcpyAsync(dwords, skel.data(), skel.size(), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(dblob, pblob, blob_bytes, hipMemcpyHostToDevice, st));
+    launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(dblob), (uint32_t)bsegs.size(),
+                         dblob + bsegs.size() * sizeof(ByteSeg), reinterpret_cast<uint8_t*>(dwords), st);
     HIPCHECK(hipMemcpyAsync(dbook, book.data(), hist_total * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
     double hhost_ms = ms_since(hh0);
@@ -494,8 +530,9 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
-    c->host_stream.resize(lay.total_bytes);
-    HIPCHECK(hipMemcpyAsync(c->host_stream.data(), dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
+    c->out_stream = c->pin_stream.get<uint8_t>(lay.total_bytes + 1);
+    c->out_bytes = lay.total_bytes;
+    HIPCHECK(hipMemcpyAsync(c->out_stream, dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
     if (qb_out) HIPCHECK(hipMemcpyAsync(qb_out, qb, N * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> t32;
     if (t_out) {
@@ -591,7 +628,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     // decode tables and descriptors
     std::vector<uint32_t> tables;
     std::vector<DecArrayDev> da;
-    std::vector<uint32_t> sub_arr;
+    std::vector<uint32_t> wg_arr;  // decoder workgroup -> array (256 subsequences each)
     for (uint64_t k = 0; k < L + 2; k++) {
         const DecodeInfo& di = Q.arrays[k];
         if (Q.counts[k] == 0) continue;
@@ -611,13 +648,15 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
         D.offset = di.offset;
         D.root_bits = (uint32_t)std::min(di.max_len, kRootBits);
         D.max_len = (uint32_t)di.max_len;
-        D.sub0 = sub_arr.size();
         D.nsub = (di.payload_bits + kSubBits - 1) / kSubBits;
+        D.wg0 = wg_arr.size();
+        D.nwg = (D.nsub + 255) / 256;
+        D.sub0 = D.wg0 * 256;
         tables.insert(tables.end(), di.table.begin(), di.table.end());
-        for (uint64_t i = 0; i < D.nsub; i++) sub_arr.push_back((uint32_t)da.size());
+        for (uint64_t i = 0; i < D.nwg; i++) wg_arr.push_back((uint32_t)da.size());
         da.push_back(D);
     }
-    const uint64_t nsub = sub_arr.size();
+    const uint64_t nwg = wg_arr.size(), nsub = nwg * 256;
     const uint64_t nwords = (n + 3) / 4 + 4;
     uint32_t* dwords = c->words.get<uint32_t>(nwords);
     HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
@@ -626,22 +665,23 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     HIPCHECK(hipMemcpyAsync(dtab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
     DecArrayDev* dda = c->dec_arrays.get<DecArrayDev>(da.size() + 1);
     HIPCHECK(hipMemcpyAsync(dda, da.data(), da.size() * sizeof(DecArrayDev), hipMemcpyHostToDevice, st));
-    uint32_t* dsub = c->sub_arr.get<uint32_t>(nsub + 1);
-    HIPCHECK(hipMemcpyAsync(dsub, sub_arr.data(), nsub * 4, hipMemcpyHostToDevice, st));
+    uint32_t* dwg = c->sub_arr.get<uint32_t>(nwg + 1);
+    HIPCHECK(hipMemcpyAsync(dwg, wg_arr.data(), nwg * 4, hipMemcpyHostToDevice, st));
     uint64_t* dstart = c->start.get<uint64_t>(nsub + 1);
     uint64_t* dex0 = c->exit0.get<uint64_t>(nsub + 1);
     uint64_t* dex1 = c->exit1.get<uint64_t>(nsub + 1);
     uint32_t* dcnt = c->count.get<uint32_t>(nsub + 1);
-    uint64_t* dpos = c->out_pos.get<uint64_t>(nsub + 1);
+    uint64_t* dwgs = c->out_pos.get<uint64_t>(2 * nwg + 2);
     unsigned int* flags = c->misc.get<unsigned int>(8);
     HIPCHECK(hipMemsetAsync(flags, 0, 8 * 4, st));
     c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
-    launch_dec_sync(dwords, nwords, dda, dsub, nsub, dtab, dstart, dex0, dcnt, st);
+    int kd = kev_begin(c, KEV_DECODE, 0);
+    launch_dec_sync(dwords, dda, dwg, nwg, dtab, dstart, dex0, dcnt, st);
     uint64_t iters = 0;
     for (;;) {
         HIPCHECK(hipMemsetAsync(flags, 0, 4, st));
-        launch_dec_fix(dwords, nwords, dda, dsub, nsub, dtab, dstart, dex0, dex1, dcnt, flags, st);
+        launch_dec_fix(dwords, dda, dwg, nwg, dtab, dstart, dex0, dex1, dcnt, flags, st);
         std::swap(dex0, dex1);
         unsigned int changed = 0;
         HIPCHECK(hipMemcpyAsync(&changed, flags, 4, hipMemcpyDeviceToHost, st));
@@ -650,8 +690,9 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
         if (!changed) break;
         if (iters > nsub + 2) fail(ASZ_ERR_DATA, "Huffman decoder did not synchronise");
     }
-    launch_dec_scan(dda, (uint32_t)da.size(), dcnt, dpos, flags + 1, st);
-    launch_dec_write(dwords, nwords, dda, dsub, nsub, dtab, dstart, dpos, sym, flags + 1, st);
+    launch_dec_offsets(dda, (uint32_t)da.size(), nwg, dcnt, dwgs, dwgs + nwg + 1, flags + 1, st);
+    launch_dec_write(dwords, dda, dwg, nwg, dtab, dstart, dcnt, dwgs + nwg + 1, sym, flags + 1, st);
+    kev_end(c, kd);
     HIPCHECK(hipEventRecord(c->ev[1], st));
     unsigned int derr = 0;
     HIPCHECK(hipMemcpyAsync(&derr, flags + 1, 4, hipMemcpyDeviceToHost, st));
@@ -696,7 +737,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = Q.counts[k + 1];
     kev_collect(c);
     S.huffman_ms = ev_ms(c->ev[0], c->ev[1]);
-    S.decode_ms = S.huffman_ms;
+    if (S.decode_ms == 0) S.decode_ms = S.huffman_ms;
     S.rounds_ms = ev_ms(c->ev[2], c->ev[3]);
     S.total_gpu_ms = S.huffman_ms + S.rounds_ms;
     S.wall_ms = ms_since(wall0);
@@ -865,8 +906,8 @@ int asz_compress(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_pl
     return guarded(c, [&] {
         HIPCHECK(hipSetDevice(c->device));
         asz::compress_impl(c, d_work, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
-        *stream = c->host_stream.data();
-        *stream_bytes = c->host_stream.size();
+        *stream = c->out_stream;
+        *stream_bytes = c->out_bytes;
     });
 }
 
@@ -885,8 +926,8 @@ int asz_compress_host(asz_ctx* c, const float* field, const uint64_t dims[3], co
             HIPCHECK(hipMemcpyAsync(recon, w, N * 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHECK(hipStreamSynchronize(c->stream));
         }
-        *stream = c->host_stream.data();
-        *stream_bytes = c->host_stream.size();
+        *stream = c->out_stream;
+        *stream_bytes = c->out_bytes;
     });
 }
 

@@ -181,139 +181,6 @@ __global__ __launch_bounds__(256) void k_encode(const int32_t* __restrict__ sym,
     }
 }
 
-// ------------------------------------------------------------------ decode
-
-__device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ w, uint64_t p) {
-    const uint64_t i = p >> 5;
-    const uint32_t sh = (uint32_t)(p & 31);
-    uint64_t lo = (uint64_t)w[i] | ((uint64_t)w[i + 1] << 32);
-    if (sh == 0) return lo;
-    return (lo >> sh) | ((uint64_t)w[i + 2] << (64 - sh));
-}
-
-// Decode one codeword at bit p; returns its length, writes the symbol index.
-// Codes are at most 64 bits (checked on the host), so one 64-bit window holds
-// the whole codeword.
-__device__ __forceinline__ uint32_t decode_one(const uint32_t* __restrict__ w, const uint32_t* __restrict__ tab,
-                                               uint32_t root_bits, uint64_t p, uint32_t& sym) {
-    uint64_t win = window64(w, p);
-    uint32_t bits = root_bits;
-    uint32_t consumed = 0;
-    uint32_t e = tab[win & ((1u << bits) - 1)];
-    while (e & 0x80000000u) {
-        consumed += bits;
-        win >>= bits;
-        const uint32_t* t = tab + (e & 0xFFFFFF);
-        bits = (e >> 24) & 0x1F;
-        e = t[win & ((1u << bits) - 1)];
-    }
-    sym = e & 0xFFFFFF;
-    return consumed + ((e >> 24) & 0x7F);
-}
-
-__global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ words, const DecArrayDev* __restrict__ arrays,
-                                                  const uint32_t* __restrict__ sub_arr, uint64_t nsub,
-                                                  const uint32_t* __restrict__ tables, uint64_t* start,
-                                                  uint64_t* exit_pos, uint32_t* count) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nsub) return;
-    const DecArrayDev A = arrays[sub_arr[j]];
-    const uint64_t u = j - A.sub0;
-    const uint64_t end = A.payload_bit + A.bits;
-    const uint64_t lim = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    uint64_t p = A.payload_bit + u * kSubBits;
-    start[j] = p;
-    uint32_t n = 0, sym;
-    const uint32_t* tab = tables + A.table_off;
-    while (p < lim) {
-        p += decode_one(words, tab, A.root_bits, p, sym);
-        n++;
-    }
-    exit_pos[j] = min(p, end + 64);
-    count[j] = n;
-}
-
-__global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ words, const DecArrayDev* __restrict__ arrays,
-                                                 const uint32_t* __restrict__ sub_arr, uint64_t nsub,
-                                                 const uint32_t* __restrict__ tables, uint64_t* start,
-                                                 const uint64_t* __restrict__ exit_in, uint64_t* exit_out,
-                                                 uint32_t* count, unsigned int* changed) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nsub) return;
-    const DecArrayDev A = arrays[sub_arr[j]];
-    const uint64_t u = j - A.sub0;
-    if (u == 0 || start[j] == exit_in[j - 1]) {
-        exit_out[j] = exit_in[j];
-        return;
-    }
-    const uint64_t end = A.payload_bit + A.bits;
-    const uint64_t lim = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    uint64_t p = exit_in[j - 1];
-    start[j] = p;
-    uint32_t n = 0, sym;
-    const uint32_t* tab = tables + A.table_off;
-    while (p < lim) {
-        p += decode_one(words, tab, A.root_bits, p, sym);
-        n++;
-    }
-    exit_out[j] = min(p, end + 64);
-    count[j] = n;
-    atomicOr(changed, 1u);
-}
-
-// One workgroup per array: exclusive scan of subsequence symbol counts.
-__global__ __launch_bounds__(1024) void k_dec_scan(const DecArrayDev* __restrict__ arrays,
-                                                   const uint32_t* __restrict__ count, uint64_t* out_pos,
-                                                   unsigned int* err) {
-    __shared__ uint64_t buf[1024];
-    const DecArrayDev A = arrays[blockIdx.x];
-    if (A.nsub == 0) return;
-    uint64_t carry = A.out_off;
-    for (uint64_t base = 0; base < A.nsub; base += 1024) {
-        uint64_t i = base + threadIdx.x;
-        uint64_t v = i < A.nsub ? count[A.sub0 + i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint64_t add = threadIdx.x >= (uint32_t)o ? buf[threadIdx.x - o] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < A.nsub) out_pos[A.sub0 + i] = carry + buf[threadIdx.x] - v;
-        uint64_t tot = buf[1023];
-        __syncthreads();
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && carry - A.out_off != A.count) atomicOr(err, 1u);
-}
-
-__global__ __launch_bounds__(256) void k_dec_write(const uint32_t* __restrict__ words,
-                                                   const DecArrayDev* __restrict__ arrays,
-                                                   const uint32_t* __restrict__ sub_arr, uint64_t nsub,
-                                                   const uint32_t* __restrict__ tables,
-                                                   const uint64_t* __restrict__ start,
-                                                   const uint64_t* __restrict__ out_pos, int32_t* out,
-                                                   unsigned int* err) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nsub) return;
-    const DecArrayDev A = arrays[sub_arr[j]];
-    const uint64_t u = j - A.sub0;
-    const uint64_t end = A.payload_bit + A.bits;
-    const uint64_t lim = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    uint64_t p = start[j];
-    uint64_t o = out_pos[j];
-    const uint64_t oend = A.out_off + A.count;
-    const uint32_t* tab = tables + A.table_off;
-    uint32_t sym;
-    while (p < lim) {
-        p += decode_one(words, tab, A.root_bits, p, sym);
-        if (o >= oend) { atomicOr(err, 2u); return; }
-        out[o++] = (int32_t)(A.offset + (int64_t)sym);
-    }
-    if (u + 1 == A.nsub && p != end) atomicOr(err, 4u);  // payload ends inside a code
-}
-
 }  // namespace dev
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
@@ -337,31 +204,6 @@ void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, const uint32_
     if (!nchunks) return;
     hipLaunchKernelGGL(dev::k_encode, dim3((uint32_t)nchunks), dim3(256), 0, st, sym, arrays, chunk_arr, book,
                        chunk_off, stream_words, max_len <= 32 ? 1 : 0);
-}
-void launch_dec_sync(const uint32_t* words, uint64_t, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                     uint64_t nsub, const uint32_t* tables, uint64_t* start, uint64_t* exit_pos, uint32_t* count,
-                     hipStream_t st) {
-    if (!nsub) return;
-    hipLaunchKernelGGL(dev::k_dec_sync, dim3((uint32_t)((nsub + 255) / 256)), dim3(256), 0, st, words, arrays, sub_arr,
-                       nsub, tables, start, exit_pos, count);
-}
-void launch_dec_fix(const uint32_t* words, uint64_t, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                    uint64_t nsub, const uint32_t* tables, uint64_t* start, const uint64_t* exit_in,
-                    uint64_t* exit_out, uint32_t* count, unsigned int* changed, hipStream_t st) {
-    if (!nsub) return;
-    hipLaunchKernelGGL(dev::k_dec_fix, dim3((uint32_t)((nsub + 255) / 256)), dim3(256), 0, st, words, arrays, sub_arr,
-                       nsub, tables, start, exit_in, exit_out, count, changed);
-}
-void launch_dec_scan(const DecArrayDev* arrays, uint32_t narrays, const uint32_t* count, uint64_t* out_pos,
-                     unsigned int* err, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_dec_scan, dim3(narrays), dim3(1024), 0, st, arrays, count, out_pos, err);
-}
-void launch_dec_write(const uint32_t* words, uint64_t, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                      uint64_t nsub, const uint32_t* tables, const uint64_t* start, const uint64_t* out_pos,
-                      int32_t* out, unsigned int* err, hipStream_t st) {
-    if (!nsub) return;
-    hipLaunchKernelGGL(dev::k_dec_write, dim3((uint32_t)((nsub + 255) / 256)), dim3(256), 0, st, words, arrays,
-                       sub_arr, nsub, tables, start, out_pos, out, err);
 }
 
 }  // namespace asz

@@ -63,7 +63,7 @@ struct RoundDesc {
 
 constexpr uint32_t kRoundDirect = 0;  // one thread per target (round axis 2)
 constexpr uint32_t kRoundSweep = 1;   // register-window sweep along the round axis (axis 0 / 1)
-constexpr uint32_t kSweepSeg = 32;    // targets per sweep segment
+constexpr uint32_t kSweepSeg = 16;    // targets per sweep segment
 
 // A sweep segment: targets k0 .. k0+cnt-1 (global indices along the round
 // axis) inside block b along that axis.

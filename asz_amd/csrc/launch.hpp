@@ -85,24 +85,25 @@ struct DecArrayDev {
     uint64_t count;        // symbols to produce
     uint64_t out_off;      // first output slot
     uint64_t table_off;    // decode table offset (words)
-    uint64_t sub0;         // first subsequence index of this array
+    uint64_t sub0;         // first subsequence index of this array (= wg0 * 256)
     uint64_t nsub;         // subsequences
+    uint64_t wg0, nwg;     // decoder workgroups of this array
     int64_t offset;        // symbol offset
     uint32_t root_bits;
     uint32_t max_len;
 };
-constexpr uint32_t kSubBits = 1024;  // bits per subsequence
+constexpr uint32_t kSubBits = 512;  // bits per subsequence
 
-void launch_dec_sync(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                     uint64_t nsub, const uint32_t* tables, uint64_t* start, uint64_t* exit_pos, uint32_t* count,
-                     hipStream_t st);
-void launch_dec_fix(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                    uint64_t nsub, const uint32_t* tables, uint64_t* start, const uint64_t* exit_in,
-                    uint64_t* exit_out, uint32_t* count, unsigned int* changed, hipStream_t st);
-void launch_dec_scan(const DecArrayDev* arrays, uint32_t narrays, const uint32_t* count, uint64_t* out_pos,
-                     unsigned int* err, hipStream_t st);
-void launch_dec_write(const uint32_t* words, uint64_t nwords, const DecArrayDev* arrays, const uint32_t* sub_arr,
-                      uint64_t nsub, const uint32_t* tables, const uint64_t* start, const uint64_t* out_pos,
+// wg_arr[w] = array of decoder workgroup w (each workgroup holds 256 subsequences of one array)
+void launch_dec_sync(const uint32_t* words, const DecArrayDev* arrays, const uint32_t* wg_arr, uint64_t nwg,
+                     const uint32_t* tables, uint64_t* start, uint64_t* exit_pos, uint32_t* count, hipStream_t st);
+void launch_dec_fix(const uint32_t* words, const DecArrayDev* arrays, const uint32_t* wg_arr, uint64_t nwg,
+                    const uint32_t* tables, uint64_t* start, const uint64_t* exit_in, uint64_t* exit_out,
+                    uint32_t* count, unsigned int* changed, hipStream_t st);
+void launch_dec_offsets(const DecArrayDev* arrays, uint32_t narrays, uint64_t nwg, const uint32_t* count,
+                        uint64_t* wg_sum, uint64_t* wg_off, unsigned int* err, hipStream_t st);
+void launch_dec_write(const uint32_t* words, const DecArrayDev* arrays, const uint32_t* wg_arr, uint64_t nwg,
+                      const uint32_t* tables, const uint64_t* start, const uint32_t* count, const uint64_t* wg_off,
                       int32_t* out, unsigned int* err, hipStream_t st);
 
 }  // namespace asz

@@ -6,6 +6,12 @@
 
 namespace asz {
 
+// Copy job: len bytes from src[src_off] to the stream at byte dst.
+struct ByteSeg {
+    uint64_t dst, src_off, len;
+};
+void launch_scatter_bytes(const ByteSeg* segs, uint32_t nsegs, const uint8_t* src, uint8_t* dst, hipStream_t st);
+
 void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st);
 void launch_fill_i32(int32_t* v, uint64_t n, int32_t val, hipStream_t st);
 void launch_t_float(const int32_t* t_int, uint64_t nblocks, uint32_t L, float prec, float* tblk, hipStream_t st);

@@ -33,7 +33,19 @@ __global__ __launch_bounds__(256) void k_t_float(const int32_t* __restrict__ t_i
     tblk[r * nblocks + b] = (float)t_int[i] / prec;
 }
 
+// One workgroup per segment: copy header bytes into the (zeroed) stream buffer.
+__global__ __launch_bounds__(256) void k_scatter_bytes(const ByteSeg* __restrict__ segs, const uint8_t* __restrict__ src,
+                                                       uint8_t* __restrict__ dst) {
+    const ByteSeg S = segs[blockIdx.x];
+    for (uint64_t i = threadIdx.x; i < S.len; i += blockDim.x) dst[S.dst + i] = src[S.src_off + i];
+}
+
 }  // namespace dev
+
+void launch_scatter_bytes(const ByteSeg* segs, uint32_t nsegs, const uint8_t* src, uint8_t* dst, hipStream_t st) {
+    if (!nsegs) return;
+    hipLaunchKernelGGL(dev::k_scatter_bytes, dim3(nsegs), dim3(256), 0, st, segs, src, dst);
+}
 
 void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st) {
     if (!n) return;

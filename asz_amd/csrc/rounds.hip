@@ -354,24 +354,37 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
     if (valid) {
-        auto ld = [&](int64_t c) -> float { return (c >= 0 && c < Dd) ? work[col + (uint64_t)c * sd] : 0.0f; };
-        uint32_t i = h + 2 * h * S.k0;
-        const int64_t ih = (int64_t)i, hh = (int64_t)h;
-        float w[6];
-        w[0] = QUIN ? ld(ih - 5 * hh) : 0.0f;
-        w[1] = ld(ih - 3 * hh);
-        w[2] = ld(ih - hh);
-        w[3] = ld(ih + hh);
-        w[4] = ld(ih + 3 * hh);
-        w[5] = QUIN ? ld(ih + 5 * hh) : 0.0f;
-        for (uint32_t c = 0; c < S.cnt; c++, i += 2 * h) {
+        // Every load of the segment is issued before any use: cw[m] holds the
+        // coarse value at i0 + (2m - 5) h, xs[c] the original value of target c.
+        constexpr int SEG = (int)kSweepSeg;
+        const uint32_t cnt_s = S.cnt;
+        const uint32_t i0 = h + 2 * h * S.k0;
+        const int64_t hh = (int64_t)h;
+        float cw[SEG + 5];
+#pragma unroll
+        for (int m = 0; m < SEG + 5; m++) {
+            const int64_t c = (int64_t)i0 + (2 * m - 5) * hh;
+            const bool need = QUIN ? m <= (int)cnt_s + 4 : (m >= 1 && m <= (int)cnt_s + 3);
+            cw[m] = (need && c >= 0 && c < Dd) ? work[col + (uint64_t)c * sd] : 0.0f;
+        }
+        float xs[SEG];
+        if (!DEC) {
+#pragma unroll
+            for (int c = 0; c < SEG; c++)
+                xs[c] = c < (int)cnt_s ? work[col + (uint64_t)(i0 + 2 * h * c) * sd] : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < SEG; c++) {
+            if (c >= (int)cnt_s) break;
+            const uint32_t i = i0 + 2 * h * c;
             float* rp = work + col + (uint64_t)i * sd;
+            const float w[6] = {cw[c], cw[c + 1], cw[c + 2], cw[c + 3], cw[c + 4], cw[c + 5]};
             const float p = predict_w<QUIN>(i, h, right, w, t);
             if (DEC) {
                 const int32_t q = WMODE == 1 ? tile[wave][c][lane] : qb[slot_of(c)];
                 *rp = reconstruct(p, q, R.neg_two_eb);
             } else {
-                const float x = *rp;
+                const float x = xs[c];
                 const int32_t q = quantize<AX>(p, x, R);
                 const float xr = reconstruct(p, q, R.neg_two_eb);
                 if (write_recon) *rp = xr;
@@ -382,18 +395,6 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                 sq += e * e;
                 qmin = min(qmin, q);
                 qmax = max(qmax, q);
-            }
-            // slide the window by 2h
-            w[0] = w[1];
-            w[1] = w[2];
-            w[2] = w[3];
-            w[3] = w[4];
-            const int64_t inext = (int64_t)i + 2 * hh;
-            if (QUIN) {
-                w[4] = w[5];
-                w[5] = ld(inext + 5 * hh);
-            } else {
-                w[4] = ld(inext + 3 * hh);
             }
         }
     }
