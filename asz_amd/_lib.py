@@ -48,6 +48,7 @@ class Stats(ctypes.Structure):
                 ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double)]
 
 ASZ_OPT_KERNEL_EVENTS = 1
+ASZ_OPT_STREAM_ON_DEVICE = 2
 
 
 # exported entry points (must match include/asz_gpu.h)

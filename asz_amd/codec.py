@@ -108,6 +108,11 @@ class Context:
     def set_kernel_events(self, on: bool = True):
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_KERNEL_EVENTS, 1 if on else 0))
 
+    def set_stream_on_device(self, on: bool = True):
+        """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))
+        self._stream_dev = on
+
     def stats(self) -> Stats:
         s = Stats()
         self._L.asz_get_stats(self._c, ctypes.byref(s))
@@ -153,13 +158,21 @@ class Context:
                                          ctypes.byref(params), mn, mx, TSEL[tsel], _ptr(ti), ctypes.byref(sp),
                                          ctypes.byref(sn), _ptr(qb), _ptr(t)))
         self._last_stream = (sp.value, sn.value)
-        stream = ctypes.string_at(sp, sn.value) if copy_stream else None
+        stream = None
+        if copy_stream:
+            stream = self.stream_view().tobytes() if getattr(self, "_stream_dev", False) else \
+                ctypes.string_at(sp, sn.value)
         st = self.stats()
         return CompressResult(stream, mn, mx, st, qb, t, None, [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
     def stream_view(self) -> np.ndarray:
-        """The last compressed stream in the context's pinned buffer (no copy)."""
+        """The last compressed stream: the context's pinned buffer (no copy), or
+        a host copy when the stream was left in device memory."""
         ptr, n = self._last_stream
+        if getattr(self, "_stream_dev", False):
+            out = np.empty(n, np.uint8)
+            self._check(self._L.asz_copy_to_host(self._c, _ptr(out), ctypes.c_void_p(ptr), n))
+            return out
         return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)), shape=(n,))
 
     def compress(self, field_host: np.ndarray, plan: Plan, params: Params, tsel="gpu", t_inject=None,

@@ -2,10 +2,14 @@
 //
 // compress (param_c::compress, compress.tpp:146-299 without Zstd):
 //   pivots -> for each round: t (GPU selection / exact host scoring / injected)
-//   -> round kernel -> per-array QB range (fused into the round kernels)
-//   -> one histogram launch over all arrays -> host reference-tree codebooks
-//   -> chunk bit counts -> per-array scan -> encode into a stream buffer that
-//   already holds every header -> one D2H of the whole pre-Zstd stream.
+//   -> round kernel (QBs, reconstruction, per-workgroup error / QB range /
+//   histogram window) -> fold the partials and histograms on the GPU -> one
+//   read-back of accumulators + compacted histograms -> host reference-tree
+//   codebooks -> one upload of headers and codebooks -> chunk-local encode
+//   -> per-array scan of chunk bit counts -> bit-exact compaction into a
+//   stream buffer that already holds every header.  The stream stays
+//   in HBM (ASZ_OPT_STREAM_ON_DEVICE) or is copied once to pinned memory.
+//   Plan tables are uploaded once and reused while the plan inputs repeat.
 // decompress (param_c::decompress, decompress.tpp:88-196): host parses the
 //   headers and builds decode tables; the GPU decodes every array and runs
 //   pivots and rounds.
@@ -73,6 +77,24 @@ struct PinnedBuf {
     }
 };
 
+// Inputs that shape a compression plan and its device tables.  Compared
+// bytewise (the struct is zeroed before it is filled).
+struct PlanKey {
+    uint64_t dims[3];
+    asz_plan plan;
+    uint64_t bpr, h1;
+    float aeb, pivot_eb_ratio, eb_ratio, slope_ratio, default_slope;
+};
+
+struct PlanCache {
+    bool valid = false;
+    PlanKey key;
+    Plan P;
+    struct {
+        uint64_t tabs, ss, segs, ranges, bytes;
+    } up{};
+};
+
 }  // namespace asz
 
 struct asz_ctx {
@@ -80,11 +102,13 @@ struct asz_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     asz_stats stats{};
-    asz::DevBuf work, sym, tabs, ss, tblk, acc, hist, book, arrays, chunk_arr, hchunk_arr, chunk_bits, chunk_off,
-        words, keys, key_off, misc, dec_tab, dec_arrays, sub_arr, start, exit0, exit1, count, out_pos, segs,
-        partials, pranges, blob;
-    asz::PinnedBuf pin_blob, pin_stream;
-    uint8_t* out_stream = nullptr;  // last compressed stream (in pin_stream)
+    asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
+        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, cmeta, phist, ghist, xfer, hmeta, scratch,
+        chunk_bits, chunk_off;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream;
+    asz::PlanCache pc;
+    bool stream_on_device = false;
+    uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;
     hipEvent_t ev[8];
     // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
@@ -108,7 +132,7 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_CHUNK, KEV_ENCODE, KEV_DECODE };
+enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE };
 
 int kev_begin(asz_ctx* c, int kind, int r) {
     if (!c->kev_on || c->kev_n * 2 + 2 > (int)c->kev.size()) return -1;
@@ -132,7 +156,6 @@ void kev_collect(asz_ctx* c) {
             case KEV_TSEL: S.tsel_round_ms[r] += ms; break;
             case KEV_ROUND: S.round_ms[r] += ms; break;
             case KEV_HIST: S.hist_ms += ms; break;
-            case KEV_CHUNK: S.chunk_ms += ms; break;
             case KEV_ENCODE: S.encode_ms += ms; break;
             default: S.decode_ms += ms; break;
         }
@@ -181,7 +204,6 @@ float abs_eb_of(float rel_eb, float mn, float mx) {
 struct Layout {
     std::vector<HuffArray> arrays;
     std::vector<HuffArrayDev> dev;
-    std::vector<uint64_t> block_off;  // byte offset of each array's HUF block (after its u64 count)
     uint64_t total_bytes = 0;
     int max_len = 0;
 };
@@ -266,15 +288,83 @@ static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_
 
 // ------------------------------------------------------------------ compress
 
+namespace {
+
+// Device layout of a plan's tables, uploaded once per plan (offsets in bytes).
+using PlanUpload = decltype(PlanCache::up);
+
+PlanUpload plan_upload_layout(const Plan& P) {
+    auto al = [](uint64_t v) { return (v + 255) / 256 * 256; };
+    PlanUpload u{};
+    u.tabs = 0;
+    u.ss = al(u.tabs + P.tables.size() * 4);
+    u.segs = al(u.ss + P.ss.size() * 4);
+    u.ranges = al(u.segs + P.segs.size() * sizeof(SweepSeg));
+    u.bytes = al(u.ranges + 4 * P.rounds.size() * 8) + 256;
+    return u;
+}
+
+}  // namespace
+
+// Same bin span rule as the compaction kernel: arrays inside the global window.
+static uint64_t compact_span_host(const ArrayAcc& a) {
+    if (a.qmin > a.qmax || a.qmin < -kHistCentre || a.qmax >= kHistCentre) return 0;
+    return (uint64_t)((int64_t)a.qmax - a.qmin + 1);
+}
+
 static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
                           const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
                           int32_t* qb_out, int64_t* t_out) {
     auto wall0 = Clock::now();
     validate_params(p);
     hipStream_t st = c->stream;
-    Geometry g = make_geometry(dims, plan, p.block_pivot_ratio);
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
-    Plan P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope, p.h1_block_sampling_num);
+
+    // plan and its device tables, reused while the inputs that shape them repeat
+    PlanKey key;
+    std::memset(&key, 0, sizeof(key));
+    for (int a = 0; a < 3; a++) key.dims[a] = dims[a];
+    key.plan = plan;
+    key.bpr = p.block_pivot_ratio;
+    key.h1 = p.h1_block_sampling_num;
+    key.aeb = aeb;
+    key.pivot_eb_ratio = p.pivot_eb_ratio;
+    key.eb_ratio = p.eb_ratio;
+    key.slope_ratio = p.slope_ratio;
+    key.default_slope = p.default_slope;
+    PlanCache& pc = c->pc;
+    if (!pc.valid || std::memcmp(&key, &pc.key, sizeof(key)) != 0) {
+        pc.valid = false;
+        Geometry g0 = make_geometry(dims, plan, p.block_pivot_ratio);
+        pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope,
+                         p.h1_block_sampling_num);
+        const Plan& P0 = pc.P;
+        const PlanUpload u = plan_upload_layout(P0);
+        uint8_t* hm = c->pin_meta.get<uint8_t>(u.bytes);
+        std::memcpy(hm + u.tabs, P0.tables.data(), P0.tables.size() * 4);
+        std::memcpy(hm + u.ss, P0.ss.data(), P0.ss.size() * 4);
+        std::memcpy(hm + u.segs, P0.segs.data(), P0.segs.size() * sizeof(SweepSeg));
+        uint64_t* rg = reinterpret_cast<uint64_t*>(hm + u.ranges);
+        for (size_t r = 0; r < P0.rounds.size(); r++) {
+            rg[4 * r] = P0.rounds[r].part_off;
+            rg[4 * r + 1] = P0.rounds[r].nwg;
+            rg[4 * r + 2] = P0.rounds[r].hist_off;
+            rg[4 * r + 3] = P0.rounds[r].hist_win;
+        }
+        uint8_t* dm = c->cmeta.get<uint8_t>(u.bytes);
+        HIPCHECK(hipMemcpyAsync(dm, hm, u.bytes, hipMemcpyHostToDevice, st));
+        pc.up = u;
+        pc.key = key;
+        pc.valid = true;
+    }
+    const Plan& P = pc.P;
+    const Geometry& g = P.g;
+    uint8_t* dm = static_cast<uint8_t*>(c->cmeta.p);
+    const uint32_t* tabs = reinterpret_cast<const uint32_t*>(dm + pc.up.tabs);
+    const uint32_t* ss = reinterpret_cast<const uint32_t*>(dm + pc.up.ss);
+    const SweepSeg* segs = reinterpret_cast<const SweepSeg*>(dm + pc.up.segs);
+    const uint64_t* dranges = reinterpret_cast<const uint64_t*>(dm + pc.up.ranges);
+
     const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
     asz_stats& S = c->stats;
     S = asz_stats{};
@@ -284,33 +374,20 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     // symbol buffer: t array (nt) then pivot + round QB arrays (N)
     int32_t* sym = c->sym.get<int32_t>(nt + N + 16);
     int32_t* qb = sym + nt;
-    uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
-    HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
-    uint32_t* ss = c->ss.get<uint32_t>(P.ss.size() + 1);
-    HIPCHECK(hipMemcpyAsync(ss, P.ss.data(), P.ss.size() * 4, hipMemcpyHostToDevice, st));
     float* tblk = c->tblk.get<float>(nt + 1);
-    SweepSeg* segs = c->segs.get<SweepSeg>(P.segs.size() + 1);
-    if (!P.segs.empty())
-        HIPCHECK(hipMemcpyAsync(segs, P.segs.data(), P.segs.size() * sizeof(SweepSeg), hipMemcpyHostToDevice, st));
     Partial* part = c->partials.get<Partial>(P.total_partials + 1);
-    std::vector<uint64_t> pranges(2 * L);
-    for (uint64_t r = 0; r < L; r++) {
-        pranges[2 * r] = P.rounds[r].part_off;
-        pranges[2 * r + 1] = P.rounds[r].count ? (r + 1 < L ? P.rounds[r + 1].part_off : P.total_partials) -
-                                                     P.rounds[r].part_off
-                                               : 0;
-    }
-    uint64_t* dranges = c->pranges.get<uint64_t>(2 * L);
-    HIPCHECK(hipMemcpyAsync(dranges, pranges.data(), 2 * L * 8, hipMemcpyHostToDevice, st));
-
-    // accumulators: error stats, fallback counter, QB range per array (t array + L+1)
-    const uint64_t narr = L + 2;
-    DevAcc init{};
-    for (uint64_t k = 0; k < narr; k++) { init.arr[k].qmin = INT32_MAX; init.arr[k].qmax = INT32_MIN; }
-    DevAcc* acc = c->acc.get<DevAcc>(1);
-    HIPCHECK(hipMemcpyAsync(acc, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    uint32_t* phist = c->phist.get<uint32_t>(P.total_hist_words + 1);
+    const uint64_t narr = L + 2;  // t array, pivots, rounds
+    uint32_t* ghist = c->ghist.get<uint32_t>(narr * kHistGlobal);
+    // transfer area: accumulators, then the compacted histograms
+    const uint64_t acc_bytes = (sizeof(DevAcc) + 255) / 256 * 256;
+    const uint64_t compact_max = narr * kHistGlobal;
+    uint8_t* xfer = c->xfer.get<uint8_t>(acc_bytes + compact_max * 4);
+    DevAcc* acc = reinterpret_cast<DevAcc*>(xfer);
+    uint32_t* compact = reinterpret_cast<uint32_t*>(xfer + acc_bytes);
     auto aacc = [&](uint64_t k) { return (void*)&acc->arr[k]; };
-    void* eacc = (void*)&acc->err;
+    launch_init_acc(acc, st);
+    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), narr * kHistGlobal, 0, st);
 
     if (tsel_mode == ASZ_TSEL_INJECT) {
         if (!t_inject) fail(ASZ_ERR_ARG, "t_inject is required with ASZ_TSEL_INJECT");
@@ -330,7 +407,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipEventRecord(c->ev[0], st));
     PivotDesc PD = pivot_desc(P);
     int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(true, d_work, qb, PD, eacc, aacc(1), st);
+    launch_pivots(true, d_work, qb, PD, (void*)&acc->err, aacc(1), st);
     kev_end(c, kp);
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
@@ -381,18 +458,29 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             tsel_host_ms += ms_since(h0);
         }
         int kr = kev_begin(c, KEV_ROUND, (int)r);
-        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, 1, st);
+        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist,
+                              ghist + (2 + r) * kHistGlobal, st);
         kev_end(c, kr);
     }
     HIPCHECK(hipGetLastError());
+    int kh = kev_begin(c, KEV_HIST, 0);
     launch_reduce_partials(part, dranges, (uint32_t)L, acc, st);
-    // t array range
     launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
+    launch_hist_small(sym, nt, qb, P.pivot_count, ghist, ghist + kHistGlobal, st);
+    launch_hist_reduce(phist, dranges, (uint32_t)L, ghist, st);
+    launch_hist_compact(acc, (uint32_t)narr, ghist, compact, st);
+    kev_end(c, kh);
     HIPCHECK(hipEventRecord(c->ev[1], st));
 
-    DevAcc hacc;
-    HIPCHECK(hipMemcpyAsync(&hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost, st));
+    // one read-back: accumulators and the compacted histograms (up to a budget)
+    const uint64_t budget = std::min<uint64_t>(compact_max, 32768);
+    uint8_t* px = c->pin_xfer.get<uint8_t>(acc_bytes + compact_max * 4);
+    HIPCHECK(hipMemcpyAsync(px, xfer, acc_bytes + budget * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    auto hh0 = Clock::now();
+    DevAcc hacc;
+    std::memcpy(&hacc, px, sizeof(hacc));
+    const uint32_t* hcompact = reinterpret_cast<const uint32_t*>(px + acc_bytes);
     S.max_err = hacc.err.max_err;
     S.sq_err_sum = hacc.err.sq_sum;
     for (uint64_t r = 0; r < L; r++) {
@@ -404,57 +492,65 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
 
     // ---------------------------------------------------------------- Huffman
-    auto hh0 = Clock::now();
     Layout lay;
     lay.arrays.resize(narr);
     lay.dev.resize(narr);
-    uint64_t hist_total = 0, hchunks = 0, chunks = 0;
+    std::vector<const uint32_t*> hptr(narr, nullptr);
+    std::vector<uint64_t> exact;  // arrays whose QB range leaves the global window
+    uint64_t coff = 0;
     for (uint64_t k = 0; k < narr; k++) {
         HuffArray& A = lay.arrays[k];
-        HuffArrayDev& D = lay.dev[k];
-        std::memset(&D, 0, sizeof(D));
-        const int32_t q[2] = {hacc.arr[k].qmin, hacc.arr[k].qmax};
         A.count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
         A.is_int64 = k == 0;
-        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
-        D.count = A.count;
         if (A.count == 0) continue;
-        A.offset = q[0];
-        A.range = (uint64_t)((int64_t)q[1] - (int64_t)q[0] + 1);
+        const int32_t q0 = hacc.arr[k].qmin, q1 = hacc.arr[k].qmax;
+        A.offset = q0;
+        A.range = (uint64_t)((int64_t)q1 - (int64_t)q0 + 1);
         if (A.range > (1ull << 24)) fail(ASZ_ERR_DATA, "QB alphabet wider than 2^24 symbols");
-        D.offset = A.offset;
-        D.range = A.range;
-        D.hist_off = hist_total;
-        hist_total += A.range;
-        D.hchunk0 = hchunks;
-        hchunks += (A.count + kHistChunk - 1) / kHistChunk;
-        D.chunk0 = chunks;
-        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
-        chunks += D.nchunk;
+        const uint64_t span = compact_span_host(hacc.arr[k]);
+        if (span) {
+            hptr[k] = hcompact + coff;
+            coff += span;
+        } else {
+            exact.push_back(k);
+        }
     }
-    std::vector<uint32_t> harr(hchunks), carr(chunks);
-    for (uint64_t k = 0; k < narr; k++) {
-        const HuffArrayDev& D = lay.dev[k];
-        if (!D.count) continue;
-        for (uint64_t i = 0; i < (D.count + kHistChunk - 1) / kHistChunk; i++) harr[D.hchunk0 + i] = (uint32_t)k;
-        for (uint64_t i = 0; i < D.nchunk; i++) carr[D.chunk0 + i] = (uint32_t)k;
+    if (coff > budget) {
+        HIPCHECK(hipMemcpyAsync(px + acc_bytes + budget * 4, compact + budget, (coff - budget) * 4,
+                                hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
     }
-    uint32_t* dhist = c->hist.get<uint32_t>(hist_total + 1);
-    HuffArrayDev* darr = c->arrays.get<HuffArrayDev>(narr);
-    uint32_t* dharr = c->hchunk_arr.get<uint32_t>(hchunks + 1);
-    uint32_t* dcarr = c->chunk_arr.get<uint32_t>(chunks + 1);
-    HIPCHECK(hipMemsetAsync(dhist, 0, hist_total * 4, st));
-    HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(dharr, harr.data(), hchunks * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), chunks * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipEventRecord(c->ev[2], st));
-    int kh = kev_begin(c, KEV_HIST, 0);
-    launch_hist(sym, darr, dharr, hchunks, dhist, st);
-    kev_end(c, kh);
-    HIPCHECK(hipEventRecord(c->ev[3], st));
-    std::vector<uint32_t> hh(hist_total);
-    HIPCHECK(hipMemcpyAsync(hh.data(), dhist, hist_total * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<uint32_t> hexact;
+    if (!exact.empty()) {
+        // exact-range histograms of the wide arrays
+        std::vector<HuffArrayDev> ed;
+        std::vector<uint32_t> carr;
+        uint64_t htot = 0;
+        for (uint64_t k : exact) {
+            const HuffArray& A = lay.arrays[k];
+            HuffArrayDev D{};
+            D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+            D.count = A.count;
+            D.offset = A.offset;
+            D.range = A.range;
+            D.hist_off = htot;
+            htot += A.range;
+            D.hchunk0 = carr.size();
+            for (uint64_t i = 0; i < (A.count + kHistChunk - 1) / kHistChunk; i++) carr.push_back((uint32_t)ed.size());
+            ed.push_back(D);
+        }
+        HuffArrayDev* ded = c->arrays.get<HuffArrayDev>(ed.size());
+        uint32_t* dcarr = c->chunk_arr.get<uint32_t>(carr.size() + 1);
+        uint32_t* dh = c->hist.get<uint32_t>(htot + 1);
+        HIPCHECK(hipMemcpyAsync(ded, ed.data(), ed.size() * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), carr.size() * 4, hipMemcpyHostToDevice, st));
+        launch_fill_i32(reinterpret_cast<int32_t*>(dh), htot, 0, st);
+        launch_hist(sym, ded, dcarr, carr.size(), dh, st);
+        hexact.resize(htot);
+        HIPCHECK(hipMemcpyAsync(hexact.data(), dh, htot * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        for (size_t e = 0; e < exact.size(); e++) hptr[exact[e]] = hexact.data() + ed[e].hist_off;
+    }
 
     // codebooks and stream layout (compress.tpp:243-291).  Only the header
     // bytes travel to the device (a compact blob scattered into the zeroed
@@ -479,60 +575,86 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         }
         emit(pre, 9 + 2 * L);
     }
-    std::vector<uint64_t> book(hist_total);
+    uint64_t book_total = 0, chunks = 0;
+    for (uint64_t k = 0; k < narr; k++) book_total += lay.arrays[k].count ? lay.arrays[k].range : 0;
+    std::vector<uint64_t> book(book_total + 1);
+    book_total = 0;
     for (uint64_t k = 0; k < narr; k++) {
         HuffArray& A = lay.arrays[k];
         HuffArrayDev& D = lay.dev[k];
+        std::memset(&D, 0, sizeof(D));
         uint8_t cnt8[8];
         for (int b = 0; b < 8; b++) cnt8[b] = (uint8_t)(A.count >> (8 * b));
         emit(cnt8, 8);
         if (!A.count) continue;
-        build_codebook(A, hh.data() + D.hist_off);
+        build_codebook(A, hptr[k]);
         emit(A.header.data(), A.header.size());
-        D.single = A.single ? 1 : 0;
-        if (!A.single) {
-            D.payload_bit = spos * 8;
-            spos += (A.payload_bits + 7) >> 3;
-            for (uint64_t s = 0; s < A.range; s++) {
-                if (A.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
-                book[D.hist_off + s] = A.code[s] | ((uint64_t)A.len[s] << 56);
-                lay.max_len = std::max<int>(lay.max_len, A.len[s]);
-            }
+        if (A.single) continue;
+        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+        D.count = A.count;
+        D.offset = A.offset;
+        D.range = A.range;
+        D.book_off = book_total;
+        D.chunk0 = chunks;
+        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
+        chunks += D.nchunk;
+        D.payload_bit = spos * 8;
+        spos += (A.payload_bits + 7) >> 3;
+        int alen = 0;
+        for (uint64_t s = 0; s < A.range; s++) {
+            if (A.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
+            book[book_total + s] = A.code[s] | ((uint64_t)A.len[s] << 56);
+            alen = std::max<int>(alen, A.len[s]);
         }
+        D.lds_book = (A.range <= kLdsBook && alen <= 24) ? 1 : 0;
+        lay.max_len = std::max(lay.max_len, alen);
+        book_total += A.range;
     }
     lay.total_bytes = spos;
     S.stream_bytes = lay.total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lay.arrays[k + 1].count;
+
+    // one upload: header segments, header bytes, codebooks, array descriptors
+    auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
+    const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
+                   o_arr = al8(o_book + book_total * 8), up_bytes = o_arr + narr * sizeof(HuffArrayDev);
+    uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
+    std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
+    std::memcpy(hu + o_blob, blob.data(), blob.size());
+    std::memcpy(hu + o_book, book.data(), book_total * 8);
+    std::memcpy(hu + o_arr, lay.dev.data(), narr * sizeof(HuffArrayDev));
+    uint8_t* du = c->hmeta.get<uint8_t>(up_bytes);
+    HIPCHECK(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, st));
+    S.huffman_host_ms = ms_since(hh0);
+
     const uint64_t nwords = (lay.total_bytes + 3) / 4 + 4;
     uint32_t* dwords = c->words.get<uint32_t>(nwords);
-    uint64_t* dbook = c->book.get<uint64_t>(hist_total + 1);
-    uint32_t* dcbits = c->chunk_bits.get<uint32_t>(chunks + 1);
-    uint64_t* dcoff = c->chunk_off.get<uint64_t>(chunks + 1);
-    uint8_t* pblob = c->pin_blob.get<uint8_t>(blob.size() + bsegs.size() * sizeof(ByteSeg) + 64);
-    std::memcpy(pblob, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
-    std::memcpy(pblob + bsegs.size() * sizeof(ByteSeg), blob.data(), blob.size());
-    const uint64_t blob_bytes = bsegs.size() * sizeof(ByteSeg) + blob.size();
-    uint8_t* dblob = c->blob.get<uint8_t>(blob_bytes + 64);
-    HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
-    HIPCHECK(hipMemcpyAsync(dblob, pblob, blob_bytes, hipMemcpyHostToDevice, st));
-    launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(dblob), (uint32_t)bsegs.size(),
-                         dblob + bsegs.size() * sizeof(ByteSeg), reinterpret_cast<uint8_t*>(dwords), st);
-    HIPCHECK(hipMemcpyAsync(dbook, book.data(), hist_total * 8, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(darr, lay.dev.data(), narr * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
-    double hhost_ms = ms_since(hh0);
+    // chunk scratch: each chunk's bits at the start of its own slot
+    const uint64_t slot_words = (uint64_t)kEncChunk * std::max(lay.max_len, 1) / 32 + 2;
+    uint32_t* scratch = c->scratch.get<uint32_t>(chunks * slot_words + 1);
+    uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
+    uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
     HIPCHECK(hipEventRecord(c->ev[4], st));
-    int kc = kev_begin(c, KEV_CHUNK, 0);
-    launch_chunk_bits(sym, darr, dcarr, chunks, dbook, dcbits, st);
-    launch_chunk_scan(darr, (uint32_t)narr, dcbits, dcoff, st);
-    kev_end(c, kc);
+    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), nwords, 0, st);
+    if (lay.max_len > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), chunks * slot_words, 0, st);
+    launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
+                         reinterpret_cast<uint8_t*>(dwords), st);
+    const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
     int ke = kev_begin(c, KEV_ENCODE, 0);
-    launch_encode(sym, darr, dcarr, chunks, dbook, dcoff, dwords, lay.max_len, st);
+    launch_encode_chunks(sym, darr, (uint32_t)narr, chunks, reinterpret_cast<const uint64_t*>(du + o_book), scratch,
+                         slot_words, cbits, lay.max_len, st);
+    launch_chunk_scan(darr, (uint32_t)narr, cbits, coffs, st);
+    launch_compact_bits(scratch, slot_words, cbits, coffs, chunks, dwords, st);
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
-    c->out_stream = c->pin_stream.get<uint8_t>(lay.total_bytes + 1);
     c->out_bytes = lay.total_bytes;
-    HIPCHECK(hipMemcpyAsync(c->out_stream, dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
+    if (c->stream_on_device) {
+        c->out_stream = reinterpret_cast<uint8_t*>(dwords);
+    } else {
+        c->out_stream = c->pin_stream.get<uint8_t>(lay.total_bytes + 1);
+        HIPCHECK(hipMemcpyAsync(c->out_stream, dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
+    }
     if (qb_out) HIPCHECK(hipMemcpyAsync(qb_out, qb, N * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> t32;
     if (t_out) {
@@ -544,8 +666,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         for (uint64_t i = 0; i < nt; i++) t_out[i] = t32[i];
     kev_collect(c);
     S.rounds_ms = ev_ms(c->ev[0], c->ev[1]);
-    S.huffman_ms = ev_ms(c->ev[2], c->ev[3]) + ev_ms(c->ev[4], c->ev[5]);
-    S.huffman_host_ms = hhost_ms;
+    S.huffman_ms = ev_ms(c->ev[4], c->ev[5]);
     S.tsel_ms = tsel_host_ms;
     S.total_gpu_ms = S.rounds_ms + S.huffman_ms;
     S.wall_ms = ms_since(wall0);
@@ -823,6 +944,10 @@ const char* asz_last_error(asz_ctx* c) { return c ? c->err.c_str() : "no context
 
 int asz_set_option(asz_ctx* c, int option, int64_t value) {
     return guarded(c, [&] {
+        if (option == ASZ_OPT_STREAM_ON_DEVICE) {
+            c->stream_on_device = value != 0;
+            return;
+        }
         if (option != ASZ_OPT_KERNEL_EVENTS) fail(ASZ_ERR_ARG, "unknown option");
         HIPCHECK(hipSetDevice(c->device));
         c->kev_on = value != 0;
@@ -921,7 +1046,15 @@ int asz_compress_host(asz_ctx* c, const float* field, const uint64_t dims[3], co
         HIPCHECK(hipMemcpyAsync(w, field, N * 4, hipMemcpyHostToDevice, c->stream));
         float mn, mx;
         asz::range_scan(c, w, N, &mn, &mx);
-        asz::compress_impl(c, w, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
+        const bool on_dev = c->stream_on_device;
+        c->stream_on_device = false;  // this entry point returns a host stream
+        try {
+            asz::compress_impl(c, w, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
+        } catch (...) {
+            c->stream_on_device = on_dev;
+            throw;
+        }
+        c->stream_on_device = on_dev;
         if (recon) {
             HIPCHECK(hipMemcpyAsync(recon, w, N * 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHECK(hipStreamSynchronize(c->stream));

@@ -31,35 +31,36 @@ __device__ __forceinline__ int64_t lround_f64(float y) {
 }
 
 // The reference predictor with GCC's contraction of compress.tpp:9-34
-// (identical in decompress.tpp:8-33).  r points at the target in the
-// reconstruction; dist is the element distance of one half stride.
+// (identical in decompress.tpp:8-33), over a register window of coarse values
+// w[m] = value at i + (2m - 5) h, m = 0..5 (w[0] = i-5h ... w[5] = i+5h).  i is
+// the target's coordinate along the round axis and right the end of its
+// block's extent (min((b+1) B + 1, D)); entries outside what the branch taken
+// reads may hold anything.
 template <bool QUIN>
-__device__ __forceinline__ float predict(const float* __restrict__ r, uint32_t i, uint32_t h, uint32_t right,
-                                         int64_t dist, float t) {
+__device__ __forceinline__ float predict_w(uint32_t i, uint32_t h, uint32_t right, const float w[6], float t) {
     if (QUIN && i >= 5 * h && i + 5 * h < right) {
-        float s1 = r[-dist] + r[dist];
-        float s3 = r[-3 * dist] + r[3 * dist];
-        float s5 = r[-5 * dist] + r[5 * dist];
+        float s1 = w[2] + w[3];
+        float s3 = w[1] + w[4];
+        float s5 = w[0] + w[5];
         double acc = (double)s3 * (-25.0 / 256);
         acc = __fma_rn((double)s1, 150.0 / 256, acc);
         acc = __fma_rn((double)s5, 3.0 / 256, acc);
         return (float)acc;
     }
-    float a = r[-dist];
+    const float a = w[2];
     if (i <= h) {
         if (i + h >= right) return a;
-        float b = r[dist];
+        const float b = w[3];
         if (i + 3 * h >= right) return (a + b) * 0.5f;
-        float e = r[3 * dist];
-        return (__fmaf_rn(a, 3.0f, b * 6.0f) - e) * 0.125f;
+        return (__fmaf_rn(a, 3.0f, b * 6.0f) - w[4]) * 0.125f;
     }
-    float c = r[-3 * dist];
+    const float c = w[1];
     if (i + h >= right) return __fmaf_rn(a, 3.0f, -c) * 0.5f;
-    float b = r[dist];
+    const float b = w[3];
     if (i + 3 * h >= right) return __fmaf_rn(b, 3.0f, __fmaf_rn(a, 6.0f, -c)) * 0.125f;
-    float e = r[3 * dist];
-    double td = (double)t;
-    float p0 = (float)((0.25 - td) * (double)(e + c));
+    const float e = w[4];
+    const double td = (double)t;
+    const float p0 = (float)((0.25 - td) * (double)(e + c));
     return (float)__fma_rn(td + 0.25, (double)(a + b), (double)p0);
 }
 
@@ -99,6 +100,23 @@ __device__ __forceinline__ T wave_min(T v) {
 
 __device__ __forceinline__ void atomic_max_pos_double(double* p, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    return (uint32_t)(((uint64_t)n * f.m) >> f.sh);
+}
+
+// One QB into the workgroup's LDS window of win bins (symbols [-win/2, win/2)),
+// or into the array's global window with an atomic when it falls outside.
+// Symbols outside the global window are left to the dedicated histogram pass.
+__device__ __forceinline__ void hist_add(uint32_t* lh, uint32_t win, uint32_t* gwin, int32_t q) {
+    const uint32_t w = (uint32_t)q + (win >> 1);
+    if (w < win) {
+        atomicAdd(&lh[w], 1u);
+    } else {
+        const uint32_t g = (uint32_t)q + (uint32_t)kHistCentre;
+        if (g < kHistGlobal) atomicAdd(&gwin[g], 1u);
+    }
 }
 
 using asz::ArrayAcc;

@@ -31,6 +31,24 @@ struct Geometry {
 };
 Geometry make_geometry(const uint64_t dims[3], const asz_plan& plan, uint64_t bpr);
 
+// Exact unsigned division by a constant 1 <= d < 2^31: n / d == (n * m) >> sh
+// for every n < 2^31, with sh = 32 + ceil(log2 d) and m = ceil(2^sh / d)
+// (m < 2^33, so n * m < 2^64; the rounding error n(m - 2^sh/d)/2^sh stays
+// below 1/d).
+struct FastDiv {
+    uint64_t m;
+    uint32_t sh, d;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    FastDiv f;
+    f.sh = 32 + l;
+    f.m = ((1ull << f.sh) + d - 1) / d;
+    f.d = d;
+    return f;
+}
+
 // Per-axis block tables of one round.  cnt[b] = targets of block b along the
 // axis, pre[b] = exclusive prefix of cnt (asz_param.hpp:105-115 per axis).
 // The QB base of block (b0,b1,b2) in the round's array is
@@ -54,16 +72,40 @@ struct RoundDesc {
     double inv_half_eb; // 0.5 / eb           (axis-0 quantizer form)
     double eb_d;        // (double) eb         (axes-1/2 quantizer form)
     double neg_two_eb;  // -(2.0 * (double) eb) (reconstruction)
-    uint32_t kind;      // kRoundDirect or kRoundSweep
+    uint32_t kind;      // kRoundRow or kRoundSweep
     uint32_t seg_off;   // first sweep segment of this round
     uint32_t nseg;      // sweep segments of this round
-    uint32_t pad;
+    uint32_t rpw;       // row kernel: lattice rows per workgroup (multiple of kRowsPerWave * 4)
     uint64_t part_off;  // first per-workgroup partial of this round
+    uint64_t nwg;       // workgroups of this round's kernel
+    uint64_t hist_off;  // first word of this round's per-workgroup histogram windows
+    uint32_t hist_win;  // bins of each workgroup's stored (partial) window (power of 2)
+    uint32_t lds_win;   // bins of each workgroup's LDS histogram (power of 2, >= hist_win)
+    FastDiv tpb;        // targets per full block along the round axis, B[d] / (2h)
 };
 
-constexpr uint32_t kRoundDirect = 0;  // one thread per target (round axis 2)
+constexpr uint32_t kRoundRow = 0;     // round axis 2: waves walk lattice rows along DIM2
 constexpr uint32_t kRoundSweep = 1;   // register-window sweep along the round axis (axis 0 / 1)
 constexpr uint32_t kSweepSeg = 16;    // targets per sweep segment
+constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave processes together
+
+// QB histograms.  Every round kernel bins its QBs into a per-workgroup LDS
+// histogram of lds_win symbols around 0.  At the end of the workgroup the
+// central hist_win bins are stored as its partial histogram (folded later
+// without atomics) and the other nonzero bins are added to the array's global
+// window of kHistGlobal bins (centre kHistCentre) with atomics; the rare QB
+// outside the LDS histogram goes there directly.  No global atomic sits in a
+// round kernel's main loop: gfx9 counts them in vmcnt with the loads, so a
+// slow one would stall every later load.  Rounds with few workgroups (the
+// coarse ones, whose QBs spread widest) get the widest partial windows, so the
+// partials stay near kHistPartialWords words per round.  Arrays whose QB
+// range leaves the global window are histogrammed again by a dedicated pass.
+constexpr int kHistWin = 128;                   // window of the t / pivot arrays
+constexpr uint32_t kHistWinMin = 128, kHistWinMax = 4096;
+constexpr uint32_t kLdsWinMin = 2048;
+constexpr uint64_t kHistPartialWords = 1 << 20;
+constexpr uint32_t kHistGlobal = 65536;
+constexpr int32_t kHistCentre = 32768;
 
 // A sweep segment: targets k0 .. k0+cnt-1 (global indices along the round
 // axis) inside block b along that axis.
@@ -85,6 +127,7 @@ struct Plan {
     std::vector<uint32_t> ns;      // t-selection sample count per (round, block), 0 for fallback blocks
     std::vector<SweepSeg> segs;    // sweep segments of every sweep round
     uint64_t total_partials = 0;   // per-workgroup partials over all rounds
+    uint64_t total_hist_words = 0; // per-workgroup histogram windows over all rounds
     uint64_t pivot_count;
     uint32_t pn[3];                // pivot lattice counts per axis
     float pivot_eb;

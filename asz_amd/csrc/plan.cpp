@@ -68,7 +68,8 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         for (int a = 0; a < 3; a++) {
             R.D[a] = (uint32_t)g.D[a];
             R.s[a] = (uint32_t)s[a];
-            R.B[a] = (uint32_t)g.block_shape[a];
+            // blocks wider than the field behave the same under any width >= D[a]
+            R.B[a] = (uint32_t)std::min<uint64_t>(g.block_shape[a], 1ull << 31);
             R.nb[a] = (uint32_t)g.block_num[a];
             if (a == d) R.n[a] = g.D[a] > R.h ? (uint32_t)((g.D[a] - R.h + 2 * R.h - 1) / (2 * R.h)) : 0;
             else R.n[a] = (uint32_t)((g.D[a] + s[a] - 1) / s[a]);
@@ -93,8 +94,10 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.count = count;
         R.qb_off = qb_off;
         qb_off += count;
-        // kernel shape: register-window sweep for rounds along axis 0/1, direct otherwise
-        R.kind = d == 2 ? kRoundDirect : kRoundSweep;
+        // kernel shape: register-window sweep for rounds along axis 0/1, row walk along DIM2 otherwise
+        R.kind = d == 2 ? kRoundRow : kRoundSweep;
+        // targets per block along d; any value above the target count gives block 0
+        R.tpb = make_fastdiv((uint32_t)std::min<uint64_t>(g.block_shape[d] / s[d], 1ull << 30));
         if (R.kind == kRoundSweep) {
             R.seg_off = (uint32_t)P.segs.size();
             const uint32_t per_block = (uint32_t)(g.block_shape[d] / s[d]);  // targets per full block along d
@@ -106,9 +109,28 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             }
             R.nseg = (uint32_t)P.segs.size() - R.seg_off;
         }
+        if (R.kind == kRoundSweep) {
+            R.nwg = (uint64_t)((R.n[2] + 255) / 256) * R.n[1 - d] * R.nseg;
+        } else {
+            // about 4096 targets per workgroup, but at least ~2048 workgroups when the round has the rows
+            const uint64_t rows = (uint64_t)R.n[0] * R.n[1];
+            auto up4 = [](uint64_t v) { return (std::max<uint64_t>(v, 1) + 3) / 4 * 4; };
+            const uint64_t want = up4((4096 + R.n[2] - 1) / std::max<uint32_t>(R.n[2], 1));
+            const uint64_t cap = up4((rows + 2047) / 2048);
+            const uint64_t g = 4 * kRowsPerWave;  // rows of one pass of the four waves
+            uint64_t rpw = std::min<uint64_t>(std::max<uint64_t>(4, std::min(want, cap)), 1024);
+            R.rpw = (uint32_t)((rpw + g - 1) / g * g);
+            R.nwg = (rows + R.rpw - 1) / R.rpw;
+        }
+        if (R.count == 0) R.nwg = 0;
         R.part_off = P.total_partials;
-        const uint64_t gx = (R.n[2] + 255) / 256;
-        P.total_partials += R.kind == kRoundSweep ? gx * R.n[1 - d] * R.nseg : gx * R.n[1] * R.n[0];
+        P.total_partials += R.nwg;
+        uint32_t win = kHistWinMin;
+        while (win < kHistWinMax && (uint64_t)win * 2 * std::max<uint64_t>(R.nwg, 1) <= kHistPartialWords) win *= 2;
+        R.hist_win = win;
+        R.lds_win = std::max(win, kLdsWinMin);
+        R.hist_off = P.total_hist_words;
+        P.total_hist_words += R.nwg * win;
         // t-selection sample strides (compress.tpp:96-116, get_AB_map.tpp:9-12); 0 marks a fallback block
         for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
             for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)

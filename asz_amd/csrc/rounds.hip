@@ -16,10 +16,11 @@
 //    to their slots when the serialization order runs along DIM2 ("012"), are
 //    staged in LDS and written as contiguous runs when it runs along the round
 //    axis ("120" on axis 0, "021" on axis 1), and are scattered otherwise.
-//  * k_round_direct (round axis 2, and every order): one thread per target,
-//    lanes along DIM2.
+//  * k_round_row (round axis 2, every order): waves walk whole lattice rows
+//    along DIM2, one target per lane per step.
 // Error statistics and QB ranges are reduced per workgroup into partials and
-// folded by one small kernel after the last round.
+// folded by one small kernel after the last round; each compress workgroup
+// also leaves an LDS histogram window of its QBs for the Huffman coder.
 #include "device.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
@@ -97,8 +98,12 @@ __global__ __launch_bounds__(256) void k_zero_index(const float* __restrict__ x,
 
 // ------------------------------------------------------------------ partial reductions
 
-// One Partial per workgroup: max |x - x'|, sum (x - x')^2, QB range.
-__device__ __forceinline__ void write_partial(double err, double sq, int32_t qmin, int32_t qmax, Partial* out) {
+// Per-workgroup results of a compress round kernel, stored at the
+// workgroup's linear index: one Partial (max |x - x'|, sum (x - x')^2, QB
+// range) and the LDS histogram window.  Called by every thread after its last
+// QB; the barrier inside also completes the LDS histogram.
+__device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, int32_t qmax, const RoundDesc& R,
+                                          Partial* part, const uint32_t* lh, uint32_t* phist, uint32_t* gwin) {
     __shared__ double s_err[4], s_sq[4];
     __shared__ int32_t s_min[4], s_max[4];
     err = wave_max(err);
@@ -108,6 +113,8 @@ __device__ __forceinline__ void write_partial(double err, double sq, int32_t qmi
     const int w = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
     __syncthreads();
+    const uint64_t wg =
+        R.part_off + blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; k++) {
             err = fmax(err, s_err[k]);
@@ -115,16 +122,34 @@ __device__ __forceinline__ void write_partial(double err, double sq, int32_t qmi
             qmin = min(qmin, s_min[k]);
             qmax = max(qmax, s_max[k]);
         }
-        const uint64_t wg = blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
-        out[wg] = Partial{err, sq, qmin, qmax};
+        part[wg] = Partial{err, sq, qmin, qmax};
+    }
+    // central bins to the partial, the others (nonzero) to the global window
+    uint32_t* out = phist + R.hist_off + (wg - R.part_off) * R.hist_win;
+    const uint32_t c0 = (R.lds_win - R.hist_win) / 2;
+    for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) {
+        const uint32_t v = lh[i];
+        if (i - c0 < R.hist_win) out[i - c0] = v;
+        else if (v) atomicAdd(&gwin[(uint32_t)kHistCentre - R.lds_win / 2 + i], v);
     }
 }
 
+__global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
+    const uint32_t i = threadIdx.x;
+    if (i == 0) {
+        acc->err = ErrAcc{0.0, 0.0};
+        acc->fallbacks = 0;
+    }
+    if (i < ASZ_MAX_LAYERS + 2) acc->arr[i] = ArrayAcc{INT32_MAX, INT32_MIN, {0, 0}};
+    if (i < ASZ_MAX_LAYERS) acc->rerr[i] = ErrAcc{0.0, 0.0};
+}
+
 // One workgroup per round: fold that round's partials into arr[2 + r] / rerr[r].
+// ranges[4r], ranges[4r + 1]: first partial and partial count of round r.
 __global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restrict__ part,
                                                          const uint64_t* __restrict__ ranges, DevAcc* acc) {
     const uint32_t r = blockIdx.x;
-    const uint64_t off = ranges[2 * r], n = ranges[2 * r + 1];
+    const uint64_t off = ranges[4 * r], n = ranges[4 * r + 1];
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
     for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -214,75 +239,117 @@ __global__ __launch_bounds__(256) void k_pivots_decompress(float* __restrict__ o
     out[pos] = (float)((double)q * P.two_pe);
 }
 
-// ------------------------------------------------------------------ direct rounds
+// ------------------------------------------------------------------ row rounds (axis 2)
 
-template <int AX, bool QUIN, bool DEC>
-__global__ __launch_bounds__(256) void k_round_direct(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
-                                                      const uint32_t* __restrict__ tabs,
-                                                      const float* __restrict__ tblk, Partial* part, int write_recon) {
-    uint32_t k[3] = {blockIdx.z, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x};
+// Round along DIM2.  Workgroup x owns lattice rows [x rpw, (x+1) rpw) of the
+// (k0, k1) lattice; wave w takes kRowsPerWave consecutive rows at a time and
+// lane l the targets k2 = l, l+64, ... of each, so loads and the "012" QB
+// stores are coalesced along DIM2, and the loads of all the rows are issued
+// before any of them is used.  Per-row quantities (blocks and QB bases along
+// axes 0/1) are wave-uniform.  With h == 1 and an even row length a row is
+// rewritten whole: each lane stores the pair {coarse left neighbour, target},
+// so no cache line is written partially (the neighbour is rewritten with the
+// value it holds).
+//
+// The serialization order is a template parameter: with a run-time switch on
+// R.order inside this divergent loop, the gfx950 code generated by this
+// toolchain added j1 instead of j2 on the "012" path (seen as QBs stored one
+// row-stride off); compile-time orders leave no switch to lower.
+template <bool QUIN, bool DEC, int ORDER>
+__global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
+                                                   const uint32_t* __restrict__ tabs, const float* __restrict__ tblk,
+                                                   Partial* part, uint32_t* __restrict__ phist,
+                                                   uint32_t* __restrict__ gwin) {
+    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
+    constexpr int RU = (int)kRowsPerWave;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t win = R.lds_win;
+    if (!DEC) {
+        for (uint32_t i = threadIdx.x; i < win; i += blockDim.x) lh[i] = 0;
+        __syncthreads();
+    }
+    const uint32_t h = R.h, n1 = R.n[1], n2 = R.n[2];
+    const int64_t D2 = R.D[2], hh = h;
+    const uint64_t rows = (uint64_t)R.n[0] * n1;
+    const bool pair = h == 1 && (R.D[2] & 1u) == 0 && (reinterpret_cast<uintptr_t>(work) & 7u) == 0;
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
-    if (k[2] < R.n[2]) {
-        uint32_t i[3];
-        target_coords(R, AX, k, i);
-        TargetLoc L;
-        locate(R, AX, k, i, L);
-        const uint32_t right = min((L.b[AX] + 1) * R.B[AX] + 1, R.D[AX]);
-        const float t = tblk[((uint64_t)L.b[0] * R.nb[1] + L.b[1]) * R.nb[2] + L.b[2]];
-        const uint64_t pos = ((uint64_t)i[0] * R.D[1] + i[1]) * R.D[2] + i[2];
-        float* rp = work + pos;
-        const float p = predict<QUIN>(rp, i[AX], R.h, right, (int64_t)R.dist, t);
-        const uint64_t slot = R.qb_off + qb_slot(R, tabs, L);
-        if (DEC) {
-            *rp = reconstruct(p, qb[slot], R.neg_two_eb);
-        } else {
-            const float x = *rp;
-            const int32_t q = quantize<AX>(p, x, R);
-            const float xr = reconstruct(p, q, R.neg_two_eb);
-            if (write_recon) *rp = xr;
-            qb[slot] = q;
-            double e = (double)daz(x) - (double)xr;
-            err = fabs(e);
-            sq = e * e;
-            qmin = qmax = q;
+    for (uint32_t rr0 = wave * RU; rr0 < R.rpw; rr0 += 4 * RU) {
+        const uint64_t row0 = (uint64_t)blockIdx.x * R.rpw + rr0;
+        if (row0 >= rows) break;
+        float* rowp[RU];
+        const float* trow[RU];
+        uint32_t j0[RU], j1[RU], c0[RU], c1[RU], p0[RU], p1[RU];
+        bool ok[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            ok[u] = row0 + u < rows;
+            const uint64_t row = ok[u] ? row0 + u : row0;
+            const uint32_t k0 = (uint32_t)(row / n1), k1 = (uint32_t)(row - (uint64_t)k0 * n1);
+            uint32_t b0, b1;
+            side_loc(R.s[0] * k0, k0, R.B[0], R.s[0], b0, j0[u]);
+            side_loc(R.s[1] * k1, k1, R.B[1], R.s[1], b1, j1[u]);
+            c0[u] = tabs[R.tab_off[0] + b0];
+            p0[u] = tabs[R.tab_off[0] + R.nb[0] + b0];
+            c1[u] = tabs[R.tab_off[1] + b1];
+            p1[u] = tabs[R.tab_off[1] + R.nb[1] + b1];
+            rowp[u] = work + ((uint64_t)(R.s[0] * k0) * R.D[1] + R.s[1] * k1) * R.D[2];
+            trow[u] = tblk + ((uint64_t)b0 * R.nb[1] + b1) * R.nb[2];
+        }
+        for (uint32_t k2 = lane; k2 < n2; k2 += 64) {
+            const uint32_t i2 = h + 2 * h * k2;
+            const uint32_t b2 = fdiv(k2, R.tpb);
+            const uint32_t j2 = k2 - b2 * R.tpb.d;
+            const uint32_t right = (uint32_t)min<uint64_t>((uint64_t)(b2 + 1) * R.B[2] + 1, R.D[2]);
+            const uint32_t c2 = tabs[R.tab_off[2] + b2], p2 = tabs[R.tab_off[2] + R.nb[2] + b2];
+            // every load of the RU rows first, unconditionally (positions clamped into
+            // the row; predict_w reads only the in-block ones): window w[u][m] at
+            // i2 + (2m - 5) h, the target, and the block's t
+            float w[RU][6], xv[RU], tv[RU];
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+#pragma unroll
+                for (int m = 0; m < 6; m++) {
+                    if (!QUIN && (m == 0 || m == 5)) {
+                        w[u][m] = 0.0f;
+                        continue;
+                    }
+                    const int64_t pos = min(max((int64_t)i2 + (2 * m - 5) * hh, (int64_t)0), D2 - 1);
+                    w[u][m] = rowp[u][pos];
+                }
+                xv[u] = DEC ? 0.0f : rowp[u][i2];
+                tv[u] = trow[u][b2];
+            }
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+                if (!ok[u]) continue;
+                const uint32_t cnt[3] = {c0[u], c1[u], c2}, pre[3] = {p0[u], p1[u], p2}, jj[3] = {j0[u], j1[u], j2};
+                const uint64_t slot = R.qb_off + block_base(R, cnt, pre) + local_slot(ORDER, jj, cnt);
+                const float p = predict_w<QUIN>(i2, h, right, w[u], tv[u]);
+                float xr;
+                if (DEC) {
+                    xr = reconstruct(p, qb[slot], R.neg_two_eb);
+                } else {
+                    const float x = xv[u];
+                    const int32_t q = quantize<2>(p, x, R);
+                    xr = reconstruct(p, q, R.neg_two_eb);
+                    qb[slot] = q;
+                    const double e = (double)daz(x) - (double)xr;
+                    err = fmax(err, fabs(e));
+                    sq += e * e;
+                    qmin = min(qmin, q);
+                    qmax = max(qmax, q);
+                    hist_add(lh, win, gwin, q);
+                }
+                if (pair) *reinterpret_cast<float2*>(rowp[u] + i2 - 1) = make_float2(w[u][2], xr);
+                else rowp[u][i2] = xr;
+            }
         }
     }
-    if (!DEC) write_partial(err, sq, qmin, qmax, part + R.part_off);
+    if (!DEC) finish_wg(err, sq, qmin, qmax, R, part, lh, phist, gwin);
 }
 
 // ------------------------------------------------------------------ sweep rounds (axis 0 / 1)
-
-// The reference predictor over a register window w[m] = coarse value at
-// i + (2m - 5) h, m = 0..5 (w[0] = i-5h ... w[5] = i+5h).  Same branches and
-// arithmetic as predict<QUIN>.
-template <bool QUIN>
-__device__ __forceinline__ float predict_w(uint32_t i, uint32_t h, uint32_t right, const float w[6], float t) {
-    if (QUIN && i >= 5 * h && i + 5 * h < right) {
-        float s1 = w[2] + w[3];
-        float s3 = w[1] + w[4];
-        float s5 = w[0] + w[5];
-        double acc = (double)s3 * (-25.0 / 256);
-        acc = __fma_rn((double)s1, 150.0 / 256, acc);
-        acc = __fma_rn((double)s5, 3.0 / 256, acc);
-        return (float)acc;
-    }
-    const float a = w[2];
-    if (i <= h) {
-        if (i + h >= right) return a;
-        const float b = w[3];
-        if (i + 3 * h >= right) return (a + b) * 0.5f;
-        return (__fmaf_rn(a, 3.0f, b * 6.0f) - w[4]) * 0.125f;
-    }
-    const float c = w[1];
-    if (i + h >= right) return __fmaf_rn(a, 3.0f, -c) * 0.5f;
-    const float b = w[3];
-    if (i + 3 * h >= right) return __fmaf_rn(b, 3.0f, __fmaf_rn(a, 6.0f, -c)) * 0.125f;
-    const float e = w[4];
-    const double td = (double)t;
-    const float p0 = (float)((0.25 - td) * (double)(e + c));
-    return (float)__fma_rn(td + 0.25, (double)(a + b), (double)p0);
-}
 
 // WMODE 0: order "012" (QB runs along DIM2)      -> direct per-lane writes
 // WMODE 1: order runs along the round axis       -> LDS tile, contiguous runs
@@ -291,11 +358,17 @@ template <int AX, bool QUIN, bool DEC, int WMODE>
 __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
                                                      const uint32_t* __restrict__ tabs,
                                                      const float* __restrict__ tblk, const SweepSeg* __restrict__ segs,
-                                                     Partial* part, int write_recon) {
+                                                     Partial* part, uint32_t* __restrict__ phist,
+                                                     uint32_t* __restrict__ gwin) {
     constexpr int OA = 1 - AX;  // the non-round axis other than DIM2
     __shared__ int32_t tile[WMODE == 1 ? 4 : 1][WMODE == 1 ? kSweepSeg : 1][65];
     __shared__ uint64_t lbase[WMODE == 1 ? 4 : 1][64];
+    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (!DEC) {
+        for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) lh[i] = 0;
+        if (WMODE != 1) __syncthreads();
+    }
     const SweepSeg S = segs[R.seg_off + blockIdx.z];
     const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ko = blockIdx.y;
@@ -307,22 +380,15 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     uint32_t b[3], j[3];
     b[AX] = S.b;
     j[AX] = S.k0 - S.b * (R.B[AX] / (2 * h));
-    {
-        const uint32_t i2 = R.s[2] * kk2;
-        b[2] = i2 == 0 ? 0 : (i2 - 1) / R.B[2];
-        j[2] = i2 == 0 ? 0 : (b[2] == 0 ? kk2 : kk2 - b[2] * (R.B[2] / R.s[2]) - 1);
-        const uint32_t io = R.s[OA] * ko;
-        b[OA] = io == 0 ? 0 : (io - 1) / R.B[OA];
-        j[OA] = io == 0 ? 0 : (b[OA] == 0 ? ko : ko - b[OA] * (R.B[OA] / R.s[OA]) - 1);
-    }
+    side_loc(R.s[2] * kk2, kk2, R.B[2], R.s[2], b[2], j[2]);
+    side_loc(R.s[OA] * ko, ko, R.B[OA], R.s[OA], b[OA], j[OA]);
     uint32_t cnt[3], pre[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         cnt[a] = tabs[R.tab_off[a] + b[a]];
         pre[a] = tabs[R.tab_off[a] + R.nb[a] + b[a]];
     }
-    const uint64_t base = R.qb_off + (uint64_t)pre[0] * R.C[1] * R.C[2] +
-                          (uint64_t)cnt[0] * ((uint64_t)pre[1] * R.C[2] + (uint64_t)cnt[1] * pre[2]);
+    const uint64_t base = R.qb_off + block_base(R, cnt, pre);
     const uint32_t right = min((S.b + 1) * R.B[AX] + 1, R.D[AX]);
     const float t = tblk[((uint64_t)b[0] * R.nb[1] + b[1]) * R.nb[2] + b[2]];
     const uint64_t sd = AX == 0 ? (uint64_t)R.D[1] * R.D[2] : (uint64_t)R.D[2];
@@ -330,13 +396,11 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     const uint64_t col = (uint64_t)(R.s[OA] * ko) * so + (uint64_t)R.s[2] * kk2;
     const int64_t Dd = R.D[AX];
 
-    // QB slot of target c of the segment (orders of asz_param.hpp:122-129)
+    // QB slot of target c of the segment
     auto slot_of = [&](uint32_t c) -> uint64_t {
         uint32_t jj[3] = {j[0], j[1], j[2]};
         jj[AX] += c;
-        if (R.order == 0) return base + ((uint64_t)jj[1] * cnt[2] + jj[2]) * cnt[0] + jj[0];
-        if (R.order == 1) return base + ((uint64_t)jj[0] * cnt[2] + jj[2]) * cnt[1] + jj[1];
-        return base + ((uint64_t)jj[0] * cnt[1] + jj[1]) * cnt[2] + jj[2];
+        return base + local_slot(R.order, jj, cnt);
     };
     if (WMODE == 1) {
         lbase[wave][lane] = slot_of(0);
@@ -387,7 +451,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                 const float x = xs[c];
                 const int32_t q = quantize<AX>(p, x, R);
                 const float xr = reconstruct(p, q, R.neg_two_eb);
-                if (write_recon) *rp = xr;
+                *rp = xr;
                 if (WMODE == 1) tile[wave][c][lane] = q;
                 else qb[slot_of(c)] = q;
                 const double e = (double)daz(x) - (double)xr;
@@ -395,6 +459,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                 sq += e * e;
                 qmin = min(qmin, q);
                 qmax = max(qmax, q);
+                hist_add(lh, R.lds_win, gwin, q);
             }
         }
     }
@@ -407,7 +472,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                 if (kl < R.n[2]) qb[lbase[wave][jl] + c] = tile[wave][c][jl];
             }
         }
-        write_partial(err, sq, qmin, qmax, part + R.part_off);
+        finish_wg(err, sq, qmin, qmax, R, part, lh, phist, gwin);
     }
 }
 
@@ -434,47 +499,54 @@ void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, 
         hipLaunchKernelGGL(dev::k_pivots_decompress, grid, dim3(256), 0, st, work, (const int32_t*)qb, P);
 }
 
+void launch_init_acc(DevAcc* acc, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_init_acc, dim3(1), dim3(128), 0, st, acc);
+}
+
 void launch_reduce_partials(const Partial* part, const uint64_t* ranges, uint32_t rounds, DevAcc* acc,
                             hipStream_t st) {
     if (!rounds) return;
     hipLaunchKernelGGL(dev::k_reduce_partials, dim3(rounds), dim3(256), 0, st, part, ranges, acc);
 }
 
-uint64_t round_workgroups(const RoundDesc& R) {
-    const uint64_t gx = (R.n[2] + 255) / 256;
-    if (R.kind == kRoundSweep) return gx * R.n[1 - R.d] * R.nseg;
-    return gx * R.n[1] * R.n[0];
-}
-
 namespace {
 
-template <int AX, bool Q, bool DEC>
-void direct(const RoundDesc& R, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk, Partial* part, int wr,
-            hipStream_t st) {
-    dim3 grid((R.n[2] + 255) / 256, R.n[1], R.n[0]);
-    hipLaunchKernelGGL((dev::k_round_direct<AX, Q, DEC>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk, part, wr);
+struct RoundArgs {
+    float* w;
+    int32_t* qb;
+    const uint32_t* tabs;
+    const float* tblk;
+    const SweepSeg* segs;
+    Partial* part;
+    uint32_t* phist;
+    uint32_t* gwin;
+};
+
+template <bool Q, bool DEC, int ORDER>
+void row(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((dev::k_round_row<Q, DEC, ORDER>), dim3((uint32_t)R.nwg), dim3(256), DEC ? 0 : R.lds_win * 4,
+                       st, R, a.w, a.qb, a.tabs, a.tblk, a.part, a.phist, a.gwin);
+}
+
+template <bool Q, bool DEC>
+void row_order(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
+    if (R.order == 0) row<Q, DEC, 0>(R, a, st);
+    else if (R.order == 1) row<Q, DEC, 1>(R, a, st);
+    else row<Q, DEC, 2>(R, a, st);
 }
 
 template <int AX, bool Q, bool DEC, int WM>
-void sweep(const RoundDesc& R, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk, const SweepSeg* segs,
-           Partial* part, int wr, hipStream_t st) {
+void sweep(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
     dim3 grid((R.n[2] + 255) / 256, R.n[1 - AX], R.nseg);
-    hipLaunchKernelGGL((dev::k_round_sweep<AX, Q, DEC, WM>), grid, dim3(256), 0, st, R, w, qb, tabs, tblk, segs,
-                       part, wr);
+    hipLaunchKernelGGL((dev::k_round_sweep<AX, Q, DEC, WM>), grid, dim3(256), DEC ? 0 : R.lds_win * 4, st, R, a.w,
+                       a.qb, a.tabs, a.tblk, a.segs, a.part, a.phist, a.gwin);
 }
 
 template <bool DEC>
-void dispatch(const RoundDesc& R, bool q, float* w, int32_t* qb, const uint32_t* tabs, const float* tblk,
-              const SweepSeg* segs, Partial* part, int wr, hipStream_t st) {
-    if (R.kind == kRoundDirect) {
-        switch (R.d * 2 + (q ? 1 : 0)) {
-            case 0: direct<0, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-            case 1: direct<0, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-            case 2: direct<1, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-            case 3: direct<1, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-            case 4: direct<2, false, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-            default: direct<2, true, DEC>(R, w, qb, tabs, tblk, part, wr, st); break;
-        }
+void dispatch(const RoundDesc& R, bool q, const RoundArgs& a, hipStream_t st) {
+    if (R.kind == kRoundRow) {
+        if (q) row_order<true, DEC>(R, a, st);
+        else row_order<false, DEC>(R, a, st);
         return;
     }
     // sweep: write mode from the serialization order
@@ -482,33 +554,35 @@ void dispatch(const RoundDesc& R, bool q, float* w, int32_t* qb, const uint32_t*
     const int wm = fastest == 2 ? 0 : (fastest == (int)R.d ? 1 : 2);
     const int key = ((int)R.d * 2 + (q ? 1 : 0)) * 3 + wm;
     switch (key) {
-        case 0: sweep<0, false, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 1: sweep<0, false, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 2: sweep<0, false, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 3: sweep<0, true, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 4: sweep<0, true, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 5: sweep<0, true, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 6: sweep<1, false, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 7: sweep<1, false, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 8: sweep<1, false, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 9: sweep<1, true, DEC, 0>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        case 10: sweep<1, true, DEC, 1>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
-        default: sweep<1, true, DEC, 2>(R, w, qb, tabs, tblk, segs, part, wr, st); break;
+        case 0: sweep<0, false, DEC, 0>(R, a, st); break;
+        case 1: sweep<0, false, DEC, 1>(R, a, st); break;
+        case 2: sweep<0, false, DEC, 2>(R, a, st); break;
+        case 3: sweep<0, true, DEC, 0>(R, a, st); break;
+        case 4: sweep<0, true, DEC, 1>(R, a, st); break;
+        case 5: sweep<0, true, DEC, 2>(R, a, st); break;
+        case 6: sweep<1, false, DEC, 0>(R, a, st); break;
+        case 7: sweep<1, false, DEC, 1>(R, a, st); break;
+        case 8: sweep<1, false, DEC, 2>(R, a, st); break;
+        case 9: sweep<1, true, DEC, 0>(R, a, st); break;
+        case 10: sweep<1, true, DEC, 1>(R, a, st); break;
+        default: sweep<1, true, DEC, 2>(R, a, st); break;
     }
 }
 
 }  // namespace
 
 void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
-                           const float* tblk, const SweepSeg* segs, Partial* part, int write_recon, hipStream_t st) {
+                           const float* tblk, const SweepSeg* segs, Partial* part, uint32_t* phist, uint32_t* gwin,
+                           hipStream_t st) {
     if (R.count == 0) return;
-    dispatch<false>(R, quintic, work, qb, tabs, tblk, segs, part, write_recon, st);
+    dispatch<false>(R, quintic, RoundArgs{work, qb, tabs, tblk, segs, part, phist, gwin}, st);
 }
 
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
                              const float* tblk, const SweepSeg* segs, hipStream_t st) {
     if (R.count == 0) return;
-    dispatch<true>(R, quintic, out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, 1, st);
+    dispatch<true>(R, quintic, RoundArgs{out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, nullptr, nullptr},
+                   st);
 }
 
 }  // namespace asz

@@ -84,6 +84,11 @@ typedef struct asz_stats {
  * the context's stream around every kernel of asz_compress / asz_decompress
  * and report each kernel's device time in asz_stats. */
 #define ASZ_OPT_KERNEL_EVENTS 1
+/* ASZ_OPT_STREAM_ON_DEVICE: when nonzero, asz_compress leaves the pre-Zstd
+ * stream in device memory and returns a device pointer in *stream (no copy to
+ * the host); the caller copies it out (asz_copy_to_host) or hands it to other
+ * device code.  Default 0: *stream is page-locked host memory. */
+#define ASZ_OPT_STREAM_ON_DEVICE 2
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 
 /* ------------------------------------------------------------------ context */
@@ -141,7 +146,8 @@ int asz_default_slope(asz_ctx *ctx, const float *d_field, const uint64_t dims[3]
  * min_val/max_val come from asz_range_f32.  tsel_mode picks the t source; with
  * ASZ_TSEL_INJECT, t_inject holds #blocks*layers t_int values.
  * On success *stream / *stream_bytes hold the pre-Zstd stream in a buffer the
- * context owns (valid until the next call on ctx).
+ * context owns (valid until the next call on ctx): page-locked host memory,
+ * or device memory when ASZ_OPT_STREAM_ON_DEVICE is set.
  * If qb_out is not NULL it receives the N QB values (pivot array then rounds);
  * if t_out is not NULL it receives the #t t_int values. */
 int asz_compress(asz_ctx *ctx, float *d_work, const uint64_t dims[3], const asz_plan *plan, const asz_params *p,
