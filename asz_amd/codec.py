@@ -108,6 +108,10 @@ class Context:
     def set_kernel_events(self, on: bool = True):
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_KERNEL_EVENTS, 1 if on else 0))
 
+    def set_timed_round(self, r: int):
+        """Time round r's kernel only (HIP events around that launch); r < 0: no events."""
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_TIMED_ROUND, int(r)))
+
     def set_stream_on_device(self, on: bool = True):
         """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))

@@ -3,13 +3,17 @@
 // compress (param_c::compress, compress.tpp:146-299 without Zstd):
 //   pivots -> for each round: t (GPU selection / exact host scoring / injected)
 //   -> round kernel (QBs, reconstruction, per-workgroup error / QB range /
-//   histogram window) -> fold the partials and histograms on the GPU -> one
-//   read-back of accumulators + compacted histograms -> host reference-tree
-//   codebooks -> one upload of headers and codebooks -> chunk-local encode
-//   -> per-array scan of chunk bit counts -> bit-exact compaction into a
-//   stream buffer that already holds every header.  The stream stays
-//   in HBM (ASZ_OPT_STREAM_ON_DEVICE) or is copied once to pinned memory.
-//   Plan tables are uploaded once and reused while the plan inputs repeat.
+//   histogram window) -> fold of the round's partials and histograms (the
+//   small coarse rounds in one batch).  The kernel that finishes an array
+//   writes its QB range and used histogram bins straight into pinned host
+//   memory and raises a flag; host worker threads build the reference-tree
+//   codebooks as the flags come up, while this thread is still launching and
+//   the GPU runs the later rounds.  Then one upload of headers and codebooks
+//   -> persistent chunk encoder -> per-array scan of chunk bit counts ->
+//   bit-exact compaction into a stream buffer that already holds every
+//   header.  The stream stays in HBM (ASZ_OPT_STREAM_ON_DEVICE) or is copied
+//   once to pinned memory.  Plan tables are uploaded once and reused while
+//   the plan inputs repeat.
 // decompress (param_c::decompress, decompress.tpp:88-196): host parses the
 //   headers and builds decode tables; the GPU decodes every array and runs
 //   pivots and rounds.
@@ -19,7 +23,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <immintrin.h>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "internal.hpp"
 #include "launch.hpp"
@@ -90,8 +98,10 @@ struct PlanCache {
     bool valid = false;
     PlanKey key;
     Plan P;
+    std::vector<FoldDesc> folds;   // per round (make_fold_descs)
+    uint64_t fpart_total = 0;      // fold scratch Partials over all rounds
     struct {
-        uint64_t tabs, ss, segs, ranges, bytes;
+        uint64_t tabs, ss, segs, folds, bytes;
     } up{};
 };
 
@@ -103,9 +113,15 @@ struct asz_ctx {
     std::string err;
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
-        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, cmeta, phist, ghist, xfer, hmeta, scratch,
+        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, cmeta, phist, ghist, xfer, hmeta, scratch,
         chunk_bits, chunk_off;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream;
+    hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
+    // host result slots (pinned, written by the GPU): kHostSlotWords per array
+    uint32_t* hres_host = nullptr;
+    uint32_t* hres_dev = nullptr;
+    uint64_t hres_arrays = 0;
+    uint32_t epoch = 0;
     asz::PlanCache pc;
     bool stream_on_device = false;
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
@@ -113,6 +129,7 @@ struct asz_ctx {
     hipEvent_t ev[8];
     // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
     bool kev_on = false;
+    int kev_round = -1;  // >= 0: events around that round's kernel only (ASZ_OPT_TIMED_ROUND)
     std::vector<hipEvent_t> kev;
     std::vector<std::pair<int, int>> kev_tag;
     int kev_n = 0;
@@ -136,6 +153,7 @@ enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE 
 
 int kev_begin(asz_ctx* c, int kind, int r) {
     if (!c->kev_on || c->kev_n * 2 + 2 > (int)c->kev.size()) return -1;
+    if (c->kev_round >= 0 && !(kind == KEV_ROUND && r == c->kev_round)) return -1;
     int i = c->kev_n++;
     c->kev_tag[i] = {kind, r};
     (void)hipEventRecord(c->kev[2 * i], c->stream);
@@ -185,6 +203,7 @@ PivotDesc pivot_desc(const Plan& P) {
         D.ps[a] = (uint32_t)P.g.pivot_stride[a];
         D.pn[a] = P.pn[a];
     }
+    D.count = P.pivot_count;
     D.inv_half_pe = 0.5 / (double)P.pivot_eb;
     D.two_pe = (double)P.pivot_eb * 2.0;
     return D;
@@ -299,17 +318,32 @@ PlanUpload plan_upload_layout(const Plan& P) {
     u.tabs = 0;
     u.ss = al(u.tabs + P.tables.size() * 4);
     u.segs = al(u.ss + P.ss.size() * 4);
-    u.ranges = al(u.segs + P.segs.size() * sizeof(SweepSeg));
-    u.bytes = al(u.ranges + 4 * P.rounds.size() * 8) + 256;
+    u.folds = al(u.segs + P.segs.size() * sizeof(SweepSeg));
+    u.bytes = al(u.folds + P.rounds.size() * sizeof(FoldDesc)) + 256;
     return u;
 }
 
 }  // namespace
 
-// Same bin span rule as the compaction kernel: arrays inside the global window.
-static uint64_t compact_span_host(const ArrayAcc& a) {
-    if (a.qmin > a.qmax || a.qmin < -kHistCentre || a.qmax >= kHistCentre) return 0;
-    return (uint64_t)((int64_t)a.qmax - a.qmin + 1);
+// Pinned host slots the GPU writes each array's histogram results into
+// (coherent, mapped: written by kernels while they run, read by the host).
+static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
+    if (c->hres_arrays >= narr) return;
+    if (c->hres_host) HIPCHECK(hipHostFree(c->hres_host));
+    c->hres_host = nullptr;
+    c->hres_dev = nullptr;
+    c->hres_arrays = 0;
+    const uint64_t n = std::max<uint64_t>(narr, 16);
+    void* h = nullptr;
+    if (hipHostMalloc(&h, n * kHostSlotWords * 4, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        fail(ASZ_ERR_NOMEM, "pinned host allocation of the histogram slots failed");
+    std::memset(h, 0, n * kHostSlotWords * 4);
+    void* d = nullptr;
+    HIPCHECK(hipHostGetDevicePointer(&d, h, 0));
+    c->hres_host = static_cast<uint32_t*>(h);
+    c->hres_dev = static_cast<uint32_t*>(d);
+    c->hres_arrays = n;
+    c->epoch = 0;
 }
 
 static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
@@ -318,6 +352,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     auto wall0 = Clock::now();
     validate_params(p);
     hipStream_t st = c->stream;
+    HIPCHECK(hipStreamSynchronize(c->side));  // no copy of an earlier call is still in flight
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
 
     // plan and its device tables, reused while the inputs that shape them repeat
@@ -344,13 +379,8 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         std::memcpy(hm + u.tabs, P0.tables.data(), P0.tables.size() * 4);
         std::memcpy(hm + u.ss, P0.ss.data(), P0.ss.size() * 4);
         std::memcpy(hm + u.segs, P0.segs.data(), P0.segs.size() * sizeof(SweepSeg));
-        uint64_t* rg = reinterpret_cast<uint64_t*>(hm + u.ranges);
-        for (size_t r = 0; r < P0.rounds.size(); r++) {
-            rg[4 * r] = P0.rounds[r].part_off;
-            rg[4 * r + 1] = P0.rounds[r].nwg;
-            rg[4 * r + 2] = P0.rounds[r].hist_off;
-            rg[4 * r + 3] = P0.rounds[r].hist_win;
-        }
+        make_fold_descs(P0.rounds, pc.folds, pc.fpart_total);
+        std::memcpy(hm + u.folds, pc.folds.data(), pc.folds.size() * sizeof(FoldDesc));
         uint8_t* dm = c->cmeta.get<uint8_t>(u.bytes);
         HIPCHECK(hipMemcpyAsync(dm, hm, u.bytes, hipMemcpyHostToDevice, st));
         pc.up = u;
@@ -363,7 +393,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     const uint32_t* tabs = reinterpret_cast<const uint32_t*>(dm + pc.up.tabs);
     const uint32_t* ss = reinterpret_cast<const uint32_t*>(dm + pc.up.ss);
     const SweepSeg* segs = reinterpret_cast<const SweepSeg*>(dm + pc.up.segs);
-    const uint64_t* dranges = reinterpret_cast<const uint64_t*>(dm + pc.up.ranges);
+    const FoldDesc* dfolds = reinterpret_cast<const FoldDesc*>(dm + pc.up.folds);
 
     const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
     asz_stats& S = c->stats;
@@ -376,18 +406,20 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     int32_t* qb = sym + nt;
     float* tblk = c->tblk.get<float>(nt + 1);
     Partial* part = c->partials.get<Partial>(P.total_partials + 1);
+    const uint64_t npp = (P.pivot_count + 255) / 256;  // pivot kernel workgroups
+    Partial* ppart = c->fparts.get<Partial>(npp + pc.fpart_total + 1);
+    Partial* fpart = ppart + npp;
     uint32_t* phist = c->phist.get<uint32_t>(P.total_hist_words + 1);
     const uint64_t narr = L + 2;  // t array, pivots, rounds
-    uint32_t* ghist = c->ghist.get<uint32_t>(narr * kHistGlobal);
-    // transfer area: accumulators, then the compacted histograms
-    const uint64_t acc_bytes = (sizeof(DevAcc) + 255) / 256 * 256;
-    const uint64_t compact_max = narr * kHistGlobal;
-    uint8_t* xfer = c->xfer.get<uint8_t>(acc_bytes + compact_max * 4);
-    DevAcc* acc = reinterpret_cast<DevAcc*>(xfer);
-    uint32_t* compact = reinterpret_cast<uint32_t*>(xfer + acc_bytes);
+    // global histogram windows, one kHistStride slot per array
+    uint32_t* ghist = c->ghist.get<uint32_t>(narr * kHistStride);
+    DevAcc* acc = c->xfer.get<DevAcc>(1);
     auto aacc = [&](uint64_t k) { return (void*)&acc->arr[k]; };
+    auto gwin = [&](uint64_t k) { return ghist + k * kHistStride; };
+    ensure_host_slots(c, narr);
+    const uint32_t epoch = ++c->epoch ? c->epoch : ++c->epoch;  // never 0 (the slots start zeroed)
     launch_init_acc(acc, st);
-    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), narr * kHistGlobal, 0, st);
+    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), narr * kHistStride, 0, st);
 
     if (tsel_mode == ASZ_TSEL_INJECT) {
         if (!t_inject) fail(ASZ_ERR_ARG, "t_inject is required with ASZ_TSEL_INJECT");
@@ -403,12 +435,132 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         HIPCHECK(hipMemcpyAsync(sym, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
     }
 
+    // Rounds [0, batch_end) are folded together after the last of them (one
+    // launch for the small coarse rounds, whose codebooks are then built while
+    // the six finest rounds run); every later round is folded on its own.
+    const uint64_t batch_end = L >= 7 ? L - 6 : 0;
+    // Arrays in the order the GPU finishes them: pivots, the batch, the later
+    // rounds, with the t array (final after the last t selection) before the
+    // last round.
+    std::vector<uint64_t> order{1};
+    for (uint64_t r = 0; r < L; r++) {
+        if (r + 1 == L) order.push_back(0);
+        if (r + 1 == batch_end)
+            for (uint64_t q = 0; q <= r; q++) order.push_back(2 + q);
+        else if (r + 1 > batch_end)
+            order.push_back(2 + r);
+    }
+
+    // ---------------------------------------------------------------- Huffman codebooks
+    // Host worker threads build the reference-tree codebook of each array as
+    // soon as the GPU has written that array's range and bins into its host
+    // slot (flag == epoch), concurrently with this thread's launches and with
+    // the later rounds on the GPU.
+    Layout lay;
+    lay.arrays.resize(narr);
+    lay.dev.resize(narr);
+    for (uint64_t k = 0; k < narr; k++) {
+        lay.arrays[k].count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
+        lay.arrays[k].is_int64 = k == 0;
+    }
+    std::atomic<bool> enqueued{false}, abort_all{false};
+    std::mutex side_mu;  // the exact-histogram path uses the side stream and shared scratch
+    std::vector<double> host_ms_w;
+    auto slot = [&](uint64_t k) { return c->hres_host + k * kHostSlotWords; };
+    auto wait_slot = [&](uint64_t k) {
+        volatile uint32_t* flag = slot(k);
+        uint64_t spins = 0;
+        while (*flag != epoch) {
+            if (abort_all.load(std::memory_order_relaxed)) fail(ASZ_ERR_HIP, "compression aborted");
+            if ((++spins & 0xFFFF) == 0) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    fail(ASZ_ERR_HIP, std::string("GPU stream failed: ") + hipGetErrorString(q));
+                if (q == hipSuccess && enqueued.load() && *flag != epoch) {
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                    if (*flag != epoch) fail(ASZ_ERR_HIP, "histogram results never arrived from the GPU");
+                }
+            }
+            _mm_pause();
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+    };
+    auto build = [&](uint64_t k) {
+        HuffArray& A = lay.arrays[k];
+        if (A.count == 0) return;
+        const uint32_t* hs = slot(k);
+        HistTail tail;
+        std::memcpy(&tail, hs + kHostSlotHead, sizeof(tail));
+        const int32_t q0 = tail.qmin, q1 = tail.qmax;
+        A.offset = q0;
+        A.range = (uint64_t)((int64_t)q1 - (int64_t)q0 + 1);
+        if (q0 > q1 || A.range > (1ull << 24)) fail(ASZ_ERR_DATA, "QB alphabet wider than 2^24 symbols");
+        if (q0 >= -kHistCentre && q1 < kHistCentre) {
+            build_codebook(A, hs + kHostSlotBins);
+            return;
+        }
+        // the array's range leaves the global window: bin it again over its
+        // exact range on the side stream (its symbols are final: the flag is up)
+        std::lock_guard<std::mutex> lk(side_mu);
+        HIPCHECK(hipSetDevice(c->device));
+        HuffArrayDev D{};
+        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+        D.count = A.count;
+        D.offset = A.offset;
+        D.range = A.range;
+        std::vector<uint32_t> carr((A.count + kHistChunk - 1) / kHistChunk, 0);
+        HuffArrayDev* ded = c->arrays.get<HuffArrayDev>(1);
+        uint32_t* dcarr = c->chunk_arr.get<uint32_t>(carr.size() + 1);
+        uint32_t* dh = c->hist.get<uint32_t>(A.range + 1);
+        HIPCHECK(hipMemcpyAsync(ded, &D, sizeof(D), hipMemcpyHostToDevice, c->side));
+        HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), carr.size() * 4, hipMemcpyHostToDevice, c->side));
+        launch_fill_i32(reinterpret_cast<int32_t*>(dh), A.range, 0, c->side);
+        launch_hist(sym, ded, dcarr, carr.size(), dh, c->side);
+        std::vector<uint32_t> hexact(A.range);
+        HIPCHECK(hipMemcpyAsync(hexact.data(), dh, A.range * 4, hipMemcpyDeviceToHost, c->side));
+        HIPCHECK(hipStreamSynchronize(c->side));
+        build_codebook(A, hexact.data());
+    };
     c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
     PivotDesc PD = pivot_desc(P);
     int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(true, d_work, qb, PD, (void*)&acc->err, aacc(1), st);
+    launch_pivots(true, d_work, qb, PD, ppart, st);
     kev_end(c, kp);
+    launch_hist_small(nullptr, 0, qb, P.pivot_count, nullptr, gwin(1), st);
+    launch_hist_tail(acc, 1, ppart, npp, gwin(1), c->hres_dev, epoch, st);
+    // the workers start once the GPU has work queued (thread start-up overlaps it)
+    const unsigned nworkers = (unsigned)std::min<uint64_t>(order.size(), kCodebookThreads);
+    std::vector<std::exception_ptr> werr(nworkers);
+    host_ms_w.assign(nworkers, 0.0);
+    std::vector<std::thread> workers;
+    for (unsigned w = 0; w < nworkers; w++)
+        workers.emplace_back([&, w] {
+            try {
+                for (size_t i = w; i < order.size(); i += nworkers) {
+                    if (lay.arrays[order[i]].count == 0) continue;  // an empty round has no slot
+                    wait_slot(order[i]);
+                    auto h0 = Clock::now();
+                    build(order[i]);
+                    host_ms_w[w] += ms_since(h0);
+                }
+            } catch (...) {
+                werr[w] = std::current_exception();
+                abort_all = true;
+            }
+        });
+    // joins the workers on every exit path of the launch loop below
+    struct Joiner {
+        std::vector<std::thread>& ts;
+        std::atomic<bool>& ab;
+        bool armed = true;
+        ~Joiner() {
+            if (!armed) return;
+            ab = true;
+            for (auto& t : ts) if (t.joinable()) t.join();
+        }
+    } joiner{workers, abort_all};
+
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
     std::vector<uint64_t> koff;
@@ -457,100 +609,52 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             HIPCHECK(hipStreamSynchronize(st));
             tsel_host_ms += ms_since(h0);
         }
+        if (r + 1 == L) {
+            // every t is final: the t array's range, histogram and tail
+            launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
+            launch_hist_small(sym, nt, nullptr, 0, gwin(0), nullptr, st);
+            launch_hist_tail(acc, 0, nullptr, 0, gwin(0), c->hres_dev, epoch, st);
+        }
         int kr = kev_begin(c, KEV_ROUND, (int)r);
-        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist,
-                              ghist + (2 + r) * kHistGlobal, st);
+        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist, gwin(2 + r), st);
         kev_end(c, kr);
+        if (r + 1 >= batch_end) {
+            const uint64_t f0 = r + 1 == batch_end ? 0 : r;
+            int kh = kev_begin(c, KEV_HIST, (int)r);
+            uint32_t mg = 0;
+            for (uint64_t q = f0; q <= r; q++) mg = std::max(mg, pc.folds[q].groups);
+            launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), mg, part, phist, fpart, acc, ghist,
+                               c->hres_dev, epoch, st);
+            kev_end(c, kh);
+        }
     }
     HIPCHECK(hipGetLastError());
-    int kh = kev_begin(c, KEV_HIST, 0);
-    launch_reduce_partials(part, dranges, (uint32_t)L, acc, st);
-    launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
-    launch_hist_small(sym, nt, qb, P.pivot_count, ghist, ghist + kHistGlobal, st);
-    launch_hist_reduce(phist, dranges, (uint32_t)L, ghist, st);
-    launch_hist_compact(acc, (uint32_t)narr, ghist, compact, st);
-    kev_end(c, kh);
     HIPCHECK(hipEventRecord(c->ev[1], st));
-
-    // one read-back: accumulators and the compacted histograms (up to a budget)
-    const uint64_t budget = std::min<uint64_t>(compact_max, 32768);
-    uint8_t* px = c->pin_xfer.get<uint8_t>(acc_bytes + compact_max * 4);
-    HIPCHECK(hipMemcpyAsync(px, xfer, acc_bytes + budget * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    auto hh0 = Clock::now();
-    DevAcc hacc;
-    std::memcpy(&hacc, px, sizeof(hacc));
-    const uint32_t* hcompact = reinterpret_cast<const uint32_t*>(px + acc_bytes);
-    S.max_err = hacc.err.max_err;
-    S.sq_err_sum = hacc.err.sq_sum;
-    for (uint64_t r = 0; r < L; r++) {
-        if (!P.rounds[r].count) continue;
-        S.max_err = std::max(S.max_err, hacc.rerr[r].max_err);
-        S.sq_err_sum += hacc.rerr[r].sq_sum;
-    }
-    if (tsel_mode == ASZ_TSEL_GPU) S.t_fallbacks = hacc.fallbacks;
-    if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
-
-    // ---------------------------------------------------------------- Huffman
-    Layout lay;
-    lay.arrays.resize(narr);
-    lay.dev.resize(narr);
-    std::vector<const uint32_t*> hptr(narr, nullptr);
-    std::vector<uint64_t> exact;  // arrays whose QB range leaves the global window
-    uint64_t coff = 0;
+    enqueued = true;
+    const double t_enq = ms_since(wall0);
+    joiner.armed = false;
+    for (auto& t : workers) t.join();
+    for (auto& e : werr)
+        if (e) std::rethrow_exception(e);
+    double host_ms = 0;
+    for (double v : host_ms_w) host_ms = std::max(host_ms, v);
+    // errors and fallbacks from the slots
+    S.max_err = 0;
+    S.sq_err_sum = 0;
     for (uint64_t k = 0; k < narr; k++) {
-        HuffArray& A = lay.arrays[k];
-        A.count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
-        A.is_int64 = k == 0;
-        if (A.count == 0) continue;
-        const int32_t q0 = hacc.arr[k].qmin, q1 = hacc.arr[k].qmax;
-        A.offset = q0;
-        A.range = (uint64_t)((int64_t)q1 - (int64_t)q0 + 1);
-        if (A.range > (1ull << 24)) fail(ASZ_ERR_DATA, "QB alphabet wider than 2^24 symbols");
-        const uint64_t span = compact_span_host(hacc.arr[k]);
-        if (span) {
-            hptr[k] = hcompact + coff;
-            coff += span;
+        if (lay.arrays[k].count == 0) continue;
+        HistTail tail;
+        std::memcpy(&tail, slot(k) + kHostSlotHead, sizeof(tail));
+        if (k == 0) {
+            if (tsel_mode == ASZ_TSEL_GPU) S.t_fallbacks = tail.fallbacks;
+            if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
         } else {
-            exact.push_back(k);
+            S.max_err = std::max(S.max_err, tail.max_err);
+            S.sq_err_sum += tail.sq;
         }
     }
-    if (coff > budget) {
-        HIPCHECK(hipMemcpyAsync(px + acc_bytes + budget * 4, compact + budget, (coff - budget) * 4,
-                                hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-    }
-    std::vector<uint32_t> hexact;
-    if (!exact.empty()) {
-        // exact-range histograms of the wide arrays
-        std::vector<HuffArrayDev> ed;
-        std::vector<uint32_t> carr;
-        uint64_t htot = 0;
-        for (uint64_t k : exact) {
-            const HuffArray& A = lay.arrays[k];
-            HuffArrayDev D{};
-            D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
-            D.count = A.count;
-            D.offset = A.offset;
-            D.range = A.range;
-            D.hist_off = htot;
-            htot += A.range;
-            D.hchunk0 = carr.size();
-            for (uint64_t i = 0; i < (A.count + kHistChunk - 1) / kHistChunk; i++) carr.push_back((uint32_t)ed.size());
-            ed.push_back(D);
-        }
-        HuffArrayDev* ded = c->arrays.get<HuffArrayDev>(ed.size());
-        uint32_t* dcarr = c->chunk_arr.get<uint32_t>(carr.size() + 1);
-        uint32_t* dh = c->hist.get<uint32_t>(htot + 1);
-        HIPCHECK(hipMemcpyAsync(ded, ed.data(), ed.size() * sizeof(HuffArrayDev), hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), carr.size() * 4, hipMemcpyHostToDevice, st));
-        launch_fill_i32(reinterpret_cast<int32_t*>(dh), htot, 0, st);
-        launch_hist(sym, ded, dcarr, carr.size(), dh, st);
-        hexact.resize(htot);
-        HIPCHECK(hipMemcpyAsync(hexact.data(), dh, htot * 4, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        for (size_t e = 0; e < exact.size(); e++) hptr[exact[e]] = hexact.data() + ed[e].hist_off;
-    }
+    if (getenv("ASZ_HOST_TRACE")) fprintf(stderr, "[trace] enqueue done %.3f ms, codebooks done %.3f ms\n", t_enq, ms_since(wall0));
+    auto hh0 = Clock::now();
 
     // codebooks and stream layout (compress.tpp:243-291).  Only the header
     // bytes travel to the device (a compact blob scattered into the zeroed
@@ -587,7 +691,6 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         for (int b = 0; b < 8; b++) cnt8[b] = (uint8_t)(A.count >> (8 * b));
         emit(cnt8, 8);
         if (!A.count) continue;
-        build_codebook(A, hptr[k]);
         emit(A.header.data(), A.header.size());
         if (A.single) continue;
         D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
@@ -625,7 +728,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     std::memcpy(hu + o_arr, lay.dev.data(), narr * sizeof(HuffArrayDev));
     uint8_t* du = c->hmeta.get<uint8_t>(up_bytes);
     HIPCHECK(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, st));
-    S.huffman_host_ms = ms_since(hh0);
+    S.huffman_host_ms = host_ms + ms_since(hh0);
 
     const uint64_t nwords = (lay.total_bytes + 3) / 4 + 4;
     uint32_t* dwords = c->words.get<uint32_t>(nwords);
@@ -833,7 +936,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     launch_t_float(sym, nbl, (uint32_t)L, (float)p.t_precision, tblk, st);
     PivotDesc PD = pivot_desc(P);
     int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, nullptr, st);
+    launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, st);
     kev_end(c, kp);
     for (uint64_t r = 0; r < L; r++) {
         RoundDesc R = P.rounds[r];
@@ -918,6 +1021,7 @@ int asz_ctx_create(int device, asz_ctx** out) {
         HIPCHECK(hipSetDevice(device));
         c->device = device;
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
     });
     if (rc != ASZ_OK) {
@@ -934,7 +1038,10 @@ void asz_ctx_destroy(asz_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->hres_host) (void)hipHostFree(c->hres_host);
+    if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -948,8 +1055,15 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
             c->stream_on_device = value != 0;
             return;
         }
-        if (option != ASZ_OPT_KERNEL_EVENTS) fail(ASZ_ERR_ARG, "unknown option");
+        if (option != ASZ_OPT_KERNEL_EVENTS && option != ASZ_OPT_TIMED_ROUND) fail(ASZ_ERR_ARG, "unknown option");
         HIPCHECK(hipSetDevice(c->device));
+        if (option == ASZ_OPT_TIMED_ROUND) {
+            if (value >= ASZ_MAX_LAYERS) fail(ASZ_ERR_ARG, "no such round");
+            c->kev_round = value < 0 ? -1 : (int)value;
+            value = value >= 0 ? 1 : 0;
+        } else {
+            c->kev_round = -1;
+        }
         c->kev_on = value != 0;
         if (c->kev_on && c->kev.empty()) {
             c->kev.resize(2 * 256);

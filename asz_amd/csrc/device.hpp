@@ -77,6 +77,12 @@ __device__ __forceinline__ float reconstruct(float p, int32_t q, double neg_two_
     return (float)__fma_rn(neg_two_eb, (double)q, (double)p);
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, then s_barrier.  Unlike __syncthreads() it does not wait for
+// outstanding global loads, so loads issued for the next piece of work stay
+// in flight across it.  Only for barriers that order LDS accesses.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // ---------------------------------------------------------------- reductions
 
 template <typename T>

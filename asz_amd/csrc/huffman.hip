@@ -3,13 +3,14 @@
 // Histogram.  The round kernels leave one LDS window histogram per workgroup
 // (symbols [-win/2, win/2), win chosen per round) and add the rest of their
 // QBs to the array's global window (kHistGlobal bins around 0) with atomics.
-// k_hist_reduce folds the per-workgroup windows into the global windows
-// without contended atomics (each workgroup of the reduction owns a slice of
-// the partials), k_hist_small bins the two short arrays (t values and
-// pivots) the same way, and k_hist_compact packs each array's [min, max]
-// range of bins next to the accumulators so that the host reads everything
-// with one copy.  An array whose range leaves the global window is binned
-// again by k_hist over its exact range.
+// k_fold_round, launched right after each round, folds that round's windows
+// into the global window (each workgroup of the fold owns a slice of the
+// windows, so the global atomics are few); k_hist_small bins the two short
+// arrays (t values and pivots) the same way.  The workgroup that finishes an
+// array writes its QB range, errors and used bins straight into pinned host
+// memory and raises a flag, so the host builds that codebook while later
+// rounds run, without a copy or an API call in between.  An array whose range
+// leaves the global window is binned again by k_hist over its exact range.
 //
 // Encoding.  The host builds the reference-tree codebooks.  k_encode_chunks
 // codes each chunk of kEncChunk symbols (one workgroup) from an LDS copy of
@@ -79,41 +80,142 @@ __global__ __launch_bounds__(256) void k_hist_small(const int32_t* __restrict__ 
         atomicAdd(&gw[kHistCentre - kHistWin / 2 + threadIdx.x], lh[threadIdx.x]);
 }
 
-// grid (kHistReduceGroups, rounds): workgroup (g, r) sums slice g of round r's
-// per-workgroup windows and adds the slice total to the global window.
-// ranges[4r ..]: first partial, workgroups, first histogram word, window bins.
-constexpr uint32_t kHistReduceGroups = 64;
-
-__global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* __restrict__ phist,
-                                                     const uint64_t* __restrict__ ranges, uint32_t* ghist) {
-    const uint32_t r = blockIdx.y;
-    const uint64_t n = ranges[4 * r + 1], off = ranges[4 * r + 2], win = ranges[4 * r + 3];
-    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
-    const uint64_t p0 = (uint64_t)blockIdx.x * per, p1 = min(n, p0 + per);
-    if (p0 >= p1) return;
-    uint32_t* gw = ghist + (uint64_t)(2 + r) * kHistGlobal + kHistCentre - win / 2;
-    for (uint64_t bin = threadIdx.x; bin < win; bin += blockDim.x) {
-        uint32_t s = 0;
-        for (uint64_t p = p0; p < p1; p++) s += phist[off + p * win + bin];
-        if (s) atomicAdd(&gw[bin], s);
+// Fold of rounds r0 .. r0 + gridDim.y - 1 (one launch after a round kernel
+// covers one round, or a batch of finished rounds).  Workgroup (g, y) owns
+// round r's per-workgroup results [g F.pper, (g+1) F.pper): it sums their
+// histogram windows (about 4096 words per workgroup, coalesced along the
+// bins), adds the nonzero sums to the array's global window, and folds their
+// Partials into fpart[F.fpart_off + g].  The last workgroup of the round to
+// finish (counted in acc->fold_count[r], which it resets) folds those into
+// acc->arr[2 + r] and the window's tail.
+// Finishes array k for the host (whole workgroup): the tail, then the bins
+// of [qmin, qmax] from the global window gw when they lie inside it, into the
+// host slot; after a system-scope fence, the epoch into the slot's flag.
+__device__ void publish_host(const uint32_t* gw, HistTail t, uint32_t* hslot, uint32_t epoch) {
+    if (t.qmin <= t.qmax && t.qmin >= -kHistCentre && t.qmax < kHistCentre) {
+        const uint32_t n = (uint32_t)(t.qmax - t.qmin + 1);
+        const uint32_t* src = gw + (t.qmin + kHistCentre);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) hslot[kHostSlotBins + i] = src[i];
+    }
+    if (threadIdx.x == 0) {
+        uint32_t* o = hslot + kHostSlotHead;
+        o[0] = (uint32_t)t.qmin;
+        o[1] = (uint32_t)t.qmax;
+        o[2] = t.fallbacks;
+        o[3] = 0;
+        const uint64_t e = (uint64_t)__double_as_longlong(t.max_err), q = (uint64_t)__double_as_longlong(t.sq);
+        o[4] = (uint32_t)e;
+        o[5] = (uint32_t)(e >> 32);
+        o[6] = (uint32_t)q;
+        o[7] = (uint32_t)(q >> 32);
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(hslot, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// One workgroup per array: bins [qmin, qmax] of arrays inside the global
-// window, packed array after array (arrays outside it contribute nothing).
-__device__ __forceinline__ uint64_t compact_span(const ArrayAcc& a) {
-    if (a.qmin > a.qmax || a.qmin < -kHistCentre || a.qmax >= kHistCentre) return 0;
-    return (uint64_t)((int64_t)a.qmax - a.qmin + 1);
+// Workgroup-wide fold of Partials [p0, p1) (every thread gets the result).
+__device__ Partial fold_partials(const Partial* __restrict__ part, uint64_t p0, uint64_t p1) {
+    __shared__ double s_err[4], s_sq[4];
+    __shared__ int32_t s_min[4], s_max[4];
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    for (uint64_t i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
+        const Partial p = part[i];
+        err = fmax(err, p.max_err);
+        sq += p.sq;
+        qmin = min(qmin, p.qmin);
+        qmax = max(qmax, p.qmax);
+    }
+    err = wave_max(err);
+    sq = wave_sum(sq);
+    qmin = wave_min(qmin);
+    qmax = wave_max(qmax);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
+    __syncthreads();
+    Partial r{s_err[0], s_sq[0], s_min[0], s_max[0]};
+    for (int k = 1; k < (int)(blockDim.x / kWave); k++) {
+        r.max_err = fmax(r.max_err, s_err[k]);
+        r.sq += s_sq[k];
+        r.qmin = min(r.qmin, s_min[k]);
+        r.qmax = max(r.qmax, s_max[k]);
+    }
+    __syncthreads();  // the shared slots may be reused by a later call
+    return r;
 }
 
-__global__ __launch_bounds__(256) void k_hist_compact(const DevAcc* __restrict__ acc,
-                                                      const uint32_t* __restrict__ ghist, uint32_t* out) {
-    const uint32_t a = blockIdx.x;
-    uint64_t off = 0;
-    for (uint32_t k = 0; k < a; k++) off += compact_span(acc->arr[k]);
-    const uint64_t n = compact_span(acc->arr[a]);
-    const uint32_t* src = ghist + (uint64_t)a * kHistGlobal + (acc->arr[a].qmin + kHistCentre);
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) out[off + i] = src[i];
+__global__ __launch_bounds__(256) void k_fold_rounds(const FoldDesc* __restrict__ descs, uint32_t r0,
+                                                     const Partial* __restrict__ part,
+                                                     const uint32_t* __restrict__ phist, Partial* fpart, DevAcc* acc,
+                                                     uint32_t* ghist, uint32_t* hres, uint32_t epoch) {
+    __shared__ uint32_t lsum[256];
+    __shared__ int s_last;
+    const uint32_t r = r0 + blockIdx.y;
+    const FoldDesc F = descs[r];
+    if (blockIdx.x >= F.groups) return;
+    uint32_t* gw = ghist + (uint64_t)(2 + r) * kHistStride;
+    const uint64_t p0 = (uint64_t)blockIdx.x * F.pper, p1 = min<uint64_t>(F.nwg, p0 + F.pper);
+    const uint32_t win = F.win;
+    const uint32_t* src = phist + F.hist_off;
+    uint32_t* g = gw + kHistCentre - win / 2;
+    if (win <= 256) {
+        // a thread keeps one bin; 256 / win windows per step
+        const uint32_t bin = threadIdx.x & (win - 1), sub = threadIdx.x / win, step = 256 / win;
+        uint32_t s = 0;
+#pragma unroll 8
+        for (uint64_t p = p0 + sub; p < p1; p += step) s += src[p * win + bin];
+        lsum[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x < win) {
+            for (uint32_t k = 1; k < step; k++) s += lsum[threadIdx.x + k * win];
+            if (s) atomicAdd(&g[bin], s);
+        }
+    } else {
+        for (uint32_t bin = threadIdx.x; bin < win; bin += blockDim.x) {
+            uint32_t s = 0;
+#pragma unroll 4
+            for (uint64_t p = p0; p < p1; p++) s += src[p * win + bin];
+            if (s) atomicAdd(&g[bin], s);
+        }
+    }
+    Partial* fp = fpart + F.fpart_off;
+    const Partial mine = fold_partials(part + F.part_off, p0, p1);
+    if (threadIdx.x == 0) {
+        fp[blockIdx.x] = mine;
+        __threadfence();
+        s_last = atomicAdd(&acc->fold_count[r], 1u) == F.groups - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    const Partial all = fold_partials(fp, 0, F.groups);
+    if (threadIdx.x == 0) {
+        acc->rerr[r] = ErrAcc{all.max_err, all.sq};
+        acc->arr[2 + r].qmin = all.qmin;
+        acc->arr[2 + r].qmax = all.qmax;
+        acc->fold_count[r] = 0;
+    }
+    publish_host(gw, HistTail{all.qmin, all.qmax, 0, 0, all.max_err, all.sq}, hres + (2 + r) * kHostSlotWords,
+                 epoch);
+}
+
+// Finishes the t array (k = 0: range and fallback count from acc) or the
+// pivot array (k = 1: fold of the pivot kernel's Partials) for the host.
+__global__ __launch_bounds__(256) void k_hist_tail(const DevAcc* __restrict__ acc, uint32_t k,
+                                                   const Partial* __restrict__ part, uint64_t nparts,
+                                                   const uint32_t* gw, uint32_t* hres, uint32_t epoch) {
+    HistTail t{};
+    if (k == 0) {
+        t = HistTail{acc->arr[0].qmin, acc->arr[0].qmax, acc->fallbacks, 0, 0.0, 0.0};
+    } else {
+        const Partial all = fold_partials(part, 0, nparts);
+        t = HistTail{all.qmin, all.qmax, 0, 0, all.max_err, all.sq};
+    }
+    publish_host(gw, t, hres + k * kHostSlotWords, epoch);
 }
 
 // ------------------------------------------------------------------ encode
@@ -123,151 +225,261 @@ __device__ __forceinline__ uint32_t book_len(uint64_t e) { return (uint32_t)(e >
 __device__ __forceinline__ uint64_t book_code(uint64_t e) { return e & ((1ull << 56) - 1); }
 
 constexpr int kEncPerThread = kEncChunk / 256;  // 16
+constexpr int kEncPad = kEncChunk + kEncChunk / kEncPerThread;  // staging with one pad word per 16
 constexpr int kEncWords = kEncChunk + 2;        // LDS bit buffer for codes up to 32 bits
+constexpr uint32_t kEncMaxArrays = ASZ_MAX_LAYERS + 2;
 
+// Persistent encoder: workgroup g codes chunks g, g + G, g + 2G, ... of
+// kEncChunk symbols (G = gridDim.x), with the array table in LDS and the
+// codebook of the current array in LDS when it is small.  The symbols of the
+// next chunk are loaded (coalesced: lane l of step j reads symbol 256 j + l)
+// while the current one is coded.  Per chunk: codebook entries are staged in
+// LDS so that thread t owns the 16 consecutive symbols 16 t .. 16 t + 15
+// (padded rows, no bank conflicts); after a workgroup scan of the per-thread
+// bit counts each thread packs its codes into a 64-bit accumulator and emits
+// whole 32-bit words: only the first word (shared with the previous thread)
+// and the last partial word (shared with the next) are ORed into the LDS
+// buffer, the others are stored.  The chunk's bits go to the start of its
+// own scratch slot, its bit count to chunk_bits.  Codes longer than 32 bits
+// (short_codes false) are ORed straight into the zeroed scratch slot.
+template <bool short_codes>
 __global__ __launch_bounds__(256) void k_encode_chunks(const int32_t* __restrict__ sym,
                                                        const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                       const uint64_t* __restrict__ book, uint32_t* scratch,
-                                                       uint64_t slot_words, uint32_t* __restrict__ chunk_bits,
-                                                       int use_lds) {
-    __shared__ uint32_t lbuf[kEncWords];
+                                                       uint64_t nchunks, const uint64_t* __restrict__ book,
+                                                       uint32_t* scratch, uint64_t slot_words,
+                                                       uint32_t* __restrict__ chunk_bits) {
+    __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
     __shared__ uint32_t lbook[kLdsBook];
+    __shared__ HuffArrayDev sA[kEncMaxArrays];
     __shared__ uint32_t s_scan[4];
-    __shared__ uint32_t s_k;
-    const uint32_t g = blockIdx.x;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (threadIdx.x == 0) {
-        uint32_t k = 0;
-        while (k + 1 < narr && !(g >= arrays[k].chunk0 && g < arrays[k].chunk0 + arrays[k].nchunk)) k++;
-        s_k = k;
+    {
+        const uint32_t nw = narr * (uint32_t)(sizeof(HuffArrayDev) / 4);
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x)
+            reinterpret_cast<uint32_t*>(sA)[i] = reinterpret_cast<const uint32_t*>(arrays)[i];
     }
     __syncthreads();
-    const HuffArrayDev A = arrays[s_k];
-    const uint64_t* b = book + A.book_off;
-    if (A.lds_book) {
-        for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) {
-            const uint64_t e = b[i];
-            lbook[i] = (uint32_t)(e & 0xFFFFFFu) | (book_len(e) << 24);
-        }
-        __syncthreads();
-    }
-    const uint64_t first = (uint64_t)(g - A.chunk0) * kEncChunk;
-    const uint64_t last = min<uint64_t>(first + kEncChunk, A.count);
-    const int32_t* s = sym + A.src_off;
-    // thread t codes symbols [first + 16 t, first + 16 t + 16)
-    uint64_t e[kEncPerThread];
-    uint32_t mybits = 0;
-    const uint64_t my0 = first + (uint64_t)threadIdx.x * kEncPerThread;
+    const uint64_t G = gridDim.x;
+    // chunks of one workgroup increase, so its array index only moves forward
+    auto array_of = [&](uint64_t c, uint32_t k) {
+        while (k + 1 < narr && c >= sA[k].chunk0 + sA[k].nchunk) k++;
+        return k;
+    };
+    auto load_chunk = [&](uint64_t c, uint32_t k, int32_t v[kEncPerThread]) {
+        const HuffArrayDev& A = sA[k];
+        const uint64_t first = (c - A.chunk0) * kEncChunk;
+        const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
+        const int32_t* s = sym + A.src_off + first;
 #pragma unroll
-    for (int j = 0; j < kEncPerThread; j++) {
-        const uint64_t i = my0 + j;
-        uint64_t v = 0;
-        if (i < last) {
-            const int64_t idx = (int64_t)s[i] - A.offset;
-            if (A.lds_book) {
-                const uint32_t l = lbook[idx];
-                v = (uint64_t)(l & 0xFFFFFFu) | ((uint64_t)(l >> 24) << 56);
-            } else {
-                v = b[idx];
+        for (int j = 0; j < kEncPerThread; j++) {
+            const uint32_t i = j * 256 + threadIdx.x;
+            v[j] = i < n ? s[i] : 0;
+        }
+    };
+    uint64_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    uint32_t k = array_of(c, 0);
+    int32_t v[kEncPerThread];
+    if (short_codes) load_chunk(c, k, v);
+    int book_k = -1;
+    while (true) {
+        const HuffArrayDev& A = sA[k];
+        const uint64_t* b = book + A.book_off;
+        const uint64_t first = (c - A.chunk0) * kEncChunk;
+        const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
+        const uint64_t cn = c + G;
+        const uint32_t kn = cn < nchunks ? array_of(cn, k) : k;
+        uint64_t e[kEncPerThread];
+        uint32_t mybits = 0;
+        if (short_codes) {
+            if (A.lds_book && book_k != (int)k) {
+                for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) {
+                    const uint64_t be = b[i];
+                    lbook[i] = (uint32_t)(be & 0xFFFFFFu) | (book_len(be) << 24);
+                }
+                book_k = (int)k;
+            }
+            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free
+#pragma unroll
+            for (int j = 0; j < kEncPerThread; j++) {
+                const uint32_t i = j * 256 + threadIdx.x;
+                uint32_t ent = 0;
+                if (i < n) {
+                    const int64_t idx = (int64_t)v[j] - A.offset;
+                    if (A.lds_book) {
+                        ent = lbook[idx];
+                    } else {
+                        // codes of 25..32 bits keep the symbol index, resolved after staging
+                        const uint64_t be = b[idx];
+                        ent = book_len(be) <= 24 ? ((uint32_t)(be & 0xFFFFFFu) | (book_len(be) << 24))
+                                                 : (0x80000000u | (uint32_t)idx);
+                    }
+                }
+                lbuf[i + i / kEncPerThread] = ent;
+            }
+            // the next chunk's symbols stay in flight while this one is packed
+            // (LDS-only barriers below do not wait for them)
+            if (cn < nchunks) load_chunk(cn, kn, v);
+            lds_barrier();
+#pragma unroll
+            for (int j = 0; j < kEncPerThread; j++) {
+                const uint32_t i = threadIdx.x * kEncPerThread + j;
+                const uint32_t ent = lbuf[i + i / kEncPerThread];
+                uint64_t v64;
+                if (ent & 0x80000000u) v64 = b[ent & 0x7FFFFFFFu];
+                else v64 = (uint64_t)(ent & 0xFFFFFFu) | ((uint64_t)(ent >> 24) << 56);
+                e[j] = v64;
+                mybits += book_len(v64);
+            }
+        } else {
+            lds_barrier();
+            const uint32_t my0 = threadIdx.x * kEncPerThread;
+            const int32_t* s = sym + A.src_off + first;
+#pragma unroll
+            for (int j = 0; j < kEncPerThread; j++) {
+                const uint32_t i = my0 + j;
+                uint64_t v64 = 0;
+                if (i < n) v64 = b[(int64_t)s[i] - A.offset];
+                e[j] = v64;
+                mybits += book_len(v64);
             }
         }
-        e[j] = v;
-        mybits += book_len(v);
-    }
-    // exclusive scan of the per-thread bit counts
-    uint32_t incl = mybits;
+        // exclusive scan of the per-thread bit counts
+        uint32_t incl = mybits;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t v = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o) incl += v;
-    }
-    if (lane == 63) s_scan[wave] = incl;
-    __syncthreads();
-    uint32_t wave_off = 0;
-    for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
-    const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
-    const uint32_t my_off = wave_off + incl - mybits;
-    if (threadIdx.x == 0) chunk_bits[g] = total;
-    uint32_t* out = scratch + (uint64_t)g * slot_words;
-    if (use_lds) {
-        const uint32_t nw = (total + 31) / 32;
-        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lbuf[w] = 0;
-        __syncthreads();
-        uint32_t p = my_off;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += t;
+        }
+        if (lane == 63) s_scan[wave] = incl;
+        lds_barrier();  // also: every thread has read its staged entries
+        uint32_t wave_off = 0;
+        for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
+        const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
+        const uint32_t my_off = wave_off + incl - mybits;
+        if (threadIdx.x == 0) chunk_bits[c] = total;
+        uint32_t* out = scratch + c * slot_words;
+        if (short_codes) {
+            const uint32_t nw = (total + 31) / 32;
+            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lbuf[w] = 0;
+            lds_barrier();
+            if (mybits) {
+                uint32_t word = my_off >> 5;
+                uint32_t nbits = my_off & 31;
+                const bool shared_first = nbits != 0;
+                bool first_word = true;
+                uint64_t accb = 0;
 #pragma unroll
-        for (int j = 0; j < kEncPerThread; j++) {
-            const uint32_t len = book_len(e[j]);
-            if (!len) continue;
-            const uint64_t code = book_code(e[j]);
-            const uint32_t w = p >> 5, sh = p & 31;
-            atomicOr(&lbuf[w], (uint32_t)(code << sh));
-            if (sh + len > 32) atomicOr(&lbuf[w + 1], (uint32_t)(code >> (32 - sh)));
-            p += len;
+                for (int j = 0; j < kEncPerThread; j++) {
+                    accb |= book_code(e[j]) << nbits;
+                    nbits += book_len(e[j]);
+                    if (nbits >= 32) {  // at most one whole word per code (nbits < 64)
+                        const uint32_t lo = (uint32_t)accb;
+                        if (first_word && shared_first) atomicOr(&lbuf[word], lo);
+                        else lbuf[word] = lo;
+                        first_word = false;
+                        word++;
+                        accb >>= 32;
+                        nbits -= 32;
+                    }
+                }
+                if (nbits) atomicOr(&lbuf[word], (uint32_t)accb);
+            }
+            lds_barrier();
+            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) out[w] = lbuf[w];
+        } else {
+            uint64_t p = my_off;
+            for (int j = 0; j < kEncPerThread; j++) {
+                const uint32_t len = book_len(e[j]);
+                if (!len) continue;
+                const uint64_t code = book_code(e[j]);
+                const uint32_t sh = (uint32_t)(p & 31);
+                uint32_t* w = out + (p >> 5);
+                atomicOr(&w[0], (uint32_t)(code << sh));
+                if (sh + len > 32) atomicOr(&w[1], (uint32_t)(code >> (32 - sh)));
+                if (sh + len > 64) atomicOr(&w[2], (uint32_t)(code >> (64 - sh)));
+                p += len;
+            }
         }
-        __syncthreads();
-        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) out[w] = lbuf[w];
-    } else {
-        // codes longer than 32 bits: OR straight into the zeroed slot
-        uint64_t p = my_off;
-        for (int j = 0; j < kEncPerThread; j++) {
-            const uint32_t len = book_len(e[j]);
-            if (!len) continue;
-            const uint64_t code = book_code(e[j]);
-            const uint32_t sh = (uint32_t)(p & 31);
-            uint32_t* w = out + (p >> 5);
-            atomicOr(&w[0], (uint32_t)(code << sh));
-            if (sh + len > 32) atomicOr(&w[1], (uint32_t)(code >> (32 - sh)));
-            if (sh + len > 64) atomicOr(&w[2], (uint32_t)(code >> (64 - sh)));
-            p += len;
-        }
+        if (cn >= nchunks) break;
+        c = cn;
+        k = kn;
     }
 }
 
-// One workgroup per array: chunk_off[c] = payload start + exclusive scan of
-// the array's chunk bit counts.
+// One workgroup (1024 threads) per array: chunk_off[c] = payload start +
+// exclusive scan of the array's chunk bit counts.  Thread t owns a
+// contiguous run of chunks: it sums the run, the workgroup scans the 1024
+// sums (wave shuffles, then the 16 wave totals), and the thread writes its
+// run's offsets.
 __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restrict__ arrays,
                                                      const uint32_t* __restrict__ chunk_bits, uint64_t* chunk_off) {
-    __shared__ uint64_t buf[1024];
+    __shared__ uint64_t wsum[16];
     const HuffArrayDev A = arrays[blockIdx.x];
     if (A.nchunk == 0) return;
-    uint64_t carry = A.payload_bit;
-    for (uint64_t base = 0; base < A.nchunk; base += 1024) {
-        const uint64_t i = base + threadIdx.x;
-        const uint64_t v = i < A.nchunk ? chunk_bits[A.chunk0 + i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const uint64_t add = threadIdx.x >= (uint32_t)o ? buf[threadIdx.x - o] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < A.nchunk) chunk_off[A.chunk0 + i] = carry + buf[threadIdx.x] - v;
-        const uint64_t tot = buf[1023];
-        __syncthreads();
-        carry += tot;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t per = (A.nchunk + 1023) / 1024;
+    const uint64_t r0 = min<uint64_t>(A.nchunk, threadIdx.x * per), r1 = min<uint64_t>(A.nchunk, r0 + per);
+    const uint32_t* cb = chunk_bits + A.chunk0;
+    uint64_t mine = 0;
+    for (uint64_t i = r0; i < r1; i++) mine += cb[i];
+    uint64_t incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t t = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint64_t off = A.payload_bit + incl - mine;
+    for (uint32_t w = 0; w < wave; w++) off += wsum[w];
+    uint64_t* co = chunk_off + A.chunk0;
+    for (uint64_t i = r0; i < r1; i++) {
+        co[i] = off;
+        off += cb[i];
     }
 }
 
-// One workgroup per chunk: the chunk's bits from its scratch slot to bit
-// position chunk_off in the stream.
+// One wave per chunk (four chunks per workgroup): the chunk's bits from its
+// scratch slot to bit position chunk_off in the stream.  Each pass a lane
+// loads 4 consecutive source words (all loads of the pass in flight at
+// once); the word below a lane's first comes from the neighbouring lane.
+// Interior output words are stored, the two boundary words shared with the
+// neighbouring chunks are ORed in.
 __global__ __launch_bounds__(256) void k_compact_bits(const uint32_t* __restrict__ scratch, uint64_t slot_words,
                                                       const uint32_t* __restrict__ chunk_bits,
-                                                      const uint64_t* __restrict__ chunk_off, uint32_t* words) {
-    const uint32_t g = blockIdx.x;
+                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
+                                                      uint32_t* words) {
+    constexpr uint32_t kW = 4;  // words per lane per pass
+    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (g >= nchunks) return;
     const uint32_t nb = chunk_bits[g];
     if (nb == 0) return;
     const uint64_t o = chunk_off[g];
     const uint32_t sh = (uint32_t)(o & 31);
     const uint32_t nsrc = (nb + 31) / 32, nout = (sh + nb + 31) / 32;
-    const uint32_t* src = scratch + (uint64_t)g * slot_words;
+    const uint32_t* src = scratch + g * slot_words;
     uint32_t* dst = words + (o >> 5);
-    for (uint32_t i = threadIdx.x; i < nout; i += blockDim.x) {
-        const uint32_t cur = i < nsrc ? src[i] : 0u;
-        uint32_t v = cur << sh;
-        if (sh && i > 0) v |= src[i - 1] >> (32 - sh);
-        if (i == 0 || i == nout - 1) atomicOr(&dst[i], v);
-        else dst[i] = v;
+    uint32_t carry = 0;  // source word below this pass's first word
+    for (uint32_t base = 0; base < nout; base += 64 * kW) {
+        const uint32_t i0 = base + lane * kW;
+        uint32_t cur[kW];
+#pragma unroll
+        for (uint32_t q = 0; q < kW; q++) cur[q] = i0 + q < nsrc ? src[i0 + q] : 0u;
+        uint32_t below = __shfl_up(cur[kW - 1], 1, 64);
+        if (lane == 0) below = carry;
+        carry = __shfl(cur[kW - 1], 63, 64);
+#pragma unroll
+        for (uint32_t q = 0; q < kW; q++) {
+            const uint32_t i = i0 + q;
+            if (i >= nout) break;
+            const uint32_t prev = q ? cur[q - 1] : below;
+            uint32_t v = cur[q] << sh;
+            if (sh && i > 0) v |= prev >> (32 - sh);
+            if (i == 0 || i == nout - 1) atomicOr(&dst[i], v);
+            else dst[i] = v;
+        }
     }
 }
 
@@ -284,21 +496,38 @@ void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64
     if (!nc) return;
     hipLaunchKernelGGL(dev::k_hist_small, dim3((uint32_t)nc, 2), dim3(256), 0, st, s0, n0, s1, n1, g0, g1);
 }
-void launch_hist_reduce(const uint32_t* phist, const uint64_t* ranges, uint32_t rounds, uint32_t* ghist,
-                        hipStream_t st) {
-    if (!rounds) return;
-    hipLaunchKernelGGL(dev::k_hist_reduce, dim3(dev::kHistReduceGroups, rounds), dim3(256), 0, st, phist, ranges,
-                       ghist);
+void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_t max_groups, const Partial* part,
+                        const uint32_t* phist, Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres,
+                        uint32_t epoch, hipStream_t st) {
+    if (!nr || !max_groups) return;
+    hipLaunchKernelGGL(dev::k_fold_rounds, dim3(max_groups, nr), dim3(256), 0, st, descs, r0, part, phist, fpart, acc,
+                       ghist, hres, epoch);
 }
-void launch_hist_compact(const DevAcc* acc, uint32_t narr, const uint32_t* ghist, uint32_t* out, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_hist_compact, dim3(narr), dim3(256), 0, st, acc, ghist, out);
+void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
+                      uint32_t* hres, uint32_t epoch, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_hist_tail, dim3(1), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch);
 }
 void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
                           const uint64_t* book, uint32_t* scratch, uint64_t slot_words, uint32_t* chunk_bits,
                           int max_len, hipStream_t st) {
     if (!nchunks) return;
-    hipLaunchKernelGGL(dev::k_encode_chunks, dim3((uint32_t)nchunks), dim3(256), 0, st, sym, arrays, narr, book,
-                       scratch, slot_words, chunk_bits, max_len <= 32 ? 1 : 0);
+    if (narr > dev::kEncMaxArrays) throw Error(ASZ_ERR_ARG, "too many Huffman arrays");
+    // one resident wave of workgroups: CUs x workgroups that fit on a CU
+    int dev_id = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev_id);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+    if (max_len <= 32)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_chunks<true>, 256, 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_chunks<false>, 256, 0);
+    const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
+    const uint64_t groups = std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
+    if (max_len <= 32)
+        hipLaunchKernelGGL(dev::k_encode_chunks<true>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr,
+                           nchunks, book, scratch, slot_words, chunk_bits);
+    else
+        hipLaunchKernelGGL(dev::k_encode_chunks<false>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr,
+                           nchunks, book, scratch, slot_words, chunk_bits);
 }
 void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint32_t* chunk_bits, uint64_t* chunk_off,
                        hipStream_t st) {
@@ -307,8 +536,8 @@ void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint3
 void launch_compact_bits(const uint32_t* scratch, uint64_t slot_words, const uint32_t* chunk_bits,
                          const uint64_t* chunk_off, uint64_t nchunks, uint32_t* stream_words, hipStream_t st) {
     if (!nchunks) return;
-    hipLaunchKernelGGL(dev::k_compact_bits, dim3((uint32_t)nchunks), dim3(256), 0, st, scratch, slot_words,
-                       chunk_bits, chunk_off, stream_words);
+    hipLaunchKernelGGL(dev::k_compact_bits, dim3((uint32_t)((nchunks + 3) / 4)), dim3(256), 0, st, scratch,
+                       slot_words, chunk_bits, chunk_off, nchunks, stream_words);
 }
 
 }  // namespace asz

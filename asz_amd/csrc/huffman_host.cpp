@@ -28,18 +28,33 @@ uint64_t get_le64(const uint8_t* p) {
     return v;
 }
 
+// LSB-first bit writer (n <= 32 bits per call).
 struct BitSink {
     std::vector<uint8_t>& o;
-    uint8_t cur = 0;
+    uint64_t acc = 0;
     int fill = 0;
     void put(uint64_t v, int n) {
-        for (int k = 0; k < n; k++) {
-            cur |= (uint8_t)(((v >> k) & 1u) << fill);
-            if (++fill == 8) { o.push_back(cur); cur = 0; fill = 0; }
-        }
+        acc |= (v & ((1ull << n) - 1)) << fill;
+        fill += n;
+        while (fill >= 8) { o.push_back((uint8_t)acc); acc >>= 8; fill -= 8; }
     }
-    void flush() { if (fill) { o.push_back(cur); cur = 0; fill = 0; } }
+    void flush() { if (fill) { o.push_back((uint8_t)acc); acc = 0; fill = 0; } }
 };
+
+// Ascending sort of 64-bit keys, LSD radix with 11-bit digits over the bits in use.
+void radix_sort(std::vector<uint64_t>& k) {
+    if (k.size() < 64) { std::sort(k.begin(), k.end()); return; }
+    uint64_t all = 0;
+    for (uint64_t v : k) all |= v;
+    std::vector<uint64_t> tmp(k.size());
+    for (int sh = 0; sh < 64 && (all >> sh); sh += 11) {
+        size_t cnt[2049] = {0};
+        for (uint64_t v : k) cnt[((v >> sh) & 2047) + 1]++;
+        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (uint64_t v : k) tmp[cnt[(v >> sh) & 2047]++] = v;
+        k.swap(tmp);
+    }
+}
 
 constexpr uint64_t kLenMask = 0x1234abcdull;
 
@@ -95,8 +110,12 @@ void build_codebook(HuffArray& a, const uint32_t* hist) {
     // sorted by (freq, index); joined nodes are created with non-decreasing
     // freq and increasing index, so two FIFO queues give the same sequence.
     std::vector<uint32_t> lq(n);
-    for (uint64_t i = 0; i < n; i++) lq[i] = (uint32_t)i;
-    std::stable_sort(lq.begin(), lq.end(), [&](uint32_t x, uint32_t y) { return freq[x] < freq[y]; });
+    {
+        std::vector<uint64_t> key(n);  // (freq, leaf index), ascending
+        for (uint64_t i = 0; i < n; i++) key[i] = (freq[i] << 32) | i;
+        radix_sort(key);
+        for (uint64_t i = 0; i < n; i++) lq[i] = (uint32_t)key[i];
+    }
     const uint64_t total_nodes = 2 * n - 1;
     std::vector<uint64_t> nf(total_nodes);
     std::vector<int64_t> left(total_nodes, -1), right(total_nodes, -1);

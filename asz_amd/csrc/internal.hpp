@@ -86,7 +86,7 @@ struct RoundDesc {
 
 constexpr uint32_t kRoundRow = 0;     // round axis 2: waves walk lattice rows along DIM2
 constexpr uint32_t kRoundSweep = 1;   // register-window sweep along the round axis (axis 0 / 1)
-constexpr uint32_t kSweepSeg = 16;    // targets per sweep segment
+constexpr uint32_t kSweepSeg = 24;    // most targets per sweep segment (register window length)
 constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave processes together
 
 // QB histograms.  Every round kernel bins its QBs into a per-workgroup LDS
@@ -106,6 +106,27 @@ constexpr uint32_t kLdsWinMin = 2048;
 constexpr uint64_t kHistPartialWords = 1 << 20;
 constexpr uint32_t kHistGlobal = 65536;
 constexpr int32_t kHistCentre = 32768;
+
+// Each array's global window is followed by a tail that carries what the host
+// needs to build that array's codebook and to report errors, so that one
+// copy per array brings everything back (while later rounds still run).
+struct HistTail {
+    int32_t qmin, qmax;   // QB range of the array (empty: qmin > qmax)
+    uint32_t fallbacks;   // t array only: blocks that fell back to t = 4/16
+    uint32_t pad;
+    double max_err, sq;   // pivots and rounds: max |x - x'| and sum (x - x')^2
+};
+constexpr uint64_t kHistTailWords = sizeof(HistTail) / 4;
+constexpr uint64_t kHistStride = kHistGlobal + kHistTailWords;  // words per array
+
+// Host-visible result slot of one array (pinned, device-written): the
+// kernel that finishes the array writes the tail and the bins of [qmin, qmax]
+// (when inside the global window), then, after a system-scope fence, the
+// call's epoch into the flag.  The host builds the codebook once it sees it.
+constexpr uint64_t kHostSlotHead = 4;                                  // flag + pad (the tail follows)
+constexpr uint64_t kHostSlotBins = kHostSlotHead + kHistTailWords;     // first bin word
+constexpr uint64_t kHostSlotWords = kHostSlotBins + kHistGlobal;
+constexpr uint64_t kCodebookThreads = 4;  // host threads building codebooks
 
 // A sweep segment: targets k0 .. k0+cnt-1 (global indices along the round
 // axis) inside block b along that axis.
@@ -146,7 +167,9 @@ struct ErrAcc {
 };
 struct DevAcc {
     ErrAcc err;                        // pivots
-    uint32_t fallbacks, pad;
+    uint32_t fallbacks;                // t selection: blocks that fell back to t = 4/16
+    uint32_t pad;
+    uint32_t fold_count[ASZ_MAX_LAYERS];  // k_fold_rounds: workgroups done per round (reset by the last one)
     ArrayAcc arr[ASZ_MAX_LAYERS + 2];  // [0] t array, [1] pivots, [2 + r] round r
     ErrAcc rerr[ASZ_MAX_LAYERS];       // round r
 };

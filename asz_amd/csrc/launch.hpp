@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "internal.hpp"
 
 namespace asz {
@@ -11,6 +14,7 @@ struct PivotDesc {
     uint32_t D[3];
     uint32_t ps[3];   // pivot stride
     uint32_t pn[3];   // pivot lattice counts
+    uint64_t count;   // pn[0] * pn[1] * pn[2]
     double inv_half_pe;  // 0.5 / (double) pivot_eb
     double two_pe;       // (double) pivot_eb * 2
 };
@@ -35,8 +39,8 @@ struct SlopeSampleDesc {
 
 void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st);
 void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st);
-void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, void* eacc, void* aacc,
-                   hipStream_t st);
+// compress: one Partial per 256 pivots into part[0 ..]; decompress: part unused.
+void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st);
 // part / phist: this round's per-workgroup partials and histogram windows (at
 // R.part_off); gwin: the round array's global histogram window.
 void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
@@ -46,9 +50,6 @@ void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const
                              const float* tblk, const SweepSeg* segs, hipStream_t st);
 // Resets the accumulators: errors and fallbacks to 0, every QB range to empty.
 void launch_init_acc(DevAcc* acc, hipStream_t st);
-// ranges: 4 words per round (see launch_hist_reduce); writes DevAcc::arr[2 + r] and rerr[r]
-void launch_reduce_partials(const Partial* part, const uint64_t* ranges, uint32_t rounds, DevAcc* acc,
-                            hipStream_t st);
 
 void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
                  unsigned int* fallbacks, uint64_t nblocks, hipStream_t st);
@@ -73,16 +74,41 @@ struct HuffArrayDev {
 };
 constexpr uint32_t kHistChunk = 16384;   // symbols per histogram workgroup
 constexpr uint32_t kEncChunk = 4096;     // symbols per encode workgroup (256 threads x 16)
-constexpr uint32_t kLdsBook = 4096;      // LDS codebook entries of the encoder
+constexpr uint32_t kLdsBook = 1024;      // LDS codebook entries of the encoder
+constexpr uint64_t kEncGroups = 2048;    // workgroups of the persistent encoder
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
                  uint32_t* hist, hipStream_t st);
 void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64_t n1, uint32_t* g0, uint32_t* g1,
                        hipStream_t st);
-// ranges: (first partial, workgroups, first histogram word, window bins) per round
-void launch_hist_reduce(const uint32_t* phist, const uint64_t* ranges, uint32_t rounds, uint32_t* ghist,
-                        hipStream_t st);
-void launch_hist_compact(const DevAcc* acc, uint32_t narr, const uint32_t* ghist, uint32_t* out, hipStream_t st);
+// Fold of a round's per-workgroup partials (error, QB range) and histogram
+// windows into the round array's global window (kHistStride words per array
+// in ghist: bins, then the HistTail) and acc->arr[2 + r].  One descriptor per
+// round, built once per plan (make_fold_descs) and kept in device memory.
+struct FoldDesc {
+    uint64_t part_off, nwg, hist_off, fpart_off;
+    uint32_t win, pper, groups, pad;
+};
+inline uint32_t fold_per_group(uint32_t win) { return win <= 256 ? 16 * (256 / win) : std::max<uint32_t>(1, 4096 / win); }
+inline void make_fold_descs(const std::vector<RoundDesc>& rounds, std::vector<FoldDesc>& out, uint64_t& fpart_total) {
+    out.clear();
+    fpart_total = 0;
+    for (const RoundDesc& R : rounds) {
+        FoldDesc F{R.part_off, R.nwg, R.hist_off, fpart_total, R.hist_win, fold_per_group(R.hist_win), 0, 0};
+        F.groups = (uint32_t)((R.nwg + F.pper - 1) / F.pper);
+        fpart_total += F.groups;
+        out.push_back(F);
+    }
+}
+// Rounds r0 .. r0 + nr - 1 in one launch; max_groups = their largest groups.
+// hres: device address of the pinned host result slots (kHostSlotWords per array).
+void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_t max_groups, const Partial* part,
+                        const uint32_t* phist, Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres,
+                        uint32_t epoch, hipStream_t st);
+// Finishes the t array (k = 0, range and fallbacks from acc) or the pivot
+// array (k = 1, folding the pivot kernel's nparts Partials) into its host slot.
+void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
+                      uint32_t* hres, uint32_t epoch, hipStream_t st);
 // slot_words: scratch words per chunk (>= kEncChunk * max_len / 32 + 2); with
 // max_len > 32 the slots must be zeroed first.
 void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,

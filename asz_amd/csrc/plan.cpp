@@ -104,8 +104,15 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             const uint32_t* cntd = P.tables.data() + R.tab_off[d];
             for (uint32_t b = 0; b < R.nb[d]; b++) {
                 const uint32_t k0 = b * per_block;
-                for (uint32_t c = 0; c < cntd[b]; c += kSweepSeg)
-                    P.segs.push_back(SweepSeg{k0 + c, std::min<uint32_t>(kSweepSeg, cntd[b] - c), b, 0});
+                // equal segments of at most kSweepSeg targets: a block run that fits one
+                // segment is walked whole, so its QB run (orders along the round axis)
+                // is written by one lane and a wave's runs form one contiguous span
+                const uint32_t nseg = (cntd[b] + kSweepSeg - 1) / kSweepSeg;
+                for (uint32_t q = 0; q < nseg; q++) {
+                    const uint32_t c0 = (uint32_t)((uint64_t)cntd[b] * q / nseg);
+                    const uint32_t c1 = (uint32_t)((uint64_t)cntd[b] * (q + 1) / nseg);
+                    P.segs.push_back(SweepSeg{k0 + c0, c1 - c0, b, 0});
+                }
             }
             R.nseg = (uint32_t)P.segs.size() - R.seg_off;
         }

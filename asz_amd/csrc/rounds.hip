@@ -19,8 +19,8 @@
 //  * k_round_row (round axis 2, every order): waves walk whole lattice rows
 //    along DIM2, one target per lane per step.
 // Error statistics and QB ranges are reduced per workgroup into partials and
-// folded by one small kernel after the last round; each compress workgroup
-// also leaves an LDS histogram window of its QBs for the Huffman coder.
+// folded right after each round (k_fold_round); each compress workgroup also
+// leaves an LDS histogram window of its QBs for the Huffman coder.
 #include "device.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
@@ -141,52 +141,43 @@ __global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
         acc->fallbacks = 0;
     }
     if (i < ASZ_MAX_LAYERS + 2) acc->arr[i] = ArrayAcc{INT32_MAX, INT32_MIN, {0, 0}};
-    if (i < ASZ_MAX_LAYERS) acc->rerr[i] = ErrAcc{0.0, 0.0};
-}
-
-// One workgroup per round: fold that round's partials into arr[2 + r] / rerr[r].
-// ranges[4r], ranges[4r + 1]: first partial and partial count of round r.
-__global__ __launch_bounds__(256) void k_reduce_partials(const Partial* __restrict__ part,
-                                                         const uint64_t* __restrict__ ranges, DevAcc* acc) {
-    const uint32_t r = blockIdx.x;
-    const uint64_t off = ranges[4 * r], n = ranges[4 * r + 1];
-    double err = 0, sq = 0;
-    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        Partial p = part[off + i];
-        err = fmax(err, p.max_err);
-        sq += p.sq;
-        qmin = min(qmin, p.qmin);
-        qmax = max(qmax, p.qmax);
-    }
-    __shared__ double s_err[4], s_sq[4];
-    __shared__ int32_t s_min[4], s_max[4];
-    err = wave_max(err);
-    sq = wave_sum(sq);
-    qmin = wave_min(qmin);
-    qmax = wave_max(qmax);
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; k++) {
-            err = fmax(err, s_err[k]);
-            sq += s_sq[k];
-            qmin = min(qmin, s_min[k]);
-            qmax = max(qmax, s_max[k]);
-        }
-        acc->rerr[r] = ErrAcc{err, sq};
-        acc->arr[2 + r].qmin = qmin;
-        acc->arr[2 + r].qmax = qmax;
+    if (i < ASZ_MAX_LAYERS) {
+        acc->rerr[i] = ErrAcc{0.0, 0.0};
+        acc->fold_count[i] = 0;
     }
 }
 
 // ------------------------------------------------------------------ pivots (compress.tpp:161-175)
 
-__device__ __forceinline__ void flush_atomic(double err, double sq, int32_t qmin, int32_t qmax, ErrAcc* eacc,
-                                             ArrayAcc* aacc) {
+// One thread per pivot (lattice i_a = 0 mod pivot_stride_a, C order); every
+// workgroup leaves one Partial (error, QB range) at part[blockIdx.x].
+__device__ __forceinline__ void pivot_coords(const PivotDesc& P, uint64_t i, uint64_t& pos) {
+    const uint32_t k2 = (uint32_t)(i % P.pn[2]);
+    const uint64_t k01 = i / P.pn[2];
+    const uint32_t k1 = (uint32_t)(k01 % P.pn[1]), k0 = (uint32_t)(k01 / P.pn[1]);
+    pos = ((uint64_t)k0 * P.ps[0] * P.D[1] + (uint64_t)k1 * P.ps[1]) * P.D[2] + (uint64_t)k2 * P.ps[2];
+}
+
+__global__ __launch_bounds__(256) void k_pivots_compress(float* __restrict__ work, int32_t* __restrict__ qb,
+                                                         PivotDesc P, Partial* __restrict__ part) {
     __shared__ double s_err[4], s_sq[4];
     __shared__ int32_t s_min[4], s_max[4];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    if (i < P.count) {
+        uint64_t pos;
+        pivot_coords(P, i, pos);
+        const float x = work[pos];
+        const int32_t q = lround_q((double)daz(x) * P.inv_half_pe);
+        const float r = (float)((double)q * P.two_pe);
+        work[pos] = r;
+        qb[i] = q;
+        const double e = (double)daz(x) - (double)r;
+        err = fabs(e);
+        sq = e * e;
+        qmin = qmax = q;
+    }
     err = wave_max(err);
     sq = wave_sum(sq);
     qmin = wave_min(qmin);
@@ -201,42 +192,17 @@ __device__ __forceinline__ void flush_atomic(double err, double sq, int32_t qmin
             qmin = min(qmin, s_min[k]);
             qmax = max(qmax, s_max[k]);
         }
-        if (qmin <= qmax) {
-            atomic_max_pos_double(&eacc->max_err, err);
-            atomicAdd(&eacc->sq_sum, sq);
-            atomicMin(&aacc->qmin, qmin);
-            atomicMax(&aacc->qmax, qmax);
-        }
+        part[blockIdx.x] = Partial{err, sq, qmin, qmax};
     }
-}
-
-__global__ __launch_bounds__(256) void k_pivots_compress(float* __restrict__ work, int32_t* __restrict__ qb,
-                                                         PivotDesc P, ErrAcc* eacc, ArrayAcc* aacc) {
-    const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x, k1 = blockIdx.y, k0 = blockIdx.z;
-    double err = 0, sq = 0;
-    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
-    if (k2 < P.pn[2]) {
-        uint64_t pos = ((uint64_t)k0 * P.ps[0] * P.D[1] + (uint64_t)k1 * P.ps[1]) * P.D[2] + (uint64_t)k2 * P.ps[2];
-        float x = work[pos];
-        int32_t q = lround_q((double)daz(x) * P.inv_half_pe);
-        float r = (float)((double)q * P.two_pe);
-        work[pos] = r;
-        qb[((uint64_t)k0 * P.pn[1] + k1) * P.pn[2] + k2] = q;
-        double e = (double)daz(x) - (double)r;
-        err = fabs(e);
-        sq = e * e;
-        qmin = qmax = q;
-    }
-    flush_atomic(err, sq, qmin, qmax, eacc, aacc);
 }
 
 __global__ __launch_bounds__(256) void k_pivots_decompress(float* __restrict__ out, const int32_t* __restrict__ qb,
                                                            PivotDesc P) {
-    const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x, k1 = blockIdx.y, k0 = blockIdx.z;
-    if (k2 >= P.pn[2]) return;
-    uint64_t pos = ((uint64_t)k0 * P.ps[0] * P.D[1] + (uint64_t)k1 * P.ps[1]) * P.D[2] + (uint64_t)k2 * P.ps[2];
-    int32_t q = qb[((uint64_t)k0 * P.pn[1] + k1) * P.pn[2] + k2];
-    out[pos] = (float)((double)q * P.two_pe);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.count) return;
+    uint64_t pos;
+    pivot_coords(P, i, pos);
+    out[pos] = (float)((double)qb[i] * P.two_pe);
 }
 
 // ------------------------------------------------------------------ row rounds (axis 2)
@@ -439,7 +405,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
         }
 #pragma unroll
         for (int c = 0; c < SEG; c++) {
-            if (c >= (int)cnt_s) break;
+            if (c >= (int)cnt_s) continue;  // no early exit: every index stays a compile-time constant
             const uint32_t i = i0 + 2 * h * c;
             float* rp = work + col + (uint64_t)i * sd;
             const float w[6] = {cw[c], cw[c + 1], cw[c + 2], cw[c + 3], cw[c + 4], cw[c + 5]};
@@ -489,24 +455,16 @@ void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, h
     hipLaunchKernelGGL(dev::k_zero_index, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out2);
 }
 
-void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, void* eacc, void* aacc,
-                   hipStream_t st) {
-    dim3 grid((P.pn[2] + 255) / 256, P.pn[1], P.pn[0]);
+void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st) {
+    const dim3 grid((uint32_t)((P.count + 255) / 256));
     if (compress)
-        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, (ErrAcc*)eacc,
-                           (ArrayAcc*)aacc);
+        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, part);
     else
         hipLaunchKernelGGL(dev::k_pivots_decompress, grid, dim3(256), 0, st, work, (const int32_t*)qb, P);
 }
 
 void launch_init_acc(DevAcc* acc, hipStream_t st) {
     hipLaunchKernelGGL(dev::k_init_acc, dim3(1), dim3(128), 0, st, acc);
-}
-
-void launch_reduce_partials(const Partial* part, const uint64_t* ranges, uint32_t rounds, DevAcc* acc,
-                            hipStream_t st) {
-    if (!rounds) return;
-    hipLaunchKernelGGL(dev::k_reduce_partials, dim3(rounds), dim3(256), 0, st, part, ranges, acc);
 }
 
 namespace {

@@ -17,16 +17,18 @@ namespace asz {
 namespace dev {
 
 constexpr int kTselThreads = 256;
-constexpr int kHashSlots = 2048;
-constexpr int kMaxKeep = 4096;  // samples whose keys stay in LDS; more are recomputed
+constexpr int kNCand = 5;          // block candidates {1/16, 1/4, 9/32, 5/16, 11/32} (asz_param.hpp:203)
+constexpr int kDirect = 1024;      // direct LDS bins per candidate: QB in [-512, 512)
+constexpr int kHashSlots = 512;    // per candidate, for QBs outside the direct bins
+constexpr int kMaxKeep = 2048;     // samples whose keys stay in LDS; more are recomputed
 
 struct TselShared {
     float af[kMaxKeep], bf[kMaxKeep];
-    long long hkey[kHashSlots];
-    unsigned int hcnt[kHashSlots];
-    double red[kTselThreads / kWave];
-    double red2[kTselThreads / kWave];
-    unsigned int overflow;
+    uint32_t bins[kNCand][kDirect];
+    long long hkey[kNCand][kHashSlots];
+    uint32_t hcnt[kNCand][kHashSlots];
+    uint32_t ovf[kNCand];          // QBs that found no hash slot (counted as distinct symbols)
+    double red[kTselThreads / kWave][2 * kNCand];
 };
 
 __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
@@ -49,70 +51,34 @@ __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __re
     bf = (float)lround_f64(neg2 * B);
 }
 
-__device__ __forceinline__ void hash_insert(TselShared& sh, long long key) {
-    uint32_t hsh = (uint32_t)((unsigned long long)key * 0x9E3779B97F4A7C15ull >> 53) & (kHashSlots - 1);
+__device__ __forceinline__ void count_qb(TselShared& sh, int k, long long q) {
+    if (q >= -kDirect / 2 && q < kDirect / 2) {
+        atomicAdd(&sh.bins[k][(int)q + kDirect / 2], 1u);
+        return;
+    }
+    uint32_t hsh = (uint32_t)((unsigned long long)q * 0x9E3779B97F4A7C15ull >> 55) & (kHashSlots - 1);
     for (int probe = 0; probe < kHashSlots; probe++) {
-        uint32_t s = (hsh + probe) & (kHashSlots - 1);
-        long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&sh.hkey[s]), (unsigned long long)LLONG_MIN,
-                                   (unsigned long long)key);
-        if (prev == LLONG_MIN || prev == key) {
-            atomicAdd(&sh.hcnt[s], 1u);
+        const uint32_t slot = (hsh + probe) & (kHashSlots - 1);
+        const long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&sh.hkey[k][slot]),
+                                         (unsigned long long)LLONG_MIN, (unsigned long long)q);
+        if (prev == LLONG_MIN || prev == q) {
+            atomicAdd(&sh.hcnt[k][slot], 1u);
             return;
         }
     }
-    atomicOr(&sh.overflow, 1u);
+    atomicAdd(&sh.ovf[k], 1u);
 }
 
-__device__ double block_sum(TselShared& sh, double v, double* buf) {
-    v = wave_sum(v);
-    if ((threadIdx.x & (kWave - 1)) == 0) buf[threadIdx.x / kWave] = v;
-    __syncthreads();
-    double s = 0;
-    for (int k = 0; k < kTselThreads / kWave; k++) s += buf[k];
-    __syncthreads();
-    return s;
+__device__ __forceinline__ double entropy_term(uint32_t cnt, float rec) {
+    const float p = (float)cnt * rec;
+    return (double)p * (double)log2f(p);
 }
 
-// One candidate t: returns (h1, psnr) as the reference's float results would
-// be, up to summation order.
-__device__ void score(TselShared& sh, const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
-                      const uint32_t step[3], const uint32_t cnt[3], uint32_t ns, float neg2, float stat_eb,
-                      float t, float& h1_out, float& psnr_out) {
-    const float eb = T.R.eb;
-    const float neg_2_eb = 0.5f / eb;
-    const float two_t = t * 2.0f;
-    const float two_eb = eb * 2.0f;
-    for (int s = threadIdx.x; s < kHashSlots; s += kTselThreads) { sh.hkey[s] = LLONG_MIN; sh.hcnt[s] = 0; }
-    __syncthreads();
-    double mse = 0;
-    for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
-        float af, bf;
-        if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
-        else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf);
-        float err = __fmaf_rn(bf, 2.0f, af * two_t) * stat_eb;
-        long long q = lround_f64(err * neg_2_eb);
-        float de = __fmaf_rn(-(float)q, two_eb, err);
-        mse += (double)de * (double)de;
-        hash_insert(sh, q);
-    }
-    __syncthreads();
-    mse = block_sum(sh, mse, sh.red);
-    const float dn = (float)ns;
-    const float rec = 1.0f / dn;
-    double h = 0;
-    for (int s = threadIdx.x; s < kHashSlots; s += kTselThreads) {
-        if (sh.hcnt[s]) {
-            float p = (float)sh.hcnt[s] * rec;
-            h -= (double)p * (double)log2f(p);
-        }
-    }
-    h = block_sum(sh, h, sh.red2);
-    const float msef = (float)mse;
-    const float r2 = T.range * T.range;
-    h1_out = (float)h;
-    psnr_out = log10f((r2 / msef) * dn) * 10.0f;
-}
-
+// One workgroup per block: samples the block once (compress.tpp:102-120,
+// get_AB_map.tpp:9-22) and scores all five candidates in the same pass over
+// the samples (get_h_psnr.tpp:12-30, one QB histogram per candidate), then
+// replays the reference's sequential acceptance rule (compress.tpp:127-142)
+// on the five (H1, PSNR) pairs.
 __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
                                                        const uint32_t* __restrict__ ss_tab, float* tblk,
                                                        int32_t* t_int, unsigned int* fallbacks) {
@@ -123,7 +89,7 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
     const uint32_t bb[3] = {b0, b1, b2};
     const uint32_t ss = ss_tab[bid];
     float best_t = 0.25f;  // insufficient_sample_t (asz_param.hpp:167)
-    bool fallback = ss == 0;
+    const bool fallback = ss == 0;
     if (!fallback) {
         const int d = (int)R.d;
         uint32_t lo[3], hi[3], step[3], cnt[3];
@@ -142,20 +108,69 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         }
         const float stat_eb = T.stat_eb_ratio * R.eb;
         const float neg2 = 0.5f / stat_eb;
-        if (threadIdx.x == 0) sh.overflow = 0;
+        const float eb = R.eb;
+        const float neg_2_eb = 0.5f / eb;
+        const float two_eb = eb * 2.0f;
+        const float cand[kNCand] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
+        for (int i = threadIdx.x; i < kNCand * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
+        for (int i = threadIdx.x; i < kNCand * kHashSlots; i += kTselThreads) {
+            (&sh.hkey[0][0])[i] = LLONG_MIN;
+            (&sh.hcnt[0][0])[i] = 0;
+        }
+        if (threadIdx.x < kNCand) sh.ovf[threadIdx.x] = 0;
         for (uint32_t idx = threadIdx.x; idx < ns && idx < kMaxKeep; idx += kTselThreads)
             sample_keys(T, work, lo, step, cnt, idx, neg2, sh.af[idx], sh.bf[idx]);
         __syncthreads();
-        const float cand[5] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
-        float best_h1, best_psnr;
-        score(sh, T, work, lo, step, cnt, ns, neg2, stat_eb, best_t, best_h1, best_psnr);
-        for (int k = 0; k < 5; k++) {
-            const float t = cand[k];
-            if (t == best_t || best_psnr > FLT_MAX) continue;
-            float h1, psnr;
-            score(sh, T, work, lo, step, cnt, ns, neg2, stat_eb, t, h1, psnr);
-            const float sc = __fmaf_rn(best_h1 - h1, R.layer_slope, psnr - best_psnr);
-            if (sc > 0.0f || psnr > FLT_MAX) { best_t = t; best_h1 = h1; best_psnr = psnr; }
+        double mse[kNCand] = {0, 0, 0, 0, 0};
+        for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
+            float af, bf;
+            if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
+            else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf);
+#pragma unroll
+            for (int k = 0; k < kNCand; k++) {
+                const float err = __fmaf_rn(bf, 2.0f, af * (cand[k] * 2.0f)) * stat_eb;
+                const long long q = lround_f64(err * neg_2_eb);
+                const float de = __fmaf_rn(-(float)q, two_eb, err);
+                mse[k] += (double)de * (double)de;
+                count_qb(sh, k, q);
+            }
+        }
+        __syncthreads();
+        const float dn = (float)ns;
+        const float rec = 1.0f / dn;
+        double h[kNCand] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < kNCand; k++) {
+            for (int i = threadIdx.x; i < kDirect; i += kTselThreads)
+                if (sh.bins[k][i]) h[k] -= entropy_term(sh.bins[k][i], rec);
+            for (int i = threadIdx.x; i < kHashSlots; i += kTselThreads)
+                if (sh.hcnt[k][i]) h[k] -= entropy_term(sh.hcnt[k][i], rec);
+            if (threadIdx.x == 0 && sh.ovf[k]) h[k] -= (double)sh.ovf[k] * entropy_term(1, rec);
+        }
+        const int w = threadIdx.x / kWave;
+#pragma unroll
+        for (int k = 0; k < kNCand; k++) {
+            const double hs = wave_sum(h[k]), ms = wave_sum(mse[k]);
+            if ((threadIdx.x & (kWave - 1)) == 0) { sh.red[w][k] = hs; sh.red[w][kNCand + k] = ms; }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float h1[kNCand], psnr[kNCand];
+            const float r2 = T.range * T.range;
+            for (int k = 0; k < kNCand; k++) {
+                double hs = 0, ms = 0;
+                for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][kNCand + k]; }
+                h1[k] = (float)hs;
+                psnr[k] = log10f((r2 / (float)ms) * dn) * 10.0f;
+            }
+            // start from 1/4, then the candidates in order (compress.tpp:127-142)
+            int best = 1;
+            for (int k = 0; k < kNCand; k++) {
+                if (cand[k] == cand[best] || psnr[best] > FLT_MAX) continue;
+                const float sc = __fmaf_rn(h1[best] - h1[k], R.layer_slope, psnr[k] - psnr[best]);
+                if (sc > 0.0f || psnr[k] > FLT_MAX) best = k;
+            }
+            best_t = cand[best];
         }
     }
     if (threadIdx.x == 0) {

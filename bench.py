@@ -36,6 +36,31 @@ PLAN_REL_1E3 = ["cubic1_012", "cubic2_012", "cubic2_012", "cubic2_012", "cubic1_
 HBM_PEAK_GBPS = 8000.0
 
 
+def dom_kernel_name(plan, r):
+    """Template name of the round kernel that runs round r of a cubic plan."""
+    axis, order = int(plan[r][5]), ("120", "021", "012").index(plan[r][7:])
+    if axis == 2:
+        return f"k_round_row<false, false, {order}>"
+    fastest = 2 if order == 2 else (0 if order == 0 else 1)
+    wm = 0 if fastest == 2 else (1 if fastest == axis else 2)
+    return f"k_round_sweep<{axis}, false, false, {wm}>"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by its
+    reading on a plain copy kernel of known size); None if not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_512cube_rel1e-3.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            m = re.match(r"^\| `(.+?)` \| .*\| ([0-9]+) \|\s*$", line)
+            if m and m.group(1) == kernel:
+                return int(m.group(2))
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +134,6 @@ def main():
     torch.cuda.synchronize()
 
     ctx = codec.Context(0)
-    ctx.set_kernel_events(True)
     params = codec.make_params(rel_eb=float(np.float32(a.rel)), block_pivot_ratio=bpr, family="cubic")
     pl = codec.make_plan(plan)
     mn, mx = ctx.range(field.data_ptr(), N)
@@ -118,53 +142,67 @@ def main():
 
     # inputs resident in HBM: one copy per step (compression overwrites its input with the reconstruction)
     nsteps = a.warmup + a.steps
-    works = [field.clone() for _ in range(nsteps)]
+    works = [field.clone() for _ in range(max(nsteps, 2))]
     torch.cuda.synchronize()
 
     def step(w):
         m0, m1 = ctx.range(w.data_ptr(), N)
         return ctx.compress_device(w.data_ptr(), shape, pl, params, m0, m1, tsel=a.tsel, copy_stream=False)
 
+    def refill(k):
+        for i in range(k):
+            works[i].copy_(field)
+        torch.cuda.synchronize()
+
     ctx.set_stream_on_device(True)  # the stream stays in HBM: inputs and outputs resident
+    probe = field.clone()
+    res = step(probe)  # untimed: the per-round target counts pick the dominant round kernel
+    del probe
+    counts = res.counts
+    dom = max(range(len(plan)), key=lambda r: counts[r + 1])
     for k in range(a.warmup):
         step(works[k])
-    acc = {"pivot": 0.0, "hist": 0.0, "encode": 0.0}
-    round_ms = [0.0] * len(plan)
-    tsel_ms = [0.0] * len(plan)
+    # timed region: events only around the dominant round's kernel
+    ctx.set_timed_round(dom)
+    dom_ms = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = None
     for k in range(a.steps):
         res = step(works[a.warmup + k])
-        st = res.stats
-        acc["pivot"] += st.pivot_ms
-        acc["hist"] += st.hist_ms
-        acc["encode"] += st.encode_ms
-        for r in range(len(plan)):
-            round_ms[r] += st.round_ms[r]
-            tsel_ms[r] += st.tsel_round_ms[r]
+        dom_ms += res.stats.round_ms[dom]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    ctx.set_timed_round(-1)
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     value = nbytes / (ms_per_step / 1e3) / 1e9
-    K = a.steps
-    round_ms = [x / K for x in round_ms]
-    tsel_ms = [x / K for x in tsel_ms]
-    for k in acc:
-        acc[k] /= K
+    dom_ms /= a.steps
     st = res.stats
-    counts = res.counts
     res.stream = ctx.stream_view().tobytes()  # copied out of HBM after the timed region
     S_h = len(res.stream)
 
+    # per-kernel breakdown (untimed: an event pair around every launch)
+    K = 2
+    refill(K)
+    ctx.set_kernel_events(True)
+    acc = {"pivot": 0.0, "hist": 0.0, "encode": 0.0}
+    round_ms = [0.0] * len(plan)
+    tsel_ms = [0.0] * len(plan)
+    for k in range(K):
+        bst = step(works[k]).stats
+        acc["pivot"] += bst.pivot_ms / K
+        acc["hist"] += bst.hist_ms / K
+        acc["encode"] += bst.encode_ms / K
+        for r in range(len(plan)):
+            round_ms[r] += bst.round_ms[r] / K
+            tsel_ms[r] += bst.tsel_round_ms[r] / K
+    ctx.set_kernel_events(False)
+
     # the same steps with the stream copied to page-locked host memory (PCIe-inclusive rate)
     ctx.set_stream_on_device(False)
-    for k in range(a.steps):
-        works[a.warmup + k].copy_(field)
-    torch.cuda.synchronize()
+    refill(a.steps)
     p0 = time.perf_counter()
     for k in range(a.steps):
-        step(works[a.warmup + k])
+        step(works[k])
     torch.cuda.synchronize()
     pcie_ms = (time.perf_counter() - p0) * 1e3 / a.steps
     pcie_same_stream = ctx.stream_view().tobytes() == res.stream
@@ -188,20 +226,28 @@ def main():
     dims_arr = np.asarray(shape, np.uint64)
     torch.cuda.synchronize()
     dec_ms = []
-    dec_round = [0.0] * len(plan)
-    dec_huff = 0.0
-    for k in range(a.warmup + a.steps):
-        d0 = time.perf_counter()
+
+    def dec_once():
         ctx._check(ctx._L.asz_decompress(ctx._c, codec.ctypes.c_void_p(pinned.data_ptr()), S_h,
                                          dims_arr.ctypes.data_as(codec.ctypes.c_void_p), codec.ctypes.byref(params),
                                          codec.ctypes.c_void_p(out.data_ptr()), None, None))
+
+    for k in range(a.warmup + a.steps):
+        d0 = time.perf_counter()
+        dec_once()
         d1 = time.perf_counter()
         if k >= a.warmup:
             dec_ms.append((d1 - d0) * 1e3)
-            ds = ctx.stats()
-            dec_huff += ds.decode_ms / K
-            for r in range(len(plan)):
-                dec_round[r] += ds.round_ms[r] / K
+    dec_round = [0.0] * len(plan)
+    dec_huff = 0.0
+    ctx.set_kernel_events(True)
+    for k in range(K):
+        dec_once()
+        ds = ctx.stats()
+        dec_huff += ds.decode_ms / K
+        for r in range(len(plan)):
+            dec_round[r] += ds.round_ms[r] / K
+    ctx.set_kernel_events(False)
     dec_ms_step = sum(dec_ms) / len(dec_ms)
     recon = out.view(shape)
     err = (recon.double() - field.double()).abs()
@@ -211,12 +257,13 @@ def main():
     psnr = 20 * np.log10(rng) - 10 * np.log10(mse) if mse > 0 else float("inf")
     abs_eb = float(np.float32(a.rel) * (np.float32(mx) - np.float32(mn)))
 
-    # roofline of the dominant kernel: the largest round kernel; algorithmic bytes
-    # per target = 16 (original read, coarse read, reconstruction write, QB write)
-    dom = max(range(len(plan)), key=lambda r: round_ms[r])
+    # roofline of the dominant kernel (the round with the most targets), timed with
+    # HIP events on the library's stream inside the timed region; algorithmic
+    # bytes per target = 16 (original read, coarse read, reconstruction write, QB write)
     T_dom = counts[dom + 1]
     dom_bytes = 16 * T_dom
-    dom_gbps = dom_bytes / (round_ms[dom] / 1e3) / 1e9
+    dom_gbps = dom_bytes / (dom_ms / 1e3) / 1e9
+    traffic = measured_traffic(dom_kernel_name(plan, dom))
     P = counts[0]
     gpu_ms = acc["pivot"] + sum(round_ms) + sum(tsel_ms) + acc["hist"] + acc["encode"]
     B_c = 28 * N - 4 * P + S_h
@@ -242,9 +289,9 @@ def main():
                    "shape": list(shape), "rel_eb": a.rel, "plan": plan, "family": "cubic", "block_pivot_ratio": bpr,
                    "tsel": a.tsel},
         "roofline": {"bound": "hbm", "achieved": round(dom_gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": f"k_round_compress (round {dom}, axis {plan[dom][5]}, order {plan[dom][7:]})",
-                     "algorithmic_bytes": dom_bytes, "avg_ms": round(round_ms[dom], 5)},
+                     "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": f"{dom_kernel_name(plan, dom)} (round {dom}, axis {plan[dom][5]}, order {plan[dom][7:]})",
+                     "algorithmic_bytes": dom_bytes, "avg_ms": round(dom_ms, 5)},
         "compress_pcie_inclusive_GBps": round(nbytes / (pcie_ms / 1e3) / 1e9, 3),
         "pcie_leg_same_stream": pcie_same_stream,
         "decompress_GBps": round(nbytes / (dec_ms_step / 1e3) / 1e9, 3),

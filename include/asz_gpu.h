@@ -89,6 +89,10 @@ typedef struct asz_stats {
  * the host); the caller copies it out (asz_copy_to_host) or hands it to other
  * device code.  Default 0: *stream is page-locked host memory. */
 #define ASZ_OPT_STREAM_ON_DEVICE 2
+/* ASZ_OPT_TIMED_ROUND: when >= 0, record the event pair around that round's
+ * kernel only (asz_stats.round_ms[value]), leaving every other launch without
+ * events; a negative value turns kernel events off. */
+#define ASZ_OPT_TIMED_ROUND 3
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 
 /* ------------------------------------------------------------------ context */
