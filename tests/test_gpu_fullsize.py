@@ -1,0 +1,115 @@
+"""Full-size parity on the 512^3 anisotropic |K|^-11/3 field with an oblique
+step (0.3 x range): the reference s1_compress (built from its own sources,
+auto plan, family forced, block pivot ratio by the launcher's policy) against
+the GPU path at REL 1e-2, 1e-3, 1e-4 and 1e-5, cubic and quintic.
+
+For each case the reference stream's plan and per-block t are injected into
+the GPU compressor, whose pre-Zstd stream must equal the reference's byte for
+byte (so QB arrays, t arrays and outliers are identical); the GPU decoder must
+reproduce the reference s1_decompress output bit for bit on the reference
+stream.  One case also goes through the drop-in executables: our
+s1_compress with the reference's plan, decoded by the reference s1_decompress
+and vice versa.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from asz_amd import _lib, codec, fields, zstd
+from oracle import api
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (512, 512, 512)
+CASES = [(rel, fam) for rel in (1e-2, 1e-3, 1e-4, 1e-5) for fam in ("cubic", "quintic")]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bpr(rel):
+    return 96 if rel <= 1e-4 else 3  # t1_start_compressors.py:492-495
+
+
+@pytest.fixture(scope="module")
+def work():
+    if not api.ref_available():
+        pytest.skip("reference CLI not built (oracle/_ref)")
+    import torch
+    assert torch.cuda.is_available()
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        f = fields.to_numpy(fields.config2_field(SHAPE, seed=2, device="cuda")).astype(np.float32)
+        path = os.path.join(tmp, "c2.f32")
+        f.tofile(path)
+        ctx = codec.Context(0)
+        yield ctx, f, path, tmp
+        ctx.close()
+
+
+@pytest.mark.parametrize("rel,fam", CASES)
+def test_reference_stream_is_reproduced(work, rel, fam):
+    ctx, f, path, tmp = work
+    _say("case", SHAPE, rel, fam)
+    asz = os.path.join(tmp, f"ref_{rel:g}_{fam}.asz")
+    rec = os.path.join(tmp, f"ref_{rel:g}_{fam}.rec")
+    bpr = _bpr(rel)
+    extra = ["--block-pivot-ratio", bpr, "--predictor-family", fam]
+    api.run_ref_compress(path, SHAPE, rel, asz, extra)
+    api.run_ref_decompress(asz, SHAPE, rel, rec, ["--block-pivot-ratio", bpr, "--predictor-family", fam])
+    with open(asz, "rb") as fh:
+        payload = zstd.decompress(fh.read())
+    st = api.parse_payload(payload)
+    assert st.quintic == (fam == "quintic")
+    p = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=bpr, family=fam)
+    res = ctx.compress(f, codec.make_plan(codec.plan_strings(st.axis, st.order)), p, tsel="inject", t_inject=st.t)
+    assert len(res.stream) == len(payload)
+    assert res.stream == payload, "GPU stream differs from the reference stream"
+    ref_out = np.fromfile(rec, np.float32).reshape(SHAPE)
+    out = ctx.decompress(payload, SHAPE, p)
+    assert np.array_equal(out.view(np.uint32), ref_out.view(np.uint32)), "GPU decode differs from s1_decompress"
+    eb = float(api.abs_eb(p.rel_eb, res.min_val, res.max_val))
+    err = np.abs(f.astype(np.float64) - out.astype(np.float64))
+    bound = eb + np.spacing(np.abs(out)).astype(np.float64) / 2 + 1e-12 * eb
+    assert np.all(err <= bound)
+    _say("  stream bytes", len(payload), "max|err|/eb", float(err.max()) / eb)
+    os.remove(asz)
+    os.remove(rec)
+
+
+def test_drop_in_executables_cross_decode(work):
+    ctx, f, path, tmp = work
+    rel, fam, bpr = 1e-3, "cubic", 3
+    _say("case CLI", SHAPE, rel, fam)
+    common = ["-f", "-3", *map(str, SHAPE), "-M", "REL", f"{rel:.17g}", "--block-pivot-ratio", str(bpr),
+              "--predictor-family", fam]
+    ref_asz = os.path.join(tmp, "ref.asz")
+    out = api.run_ref_compress(path, SHAPE, rel, ref_asz, ["--block-pivot-ratio", bpr, "--predictor-family", fam])
+    plan = [ln for ln in out.splitlines() if ln.startswith("predictor_vec=")][0]
+    plan = plan[len("predictor_vec={"):-2].replace('"', "")
+    ours_asz = os.path.join(tmp, "ours.asz")
+    ours = subprocess.run([os.path.join(_lib.BIN_DIR, "s1_compress"), "-i", path, "-z", ours_asz, *common,
+                           "--predictor-plan", plan, "--tsel", "exact"], capture_output=True, text=True, timeout=300)
+    assert ours.returncode == 0, ours.stderr
+    for line in ("Data range time:", "Compression time:", "predictor_vec={", "predictor_family=cubic;",
+                 "Default slope statistics:"):
+        assert line in ours.stdout
+    # same plan and exact t selection: the payload is the reference's
+    with open(ref_asz, "rb") as a, open(ours_asz, "rb") as b:
+        assert zstd.decompress(a.read()) == zstd.decompress(b.read())
+    # our stream through the reference decoder, the reference stream through ours
+    rec_ref = os.path.join(tmp, "ours_by_ref.rec")
+    api.run_ref_decompress(ours_asz, SHAPE, rel, rec_ref, ["--block-pivot-ratio", bpr, "--predictor-family", fam])
+    rec_ours = os.path.join(tmp, "ref_by_ours.rec")
+    dec = subprocess.run([os.path.join(_lib.BIN_DIR, "s1_decompress"), "-z", ref_asz, "-o", rec_ours, *common],
+                         capture_output=True, text=True, timeout=300)
+    assert dec.returncode == 0, dec.stderr
+    assert "CR=" in dec.stdout and "Decompression time:" in dec.stdout
+    a = np.fromfile(rec_ref, np.float32)
+    b = np.fromfile(rec_ours, np.float32)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
