@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X ASZ compression core.
 
-Workload (one GPU): a 512^3 float32 field, anisotropic |K|^-11/3 Gaussian random
-field (axis0 stretched 4x, axis2 2x) plus an oblique planar step of 0.3 x range,
-compressed at REL 1e-3 with the cubic family and a fixed plan that the
-reference s1_compress chose for this field.  One step = one full compression
+Workload (one GPU, default "c2"): a 512^3 float32 field, anisotropic |K|^-11/3
+Gaussian random field (axis0 stretched 4x, axis2 2x) plus an oblique planar
+step of 0.3 x range, compressed at REL 1e-3 with the cubic family and the plan
+the reference s1_compress chose for this field.  "--workload c3": the 1024^3
+exp(GRF) density-like field at REL 1e-4 (block pivot ratio 96).  One step = one full compression
 of the field resident in HBM: range scan, pivots, every round (GPU per-block t
 selection, prediction, quantization, reconstruction, QB serialization) and
 Huffman coding (histogram, codebook, encode), with the pre-Zstd stream left
@@ -29,10 +30,18 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Plan the reference s1_compress picked for this field at REL 1e-3
-# (--block-pivot-ratio 3 --predictor-family cubic, auto plan generation).
-PLAN_REL_1E3 = ["cubic1_012", "cubic2_012", "cubic2_012", "cubic2_012", "cubic1_012", "cubic1_012", "cubic0_012",
-                "cubic2_012", "cubic2_012", "cubic0_012", "cubic1_021", "cubic0_012", "cubic2_012", "cubic0_120"]
+# Plans the reference s1_compress picked for these fields (auto plan generation,
+# --predictor-family cubic, block pivot ratio by the launcher's policy).
+PLAN_C2_REL_1E3 = ["cubic1_012", "cubic2_012", "cubic2_012", "cubic2_012", "cubic1_012", "cubic1_012", "cubic0_012",
+                   "cubic2_012", "cubic2_012", "cubic0_012", "cubic1_021", "cubic0_012", "cubic2_012", "cubic0_120"]
+PLAN_C3_REL_1E4 = ["cubic1_012", "cubic0_012", "cubic0_012", "cubic0_012", "cubic1_012", "cubic2_012", "cubic2_012",
+                   "cubic1_012", "cubic1_012", "cubic2_012", "cubic0_120", "cubic0_120", "cubic1_120", "cubic2_120"]
+WORKLOADS = {
+    "c2": dict(shape=(512, 512, 512), rel=1e-3, plan=PLAN_C2_REL_1E3, field="config2", seed=2,
+               desc="512^3 f32 anisotropic |K|^-11/3 GRF + oblique step, REL 1e-3"),
+    "c3": dict(shape=(1024, 1024, 1024), rel=1e-4, plan=PLAN_C3_REL_1E4, field="config3", seed=3,
+               desc="1024^3 f32 exp(|k|^-3 GRF) rescaled to [0, 1e3], REL 1e-4"),
+}
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -66,8 +75,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 512])
-    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--tsel", default="gpu", choices=["gpu", "exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--json-out", default=None)
@@ -125,12 +133,15 @@ def main():
     from asz_amd import codec, fields
 
     torch.cuda.set_device(0)
-    shape = tuple(a.shape)
+    W = WORKLOADS[a.workload]
+    a.rel = W["rel"]
+    shape = tuple(W["shape"])
     N = int(np.prod(shape))
     nbytes = N * 4
     bpr = 3 if a.rel > 1e-4 else 96
-    plan = PLAN_REL_1E3
-    field = fields.config2_field(shape, seed=2, device="cuda").contiguous()
+    plan = W["plan"]
+    gen = fields.config2_field if W["field"] == "config2" else fields.config3_field
+    field = gen(shape, seed=W["seed"], device="cuda").contiguous()
     torch.cuda.synchronize()
 
     ctx = codec.Context(0)
@@ -263,7 +274,7 @@ def main():
     T_dom = counts[dom + 1]
     dom_bytes = 16 * T_dom
     dom_gbps = dom_bytes / (dom_ms / 1e3) / 1e9
-    traffic = measured_traffic(dom_kernel_name(plan, dom))
+    traffic = measured_traffic(dom_kernel_name(plan, dom)) if a.workload == "c2" else None
     P = counts[0]
     gpu_ms = acc["pivot"] + sum(round_ms) + sum(tsel_ms) + acc["hist"] + acc["encode"]
     B_c = 28 * N - 4 * P + S_h
@@ -283,9 +294,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded GRF generated on the GPU)",
-        "config": {"workload": "config2: 512^3 f32 anisotropic |K|^-11/3 GRF + oblique step, REL 1e-3, cubic, "
-                               "fixed reference plan, block_pivot_ratio 3, GPU t selection; one step = range + "
-                               "compress to the pre-Zstd stream",
+        "config": {"workload": W["desc"] + ", cubic, fixed reference plan, block_pivot_ratio "
+                               f"{bpr}, GPU t selection; one step = range + compress to the pre-Zstd stream",
                    "shape": list(shape), "rel_eb": a.rel, "plan": plan, "family": "cubic", "block_pivot_ratio": bpr,
                    "tsel": a.tsel},
         "roofline": {"bound": "hbm", "achieved": round(dom_gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
