@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libasz_gpu.so")
 BIN_DIR = os.path.join(HERE, "bin")
 
 ASZ_OK = 0
-ASZ_TSEL_GPU, ASZ_TSEL_EXACT, ASZ_TSEL_INJECT = 0, 1, 2
+ASZ_TSEL_GPU, ASZ_TSEL_EXACT, ASZ_TSEL_INJECT, ASZ_TSEL_GRAD = 0, 1, 2, 3
 MAX_LAYERS = 63
 
 _P = ctypes.c_void_p
@@ -57,7 +57,8 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
-           "asz_round_layout", "asz_set_option")
+           "asz_round_layout", "asz_set_option", "asz_prepare_round", "asz_round_energy",
+           "asz_round_energy_adjoint")
 
 _lib = None
 
@@ -97,6 +98,12 @@ def lib():
                                        _U64P]
     L.asz_zstd_compress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
     L.asz_zstd_decompress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
+    L.asz_prepare_round.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_int, _P]
+    L.asz_round_energy.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_int,
+                                   ctypes.c_int, _P, _P]
+    L.asz_round_energy_adjoint.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_int,
+                                           ctypes.c_int, _P, _P, _P]
     L.asz_free.argtypes = [_P]
     L.asz_free.restype = None
     _lib = L

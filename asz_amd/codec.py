@@ -14,10 +14,10 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import ASZ_TSEL_EXACT, ASZ_TSEL_GPU, ASZ_TSEL_INJECT, Params, Plan, Stats
+from ._lib import ASZ_TSEL_EXACT, ASZ_TSEL_GPU, ASZ_TSEL_GRAD, ASZ_TSEL_INJECT, Params, Plan, Stats
 
 ORDERS = ("120", "021", "012")
-TSEL = {"gpu": ASZ_TSEL_GPU, "exact": ASZ_TSEL_EXACT, "inject": ASZ_TSEL_INJECT}
+TSEL = {"gpu": ASZ_TSEL_GPU, "exact": ASZ_TSEL_EXACT, "inject": ASZ_TSEL_INJECT, "grad": ASZ_TSEL_GRAD}
 
 
 class AszError(RuntimeError):

@@ -110,7 +110,8 @@ int main(int argc, char* argv[]) try {
             string m = next();
             if (m == "gpu") s.tsel = ASZ_TSEL_GPU;
             else if (m == "exact") s.tsel = ASZ_TSEL_EXACT;
-            else throw invalid_argument("tsel must be gpu or exact");
+            else if (m == "grad") s.tsel = ASZ_TSEL_GRAD;
+            else throw invalid_argument("tsel must be gpu, exact or grad");
         }
         else if (a == "--device") s.device = stoi(next());
         else throw invalid_argument("unknown argument: " + a);

@@ -346,11 +346,14 @@ static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
     c->epoch = 0;
 }
 
+// stop_round >= 0: run the pivots and rounds [0, stop_round) only (no
+// Huffman coding), leaving d_work in the state the round stop_round reads.
 static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
                           const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
-                          int32_t* qb_out, int64_t* t_out) {
+                          int32_t* qb_out, int64_t* t_out, int stop_round = -1) {
     auto wall0 = Clock::now();
     validate_params(p);
+    if (tsel_mode < ASZ_TSEL_GPU || tsel_mode > ASZ_TSEL_GRAD) fail(ASZ_ERR_ARG, "unknown t selection mode");
     hipStream_t st = c->stream;
     HIPCHECK(hipStreamSynchronize(c->side));  // no copy of an earlier call is still in flight
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
@@ -530,7 +533,8 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     launch_hist_small(nullptr, 0, qb, P.pivot_count, nullptr, gwin(1), st);
     launch_hist_tail(acc, 1, ppart, npp, gwin(1), c->hres_dev, epoch, st);
     // the workers start once the GPU has work queued (thread start-up overlaps it)
-    const unsigned nworkers = (unsigned)std::min<uint64_t>(order.size(), kCodebookThreads);
+    const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
+    const unsigned nworkers = stop_round >= 0 ? 0u : (unsigned)std::min<uint64_t>(order.size(), kCodebookThreads);
     std::vector<std::exception_ptr> werr(nworkers);
     host_ms_w.assign(nworkers, 0.0);
     std::vector<std::thread> workers;
@@ -564,18 +568,20 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
     std::vector<uint64_t> koff;
-    for (uint64_t r = 0; r < L; r++) {
+    for (uint64_t r = 0; r < Lrun; r++) {
         const RoundDesc& R = P.rounds[r];
         check_grid(R);
         float* tb = tblk + r * nbl;
-        if (tsel_mode == ASZ_TSEL_GPU) {
-            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
+        if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) {
+            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                       p.t_min, p.t_max};
             int kt = kev_begin(c, KEV_TSEL, (int)r);
-            launch_tsel(T, d_work, ss + r * nbl, tb, sym, &acc->fallbacks, nbl, st);
+            launch_tsel(T, d_work, ss + r * nbl, tb, sym, &acc->fallbacks, nbl, tsel_mode == ASZ_TSEL_GRAD, st);
             kev_end(c, kt);
         } else if (tsel_mode == ASZ_TSEL_EXACT) {
             auto h0 = Clock::now();
-            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision};
+            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                       p.t_min, p.t_max};
             koff.assign(nbl + 1, 0);
             for (uint64_t b = 0; b < nbl; b++) koff[b + 1] = koff[b] + P.ns[r * nbl + b];
             uint64_t* dko = c->key_off.get<uint64_t>(nbl + 1);
@@ -629,6 +635,11 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         }
     }
     HIPCHECK(hipGetLastError());
+    if (stop_round >= 0) {
+        HIPCHECK(hipStreamSynchronize(st));
+        S.wall_ms = ms_since(wall0);
+        return;
+    }
     HIPCHECK(hipEventRecord(c->ev[1], st));
     enqueued = true;
     const double t_enq = ms_since(wall0);
@@ -646,7 +657,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         HistTail tail;
         std::memcpy(&tail, slot(k) + kHostSlotHead, sizeof(tail));
         if (k == 0) {
-            if (tsel_mode == ASZ_TSEL_GPU) S.t_fallbacks = tail.fallbacks;
+            if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) S.t_fallbacks = tail.fallbacks;
             if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
         } else {
             S.max_err = std::max(S.max_err, tail.max_err);
@@ -1176,6 +1187,47 @@ int asz_compress_host(asz_ctx* c, const float* field, const uint64_t dims[3], co
         *stream = c->out_stream;
         *stream_bytes = c->out_bytes;
     });
+}
+
+int asz_prepare_round(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan* plan, const asz_params* p,
+                      float mn, float mx, int round, const int64_t* t_int) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (round < 0 || round >= plan->layers) fail(ASZ_ERR_ARG, "no such round");
+        if (!t_int) fail(ASZ_ERR_ARG, "t values are required");
+        asz::compress_impl(c, d_work, dims, *plan, *p, mn, mx, ASZ_TSEL_INJECT, t_int, nullptr, nullptr, round);
+    });
+}
+
+static int round_energy(asz_ctx* c, const float* d_work, const uint64_t dims[3], const asz_plan* plan,
+                        const asz_params* p, int round, int quintic, const float* d_theta, const float* d_gE,
+                        double* d_E, float* d_g) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (round < 0 || round >= plan->layers) fail(ASZ_ERR_ARG, "no such round");
+        asz::Geometry g = asz::make_geometry(dims, *plan, p->block_pivot_ratio);
+        asz::Plan P = asz::make_plan(g, 1.0f, p->pivot_eb_ratio, p->eb_ratio, p->slope_ratio, 1.0f,
+                                     p->h1_block_sampling_num);
+        asz::launch_energy(P.rounds[round], g.nblocks(), quintic != 0, d_gE != nullptr, d_work, d_theta, d_gE, d_E,
+                           d_g, c->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int asz_round_energy(asz_ctx* c, const float* d_work, const uint64_t dims[3], const asz_plan* plan,
+                     const asz_params* p, int round, int quintic, const float* d_theta, double* d_E) {
+    return round_energy(c, d_work, dims, plan, p, round, quintic, d_theta, nullptr, d_E, nullptr);
+}
+
+int asz_round_energy_adjoint(asz_ctx* c, const float* d_work, const uint64_t dims[3], const asz_plan* plan,
+                             const asz_params* p, int round, int quintic, const float* d_theta, const float* d_gE,
+                             float* d_gtheta) {
+    if (!d_gE) {
+        if (c) c->err = "the upstream gradient is required";
+        return ASZ_ERR_ARG;
+    }
+    return round_energy(c, d_work, dims, plan, p, round, quintic, d_theta, d_gE, nullptr, d_gtheta);
 }
 
 int asz_decompress(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,

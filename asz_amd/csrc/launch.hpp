@@ -26,6 +26,7 @@ struct TselDesc {
     float stat_eb_ratio;
     float range;            // max_val - min_val (float)
     float t_precision;
+    float t_min, t_max;     // gradient-tuned form: the projection interval
 };
 
 // AB-key sampling over a whole axis for the default slope (get_default_slope.tpp:15-25).
@@ -51,9 +52,16 @@ void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const
 // Resets the accumulators: errors and fallbacks to 0, every QB range to empty.
 void launch_init_acc(DevAcc* acc, hipStream_t st);
 
+// grad: gradient-tuned candidates (projected Newton on the sampled residual
+// energy, snapped to the t grid) instead of the five block candidates.
 void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
-                 unsigned int* fallbacks, uint64_t nblocks, hipStream_t st);
+                 unsigned int* fallbacks, uint64_t nblocks, bool grad, hipStream_t st);
 void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st);
+
+// Residual energy of round R per block (forward: E[b], FP64) or its adjoint
+// (g[b * np + j] = gE[b] * dE_b/dtheta_j, FP32), np = 1 (cubic t) or 2 (quintic a, b).
+void launch_energy(const RoundDesc& R, uint64_t nblocks, bool quintic, bool adjoint, const float* work,
+                   const float* theta, const float* gE, double* E, float* g, hipStream_t st);
 void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,
                       int64_t* keys, uint64_t nblocks, hipStream_t st);
 

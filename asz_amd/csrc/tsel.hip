@@ -31,9 +31,12 @@ struct TselShared {
     double red[kTselThreads / kWave][2 * kNCand];
 };
 
+// Keys (lroundf(A / 2 sigma), lroundf(B / 2 sigma)) of sample idx; A and B
+// themselves in Araw / Braw (the residual of the interior 1D4R predictor at
+// t is A t + B).
 __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
                                             const uint32_t step[3], const uint32_t cnt[3], uint32_t idx,
-                                            float neg2, float& af, float& bf) {
+                                            float neg2, float& af, float& bf, float& Araw, float& Braw) {
     const RoundDesc& R = T.R;
     uint32_t q2 = idx % cnt[2];
     uint32_t q01 = idx / cnt[2];
@@ -49,6 +52,8 @@ __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __re
     float B = (float)__fma_rn((double)S, 0.25, -(double)x0);
     af = (float)lround_f64(neg2 * A);
     bf = (float)lround_f64(neg2 * B);
+    Araw = A;
+    Braw = B;
 }
 
 __device__ __forceinline__ void count_qb(TselShared& sh, int k, long long q) {
@@ -75,14 +80,26 @@ __device__ __forceinline__ double entropy_term(uint32_t cnt, float rec) {
 }
 
 // One workgroup per block: samples the block once (compress.tpp:102-120,
-// get_AB_map.tpp:9-22) and scores all five candidates in the same pass over
-// the samples (get_h_psnr.tpp:12-30, one QB histogram per candidate), then
+// get_AB_map.tpp:9-22) and scores all candidates in the same pass over the
+// samples (get_h_psnr.tpp:12-30, one QB histogram per candidate), then
 // replays the reference's sequential acceptance rule (compress.tpp:127-142)
-// on the five (H1, PSNR) pairs.
+// on the (H1, PSNR) pairs, starting from t = 1/4.
+//
+// GRAD = false: the five block candidates {1/16, 1/4, 9/32, 5/16, 11/32}.
+// GRAD = true (gradient-tuned form): the sampled residual energy
+// E(t) = sum (A t + B)^2 is minimised by projected Newton steps on
+// [t_min, t_max] (E is quadratic: dE/dt = 2 sum A (A t + B), d2E/dt2 =
+// 2 sum A^2, so the step is exact), the minimiser is snapped to the two
+// neighbouring points of the 1/t_precision grid, and {floor, 1/4, ceil}
+// are judged by the exact proxy.  Any t on the grid decodes with the
+// reference decoder (decompress.tpp:185).
+template <bool GRAD>
 __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
                                                        const uint32_t* __restrict__ ss_tab, float* tblk,
                                                        int32_t* t_int, unsigned int* fallbacks) {
     __shared__ TselShared sh;
+    __shared__ float s_cand[kNCand];
+    __shared__ int s_nc;
     const RoundDesc& R = T.R;
     const uint32_t bid = blockIdx.x;
     const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
@@ -111,23 +128,63 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         const float eb = R.eb;
         const float neg_2_eb = 0.5f / eb;
         const float two_eb = eb * 2.0f;
-        const float cand[kNCand] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
         for (int i = threadIdx.x; i < kNCand * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
         for (int i = threadIdx.x; i < kNCand * kHashSlots; i += kTselThreads) {
             (&sh.hkey[0][0])[i] = LLONG_MIN;
             (&sh.hcnt[0][0])[i] = 0;
         }
         if (threadIdx.x < kNCand) sh.ovf[threadIdx.x] = 0;
-        for (uint32_t idx = threadIdx.x; idx < ns && idx < kMaxKeep; idx += kTselThreads)
-            sample_keys(T, work, lo, step, cnt, idx, neg2, sh.af[idx], sh.bf[idx]);
+        double saa = 0, sab = 0;  // GRAD: sum A^2, sum A B over the samples
+        for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
+            if (!GRAD && idx >= kMaxKeep) break;
+            float af, bf, A, B;
+            sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf, A, B);
+            if (idx < kMaxKeep) { sh.af[idx] = af; sh.bf[idx] = bf; }
+            if (GRAD) { saa += (double)A * (double)A; sab += (double)A * (double)B; }
+        }
+        if (GRAD) {
+            const int w0 = threadIdx.x / kWave;
+            saa = wave_sum(saa);
+            sab = wave_sum(sab);
+            if ((threadIdx.x & (kWave - 1)) == 0) { sh.red[w0][0] = saa; sh.red[w0][1] = sab; }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double a2 = 0, ab = 0;
+                for (int v = 0; v < kTselThreads / kWave; v++) { a2 += sh.red[v][0]; ab += sh.red[v][1]; }
+                // projected Newton on [t_min, t_max] from t = 1/4
+                double t = 0.25;
+                for (int it = 0; it < 3 && a2 > 0; it++) {
+                    const double g = 2.0 * (a2 * t + ab), hs = 2.0 * a2;
+                    t = fmin(fmax(t - g / hs, (double)T.t_min), (double)T.t_max);
+                }
+                const float P = T.t_precision;
+                float lo_t = floorf((float)t * P) / P, hi_t = ceilf((float)t * P) / P;
+                lo_t = fmaxf(lo_t, T.t_min);
+                hi_t = fminf(hi_t, T.t_max);
+                int nc = 0;
+                s_cand[nc++] = 0.25f;
+                if (lo_t != 0.25f) s_cand[nc++] = lo_t;
+                if (hi_t != 0.25f && hi_t != lo_t) s_cand[nc++] = hi_t;
+                s_nc = nc;
+            }
+        } else if (threadIdx.x == 0) {
+            const float c5[kNCand] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
+            for (int k = 0; k < kNCand; k++) s_cand[k] = c5[k];
+            s_nc = kNCand;
+        }
         __syncthreads();
+        const int nc = s_nc;
+        float cand[kNCand];
+#pragma unroll
+        for (int k = 0; k < kNCand; k++) cand[k] = k < nc ? s_cand[k] : 0.0f;
         double mse[kNCand] = {0, 0, 0, 0, 0};
         for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
-            float af, bf;
+            float af, bf, A, B;
             if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
-            else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf);
+            else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf, A, B);
 #pragma unroll
             for (int k = 0; k < kNCand; k++) {
+                if (k >= nc) break;
                 const float err = __fmaf_rn(bf, 2.0f, af * (cand[k] * 2.0f)) * stat_eb;
                 const long long q = lround_f64(err * neg_2_eb);
                 const float de = __fmaf_rn(-(float)q, two_eb, err);
@@ -157,15 +214,15 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         if (threadIdx.x == 0) {
             float h1[kNCand], psnr[kNCand];
             const float r2 = T.range * T.range;
-            for (int k = 0; k < kNCand; k++) {
+            for (int k = 0; k < nc; k++) {
                 double hs = 0, ms = 0;
                 for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][kNCand + k]; }
                 h1[k] = (float)hs;
                 psnr[k] = log10f((r2 / (float)ms) * dn) * 10.0f;
             }
             // start from 1/4, then the candidates in order (compress.tpp:127-142)
-            int best = 1;
-            for (int k = 0; k < kNCand; k++) {
+            int best = GRAD ? 0 : 1;
+            for (int k = 0; k < nc; k++) {
                 if (cand[k] == cand[best] || psnr[best] > FLT_MAX) continue;
                 const float sc = __fmaf_rn(h1[best] - h1[k], R.layer_slope, psnr[k] - psnr[best]);
                 if (sc > 0.0f || psnr[k] > FLT_MAX) best = k;
@@ -249,9 +306,13 @@ __global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const f
 }  // namespace dev
 
 void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
-                 unsigned int* fallbacks, uint64_t nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_tsel, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss, tblk,
-                       t_int, fallbacks);
+                 unsigned int* fallbacks, uint64_t nblocks, bool grad, hipStream_t st) {
+    if (grad)
+        hipLaunchKernelGGL(dev::k_tsel<true>, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss,
+                           tblk, t_int, fallbacks);
+    else
+        hipLaunchKernelGGL(dev::k_tsel<false>, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss,
+                           tblk, t_int, fallbacks);
 }
 
 void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,

@@ -324,6 +324,13 @@ def main():
         "max_err_over_eb": round(max_err / abs_eb, 7), "psnr": round(psnr, 4),
         "t_fallbacks": int(st.t_fallbacks),
     }
+    # the gradient-tuned form against the discrete t scan at REL 1e-3 on the
+    # half-smooth / half-rough 512^3 field (compression ratio, max error, t-selection time)
+    if a.workload == "c2":
+        del works, out, recon, err
+        torch.cuda.empty_cache()
+        from asz_amd import tune
+        line["tuned_vs_plain"] = tune.config5_comparison(rels=(1e-3,))["0.001"]
     if not a.no_cpu_baseline:
         cb = cpu_baseline(field.cpu().numpy(), plan, a.rel, bpr)
         if cb is not None:

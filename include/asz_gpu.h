@@ -64,6 +64,13 @@ typedef struct asz_plan {
 #define ASZ_TSEL_GPU 0      /* GPU H1-PSNR selection (compress.tpp:90-144 on the GPU) */
 #define ASZ_TSEL_EXACT 1    /* GPU sampling + host scoring in the reference's map order   */
 #define ASZ_TSEL_INJECT 2   /* caller's t_int values (block-major, round-minor)           */
+/* Gradient-tuned form (no reference counterpart; it replaces the discrete
+ * five-candidate scan of compress.tpp:127-142): per block, projected Newton
+ * steps on the sampled residual energy sum (A t + B)^2 over [t_min, t_max],
+ * the minimiser snapped to the 1/t_precision grid, and {floor, 1/4, ceil}
+ * judged by the exact H1-PSNR proxy with the reference's acceptance rule.
+ * The stream format is unchanged (any grid t decodes with the reference). */
+#define ASZ_TSEL_GRAD 3
 
 /* Timings (milliseconds) and diagnostics of the last asz_compress / asz_decompress. */
 typedef struct asz_stats {
@@ -173,6 +180,34 @@ int asz_decompress(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, c
                    const asz_params *p, float *d_out, int32_t *qb_out, int64_t *t_out);
 int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, const uint64_t dims[3],
                         const asz_params *p, float *out);
+
+/* ------------------------------------------------------------------ gradient-tuned form
+ * No reference counterpart: these expose the residual-energy model behind
+ * ASZ_TSEL_GRAD for tuning harnesses (the discrete scan they replace is
+ * compress.tpp:90-144).
+ *
+ * asz_prepare_round: pivots and rounds [0, round) with the caller's t_int
+ * (block-major, round-minor, #blocks * layers values; rounds >= round unused),
+ * leaving d_work in the state round `round` reads (coarse points
+ * reconstructed, targets original).
+ *
+ * asz_round_energy: per block b, E_b = sum of r_i^2 over the block's targets
+ * of round `round` that use the parameterised predictor, r_i in FP32, the sum
+ * in FP64 (d_E: #blocks doubles).  quintic = 0: interior 1D4R targets,
+ * r = A t_b + B, d_theta = #blocks floats.  quintic = 1: 1D6R targets with
+ * free weights, r = a_b (S5 - S1) + b_b (S3 - S1) + S1/2 - x0, d_theta =
+ * 2 #blocks floats (a_b, b_b).
+ *
+ * asz_round_energy_adjoint: d_gtheta[b][j] = d_gE[b] * dE_b/dtheta_bj, FP32
+ * (the vector-Jacobian product of asz_round_energy).  All pointers are device
+ * pointers. */
+int asz_prepare_round(asz_ctx *ctx, float *d_work, const uint64_t dims[3], const asz_plan *plan, const asz_params *p,
+                      float min_val, float max_val, int round, const int64_t *t_int);
+int asz_round_energy(asz_ctx *ctx, const float *d_work, const uint64_t dims[3], const asz_plan *plan,
+                     const asz_params *p, int round, int quintic, const float *d_theta, double *d_E);
+int asz_round_energy_adjoint(asz_ctx *ctx, const float *d_work, const uint64_t dims[3], const asz_plan *plan,
+                             const asz_params *p, int round, int quintic, const float *d_theta, const float *d_gE,
+                             float *d_gtheta);
 
 /* ------------------------------------------------------------------ Huffman (host side)
  * The codebook and header of one array as huffman_encoder_class<T>::preprocess_encode

@@ -159,6 +159,25 @@ def default_slope(data, params: Params, eb_span=1.5, invalid_threshold=1.0, fall
         int(info[2])]
 
 
+def round_energy(state, axis, bpr, round_, quintic, theta):
+    """FP64 residual energy per block and its gradient (oracle_round_energy)."""
+    state = np.ascontiguousarray(state, np.float32)
+    d = np.asarray(state.shape, np.uint64)
+    ax = np.asarray(axis, np.uint8)
+    th = np.ascontiguousarray(theta, np.float64)
+    nb = th.size // (2 if quintic else 1)
+    E = np.zeros(nb, np.float64)
+    g = np.zeros(th.size, np.float64)
+    L = lib()
+    L.oracle_round_energy.restype = ctypes.c_int
+    L.oracle_round_energy.argtypes = [_P, _P, ctypes.c_int, _P, _U64, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+    rc = L.oracle_round_energy(_ptr(state), _ptr(d), len(axis), _ptr(ax), bpr, round_, int(quintic), _ptr(th),
+                               _ptr(E), _ptr(g))
+    if rc != 0:
+        raise RuntimeError(_err())
+    return E, g
+
+
 def decompress(stream: Stream, dims, params: Params):
     d = np.asarray(dims, np.uint64)
     recon = np.zeros(tuple(int(x) for x in dims), np.float32)

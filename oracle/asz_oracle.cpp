@@ -24,7 +24,11 @@
 //
 // Parity status of each function is pinned by tests/test_oracle_golden.py
 // against streams and reconstructions written by the reference CLI built from
-// its own sources (oracle/Makefile, target "ref").
+// its own sources (oracle/Makefile, target "ref").  oracle_round_energy (the
+// FP64 residual-energy model of the gradient-tuned form, which has no
+// reference code) is pinned by closed forms in tests/test_grad_cpu.py:
+// brute-force minimisation on small blocks, the quadratic identities, and
+// fields on which the interior predictors are exact.
 
 #include <xmmintrin.h>
 #include <pmmintrin.h>
@@ -734,6 +738,91 @@ int64_t oracle_parse_payload(const uint8_t* p, uint64_t n, float* min_max, int* 
         }
         return (int64_t)(p - p0);
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Gradient-tuned form, FP64 reference (no counterpart in the reference code;
+// the model follows from compress.tpp:9-32 with the neighbours of a round
+// fixed).  For round `round` of the plan, per block b, over the block's
+// targets that use the parameterised predictor (interior 1D4R: i >= 3h and
+// i + 3h < right; 1D6R: i >= 5h and i + 5h < right; ownership as in
+// compress.tpp:57-59), everything in double from the float field `state`:
+//   cubic:   r = A t + B, A = (x-h + x+h) - (x-3h + x+3h), B = (x-h + x+h + x-3h + x+3h)/4 - x0
+//   quintic: r = a (S5 - S1) + b (S3 - S1) + S1/2 - x0
+// E[b] = sum r^2, g[b * np + j] = sum 2 r dr/dtheta_j.
+int oracle_round_energy(const float* state, const uint64_t dims[3], int L, const uint8_t* axis, uint64_t bpr,
+                        int round, int quintic, const double* theta, double* E, double* g) {
+    try {
+        uint8_t ord[64] = {0};
+        Geometry G = make_geometry(dims, L, axis, ord, bpr);
+        uint64_t s[3] = {G.pivot_stride[0], G.pivot_stride[1], G.pivot_stride[2]};
+        for (int r = 0; r < round; r++) s[axis[r]] /= 2;
+        const int d = axis[round];
+        const uint64_t h = s[d] / 2;
+        const uint64_t reach = quintic ? 5 : 3;
+        const int np = quintic ? 2 : 1;
+        uint64_t dist = h;
+        for (int a = d + 1; a < 3; a++) dist *= dims[a];
+        const uint64_t nb = G.block_num[0] * G.block_num[1] * G.block_num[2];
+        for (uint64_t b = 0; b < nb; b++) {
+            const uint64_t bb[3] = {b / (G.block_num[2] * G.block_num[1]), (b / G.block_num[2]) % G.block_num[1],
+                                    b % G.block_num[2]};
+            uint64_t lo[3], hi[3], st[3];  // hi inclusive
+            bool empty = false;
+            for (int a = 0; a < 3; a++) {
+                const uint64_t B0 = bb[a] * G.block_shape[a];
+                if (a == d) {
+                    const uint64_t right = std::min(B0 + G.block_shape[a] + 1, dims[a]);
+                    const uint64_t end = std::min(B0 + G.block_shape[a], dims[a]);
+                    uint64_t first = std::max(B0, reach * h);
+                    while (first % (2 * h) != h) first++;  // odd multiple of h
+                    st[a] = 2 * h;
+                    lo[a] = first;
+                    if (end < 1 || right < 1 + reach * h) { empty = true; break; }
+                    hi[a] = std::min(end - 1, right - 1 - reach * h);
+                    if (hi[a] < lo[a]) { empty = true; break; }
+                } else {
+                    st[a] = s[a];
+                    lo[a] = bb[a] == 0 ? 0 : B0 + s[a];
+                    hi[a] = std::min(B0 + G.block_shape[a], dims[a] - 1);
+                    if (hi[a] < lo[a]) { empty = true; break; }
+                }
+            }
+            double e = 0, g0 = 0, g1 = 0;
+            const double t0 = theta[b * np], t1 = quintic ? theta[b * np + 1] : 0.0;
+            if (!empty)
+                for (uint64_t i0 = lo[0]; i0 <= hi[0]; i0 += st[0])
+                    for (uint64_t i1 = lo[1]; i1 <= hi[1]; i1 += st[1])
+                        for (uint64_t i2 = lo[2]; i2 <= hi[2]; i2 += st[2]) {
+                            const float* p = state + (i0 * dims[1] + i1) * dims[2] + i2;
+                            const double x0 = p[0], a1 = p[-(int64_t)dist], b1 = p[dist];
+                            const double c3 = p[-3 * (int64_t)dist], e3 = p[3 * dist];
+                            double r, f1, f2 = 0;
+                            if (!quintic) {
+                                f1 = (a1 + b1) - (c3 + e3);
+                                const double B = 0.25 * (a1 + b1 + c3 + e3) - x0;
+                                r = f1 * t0 + B;
+                            } else {
+                                const double g5 = p[-5 * (int64_t)dist], k5 = p[5 * dist];
+                                const double S1 = a1 + b1, S3 = c3 + e3, S5 = g5 + k5;
+                                f1 = S5 - S1;
+                                f2 = S3 - S1;
+                                r = t0 * f1 + t1 * f2 + 0.5 * S1 - x0;
+                            }
+                            e += r * r;
+                            g0 += 2 * r * f1;
+                            g1 += 2 * r * f2;
+                        }
+            E[b] = e;
+            if (g) {
+                g[b * np] = g0;
+                if (quintic) g[b * np + 1] = g1;
+            }
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
 }
 
 }  // extern "C"
