@@ -47,6 +47,12 @@ class Stats(ctypes.Structure):
                 ("round_ms", ctypes.c_double * MAX_LAYERS), ("hist_ms", ctypes.c_double),
                 ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double)]
 
+class BatchField(ctypes.Structure):
+    _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("default_slope", ctypes.c_float),
+                ("pad", ctypes.c_float), ("max_err", ctypes.c_double), ("sq_err_sum", ctypes.c_double),
+                ("t_fallbacks", ctypes.c_uint64)]
+
+
 ASZ_OPT_KERNEL_EVENTS = 1
 ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
@@ -58,7 +64,7 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
            "asz_round_layout", "asz_set_option", "asz_prepare_round", "asz_round_energy",
-           "asz_round_energy_adjoint")
+           "asz_round_energy_adjoint", "asz_compress_batch")
 
 _lib = None
 
@@ -104,6 +110,8 @@ def lib():
                                    ctypes.c_int, _P, _P]
     L.asz_round_energy_adjoint.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_int,
                                            ctypes.c_int, _P, _P, _P]
+    L.asz_compress_batch.argtypes = [_P, ctypes.c_uint32, _P, _P, _P, ctypes.POINTER(Params), ctypes.c_int,
+                                     ctypes.c_uint32, _P, _P, _P]
     L.asz_free.argtypes = [_P]
     L.asz_free.restype = None
     _lib = L

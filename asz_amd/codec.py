@@ -199,6 +199,45 @@ class Context:
         return CompressResult(stream, float(mn), float(mx), st, qb, t, recon,
                               [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
+    def compress_batch(self, fields, plans, params: Params, tsel="grad", concurrency=4, on_device=False):
+        """Compress many device fields (CUDA float32 tensors, overwritten with
+        their reconstructions) in one call; plans: one Plan per field (or one
+        for all).  Returns (streams, per-field BatchField info): bytes objects,
+        or with on_device=True (device_pointer, nbytes) pairs the caller frees
+        with free_device()."""
+        n = len(fields)
+        prev = getattr(self, "_stream_dev", False)
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on_device else 0))
+        if isinstance(plans, Plan):
+            plans = [plans] * n
+        ptrs = (ctypes.c_void_p * n)(*[f.data_ptr() for f in fields])
+        dims = np.asarray([list(f.shape) for f in fields], np.uint64).reshape(-1)
+        pl = (Plan * n)(*plans)
+        outs = (ctypes.c_void_p * n)()
+        nbytes = (ctypes.c_uint64 * n)()
+        info = (_lib.BatchField * n)()
+        try:
+            self._check(self._L.asz_compress_batch(self._c, n, ptrs, _ptr(dims), pl, ctypes.byref(params),
+                                                   TSEL[tsel], concurrency, outs, nbytes, info))
+        finally:
+            self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if prev else 0)
+        streams = []
+        for i in range(n):
+            if on_device:
+                streams.append((outs[i], int(nbytes[i])))
+            else:
+                streams.append(ctypes.string_at(outs[i], nbytes[i]))
+                self._L.asz_free(outs[i])
+        return streams, list(info)
+
+    def free_device(self, ptr):
+        self._check(self._L.asz_device_free(self._c, ctypes.c_void_p(ptr)))
+
+    def device_bytes(self, ptr, n) -> bytes:
+        out = np.empty(n, np.uint8)
+        self._check(self._L.asz_copy_to_host(self._c, _ptr(out), ctypes.c_void_p(ptr), n))
+        return out.tobytes()
+
     # ------------------------------------------------------------ decompression
     def decompress(self, stream: bytes, dims, params: Params, want_qb=False, want_t=False, nt=None):
         d = np.asarray(dims, np.uint64)

@@ -91,7 +91,6 @@ struct PlanKey {
     uint64_t dims[3];
     asz_plan plan;
     uint64_t bpr, h1;
-    float aeb, pivot_eb_ratio, eb_ratio, slope_ratio, default_slope;
 };
 
 struct PlanCache {
@@ -122,6 +121,7 @@ struct asz_ctx {
     uint32_t* hres_dev = nullptr;
     uint64_t hres_arrays = 0;
     uint32_t epoch = 0;
+    std::vector<asz_ctx*> kids;  // batch workers: one context (and HIP stream) each
     asz::PlanCache pc;
     bool stream_on_device = false;
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
@@ -272,6 +272,8 @@ static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
     const float base = aeb * p.stat_eb_ratio;
     std::vector<std::vector<int64_t>> keys(3);
+    SlopeSampleDesc SD[3];
+    uint64_t off[4] = {0, 0, 0, 0};  // key offsets of the three axes in one buffer
     for (int d = 0; d < 3; d++) {
         SlopeSampleDesc S{};
         uint64_t lo[3] = {0, 0, 0}, hi[3] = {dims[0], dims[1], dims[2]}, st[3] = {1, 1, 1};
@@ -293,13 +295,16 @@ static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_
         }
         S.dist = dist;
         S.neg_2_stat_eb = 1.0f / base;
-        uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
-        int64_t* dk = c->keys.get<int64_t>(2 * ns + 2);
-        launch_slope_sample(S, d_field, dk, c->stream);
-        keys[d].resize(2 * ns);
-        HIPCHECK(hipMemcpyAsync(keys[d].data(), dk, 2 * ns * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        SD[d] = S;
+        off[d + 1] = off[d] + 2 * (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
     }
+    // the three axes' samples in one buffer, one copy back, one synchronisation
+    int64_t* dk = c->keys.get<int64_t>(off[3] + 2);
+    for (int d = 0; d < 3; d++) launch_slope_sample(SD[d], d_field, dk + off[d], c->stream);
+    int64_t* hk = c->pin_xfer.get<int64_t>(off[3] + 2);
+    HIPCHECK(hipMemcpyAsync(hk, dk, off[3] * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; d++) keys[d].assign(hk + off[d], hk + off[d + 1]);
     return default_slope_from_keys(keys, aeb, p.stat_eb_ratio, p.default_slope_eb_span,
                                    p.default_slope_invalid_ratio_threshold, p.default_slope_fallback, p.t_min, p.t_max,
                                    p.t_precision, mn, mx);
@@ -355,21 +360,16 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     validate_params(p);
     if (tsel_mode < ASZ_TSEL_GPU || tsel_mode > ASZ_TSEL_GRAD) fail(ASZ_ERR_ARG, "unknown t selection mode");
     hipStream_t st = c->stream;
-    HIPCHECK(hipStreamSynchronize(c->side));  // no copy of an earlier call is still in flight
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
 
-    // plan and its device tables, reused while the inputs that shape them repeat
+    // plan and its device tables, reused while dims, plan, block pivot ratio and
+    // sampling count repeat (the error bounds and slopes are set per call)
     PlanKey key;
     std::memset(&key, 0, sizeof(key));
     for (int a = 0; a < 3; a++) key.dims[a] = dims[a];
     key.plan = plan;
     key.bpr = p.block_pivot_ratio;
     key.h1 = p.h1_block_sampling_num;
-    key.aeb = aeb;
-    key.pivot_eb_ratio = p.pivot_eb_ratio;
-    key.eb_ratio = p.eb_ratio;
-    key.slope_ratio = p.slope_ratio;
-    key.default_slope = p.default_slope;
     PlanCache& pc = c->pc;
     if (!pc.valid || std::memcmp(&key, &pc.key, sizeof(key)) != 0) {
         pc.valid = false;
@@ -390,6 +390,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         pc.key = key;
         pc.valid = true;
     }
+    set_plan_bounds(pc.P, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope);
     const Plan& P = pc.P;
     const Geometry& g = P.g;
     uint8_t* dm = static_cast<uint8_t*>(c->cmeta.p);
@@ -484,7 +485,10 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
                     if (*flag != epoch) fail(ASZ_ERR_HIP, "histogram results never arrived from the GPU");
                 }
             }
-            _mm_pause();
+            // spin briefly, then give the core away (batches run many of these waits at once)
+            if (spins < 4096) _mm_pause();
+            else if (spins < 16384) std::this_thread::yield();
+            else std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         std::atomic_thread_fence(std::memory_order_acquire);
     };
@@ -506,6 +510,9 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         // exact range on the side stream (its symbols are final: the flag is up)
         std::lock_guard<std::mutex> lk(side_mu);
         HIPCHECK(hipSetDevice(c->device));
+        // created on first use, so that a context normally owns one HIP stream
+        // (several contexts then spread over the device's hardware queues)
+        if (!c->side) HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HuffArrayDev D{};
         D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
         D.count = A.count;
@@ -642,7 +649,6 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     }
     HIPCHECK(hipEventRecord(c->ev[1], st));
     enqueued = true;
-    const double t_enq = ms_since(wall0);
     joiner.armed = false;
     for (auto& t : workers) t.join();
     for (auto& e : werr)
@@ -664,7 +670,6 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             S.sq_err_sum += tail.sq;
         }
     }
-    if (getenv("ASZ_HOST_TRACE")) fprintf(stderr, "[trace] enqueue done %.3f ms, codebooks done %.3f ms\n", t_enq, ms_since(wall0));
     auto hh0 = Clock::now();
 
     // codebooks and stream layout (compress.tpp:243-291).  Only the header
@@ -1032,7 +1037,6 @@ int asz_ctx_create(int device, asz_ctx** out) {
         HIPCHECK(hipSetDevice(device));
         c->device = device;
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
     });
     if (rc != ASZ_OK) {
@@ -1047,6 +1051,8 @@ int asz_ctx_create(int device, asz_ctx** out) {
 
 void asz_ctx_destroy(asz_ctx* c) {
     if (!c) return;
+    for (asz_ctx* k : c->kids) asz_ctx_destroy(k);
+    c->kids.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -1228,6 +1234,90 @@ int asz_round_energy_adjoint(asz_ctx* c, const float* d_work, const uint64_t dim
         return ASZ_ERR_ARG;
     }
     return round_energy(c, d_work, dims, plan, p, round, quintic, d_theta, d_gE, nullptr, d_gtheta);
+}
+
+int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, const uint64_t* dims,
+                       const asz_plan* plans, const asz_params* p, int tsel_mode, uint32_t concurrency,
+                       uint8_t** streams, uint64_t* stream_bytes, asz_batch_field* info) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (nfields == 0) return;
+        if (tsel_mode == ASZ_TSEL_INJECT) fail(ASZ_ERR_ARG, "t injection is not available in batches");
+        for (uint32_t i = 0; i < nfields; i++) streams[i] = nullptr;
+        const uint32_t T = std::max<uint32_t>(1, std::min<uint32_t>(concurrency ? concurrency : 4, nfields));
+        while (c->kids.size() < T) {
+            asz_ctx* k = nullptr;
+            const int rc = asz_ctx_create(c->device, &k);
+            if (rc != ASZ_OK) fail(rc, "cannot create a batch worker context");
+            c->kids.push_back(k);
+        }
+        const bool on_device = c->stream_on_device;
+        std::atomic<uint32_t> next{0};
+        std::mutex err_mu;
+        std::string first_err;
+        int first_code = ASZ_OK;
+        auto worker = [&](asz_ctx* k) {
+            (void)hipSetDevice(k->device);
+            for (;;) {
+                const uint32_t i = next.fetch_add(1);
+                if (i >= nfields) return;
+                const int rc = guarded(k, [&] {
+                    const uint64_t* d = dims + 3 * (uint64_t)i;
+                    const uint64_t n = d[0] * d[1] * d[2];
+                    float mn, mx;
+                    asz::range_scan(k, d_fields[i], n, &mn, &mx);
+                    asz_params q = *p;
+                    asz::SlopeResult sr = asz::default_slope(k, d_fields[i], d, q, mn, mx);
+                    q.default_slope = sr.slope;
+                    k->stream_on_device = on_device;
+                    asz::compress_impl(k, d_fields[i], d, plans[i], q, mn, mx, tsel_mode, nullptr, nullptr, nullptr);
+                    uint8_t* out = nullptr;
+                    if (on_device) {
+                        // a device buffer of the stream's size, filled device to device
+                        if (hipMalloc(&out, k->out_bytes ? k->out_bytes : 1) != hipSuccess)
+                            fail(ASZ_ERR_NOMEM, "device allocation failed");
+                        HIPCHECK(hipMemcpyAsync(out, k->out_stream, k->out_bytes, hipMemcpyDeviceToDevice, k->stream));
+                        HIPCHECK(hipStreamSynchronize(k->stream));
+                    } else {
+                        out = static_cast<uint8_t*>(std::malloc(k->out_bytes ? k->out_bytes : 1));
+                        if (!out) fail(ASZ_ERR_NOMEM, "host allocation failed");
+                        std::memcpy(out, k->out_stream, k->out_bytes);
+                    }
+                    streams[i] = out;
+                    stream_bytes[i] = k->out_bytes;
+                    if (info) {
+                        info[i].min_val = mn;
+                        info[i].max_val = mx;
+                        info[i].default_slope = sr.slope;
+                        info[i].pad = 0;
+                        info[i].max_err = k->stats.max_err;
+                        info[i].sq_err_sum = k->stats.sq_err_sum;
+                        info[i].t_fallbacks = k->stats.t_fallbacks;
+                    }
+                });
+                if (rc != ASZ_OK) {
+                    std::lock_guard<std::mutex> lk(err_mu);
+                    if (first_code == ASZ_OK) {
+                        first_code = rc;
+                        first_err = "field " + std::to_string(i) + ": " + k->err;
+                    }
+                    next = nfields;  // stop handing out fields
+                    return;
+                }
+            }
+        };
+        std::vector<std::thread> ts;
+        for (uint32_t t = 0; t < T; t++) ts.emplace_back(worker, c->kids[t]);
+        for (auto& t : ts) t.join();
+        if (first_code != ASZ_OK) {
+            for (uint32_t i = 0; i < nfields; i++) {
+                if (on_device) (void)hipFree(streams[i]);
+                else std::free(streams[i]);
+                streams[i] = nullptr;
+            }
+            fail(first_code, first_err);
+        }
+    });
 }
 
 int asz_decompress(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,

@@ -155,6 +155,10 @@ struct Plan {
 };
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
                float default_slope, uint64_t h1_num);
+// The per-call scalars of a plan (pivot and layer error bounds, layer slopes);
+// the tables do not depend on them.
+void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
+                     float default_slope);
 
 // Accumulators written by the pivot / round kernels (device memory).
 struct ArrayAcc {

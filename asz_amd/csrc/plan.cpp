@@ -33,11 +33,26 @@ Geometry make_geometry(const uint64_t dims[3], const asz_plan& plan, uint64_t bp
     return g;
 }
 
+void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
+                     float default_slope) {
+    P.pivot_eb = daz(abs_eb) * daz(pivot_eb_ratio);  // compress.tpp:166
+    const int L = P.g.L;
+    for (int r = 0; r < L; r++) {
+        RoundDesc& R = P.rounds[r];
+        // layer eb and slope (compress.tpp:194-195): libm pow, product in double, rounded to float
+        const double f = std::pow((double)eb_ratio, (double)(L - r - 1));
+        R.eb = (float)((double)abs_eb * f);
+        R.layer_slope = (float)((double)default_slope * std::pow((double)slope_ratio, (double)(L - r - 1)));
+        R.inv_half_eb = 0.5 / (double)R.eb;
+        R.eb_d = (double)R.eb;
+        R.neg_two_eb = -(2.0 * (double)R.eb);
+    }
+}
+
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
                float default_slope, uint64_t h1_num) {
     Plan P;
     P.g = g;
-    P.pivot_eb = daz(abs_eb) * daz(pivot_eb_ratio);  // compress.tpp:166
     P.pivot_count = 1;
     for (int a = 0; a < 3; a++) {
         P.pn[a] = (uint32_t)((g.D[a] + g.pivot_stride[a] - 1) / g.pivot_stride[a]);
@@ -57,13 +72,6 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         uint64_t dist = s[d] / 2;
         for (int a = d + 1; a < 3; a++) dist *= g.D[a];
         R.dist = dist;
-        // layer eb and slope (compress.tpp:194-195): libm pow, product in double, rounded to float
-        double f = std::pow((double)eb_ratio, (double)(g.L - r - 1));
-        R.eb = (float)((double)abs_eb * f);
-        R.layer_slope = (float)((double)default_slope * std::pow((double)slope_ratio, (double)(g.L - r - 1)));
-        R.inv_half_eb = 0.5 / (double)R.eb;
-        R.eb_d = (double)R.eb;
-        R.neg_two_eb = -(2.0 * (double)R.eb);
         uint64_t count = 1;
         for (int a = 0; a < 3; a++) {
             R.D[a] = (uint32_t)g.D[a];
@@ -174,6 +182,7 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         s[d] /= 2;
     }
     if (qb_off != g.N()) fail(ASZ_ERR_DATA, "total_qb_num!=data.size()");
+    set_plan_bounds(P, abs_eb, pivot_eb_ratio, eb_ratio, slope_ratio, default_slope);
     return P;
 }
 

@@ -181,6 +181,29 @@ int asz_decompress(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, c
 int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, const uint64_t dims[3],
                         const asz_params *p, float *out);
 
+/* ------------------------------------------------------------------ batches
+ * Many fields in one call (no reference counterpart: the reference compresses
+ * one field per s1_compress process, t1_start_compressors.py:482-542).  Each
+ * field goes through load_input's range scan, get_default_slope and
+ * param_c::compress as in asz_range_f32 + asz_default_slope + asz_compress,
+ * on `concurrency` worker contexts (0: 4), each with its own HIP stream and
+ * host thread, so the fields' pipelines overlap on the device.  d_fields[i]
+ * (device, dims[3i..3i+2]) is overwritten with its reconstruction; plans[i]
+ * is that field's plan (time steps typically share one).  On success
+ * streams[i] / stream_bytes[i] hold field i's pre-Zstd stream in host memory
+ * the caller releases with asz_free, or, with ASZ_OPT_STREAM_ON_DEVICE set on
+ * ctx, in device memory released with asz_device_free; info (may be NULL)
+ * receives per-field results.  tsel_mode: ASZ_TSEL_GPU, ASZ_TSEL_EXACT or
+ * ASZ_TSEL_GRAD. */
+typedef struct asz_batch_field {
+    float min_val, max_val, default_slope, pad;
+    double max_err, sq_err_sum;
+    uint64_t t_fallbacks;
+} asz_batch_field;
+int asz_compress_batch(asz_ctx *ctx, uint32_t nfields, float *const *d_fields, const uint64_t *dims,
+                       const asz_plan *plans, const asz_params *p, int tsel_mode, uint32_t concurrency,
+                       uint8_t **streams, uint64_t *stream_bytes, asz_batch_field *info);
+
 /* ------------------------------------------------------------------ gradient-tuned form
  * No reference counterpart: these expose the residual-energy model behind
  * ASZ_TSEL_GRAD for tuning harnesses (the discrete scan they replace is
