@@ -53,6 +53,14 @@ class BatchField(ctypes.Structure):
                 ("t_fallbacks", ctypes.c_uint64)]
 
 
+class PlanReport(ctypes.Structure):
+    _fields_ = [("layers", ctypes.c_uint32), ("searched", ctypes.c_uint32),
+                ("best_dist", ctypes.c_float * (MAX_LAYERS + 1)), ("best_axis", ctypes.c_int8 * (MAX_LAYERS + 1)),
+                ("best_order", ctypes.c_int8 * (MAX_LAYERS + 1)), ("statistics_s", ctypes.c_double),
+                ("sample_prediction_s", ctypes.c_double), ("lossless_estimation_s", ctypes.c_double),
+                ("total_s", ctypes.c_double)]
+
+
 ASZ_OPT_KERNEL_EVENTS = 1
 ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
@@ -64,7 +72,7 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
            "asz_round_layout", "asz_set_option", "asz_prepare_round", "asz_round_energy",
-           "asz_round_energy_adjoint", "asz_compress_batch")
+           "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan")
 
 _lib = None
 
@@ -112,6 +120,9 @@ def lib():
                                            ctypes.c_int, _P, _P, _P]
     L.asz_compress_batch.argtypes = [_P, ctypes.c_uint32, _P, _P, _P, ctypes.POINTER(Params), ctypes.c_int,
                                      ctypes.c_uint32, _P, _P, _P]
+    L.asz_generate_plan.argtypes = [_P, _P, _P, ctypes.POINTER(Params), ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Plan),
+                                    ctypes.POINTER(PlanReport)]
     L.asz_free.argtypes = [_P]
     L.asz_free.restype = None
     _lib = L

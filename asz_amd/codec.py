@@ -137,6 +137,26 @@ class Context:
                                               ctypes.byref(reason)))
         return s.value, inv.value, bool(fb.value), ("none", "invalid_ratio", "invalid_difference")[reason.value]
 
+    def generate_plan(self, d_ptr: int, dims, params: Params, mn: float, mx: float, max_layer_num=14,
+                      full_predictor_search_layer_num=4, sample_stride=5):
+        """Automatic plan (generate_config.tpp:443-678) for the field at d_ptr;
+        params.default_slope must hold the field's default slope.  Returns the
+        plan and the search report: per searched round (finest first) the
+        winner's predictor name ("" when none had samples) and best_dist."""
+        d = np.asarray(dims, np.uint64)
+        plan, rep = Plan(), _lib.PlanReport()
+        self._check(self._L.asz_generate_plan(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(params), mn, mx,
+                                              max_layer_num, full_predictor_search_layer_num, sample_stride,
+                                              ctypes.byref(plan), ctypes.byref(rep)))
+        lines = []
+        for k in range(rep.searched):
+            a = rep.best_axis[k]
+            name = "" if a < 0 else f"cubic{a}_{ORDERS[rep.best_order[k]]}"
+            lines.append((name, float(rep.best_dist[k])))
+        return plan, {"searched": lines, "statistics_s": rep.statistics_s,
+                      "sample_prediction_s": rep.sample_prediction_s,
+                      "lossless_estimation_s": rep.lossless_estimation_s, "total_s": rep.total_s}
+
     def geometry(self, dims, plan: Plan, bpr: int):
         d = np.asarray(dims, np.uint64)
         ps, bs, bn = (np.zeros(3, np.uint64) for _ in range(3))

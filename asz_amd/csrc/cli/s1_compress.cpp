@@ -1,8 +1,9 @@
 // Drop-in s1_compress (reference: c_codes/s1_compress.cpp).  Same command
 // line, validation, stdout lines and .asz stream; the core runs on the GPU
 // through libasz_gpu.so.  Additional flags (not in the reference):
-//   --tsel gpu|exact   per-block t from the GPU scorer (default) or from GPU
-//                      samples scored on the host in the reference's order
+//   --tsel gpu|exact|grad  per-block t from the GPU scorer (default), from GPU
+//                      samples scored on the host in the reference's order, or
+//                      from the gradient-tuned form
 //   --device N         HIP device
 #include <chrono>
 #include <cstring>
@@ -164,8 +165,24 @@ int main(int argc, char* argv[]) try {
     printf("Get default slope time: %f s\n", secs(d0, d1));
     s.p.default_slope = slope;
     printf("default_slope=%f\n", slope);
-    if (s.plan.empty())
-        throw invalid_argument("automatic plan generation is not available in this build; pass --predictor-plan");
+    if (s.plan.empty()) {
+        // generate_config (generate_config.tpp:443-678) on the GPU
+        asz_plan gen{};
+        asz_plan_report rep{};
+        check(ctx, asz_generate_plan(ctx, d_work, dims, &s.p, mn, mx, (uint32_t)s.max_layer_num,
+                                     (uint32_t)s.full_predictor_search_layer_num,
+                                     (uint32_t)s.generate_config_sample_stride, &gen, &rep));
+        static const char* ord[3] = {"120", "021", "012"};
+        for (uint32_t k = 0; k < rep.searched; k++) {
+            string name = rep.best_axis[k] < 0 ? string()
+                                               : "cubic" + to_string(rep.best_axis[k]) + "_" + ord[rep.best_order[k]];
+            printf("best_layer_predictor=%s, best_dist=%f\n", name.c_str(), rep.best_dist[k]);
+        }
+        printf("Generate config breakdown: statistics=%f s,sample prediction=%f s,lossless estimation=%f s\n",
+               rep.statistics_s, rep.sample_prediction_s, rep.lossless_estimation_s);
+        printf("Generate config time: %f s\n", rep.total_s);
+        s.plan = from_plan(gen);
+    }
     if (s.plan.size() > 63) throw invalid_argument("predictor plan cannot contain more than 63 layers");
     asz_plan plan = to_plan(s.plan);
     if (s.auto_family)

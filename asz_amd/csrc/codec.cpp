@@ -34,106 +34,7 @@
 #include "launch_misc.hpp"
 #include "scoring.hpp"
 
-#define HIPCHECK(x)                                                                               \
-    do {                                                                                          \
-        hipError_t e_ = (x);                                                                      \
-        if (e_ != hipSuccess)                                                                     \
-            ::asz::fail(ASZ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
-    } while (0)
-
-namespace asz {
-
-// ------------------------------------------------------------------ device buffers
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    template <typename T>
-    T* get(size_t n) {
-        size_t bytes = std::max<size_t>(n * sizeof(T), 16);
-        if (bytes > cap) {
-            if (p) HIPCHECK(hipFree(p));
-            p = nullptr;
-            cap = 0;
-            hipError_t e = hipMalloc(&p, bytes);
-            if (e != hipSuccess) fail(ASZ_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
-            cap = bytes;
-        }
-        return static_cast<T*>(p);
-    }
-};
-
-// Page-locked host memory for the large host <-> device transfers.
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
-    template <typename T>
-    T* get(size_t n) {
-        size_t bytes = std::max<size_t>(n * sizeof(T), 64);
-        if (bytes > cap) {
-            if (p) HIPCHECK(hipHostFree(p));
-            p = nullptr;
-            cap = 0;
-            bytes = std::max(bytes, (size_t)1 << 20);
-            if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess)
-                fail(ASZ_ERR_NOMEM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
-            cap = bytes;
-        }
-        return static_cast<T*>(p);
-    }
-};
-
-// Inputs that shape a compression plan and its device tables.  Compared
-// bytewise (the struct is zeroed before it is filled).
-struct PlanKey {
-    uint64_t dims[3];
-    asz_plan plan;
-    uint64_t bpr, h1;
-};
-
-struct PlanCache {
-    bool valid = false;
-    PlanKey key;
-    Plan P;
-    std::vector<FoldDesc> folds;   // per round (make_fold_descs)
-    uint64_t fpart_total = 0;      // fold scratch Partials over all rounds
-    struct {
-        uint64_t tabs, ss, segs, folds, bytes;
-    } up{};
-};
-
-}  // namespace asz
-
-struct asz_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    asz_stats stats{};
-    asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
-        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, cmeta, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream;
-    hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
-    // host result slots (pinned, written by the GPU): kHostSlotWords per array
-    uint32_t* hres_host = nullptr;
-    uint32_t* hres_dev = nullptr;
-    uint64_t hres_arrays = 0;
-    uint32_t epoch = 0;
-    std::vector<asz_ctx*> kids;  // batch workers: one context (and HIP stream) each
-    asz::PlanCache pc;
-    bool stream_on_device = false;
-    uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
-    uint64_t out_bytes = 0;
-    hipEvent_t ev[8];
-    // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
-    bool kev_on = false;
-    int kev_round = -1;  // >= 0: events around that round's kernel only (ASZ_OPT_TIMED_ROUND)
-    std::vector<hipEvent_t> kev;
-    std::vector<std::pair<int, int>> kev_tag;
-    int kev_n = 0;
-};
+#include "context.hpp"
 
 namespace asz {
 
@@ -213,10 +114,6 @@ void check_grid(const RoundDesc& R) {
     if (R.n[0] > 65535 || R.n[1] > 65535) fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
 }
 
-float abs_eb_of(float rel_eb, float mn, float mx) {
-    float range = daz(daz(mx) - daz(mn));
-    return daz(daz(rel_eb) * range);
-}
 
 // Layout of the Huffman-coded arrays: array 0 = t array (int64 semantics),
 // arrays 1..L+1 = pivot array and round arrays.
@@ -989,23 +886,6 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
 // ================================================================== C ABI
 
 using asz::fail;
-
-template <typename F>
-static int guarded(asz_ctx* c, F&& f) {
-    try {
-        f();
-        return ASZ_OK;
-    } catch (const asz::Error& e) {
-        if (c) c->err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (c) c->err = "host allocation failed";
-        return ASZ_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        if (c) c->err = e.what();
-        return ASZ_ERR_DATA;
-    }
-}
 
 extern "C" {
 

@@ -204,6 +204,33 @@ int asz_compress_batch(asz_ctx *ctx, uint32_t nfields, float *const *d_fields, c
                        const asz_plan *plans, const asz_params *p, int tsel_mode, uint32_t concurrency,
                        uint8_t **streams, uint64_t *stream_bytes, asz_batch_field *info);
 
+/* ------------------------------------------------------------------ plan generation
+ * Replaces param_c::generate_config (generate_config.tpp:443-678), which the
+ * reference runs when s1_compress gets no --predictor-plan
+ * (s1_compress.cpp:33-36).  Reverse greedy search from the finest round: per
+ * round, every candidate predictor's best t from the original field's (A, B)
+ * key map, an open-loop sample compression of the round (sample stride
+ * generate_config_sample_stride on two axes), the samples' Huffman + Zstd
+ * size, and the largest psnr - slope * br wins.  Uses p->rel_eb,
+ * eb_ratio, slope_ratio, stat_eb_ratio, h1_block_sampling_num, t_min,
+ * t_max, t_precision and default_slope (from asz_default_slope).  d_field is
+ * the original field in device memory (read only).  The report lists, in
+ * search order (finest first), each searched round's winner (axis -1 when no
+ * candidate had samples, which ends the search) and best_dist, for the
+ * reference's "best_layer_predictor=..." lines, and the phase times of its
+ * "Generate config breakdown" line. */
+typedef struct asz_plan_report {
+    uint32_t layers;                        /* rounds in the plan */
+    uint32_t searched;                      /* rounds searched (layers, plus one if the search ran dry) */
+    float best_dist[ASZ_MAX_LAYERS + 1];
+    int8_t best_axis[ASZ_MAX_LAYERS + 1];
+    int8_t best_order[ASZ_MAX_LAYERS + 1];
+    double statistics_s, sample_prediction_s, lossless_estimation_s, total_s;
+} asz_plan_report;
+int asz_generate_plan(asz_ctx *ctx, const float *d_field, const uint64_t dims[3], const asz_params *p, float min_val,
+                      float max_val, uint32_t max_layer_num, uint32_t full_predictor_search_layer_num,
+                      uint32_t generate_config_sample_stride, asz_plan *plan, asz_plan_report *report);
+
 /* ------------------------------------------------------------------ gradient-tuned form
  * No reference counterpart: these expose the residual-energy model behind
  * ASZ_TSEL_GRAD for tuning harnesses (the discrete scan they replace is

@@ -118,7 +118,7 @@ def main():
                 "asz_bytes": os.path.getsize(asz),
                 "compress_stdout": [l for l in cout.splitlines() if re.match(
                     r"^(Default slope statistics|predictor_vec|predictor_family=|total_qb_num|t_int_vec|T histogram"
-                    r"|T fallback|max_err|CR:|min_val)", l)],
+                    r"|T fallback|max_err|CR:|min_val|best_layer_predictor=|Predictor family selection: )", l)],
                 "decompress_cr": float(re.search(r"^CR= ([0-9.eE+-]+)$", dout, re.M).group(1)),
             }
             if recon.nbytes <= 65536:
