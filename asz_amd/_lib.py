@@ -61,6 +61,23 @@ class PlanReport(ctypes.Structure):
                 ("total_s", ctypes.c_double)]
 
 
+class FamilyOpts(ctypes.Structure):
+    _fields_ = [("suffix_layers", ctypes.c_uint64), ("sample_blocks", ctypes.c_uint64),
+                ("sample_point_limit", ctypes.c_uint64), ("sample_fraction_limit", ctypes.c_float),
+                ("fine_eb_threshold", ctypes.c_float), ("coarse_probe_bpr", ctypes.c_uint64),
+                ("fine_probe_bpr", ctypes.c_uint64)]
+
+
+class FamilyReport(ctypes.Structure):
+    _fields_ = [("suffix_layers", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("sampled_block_points", ctypes.c_uint64), ("sample_point_budget", ctypes.c_uint64),
+                ("probe_block_pivot_ratio", ctypes.c_uint64), ("points", ctypes.c_uint64),
+                ("cubic_mse", ctypes.c_longdouble), ("quintic_mse", ctypes.c_longdouble),
+                ("cubic_psnr", ctypes.c_longdouble), ("quintic_psnr", ctypes.c_longdouble),
+                ("cubic_h1", ctypes.c_longdouble), ("quintic_h1", ctypes.c_longdouble),
+                ("score_gain", ctypes.c_longdouble), ("quintic", ctypes.c_int32), ("time_s", ctypes.c_double)]
+
+
 ASZ_OPT_KERNEL_EVENTS = 1
 ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
@@ -72,7 +89,8 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
            "asz_round_layout", "asz_set_option", "asz_prepare_round", "asz_round_energy",
-           "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan")
+           "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
+           "asz_select_family")
 
 _lib = None
 
@@ -123,6 +141,8 @@ def lib():
     L.asz_generate_plan.argtypes = [_P, _P, _P, ctypes.POINTER(Params), ctypes.c_float, ctypes.c_float,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Plan),
                                     ctypes.POINTER(PlanReport)]
+    L.asz_select_family.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_float,
+                                    ctypes.c_float, ctypes.POINTER(FamilyOpts), ctypes.POINTER(FamilyReport)]
     L.asz_free.argtypes = [_P]
     L.asz_free.restype = None
     _lib = L

@@ -157,6 +157,22 @@ class Context:
                       "sample_prediction_s": rep.sample_prediction_s,
                       "lossless_estimation_s": rep.lossless_estimation_s, "total_s": rep.total_s}
 
+    def select_family(self, d_ptr: int, dims, plan: Plan, params: Params, mn: float, mx: float, suffix_layers=6,
+                      sample_blocks=8, sample_point_limit=32768, sample_fraction_limit=0.02, fine_eb_threshold=1e-4,
+                      coarse_probe_bpr=3, fine_probe_bpr=4):
+        """Predictor family of the plan for the field at d_ptr
+        (predictor_family.tpp:217-251); params.default_slope must hold the
+        field's default slope.  Returns the report (a FamilyReport: the
+        chosen family in .quintic and every value of the reference's
+        "Predictor family selection" line)."""
+        d = np.asarray(dims, np.uint64)
+        o = _lib.FamilyOpts(suffix_layers, sample_blocks, sample_point_limit, float(np.float32(sample_fraction_limit)),
+                            float(np.float32(fine_eb_threshold)), coarse_probe_bpr, fine_probe_bpr)
+        rep = _lib.FamilyReport()
+        self._check(self._L.asz_select_family(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
+                                              ctypes.byref(params), mn, mx, ctypes.byref(o), ctypes.byref(rep)))
+        return rep
+
     def geometry(self, dims, plan: Plan, bpr: int):
         d = np.asarray(dims, np.uint64)
         ps, bs, bn = (np.zeros(3, np.uint64) for _ in range(3))
@@ -277,6 +293,28 @@ class Context:
                                            ctypes.c_void_p(dev.data_ptr()), _ptr(qb), _ptr(t)))
         out[...] = dev.cpu().numpy().reshape(out.shape)
         return out, qb, t
+
+
+def family_line(rep) -> str:
+    """The reference's "Predictor family selection: ..." line for a family
+    report (predictor_family.tpp:237-247), long doubles printed by libc."""
+    libc = ctypes.CDLL(None)
+
+    def ld(name):
+        v = ctypes.c_longdouble.from_buffer(rep, getattr(_lib.FamilyReport, name).offset)
+        buf = ctypes.create_string_buffer(64)
+        libc.snprintf(buf, 64, b"%.12Le", v)
+        return buf.value.decode()
+
+    head = (f"Predictor family selection: selector=h1, suffix_layers={rep.suffix_layers}, blocks={rep.blocks}, "
+            f"sampled_block_points={rep.sampled_block_points}, sample_point_budget={rep.sample_point_budget}, "
+            f"probe_block_pivot_ratio={rep.probe_block_pivot_ratio}")
+    if rep.blocks == 0:
+        return head + ", fallback=point_budget, family=cubic"
+    return (head + f", points={rep.points}, cubic_mse={ld('cubic_mse')}, quintic_mse={ld('quintic_mse')}, "
+            f"cubic_psnr={ld('cubic_psnr')}, quintic_psnr={ld('quintic_psnr')}, cubic_h1={ld('cubic_h1')}, "
+            f"quintic_h1={ld('quintic_h1')}, six_score_gain={ld('score_gain')}, "
+            f"family={'quintic' if rep.quintic else 'cubic'}")
 
 
 def huffman_codebook(hist, offset: int, count: int, is_int64=False):

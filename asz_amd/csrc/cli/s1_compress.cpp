@@ -185,8 +185,35 @@ int main(int argc, char* argv[]) try {
     }
     if (s.plan.size() > 63) throw invalid_argument("predictor plan cannot contain more than 63 layers");
     asz_plan plan = to_plan(s.plan);
-    if (s.auto_family)
-        throw invalid_argument("automatic predictor family selection is not available in this build; pass --predictor-family");
+    if (s.auto_family) {
+        // select_predictor_family (predictor_family.tpp:217-251) on the GPU
+        asz_family_opts fo{};
+        fo.suffix_layers = s.pf_suffix_layers;
+        fo.sample_blocks = s.pf_sample_blocks;
+        fo.sample_point_limit = s.pf_sample_point_limit;
+        fo.sample_fraction_limit = s.pf_sample_fraction_limit;
+        fo.fine_eb_threshold = s.pf_fine_eb_threshold;
+        fo.coarse_probe_bpr = s.pf_coarse_bpr;
+        fo.fine_probe_bpr = s.pf_fine_bpr;
+        asz_family_report fr{};
+        check(ctx, asz_select_family(ctx, d_work, dims, &plan, &s.p, mn, mx, &fo, &fr));
+        if (fr.blocks == 0)
+            printf("Predictor family selection: selector=h1, suffix_layers=%lu, blocks=0, sampled_block_points=0, "
+                   "sample_point_budget=%lu, probe_block_pivot_ratio=%lu, fallback=point_budget, family=cubic\n",
+                   (unsigned long)fr.suffix_layers, (unsigned long)fr.sample_point_budget,
+                   (unsigned long)fr.probe_block_pivot_ratio);
+        else
+            printf("Predictor family selection: selector=h1, suffix_layers=%lu, blocks=%lu, sampled_block_points=%lu, "
+                   "sample_point_budget=%lu, probe_block_pivot_ratio=%lu, points=%lu, cubic_mse=%.12Le, "
+                   "quintic_mse=%.12Le, cubic_psnr=%.12Le, quintic_psnr=%.12Le, cubic_h1=%.12Le, quintic_h1=%.12Le, "
+                   "six_score_gain=%.12Le, family=%s\n",
+                   (unsigned long)fr.suffix_layers, (unsigned long)fr.blocks, (unsigned long)fr.sampled_block_points,
+                   (unsigned long)fr.sample_point_budget, (unsigned long)fr.probe_block_pivot_ratio,
+                   (unsigned long)fr.points, fr.cubic_mse, fr.quintic_mse, fr.cubic_psnr, fr.quintic_psnr, fr.cubic_h1,
+                   fr.quintic_h1, fr.score_gain, fr.quintic ? "quintic" : "cubic");
+        printf("Predictor family selection time: %f s\n", fr.time_s);
+        s.p.quintic = fr.quintic;
+    }
     uint64_t ps[3], bs[3], bn[3];
     check(ctx, asz_geometry(dims, &plan, s.p.block_pivot_ratio, ps, bs, bn));
     print_config(s, s.plan, ps, bs, bn);

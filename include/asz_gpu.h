@@ -231,6 +231,42 @@ int asz_generate_plan(asz_ctx *ctx, const float *d_field, const uint64_t dims[3]
                       float max_val, uint32_t max_layer_num, uint32_t full_predictor_search_layer_num,
                       uint32_t generate_config_sample_stride, asz_plan *plan, asz_plan_report *report);
 
+/* ------------------------------------------------------------------ predictor family
+ * Replaces param_c::select_predictor_family (predictor_family.tpp:217-251),
+ * which the reference runs when s1_compress gets no --predictor-family
+ * (s1_compress.cpp:44-45).  The last suffix_layers rounds of the plan at the
+ * probe block pivot ratio (fine_probe_bpr when p->rel_eb <=
+ * fine_eb_threshold, else coarse_probe_bpr); up to sample_blocks stratified
+ * blocks (mt19937_64 seeded 0) within min(sample_point_limit,
+ * floor(N * sample_fraction_limit)) points; each compressed in its own
+ * context with the cubic and with the quintic interior predictor; quintic
+ * wins when (quintic_psnr - cubic_psnr) - default_slope (quintic_h1 -
+ * cubic_h1) > 0.  Uses p->rel_eb, pivot_eb_ratio, eb_ratio, slope_ratio,
+ * stat_eb_ratio, default_slope; the probe's t sampling uses 256 samples per
+ * block whatever p->h1_block_sampling_num is (the reference's probe keeps
+ * that parameter at its default).  The report carries every value of the
+ * reference's "Predictor family selection: ..." line (blocks == 0: the
+ * point-budget fallback to cubic). */
+typedef struct asz_family_opts {
+    uint64_t suffix_layers;           /* predictor_family_suffix_layers (6) */
+    uint64_t sample_blocks;           /* predictor_family_sample_blocks (8) */
+    uint64_t sample_point_limit;      /* predictor_family_sample_point_limit (32768) */
+    float sample_fraction_limit;      /* predictor_family_sample_fraction_limit (0.02) */
+    float fine_eb_threshold;          /* predictor_family_fine_eb_threshold (1e-4) */
+    uint64_t coarse_probe_bpr;        /* predictor_family_coarse_probe_block_pivot_ratio (3) */
+    uint64_t fine_probe_bpr;          /* predictor_family_fine_probe_block_pivot_ratio (4) */
+} asz_family_opts;
+typedef struct asz_family_report {
+    uint32_t suffix_layers, blocks;
+    uint64_t sampled_block_points, sample_point_budget, probe_block_pivot_ratio, points;
+    long double cubic_mse, quintic_mse, cubic_psnr, quintic_psnr, cubic_h1, quintic_h1, score_gain;
+    int32_t quintic;                  /* the chosen family: 1 quintic, 0 cubic */
+    double time_s;
+} asz_family_report;
+int asz_select_family(asz_ctx *ctx, const float *d_field, const uint64_t dims[3], const asz_plan *plan,
+                      const asz_params *p, float min_val, float max_val, const asz_family_opts *opts,
+                      asz_family_report *report);
+
 /* ------------------------------------------------------------------ gradient-tuned form
  * No reference counterpart: these expose the residual-energy model behind
  * ASZ_TSEL_GRAD for tuning harnesses (the discrete scan they replace is
