@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -13,6 +14,7 @@
 
 #include "internal.hpp"
 #include "launch.hpp"
+#include "pool.hpp"
 
 #define HIPCHECK(x)                                                                               \
     do {                                                                                          \
@@ -113,9 +115,15 @@ struct asz_ctx {
     std::vector<hipEvent_t> kev;
     std::vector<std::pair<int, int>> kev_tag;
     int kev_n = 0;
+    std::unique_ptr<asz::TaskPool> pool;  // host workers of plan generation / family selection (lazy)
 };
 
 namespace asz {
+
+inline TaskPool& host_pool(asz_ctx* c) {
+    if (!c->pool) c->pool.reset(new TaskPool(TaskPool::default_workers()));
+    return *c->pool;
+}
 
 inline float abs_eb_of(float rel_eb, float mn, float mx) {
     float range = daz(daz(mx) - daz(mn));

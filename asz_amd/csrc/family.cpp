@@ -177,8 +177,6 @@ void select_family(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
     HIPCHECK(hipMemcpyAsync(dmeta, boxes.data(), J * sizeof(ProbeBox), hipMemcpyHostToDevice, st));
     launch_probe_init(reinterpret_cast<const ProbeBox*>(dmeta), J, max_box, d_field, orig, recon, st);
     HIPCHECK(hipGetLastError());
-    // the box descriptors must stay valid until the kernel has read them
-    HIPCHECK(hipStreamSynchronize(st));
 
     // output slots of the target blocks: QBs and records per (job, round)
     uint64_t stride0[3] = {g.pivot_stride[0], g.pivot_stride[1], g.pivot_stride[2]};
@@ -207,19 +205,26 @@ void select_family(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
     int32_t* dqb = c->pg_qb.get<int32_t>(qtotal + 64);
     float* drec = c->scratch.get<float>(3 * qtotal + 64);
 
-    std::vector<ProbeKeys> kunits;
-    std::vector<ProbeUnit> units;
+    // every round's block descriptors up front (they depend on geometry
+    // only); per round the keys are sampled, the t chosen on the host workers
+    // and written into that round's own descriptor slice, so a round's
+    // upload never overwrites descriptors a launched kernel still reads
+    struct RoundSet {
+        std::vector<ProbeKeys> keys;
+        std::vector<ProbeUnit> units;
+        std::vector<uint64_t> nkeys;  // samples per unit (0: t = 1/4 without sampling)
+        uint64_t key_total = 0;
+        float layer_eb, slope, stat_eb;
+    };
+    std::vector<RoundSet> rs(S);
     uint64_t s[3] = {stride0[0], stride0[1], stride0[2]};
     for (uint32_t r = 0; r < S; r++) {
+        RoundSet& R = rs[r];
         const int d = pp.axis[r];
         const int L1 = (int)S - (int)r - 1;
-        const float layer_eb = (float)((double)aeb * std::pow((double)p.eb_ratio, (double)L1));
-        const float slope = (float)((double)p.default_slope * std::pow((double)p.slope_ratio, (double)L1));
-        const float stat_eb = daz(layer_eb * p.stat_eb_ratio);
-        kunits.clear();
-        units.clear();
-        std::vector<uint64_t> nkeys;  // samples per unit (0: fallback t)
-        uint64_t key_total = 0;
+        R.layer_eb = (float)((double)aeb * std::pow((double)p.eb_ratio, (double)L1));
+        R.slope = (float)((double)p.default_slope * std::pow((double)p.slope_ratio, (double)L1));
+        R.stat_eb = daz(R.layer_eb * p.stat_eb_ratio);
         for (uint32_t j = 0; j < J; j++) {
             const Job& jb = jobs[j];
             for (uint64_t b0 = jb.cbb[0]; b0 <= jb.tb[0]; b0++)
@@ -262,13 +267,13 @@ void select_family(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
                                     ns *= K.cnt[a];
                                 }
                                 K.dist = dist;
-                                K.key_off = key_total;
-                                K.neg2 = 0.5f / (p.stat_eb_ratio * layer_eb);
-                                key_total += ns;
-                                kunits.push_back(K);
+                                K.key_off = R.key_total;
+                                K.neg2 = 0.5f / (p.stat_eb_ratio * R.layer_eb);
+                                R.key_total += ns;
+                                R.keys.push_back(K);
                             }
                         }
-                        nkeys.push_back(ns);
+                        R.nkeys.push_back(ns);
                         ProbeUnit U{};
                         U.base = boxes[j].base;
                         for (int a = 0; a < 3; a++) {
@@ -286,44 +291,54 @@ void select_family(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
                         U.diag = (b0 == jb.tb[0] && b1 == jb.tb[1] && b2 == jb.tb[2]) ? 1 : 0;
                         U.dist = dist;
                         U.qb_off = U.rec_off = qoff[j][r];
-                        U.eb_d = (double)layer_eb;
+                        U.eb_d = (double)R.layer_eb;
                         U.inv_half_eb = 0.5 / U.eb_d;
                         U.neg_two_eb = -(2.0 * U.eb_d);
-                        units.push_back(U);
+                        U.t = 0.25f;
+                        R.units.push_back(U);
                     }
         }
-        // keys -> host t (compress.tpp:117-143 with the reference's map order)
-        if (key_total) {
-            const uint64_t kb = al256(kunits.size() * sizeof(ProbeKeys));
-            uint8_t* hk = c->pin_meta.get<uint8_t>(kb);
-            std::memcpy(hk, kunits.data(), kunits.size() * sizeof(ProbeKeys));
-            uint8_t* dk = c->pg_meta.get<uint8_t>(kb);
-            int64_t* dkeys = c->keys.get<int64_t>(2 * key_total + 2);
-            HIPCHECK(hipMemcpyAsync(dk, hk, kunits.size() * sizeof(ProbeKeys), hipMemcpyHostToDevice, st));
-            launch_probe_keys(reinterpret_cast<const ProbeKeys*>(dk), (uint32_t)kunits.size(), recon, dkeys, st);
-            HIPCHECK(hipGetLastError());
-            int64_t* hkeys = c->pin_xfer.get<int64_t>(2 * key_total + 2);
-            HIPCHECK(hipMemcpyAsync(hkeys, dkeys, 2 * key_total * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
-            uint64_t off = 0;
-            for (size_t u = 0; u < units.size(); u++) {
-                if (!nkeys[u]) { units[u].t = 0.25f; continue; }
-                units[u].t = select_t_exact(hkeys + 2 * off, nkeys[u], layer_eb, stat_eb, slope, mn, mx);
-                off += nkeys[u];
-            }
-        } else {
-            for (auto& U : units) U.t = 0.25f;
-        }
-        const uint64_t ub = al256(units.size() * sizeof(ProbeUnit));
-        uint8_t* hu = c->pin_huff.get<uint8_t>(ub);
-        std::memcpy(hu, units.data(), units.size() * sizeof(ProbeUnit));
-        uint8_t* du = c->hmeta.get<uint8_t>(ub);
-        HIPCHECK(hipMemcpyAsync(du, hu, units.size() * sizeof(ProbeUnit), hipMemcpyHostToDevice, st));
-        launch_probe_round(reinterpret_cast<const ProbeUnit*>(du), (uint32_t)units.size(), orig, recon, dqb, drec, st);
-        HIPCHECK(hipGetLastError());
-        // the unit descriptors are overwritten by the next round's upload
-        HIPCHECK(hipStreamSynchronize(st));
         s[d] /= 2;
+    }
+    // descriptor slices: keys descriptors then unit descriptors, per round
+    std::vector<uint64_t> koff(S + 1, 0), uoff(S + 1, 0);
+    uint64_t key_max = 0;
+    for (uint32_t r = 0; r < S; r++) {
+        koff[r + 1] = koff[r] + al256(rs[r].keys.size() * sizeof(ProbeKeys));
+        key_max = std::max(key_max, rs[r].key_total);
+    }
+    uoff[0] = koff[S];
+    for (uint32_t r = 0; r < S; r++) uoff[r + 1] = uoff[r] + al256(rs[r].units.size() * sizeof(ProbeUnit));
+    uint8_t* hdesc = c->pin_huff.get<uint8_t>(uoff[S] + 256);
+    uint8_t* ddesc = c->hmeta.get<uint8_t>(uoff[S] + 256);
+    for (uint32_t r = 0; r < S; r++)
+        std::memcpy(hdesc + koff[r], rs[r].keys.data(), rs[r].keys.size() * sizeof(ProbeKeys));
+    HIPCHECK(hipMemcpyAsync(ddesc, hdesc, koff[S], hipMemcpyHostToDevice, st));
+    int64_t* dkeys = c->keys.get<int64_t>(2 * key_max + 2);
+    int64_t* hkeys = c->pin_xfer.get<int64_t>(2 * key_max + 2);
+    for (uint32_t r = 0; r < S; r++) {
+        RoundSet& R = rs[r];
+        // keys -> host t (compress.tpp:117-143 with the reference's map order)
+        if (R.key_total) {
+            launch_probe_keys(reinterpret_cast<const ProbeKeys*>(ddesc + koff[r]), (uint32_t)R.keys.size(), recon,
+                              dkeys, st);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(hkeys, dkeys, 2 * R.key_total * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            std::vector<uint64_t> off(R.units.size() + 1, 0);
+            for (size_t u = 0; u < R.units.size(); u++) off[u + 1] = off[u] + R.nkeys[u];
+            host_pool(c).run(R.units.size(), [&](size_t u) {
+                if (R.nkeys[u])
+                    R.units[u].t = select_t_exact(hkeys + 2 * off[u], R.nkeys[u], R.layer_eb, R.stat_eb, R.slope, mn,
+                                                  mx);
+            });
+        }
+        std::memcpy(hdesc + uoff[r], R.units.data(), R.units.size() * sizeof(ProbeUnit));
+        HIPCHECK(hipMemcpyAsync(ddesc + uoff[r], hdesc + uoff[r], R.units.size() * sizeof(ProbeUnit),
+                                hipMemcpyHostToDevice, st));
+        launch_probe_round(reinterpret_cast<const ProbeUnit*>(ddesc + uoff[r]), (uint32_t)R.units.size(), orig, recon,
+                           dqb, drec, st);
+        HIPCHECK(hipGetLastError());
     }
     std::vector<int32_t> hq(qtotal);
     std::vector<float> hr(3 * qtotal);

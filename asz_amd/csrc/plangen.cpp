@@ -24,7 +24,6 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
-#include <thread>
 
 #include "context.hpp"
 #include "launch_misc.hpp"
@@ -51,23 +50,19 @@ struct Candidate {
 constexpr Candidate kAll[9] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {1, 1}, {1, 2}, {2, 0}, {2, 1}, {2, 2}};
 constexpr Candidate kReduced[3] = {{0, 2}, {1, 2}, {2, 2}};
 
-// Best t of one axis from its key map (generate_config.tpp:527-548):
-// start from 1/4, keep a candidate whose fmaf(-slope, h1, psnr) is larger.
-float scan_t(const ABMap& m, float layer_eb, float stat_eb, float slope, float mn, float mx,
-             const std::vector<float>& cand_t) {
+// Best t of one axis (generate_config.tpp:527-548) from the (h1, psnr) of
+// t = 1/4 (sc[0]) and of each candidate_t (sc[1 + k]): keep a candidate
+// whose fmaf(-slope, h1, psnr) is larger.
+float pick_t(const std::vector<std::pair<float, float>>& sc, float slope, const std::vector<float>& cand_t) {
     const float initial = 4.0f / 16;
     float best_t = initial;
-    if (m.empty()) return best_t;
-    float h1, psnr;
-    h1_psnr(m, initial, layer_eb, stat_eb, mn, mx, h1, psnr);
-    float best = std::fmaf(-slope, h1, psnr);
-    for (float t : cand_t) {
-        if (t == initial) continue;
-        h1_psnr(m, t, layer_eb, stat_eb, mn, mx, h1, psnr);
-        const float dist = std::fmaf(-slope, h1, psnr);
+    float best = std::fmaf(-slope, sc[0].first, sc[0].second);
+    for (size_t k = 0; k < cand_t.size(); k++) {
+        if (cand_t[k] == initial) continue;
+        const float dist = std::fmaf(-slope, sc[1 + k].first, sc[1 + k].second);
         if (dist > best) {
             best = dist;
-            best_t = t;
+            best_t = cand_t[k];
         }
     }
     return best_t;
@@ -159,12 +154,27 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             int64_t* hk = c->pin_xfer.get<int64_t>(koff[3] + 2);
             HIPCHECK(hipMemcpyAsync(hk, dk, koff[3] * sizeof(int64_t), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
+            // the (h1, psnr) scores of every (axis, t) on the host workers
+            ABMap maps[3];
+            for (int d = 0; d < 3; d++)
+                if (has[d]) build_ab_map(hk + koff[d], (koff[d + 1] - koff[d]) / 2, maps[d]);
+            const size_t nt = cand_t.size() + 1;
+            std::vector<std::pair<float, float>> sc[3];
+            std::vector<std::pair<int, size_t>> tasks;
             for (int d = 0; d < 3; d++) {
-                if (!has[d]) continue;
-                ABMap m;
-                build_ab_map(hk + koff[d], (koff[d + 1] - koff[d]) / 2, m);
-                best_t[d] = scan_t(m, layer_eb, stat_eb, slope, mn, mx, cand_t);
+                if (!has[d] || maps[d].empty()) continue;
+                sc[d].resize(nt);
+                for (size_t k = 0; k < nt; k++)
+                    if (k == 0 || cand_t[k - 1] != 0.25f) tasks.push_back({d, k});
             }
+            host_pool(c).run(tasks.size(), [&](size_t i) {
+                const int d = tasks[i].first;
+                const size_t k = tasks[i].second;
+                const float t = k == 0 ? 0.25f : cand_t[k - 1];
+                h1_psnr(maps[d], t, layer_eb, stat_eb, mn, mx, sc[d][k].first, sc[d][k].second);
+            });
+            for (int d = 0; d < 3; d++)
+                if (!sc[d].empty()) best_t[d] = pick_t(sc[d], slope, cand_t);
         }
         stat_s += secs_since(ph);
 
@@ -265,8 +275,9 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             uint32_t* hh = c->pin_xfer.get<uint32_t>(hist_total + 1);
             HIPCHECK(hipMemcpyAsync(hh, dh, hist_total * 4, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
-            for (uint32_t k = 0; k < ncand; k++)
+            host_pool(c).run(ncand, [&](size_t k) {
                 if (S[k].count) build_codebook(A[k], hh + H[k].hist_off);
+            });
         }
         // every candidate's Huffman bytes (save + encode, huffman.hpp:322-381) back to back
         std::vector<uint64_t> boff(ncand + 1, 0);
@@ -345,17 +356,11 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             if (ZSTD_isError(r)) zerr[k] = "ZSTD_compress failed";
             else zlen[k] = r;
         };
-        {
-            std::vector<std::thread> th;
-            for (uint32_t k = 0; k < ncand; k++) {
-                if (!S[k].count) continue;
-                if (boff[k + 1] - boff[k] > (256u << 10)) th.emplace_back(zjob, k);
-                else zjob(k);
-            }
-            for (auto& t : th) t.join();
-            for (auto& e : zerr)
-                if (!e.empty()) fail(ASZ_ERR_DATA, e);
-        }
+        host_pool(c).run(ncand, [&](size_t k) {
+            if (S[k].count) zjob((uint32_t)k);
+        });
+        for (auto& e : zerr)
+            if (!e.empty()) fail(ASZ_ERR_DATA, e);
         lossless_s += secs_since(ph);
 
         // ---- 4. choose (generate_config.tpp:642-671)

@@ -96,9 +96,16 @@ __global__ __launch_bounds__(256) void k_plan_minmax(const PlanSampleDesc* __res
 // runs it while the workgroup stages the next tile of errors in LDS.
 constexpr int kChainTile = 4096;
 
+__device__ __forceinline__ float sq4(const float4 v, float s) {
+    s = __fmaf_rn(v.x, v.x, s);
+    s = __fmaf_rn(v.y, v.y, s);
+    s = __fmaf_rn(v.z, v.z, s);
+    return __fmaf_rn(v.w, v.w, s);
+}
+
 __global__ __launch_bounds__(256) void k_sq_chain(const PlanSampleDesc* __restrict__ descs,
                                                   const float* __restrict__ err, float* __restrict__ out) {
-    __shared__ float buf[2][kChainTile];
+    __shared__ __align__(16) float buf[2][kChainTile];
     const PlanSampleDesc S = descs[blockIdx.x];
     const float* e = err + S.off;
     const uint64_t n = S.count;
@@ -122,11 +129,19 @@ __global__ __launch_bounds__(256) void k_sq_chain(const PlanSampleDesc* __restri
         }
         if (threadIdx.x == 0) {
             const uint32_t m = (uint32_t)min<uint64_t>(kChainTile, n - t0);
-            const float* b = buf[cur];
-            for (uint32_t j = 0; j < m; j++) {
-                const float v = b[j];
-                s = __fmaf_rn(v, v, s);
+            // batches of 128 values: 32 LDS reads in flight, then the FMAs,
+            // so the read latency is paid once per batch
+            const float4* b4 = reinterpret_cast<const float4*>(buf[cur]);
+            const uint32_t m128 = m / 128;
+            for (uint32_t k = 0; k < m128; k++) {
+                float4 v[32];
+#pragma unroll
+                for (int u = 0; u < 32; u++) v[u] = b4[32 * k + u];
+#pragma unroll
+                for (int u = 0; u < 32; u++) s = sq4(v[u], s);
             }
+            const float* b = buf[cur];
+            for (uint32_t j = 128 * m128; j < m; j++) s = __fmaf_rn(b[j], b[j], s);
         }
 #pragma unroll
         for (int j = 0; j < kPer; j++) buf[cur ^ 1][j * 256 + threadIdx.x] = r[j];
