@@ -436,38 +436,35 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     kev_end(c, kp);
     launch_hist_small(nullptr, 0, qb, P.pivot_count, nullptr, gwin(1), st);
     launch_hist_tail(acc, 1, ppart, npp, gwin(1), c->hres_dev, epoch, st);
-    // the workers start once the GPU has work queued (thread start-up overlaps it)
+    // Codebook workers: the host pool's threads take the arrays in `order`
+    // as their results arrive, concurrently with the launches below and
+    // with the later rounds on the GPU.
     const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
-    const unsigned nworkers = stop_round >= 0 ? 0u : (unsigned)std::min<uint64_t>(order.size(), kCodebookThreads);
-    std::vector<std::exception_ptr> werr(nworkers);
-    host_ms_w.assign(nworkers, 0.0);
-    std::vector<std::thread> workers;
-    for (unsigned w = 0; w < nworkers; w++)
-        workers.emplace_back([&, w] {
-            try {
-                for (size_t i = w; i < order.size(); i += nworkers) {
-                    if (lay.arrays[order[i]].count == 0) continue;  // an empty round has no slot
-                    wait_slot(order[i]);
-                    auto h0 = Clock::now();
-                    build(order[i]);
-                    host_ms_w[w] += ms_since(h0);
-                }
-            } catch (...) {
-                werr[w] = std::current_exception();
-                abort_all = true;
-            }
+    host_ms_w.assign(order.size(), 0.0);
+    TaskPool& pool = host_pool(c);
+    if (stop_round < 0)
+        pool.start(order.size(), [&](size_t i) {
+            const uint64_t k = order[i];
+            if (lay.arrays[k].count == 0) return;  // an empty round has no slot
+            wait_slot(k);
+            auto h0 = Clock::now();
+            build(k);
+            host_ms_w[i] = ms_since(h0);
         });
-    // joins the workers on every exit path of the launch loop below
+    // stops and joins the workers on every exit path below
     struct Joiner {
-        std::vector<std::thread>& ts;
+        TaskPool& pool;
         std::atomic<bool>& ab;
         bool armed = true;
         ~Joiner() {
             if (!armed) return;
             ab = true;
-            for (auto& t : ts) if (t.joinable()) t.join();
+            try {
+                pool.wait();
+            } catch (...) {
+            }
         }
-    } joiner{workers, abort_all};
+    } joiner{pool, abort_all};
 
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
@@ -547,11 +544,10 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipEventRecord(c->ev[1], st));
     enqueued = true;
     joiner.armed = false;
-    for (auto& t : workers) t.join();
-    for (auto& e : werr)
-        if (e) std::rethrow_exception(e);
+    pool.wait();
     double host_ms = 0;
-    for (double v : host_ms_w) host_ms = std::max(host_ms, v);
+    for (double v : host_ms_w) host_ms += v;
+    host_ms /= std::max(1u, pool.workers());
     // errors and fallbacks from the slots
     S.max_err = 0;
     S.sq_err_sum = 0;
@@ -617,10 +613,10 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         D.payload_bit = spos * 8;
         spos += (A.payload_bits + 7) >> 3;
         int alen = 0;
-        for (uint64_t s = 0; s < A.range; s++) {
-            if (A.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
-            book[book_total + s] = A.code[s] | ((uint64_t)A.len[s] << 56);
-            alen = std::max<int>(alen, A.len[s]);
+        for (uint64_t s2 = 0; s2 < A.range; s2++) {
+            if (A.len[s2] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
+            book[book_total + s2] = A.code[s2] | ((uint64_t)A.len[s2] << 56);
+            alen = std::max<int>(alen, A.len[s2]);
         }
         D.lds_book = (A.range <= kLdsBook && alen <= 24) ? 1 : 0;
         lay.max_len = std::max(lay.max_len, alen);
@@ -629,7 +625,6 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     lay.total_bytes = spos;
     S.stream_bytes = lay.total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lay.arrays[k + 1].count;
-
     // one upload: header segments, header bytes, codebooks, array descriptors
     auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
     const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),

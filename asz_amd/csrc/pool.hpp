@@ -56,6 +56,40 @@ public:
         if (err_) std::rethrow_exception(err_);
     }
 
+    // Asynchronous form: the workers alone run f(0) .. f(n-1) (f is copied);
+    // wait() blocks until they are done and rethrows the first exception.
+    // No other batch may be started in between.
+    void start(size_t n, std::function<void(size_t)> f) {
+        owned_ = std::move(f);
+        if (th_.empty()) {
+            // no workers: run inline
+            for (size_t i = 0; i < n; i++) owned_(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &owned_;
+            n_ = n;
+            next_.store(0);
+            done_ = 0;
+            err_ = nullptr;
+            gen_++;
+        }
+        cv_.notify_all();
+    }
+    void wait() {
+        if (th_.empty()) return;
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return fn_ == nullptr || (done_ == n_ && active_ == 0); });
+        fn_ = nullptr;
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    unsigned workers() const { return (unsigned)th_.size(); }
+
     static unsigned default_workers() {
         const unsigned hc = std::thread::hardware_concurrency();
         return std::max(1u, std::min(hc ? hc : 1u, 8u) - 1);
@@ -101,6 +135,7 @@ private:
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     const std::function<void(size_t)>* fn_ = nullptr;
+    std::function<void(size_t)> owned_;
     size_t n_ = 0, done_ = 0, active_ = 0;
     std::atomic<size_t> next_{0};
     uint64_t gen_ = 0;
