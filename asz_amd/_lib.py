@@ -81,6 +81,7 @@ class FamilyReport(ctypes.Structure):
 ASZ_OPT_KERNEL_EVENTS = 1
 ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
+ASZ_OPT_BATCH_FIELDS = 4
 
 
 # exported entry points (must match include/asz_gpu.h)

@@ -1,12 +1,15 @@
 """Batched form: many fields of uneven shape in one call.
 
 compress_fields() runs every field's pipeline (range scan, default slope,
-rounds, Huffman) through asz_compress_batch: worker contexts, each with its
-own HIP stream and host thread, overlap the fields' launch-latency-bound
-coarse rounds and host codebook work on the device, and time steps share one
-saved plan.  The config-4 driver (python -m asz_amd.batch) compares it with
-the plain per-field form (one field after another on one stream, discrete t
-scan) on 48 time steps of 100x500x500 plus 12 mixed-shape fields at REL 1e-3.
+rounds, Huffman) through asz_compress_batch.  Fields of one shape and plan
+(time steps sharing a saved plan, same-shape fields) go through the
+multi-field kernels in chunks, one launch per round for the whole chunk, and
+worker contexts, each with its own HIP stream and host thread, run the chunks
+concurrently so that the launch-latency-bound coarse rounds and the host
+codebook work overlap on the device.  The config-4 driver (python -m
+asz_amd.batch) compares it with the plain per-field form (one field after
+another on one stream, discrete t scan) on 48 time steps of 100x500x500 plus
+12 mixed-shape fields at REL 1e-3.
 """
 from __future__ import annotations
 
@@ -45,13 +48,15 @@ def c4_fields(steps=48, seed=4, device="cuda"):
 
 
 def compress_fields(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, tsel="grad", concurrency=4,
-                    on_device=False):
+                    on_device=False, fields_per_chunk=0):
     """Batched form; returns (streams, per-field info, wall seconds).  With
-    on_device the streams stay in HBM as (pointer, bytes) pairs."""
+    on_device the streams stay in HBM as (pointer, bytes) pairs;
+    fields_per_chunk = 1 compresses one field at a time."""
     params = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=block_pivot_ratio, family="cubic")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    streams, info = ctx.compress_batch(flds, plans, params, tsel=tsel, concurrency=concurrency, on_device=on_device)
+    streams, info = ctx.compress_batch(flds, plans, params, tsel=tsel, concurrency=concurrency, on_device=on_device,
+                                       fields_per_chunk=fields_per_chunk)
     return streams, info, time.perf_counter() - t0
 
 
@@ -83,7 +88,9 @@ def config4_comparison(concurrency=4, reps=2):
     Zstd after the timed region) and max error over eb.  The plain form is
     the same C entry point with one worker (one field after another on one
     stream) and the discrete t scan, so the comparison carries no Python
-    overhead on either side; inputs and streams stay in HBM for both."""
+    overhead on either side; inputs and streams stay in HBM for both.
+    batched_per_field runs the batched configuration with one field per
+    chunk (concurrency alone, no multi-field kernels)."""
     flds = c4_fields()
     plans = c4_plans(flds)
     n_bytes = sum(f.numel() * 4 for f in flds)
@@ -91,11 +98,14 @@ def config4_comparison(concurrency=4, reps=2):
     try:
         res = {}
         for name, fn in (("plain", lambda w: compress_fields(ctx, w, plans, tsel="gpu", concurrency=1,
-                                                             on_device=True)),
+                                                             on_device=True, fields_per_chunk=1)),
                          ("batched_tuned", lambda w: compress_fields(ctx, w, plans, tsel="grad",
                                                                      concurrency=concurrency, on_device=True)),
                          ("batched_plain_t", lambda w: compress_fields(ctx, w, plans, tsel="gpu",
-                                                                       concurrency=concurrency, on_device=True))):
+                                                                       concurrency=concurrency, on_device=True)),
+                         ("batched_per_field", lambda w: compress_fields(ctx, w, plans, tsel="grad",
+                                                                         concurrency=concurrency, on_device=True,
+                                                                         fields_per_chunk=1))):
             best = None
             for _ in range(reps):
                 work = [f.clone() for f in flds]

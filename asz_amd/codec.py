@@ -235,14 +235,18 @@ class Context:
         return CompressResult(stream, float(mn), float(mx), st, qb, t, recon,
                               [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
-    def compress_batch(self, fields, plans, params: Params, tsel="grad", concurrency=4, on_device=False):
+    def compress_batch(self, fields, plans, params: Params, tsel="grad", concurrency=4, on_device=False,
+                       fields_per_chunk=0):
         """Compress many device fields (CUDA float32 tensors, overwritten with
         their reconstructions) in one call; plans: one Plan per field (or one
-        for all).  Returns (streams, per-field BatchField info): bytes objects,
-        or with on_device=True (device_pointer, nbytes) pairs the caller frees
-        with free_device()."""
+        for all).  Same-shape, same-plan fields go through the multi-field
+        kernels in chunks of at most fields_per_chunk (0: automatic, 1: one
+        field at a time).  Returns (streams, per-field BatchField info): bytes
+        objects, or with on_device=True (device_pointer, nbytes) pairs the
+        caller frees with free_device()."""
         n = len(fields)
         prev = getattr(self, "_stream_dev", False)
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_BATCH_FIELDS, int(fields_per_chunk)))
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on_device else 0))
         if isinstance(plans, Plan):
             plans = [plans] * n

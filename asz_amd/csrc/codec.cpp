@@ -110,8 +110,19 @@ PivotDesc pivot_desc(const Plan& P) {
     return D;
 }
 
-void check_grid(const RoundDesc& R) {
-    if (R.n[0] > 65535 || R.n[1] > 65535) fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
+// Largest per-axis block target counts of round R (the QB order move's grid).
+void round_cmax(const Plan& P, const RoundDesc& R, uint32_t cmax[3]) {
+    for (int a = 0; a < 3; a++) {
+        cmax[a] = 0;
+        for (uint32_t b = 0; b < R.nb[a]; b++) cmax[a] = std::max(cmax[a], P.tables[R.tab_off[a] + b]);
+    }
+}
+
+// The grid dimensions the round kernels use (sweep: y = a lattice axis, z =
+// segments x fields; row, t selection: y = fields) must fit the launch limits.
+void check_grid(const RoundDesc& R, uint32_t nf) {
+    if (R.n[0] > 65535 || R.n[1] > 65535 || (uint64_t)R.nseg * nf > 65535 || nf > 65535)
+        fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
 }
 
 
@@ -128,49 +139,60 @@ struct Layout {
 
 // ------------------------------------------------------------------ range
 
-static void range_scan(asz_ctx* c, const float* d_field, uint64_t n, float* mn, float* mx) {
-    uint32_t* r = c->misc.get<uint32_t>(8);
-    uint32_t init[3] = {0xFFFFFFFFu, 0u, 0u};
-    HIPCHECK(hipMemcpyAsync(r, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
-    launch_range(d_field, n, r, c->stream);
-    uint32_t res[3];
-    HIPCHECK(hipMemcpyAsync(res, r, sizeof(res), hipMemcpyDeviceToHost, c->stream));
+// Range scans of nf fields: one launch each, one copy back, one synchronisation.
+// names (may be null): the fields' indices, named in the error messages.
+static void range_scan_fields(asz_ctx* c, uint32_t nf, const float* const* d_fields, const uint64_t* n, float* mn,
+                              float* mx, const uint32_t* names = nullptr) {
+    uint32_t* r = c->misc.get<uint32_t>(4 * (uint64_t)nf + 4);
+    std::vector<uint32_t> init(4 * (uint64_t)nf, 0u);
+    for (uint32_t f = 0; f < nf; f++) init[4 * f] = 0xFFFFFFFFu;
+    HIPCHECK(hipMemcpyAsync(r, init.data(), init.size() * 4, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t f = 0; f < nf; f++) launch_range(d_fields[f], n[f], r + 4 * f, c->stream);
+    std::vector<uint32_t> res(4 * (uint64_t)nf);
+    HIPCHECK(hipMemcpyAsync(res.data(), r, res.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    if (res[2]) fail(ASZ_ERR_DATA, "input contains a nonfinite value");
     auto key_to_float = [](uint32_t k) {
         uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
         float f;
         std::memcpy(&f, &b, 4);
         return f;
     };
-    float vmin = key_to_float(res[0]), vmax = key_to_float(res[1]);
-    if (vmin == 0.0f || vmax == 0.0f) {
-        // ties among zeros / denormals under DAZ: minmax_element keeps the first
-        // smallest and the last largest element, with their own bits
-        unsigned long long* zi = reinterpret_cast<unsigned long long*>(r + 4);
-        unsigned long long zinit[2] = {~0ull, 0ull};
-        HIPCHECK(hipMemcpyAsync(zi, zinit, sizeof(zinit), hipMemcpyHostToDevice, c->stream));
-        launch_zero_index(d_field, n, zi, c->stream);
-        unsigned long long idx[2];
-        HIPCHECK(hipMemcpyAsync(idx, zi, sizeof(idx), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        if (vmin == 0.0f) HIPCHECK(hipMemcpy(&vmin, d_field + idx[0], 4, hipMemcpyDeviceToHost));
-        if (vmax == 0.0f) HIPCHECK(hipMemcpy(&vmax, d_field + idx[1], 4, hipMemcpyDeviceToHost));
+    for (uint32_t f = 0; f < nf; f++) {
+        const std::string who = names ? "field " + std::to_string(names[f]) + ": " : std::string();
+        if (res[4 * f + 2]) fail(ASZ_ERR_DATA, who + "input contains a nonfinite value");
+        float vmin = key_to_float(res[4 * f]), vmax = key_to_float(res[4 * f + 1]);
+        if (vmin == 0.0f || vmax == 0.0f) {
+            // ties among zeros / denormals under DAZ: minmax_element keeps the first
+            // smallest and the last largest element, with their own bits
+            unsigned long long* zi = reinterpret_cast<unsigned long long*>(r + 4 * (uint64_t)nf);
+            unsigned long long zinit[2] = {~0ull, 0ull};
+            HIPCHECK(hipMemcpyAsync(zi, zinit, sizeof(zinit), hipMemcpyHostToDevice, c->stream));
+            launch_zero_index(d_fields[f], n[f], zi, c->stream);
+            unsigned long long idx[2];
+            HIPCHECK(hipMemcpyAsync(idx, zi, sizeof(idx), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            if (vmin == 0.0f) HIPCHECK(hipMemcpy(&vmin, d_fields[f] + idx[0], 4, hipMemcpyDeviceToHost));
+            if (vmax == 0.0f) HIPCHECK(hipMemcpy(&vmax, d_fields[f] + idx[1], 4, hipMemcpyDeviceToHost));
+        }
+        if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, who + "max_val==min_val");
+        mn[f] = vmin;
+        mx[f] = vmax;
     }
-    if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, "max_val==min_val");
-    *mn = vmin;
-    *mx = vmax;
+}
+
+static void range_scan(asz_ctx* c, const float* d_field, uint64_t n, float* mn, float* mx) {
+    range_scan_fields(c, 1, &d_field, &n, mn, mx);
 }
 
 // ------------------------------------------------------------------ default slope
 
-static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_t dims[3], const asz_params& p,
-                                 float mn, float mx) {
-    const float aeb = abs_eb_of(p.rel_eb, mn, mx);
-    const float base = aeb * p.stat_eb_ratio;
-    std::vector<std::vector<int64_t>> keys(3);
+// Default slopes of nf fields of shape dims: every field's AB-key samples in
+// one buffer, one copy back and one synchronisation, then the fields' host
+// scoring in parallel on the context's pool.
+static void default_slope_fields(asz_ctx* c, uint32_t nf, const float* const* d_fields, const uint64_t dims[3],
+                                 const asz_params& p, const float* mn, const float* mx, SlopeResult* out) {
     SlopeSampleDesc SD[3];
-    uint64_t off[4] = {0, 0, 0, 0};  // key offsets of the three axes in one buffer
+    uint64_t off[4] = {0, 0, 0, 0};  // key offsets of the three axes in one field's slice
     for (int d = 0; d < 3; d++) {
         SlopeSampleDesc S{};
         uint64_t lo[3] = {0, 0, 0}, hi[3] = {dims[0], dims[1], dims[2]}, st[3] = {1, 1, 1};
@@ -191,20 +213,36 @@ static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_
             S.cnt[a] = (uint32_t)((hi[a] - lo[a] + S.step[a] - 1) / S.step[a]);
         }
         S.dist = dist;
-        S.neg_2_stat_eb = 1.0f / base;
         SD[d] = S;
         off[d + 1] = off[d] + 2 * (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
     }
-    // the three axes' samples in one buffer, one copy back, one synchronisation
-    int64_t* dk = c->keys.get<int64_t>(off[3] + 2);
-    for (int d = 0; d < 3; d++) launch_slope_sample(SD[d], d_field, dk + off[d], c->stream);
-    int64_t* hk = c->pin_xfer.get<int64_t>(off[3] + 2);
-    HIPCHECK(hipMemcpyAsync(hk, dk, off[3] * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    const uint64_t per = off[3];
+    int64_t* dk = c->keys.get<int64_t>(per * nf + 2);
+    for (uint32_t f = 0; f < nf; f++) {
+        const float base = abs_eb_of(p.rel_eb, mn[f], mx[f]) * p.stat_eb_ratio;
+        for (int d = 0; d < 3; d++) {
+            SlopeSampleDesc S = SD[d];
+            S.neg_2_stat_eb = 1.0f / base;
+            launch_slope_sample(S, d_fields[f], dk + f * per + off[d], c->stream);
+        }
+    }
+    int64_t* hk = c->pin_xfer.get<int64_t>(per * nf + 2);
+    HIPCHECK(hipMemcpyAsync(hk, dk, per * nf * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    for (int d = 0; d < 3; d++) keys[d].assign(hk + off[d], hk + off[d + 1]);
-    return default_slope_from_keys(keys, aeb, p.stat_eb_ratio, p.default_slope_eb_span,
-                                   p.default_slope_invalid_ratio_threshold, p.default_slope_fallback, p.t_min, p.t_max,
-                                   p.t_precision, mn, mx);
+    host_pool(c).run(nf, [&](size_t f) {
+        std::vector<std::vector<int64_t>> keys(3);
+        for (int d = 0; d < 3; d++) keys[d].assign(hk + f * per + off[d], hk + f * per + off[d + 1]);
+        out[f] = default_slope_from_keys(keys, abs_eb_of(p.rel_eb, mn[f], mx[f]), p.stat_eb_ratio,
+                                         p.default_slope_eb_span, p.default_slope_invalid_ratio_threshold,
+                                         p.default_slope_fallback, p.t_min, p.t_max, p.t_precision, mn[f], mx[f]);
+    });
+}
+
+static SlopeResult default_slope(asz_ctx* c, const float* d_field, const uint64_t dims[3], const asz_params& p,
+                                 float mn, float mx) {
+    SlopeResult r;
+    default_slope_fields(c, 1, &d_field, dims, p, &mn, &mx, &r);
+    return r;
 }
 
 // ------------------------------------------------------------------ compress
@@ -248,15 +286,37 @@ static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
     c->epoch = 0;
 }
 
-// stop_round >= 0: run the pivots and rounds [0, stop_round) only (no
-// Huffman coding), leaving d_work in the state the round stop_round reads.
-static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
-                          const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
-                          int32_t* qb_out, int64_t* t_out, int stop_round = -1) {
+// One field of a compress call: its array (rewritten with the
+// reconstruction), value range and default slope.
+struct FieldIn {
+    float* work;
+    float mn, mx;
+    float default_slope;
+};
+
+// Compresses nf fields of shape dims with one plan.  nf == 1 is the plain
+// drop-in (c->out_stream / c->stats); with nf > 1 every per-round kernel
+// covers all the fields (the field index is a grid dimension) and each
+// field's stream and statistics land in c->fouts / c->fstats.  Multi-field
+// calls take the GPU t selection (ASZ_TSEL_GPU or ASZ_TSEL_GRAD) and none of
+// the optional outputs.
+// stop_round >= 0 (one field): run the pivots and rounds [0, stop_round) only
+// (no Huffman coding), leaving the field in the state the round stop_round
+// reads.
+static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const uint64_t dims[3],
+                            const asz_plan& plan, const asz_params& p, int tsel_mode, const int64_t* t_inject,
+                            int32_t* qb_out, int64_t* t_out, int stop_round = -1) {
     auto wall0 = Clock::now();
     validate_params(p);
     if (tsel_mode < ASZ_TSEL_GPU || tsel_mode > ASZ_TSEL_GRAD) fail(ASZ_ERR_ARG, "unknown t selection mode");
+    if (nf == 0) fail(ASZ_ERR_ARG, "no field to compress");
+    const bool multi = nf > 1;
+    if (multi && ((tsel_mode != ASZ_TSEL_GPU && tsel_mode != ASZ_TSEL_GRAD) || t_inject || qb_out || t_out ||
+                  stop_round >= 0))
+        fail(ASZ_ERR_ARG, "multi-field calls take the GPU t selection and no optional outputs");
     hipStream_t st = c->stream;
+    float* d_work = fin[0].work;
+    const float mn = fin[0].mn, mx = fin[0].mx;
     const float aeb = abs_eb_of(p.rel_eb, mn, mx);
 
     // plan and its device tables, reused while dims, plan, block pivot ratio and
@@ -271,7 +331,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     if (!pc.valid || std::memcmp(&key, &pc.key, sizeof(key)) != 0) {
         pc.valid = false;
         Geometry g0 = make_geometry(dims, plan, p.block_pivot_ratio);
-        pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope,
+        pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, fin[0].default_slope,
                          p.h1_block_sampling_num);
         const Plan& P0 = pc.P;
         const PlanUpload u = plan_upload_layout(P0);
@@ -287,7 +347,24 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         pc.key = key;
         pc.valid = true;
     }
-    set_plan_bounds(pc.P, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, p.default_slope);
+    // per-field scalars (field 0 last: the plan keeps field 0's bounds)
+    FieldDesc* hfd = nullptr;
+    if (multi) hfd = c->pin_fdesc.get<FieldDesc>(nf);
+    for (uint32_t f = nf; f-- > 0;) {
+        set_plan_bounds(pc.P, abs_eb_of(p.rel_eb, fin[f].mn, fin[f].mx), p.pivot_eb_ratio, p.eb_ratio,
+                        p.slope_ratio, fin[f].default_slope);
+        if (!multi) break;
+        FieldDesc& F = hfd[f];
+        std::memset(&F, 0, sizeof(F));
+        F.work = fin[f].work;
+        F.inv_half_pe = 0.5 / (double)pc.P.pivot_eb;
+        F.two_pe = (double)pc.P.pivot_eb * 2.0;
+        F.range = daz(fin[f].mx) - daz(fin[f].mn);
+        for (int r = 0; r < pc.P.g.L; r++) {
+            const RoundDesc& R = pc.P.rounds[r];
+            F.r[r] = FieldRound{R.inv_half_eb, R.eb_d, R.neg_two_eb, R.eb, R.layer_slope};
+        }
+    }
     const Plan& P = pc.P;
     const Geometry& g = P.g;
     uint8_t* dm = static_cast<uint8_t*>(c->cmeta.p);
@@ -302,25 +379,46 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     S.layers = L;
     S.t_count = nt;
 
+    // Per-call buffers, one slice per field (multiples of 64 elements apart).
     // symbol buffer: t array (nt) then pivot + round QB arrays (N)
-    int32_t* sym = c->sym.get<int32_t>(nt + N + 16);
-    int32_t* qb = sym + nt;
-    float* tblk = c->tblk.get<float>(nt + 1);
-    Partial* part = c->partials.get<Partial>(P.total_partials + 1);
+    auto al64 = [](uint64_t v) { return (v + 63) / 64 * 64; };
     const uint64_t npp = (P.pivot_count + 255) / 256;  // pivot kernel workgroups
-    Partial* ppart = c->fparts.get<Partial>(npp + pc.fpart_total + 1);
+    const uint64_t narr = L + 2;                       // t array, pivots, rounds
+    FieldSet fs{};
+    fs.sym = al64(nt + N + 16);
+    fs.tblk = al64(nt + 1);
+    fs.part = al64(P.total_partials + 1);
+    fs.phist = al64(P.total_hist_words + 1);
+    fs.fpart = al64(npp + pc.fpart_total + 1);
+    fs.ghist = narr * kHistStride;  // global histogram windows, one kHistStride slot per array
+    fs.hres = narr * kHostSlotWords;
+    int32_t* sym0 = c->sym.get<int32_t>(fs.sym * nf);
+    float* tblk = c->tblk.get<float>(fs.tblk * nf);
+    Partial* part = c->partials.get<Partial>(fs.part * nf);
+    Partial* ppart = c->fparts.get<Partial>(fs.fpart * nf);
     Partial* fpart = ppart + npp;
-    uint32_t* phist = c->phist.get<uint32_t>(P.total_hist_words + 1);
-    const uint64_t narr = L + 2;  // t array, pivots, rounds
-    // global histogram windows, one kHistStride slot per array
-    uint32_t* ghist = c->ghist.get<uint32_t>(narr * kHistStride);
-    DevAcc* acc = c->xfer.get<DevAcc>(1);
-    auto aacc = [&](uint64_t k) { return (void*)&acc->arr[k]; };
-    auto gwin = [&](uint64_t k) { return ghist + k * kHistStride; };
-    ensure_host_slots(c, narr);
+    uint32_t* phist = c->phist.get<uint32_t>(fs.phist * nf);
+    uint32_t* ghist = c->ghist.get<uint32_t>(fs.ghist * nf);
+    DevAcc* acc = c->xfer.get<DevAcc>(nf);
+    if (multi) {
+        FieldDesc* dfd = c->fdesc.get<FieldDesc>(nf);
+        HIPCHECK(hipMemcpyAsync(dfd, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
+        fs.fd = dfd;
+    }
+    int32_t* sym = sym0;  // field 0
+    int32_t* qb = sym + nt;
+    // staging array of the rounds whose QBs are written in "012" order first
+    uint64_t stage_max = 0;
+    for (uint64_t r = 0; r < L; r++)
+        if (round_needs_staging(P.rounds[r])) stage_max = std::max(stage_max, P.rounds[r].count);
+    const uint64_t stage_stride = al64(stage_max + 1);
+    int32_t* stage = stage_max ? c->stage.get<int32_t>(stage_stride * nf) : nullptr;
+    auto sym_of = [&](uint32_t f) { return sym0 + f * fs.sym; };
+    auto gwin = [&](uint32_t f, uint64_t k) { return ghist + f * fs.ghist + k * kHistStride; };
+    ensure_host_slots(c, narr * nf);
     const uint32_t epoch = ++c->epoch ? c->epoch : ++c->epoch;  // never 0 (the slots start zeroed)
-    launch_init_acc(acc, st);
-    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), narr * kHistStride, 0, st);
+    for (uint32_t f = 0; f < nf; f++) launch_init_acc(acc + f, st);
+    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * fs.ghist, 0, st);
 
     if (tsel_mode == ASZ_TSEL_INJECT) {
         if (!t_inject) fail(ASZ_ERR_ARG, "t_inject is required with ASZ_TSEL_INJECT");
@@ -357,19 +455,21 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     // soon as the GPU has written that array's range and bins into its host
     // slot (flag == epoch), concurrently with this thread's launches and with
     // the later rounds on the GPU.
-    Layout lay;
-    lay.arrays.resize(narr);
-    lay.dev.resize(narr);
-    for (uint64_t k = 0; k < narr; k++) {
-        lay.arrays[k].count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
-        lay.arrays[k].is_int64 = k == 0;
+    std::vector<Layout> lays(nf);
+    for (Layout& lay : lays) {
+        lay.arrays.resize(narr);
+        lay.dev.resize(narr);
+        for (uint64_t k = 0; k < narr; k++) {
+            lay.arrays[k].count = k == 0 ? nt : (k == 1 ? P.pivot_count : P.rounds[k - 2].count);
+            lay.arrays[k].is_int64 = k == 0;
+        }
     }
     std::atomic<bool> enqueued{false}, abort_all{false};
     std::mutex side_mu;  // the exact-histogram path uses the side stream and shared scratch
     std::vector<double> host_ms_w;
-    auto slot = [&](uint64_t k) { return c->hres_host + k * kHostSlotWords; };
-    auto wait_slot = [&](uint64_t k) {
-        volatile uint32_t* flag = slot(k);
+    auto slot = [&](uint32_t f, uint64_t k) { return c->hres_host + (f * narr + k) * kHostSlotWords; };
+    auto wait_slot = [&](uint32_t f, uint64_t k) {
+        volatile uint32_t* flag = slot(f, k);
         uint64_t spins = 0;
         while (*flag != epoch) {
             if (abort_all.load(std::memory_order_relaxed)) fail(ASZ_ERR_HIP, "compression aborted");
@@ -389,10 +489,10 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         }
         std::atomic_thread_fence(std::memory_order_acquire);
     };
-    auto build = [&](uint64_t k) {
-        HuffArray& A = lay.arrays[k];
+    auto build = [&](uint32_t f, uint64_t k) {
+        HuffArray& A = lays[f].arrays[k];
         if (A.count == 0) return;
-        const uint32_t* hs = slot(k);
+        const uint32_t* hs = slot(f, k);
         HistTail tail;
         std::memcpy(&tail, hs + kHostSlotHead, sizeof(tail));
         const int32_t q0 = tail.qmin, q1 = tail.qmax;
@@ -422,7 +522,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         HIPCHECK(hipMemcpyAsync(ded, &D, sizeof(D), hipMemcpyHostToDevice, c->side));
         HIPCHECK(hipMemcpyAsync(dcarr, carr.data(), carr.size() * 4, hipMemcpyHostToDevice, c->side));
         launch_fill_i32(reinterpret_cast<int32_t*>(dh), A.range, 0, c->side);
-        launch_hist(sym, ded, dcarr, carr.size(), dh, c->side);
+        launch_hist(sym_of(f), ded, dcarr, carr.size(), dh, c->side);
         std::vector<uint32_t> hexact(A.range);
         HIPCHECK(hipMemcpyAsync(hexact.data(), dh, A.range * 4, hipMemcpyDeviceToHost, c->side));
         HIPCHECK(hipStreamSynchronize(c->side));
@@ -432,23 +532,27 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     HIPCHECK(hipEventRecord(c->ev[0], st));
     PivotDesc PD = pivot_desc(P);
     int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(true, d_work, qb, PD, ppart, st);
+    launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
     kev_end(c, kp);
-    launch_hist_small(nullptr, 0, qb, P.pivot_count, nullptr, gwin(1), st);
-    launch_hist_tail(acc, 1, ppart, npp, gwin(1), c->hres_dev, epoch, st);
+    for (uint32_t f = 0; f < nf; f++) {
+        launch_hist_small(nullptr, 0, sym_of(f) + nt, P.pivot_count, nullptr, gwin(f, 1), st);
+        launch_hist_tail(acc + f, 1, ppart + f * fs.fpart, npp, gwin(f, 1), c->hres_dev + f * fs.hres, epoch, st);
+    }
     // Codebook workers: the host pool's threads take the arrays in `order`
-    // as their results arrive, concurrently with the launches below and
-    // with the later rounds on the GPU.
+    // (every field's copy of an array in turn) as their results arrive,
+    // concurrently with the launches below and with the later rounds on the GPU.
     const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
-    host_ms_w.assign(order.size(), 0.0);
+    const uint64_t ntask = order.size() * nf;
+    host_ms_w.assign(ntask, 0.0);
     TaskPool& pool = host_pool(c);
     if (stop_round < 0)
-        pool.start(order.size(), [&](size_t i) {
-            const uint64_t k = order[i];
-            if (lay.arrays[k].count == 0) return;  // an empty round has no slot
-            wait_slot(k);
+        pool.start(ntask, [&](size_t i) {
+            const uint64_t k = order[i / nf];
+            const uint32_t f = (uint32_t)(i % nf);
+            if (lays[f].arrays[k].count == 0) return;  // an empty round has no slot
+            wait_slot(f, k);
             auto h0 = Clock::now();
-            build(k);
+            build(f, k);
             host_ms_w[i] = ms_since(h0);
         });
     // stops and joins the workers on every exit path below
@@ -471,13 +575,13 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     std::vector<uint64_t> koff;
     for (uint64_t r = 0; r < Lrun; r++) {
         const RoundDesc& R = P.rounds[r];
-        check_grid(R);
+        check_grid(R, nf);
         float* tb = tblk + r * nbl;
         if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) {
             TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
                        p.t_min, p.t_max};
             int kt = kev_begin(c, KEV_TSEL, (int)r);
-            launch_tsel(T, d_work, ss + r * nbl, tb, sym, &acc->fallbacks, nbl, tsel_mode == ASZ_TSEL_GRAD, st);
+            launch_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, st, fs, nf);
             kev_end(c, kt);
         } else if (tsel_mode == ASZ_TSEL_EXACT) {
             auto h0 = Clock::now();
@@ -518,12 +622,28 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
         }
         if (r + 1 == L) {
             // every t is final: the t array's range, histogram and tail
-            launch_minmax_i32(sym, nt, (int32_t*)aacc(0), st);
-            launch_hist_small(sym, nt, nullptr, 0, gwin(0), nullptr, st);
-            launch_hist_tail(acc, 0, nullptr, 0, gwin(0), c->hres_dev, epoch, st);
+            for (uint32_t f = 0; f < nf; f++) {
+                launch_minmax_i32(sym_of(f), nt, (int32_t*)&acc[f].arr[0], st);
+                launch_hist_small(sym_of(f), nt, nullptr, 0, gwin(f, 0), nullptr, st);
+                launch_hist_tail(acc + f, 0, nullptr, 0, gwin(f, 0), c->hres_dev + f * fs.hres, epoch, st);
+            }
         }
         int kr = kev_begin(c, KEV_ROUND, (int)r);
-        launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist, gwin(2 + r), st);
+        if (round_needs_staging(R)) {
+            RoundDesc Rs = R;
+            Rs.order = 2;
+            Rs.qb_off = 0;
+            FieldSet fss = fs;
+            fss.sym = stage_stride;
+            launch_round_compress(Rs, p.quintic != 0, d_work, stage, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
+                                  fss, nf);
+            uint32_t cmax[3];
+            round_cmax(P, R, cmax);
+            launch_qb_order(R, tabs, cmax, stage, qb + R.qb_off, stage_stride, fs.sym, nf, true, st);
+        } else {
+            launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
+                                  fs, nf);
+        }
         kev_end(c, kr);
         if (r + 1 >= batch_end) {
             const uint64_t f0 = r + 1 == batch_end ? 0 : r;
@@ -531,7 +651,7 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
             uint32_t mg = 0;
             for (uint64_t q = f0; q <= r; q++) mg = std::max(mg, pc.folds[q].groups);
             launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), mg, part, phist, fpart, acc, ghist,
-                               c->hres_dev, epoch, st);
+                               c->hres_dev, epoch, st, fs, nf);
             kev_end(c, kh);
         }
     }
@@ -549,123 +669,168 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     for (double v : host_ms_w) host_ms += v;
     host_ms /= std::max(1u, pool.workers());
     // errors and fallbacks from the slots
-    S.max_err = 0;
-    S.sq_err_sum = 0;
-    for (uint64_t k = 0; k < narr; k++) {
-        if (lay.arrays[k].count == 0) continue;
-        HistTail tail;
-        std::memcpy(&tail, slot(k) + kHostSlotHead, sizeof(tail));
-        if (k == 0) {
-            if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) S.t_fallbacks = tail.fallbacks;
-            if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
-        } else {
-            S.max_err = std::max(S.max_err, tail.max_err);
-            S.sq_err_sum += tail.sq;
+    c->fstats.assign(nf, asz_stats{});
+    for (uint32_t f = 0; f < nf; f++) {
+        asz_stats& Sf = c->fstats[f];
+        Sf.layers = L;
+        Sf.t_count = nt;
+        for (uint64_t k = 0; k < narr; k++) {
+            if (lays[f].arrays[k].count == 0) continue;
+            HistTail tail;
+            std::memcpy(&tail, slot(f, k) + kHostSlotHead, sizeof(tail));
+            if (k == 0) {
+                if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) Sf.t_fallbacks = tail.fallbacks;
+            } else {
+                Sf.max_err = std::max(Sf.max_err, tail.max_err);
+                Sf.sq_err_sum += tail.sq;
+            }
         }
     }
+    S.max_err = c->fstats[0].max_err;
+    S.sq_err_sum = c->fstats[0].sq_err_sum;
+    if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) S.t_fallbacks = c->fstats[0].t_fallbacks;
+    if (tsel_mode == ASZ_TSEL_INJECT) S.t_fallbacks = 0;
     auto hh0 = Clock::now();
 
-    // codebooks and stream layout (compress.tpp:243-291).  Only the header
-    // bytes travel to the device (a compact blob scattered into the zeroed
-    // stream buffer); payload bits are ORed in by the encoder.
-    std::vector<uint8_t> blob;
-    std::vector<ByteSeg> bsegs;
-    uint64_t spos = 0;
-    auto emit = [&](const uint8_t* q, uint64_t n) {
-        if (!bsegs.empty() && bsegs.back().dst + bsegs.back().len == spos) bsegs.back().len += n;
-        else bsegs.push_back(ByteSeg{spos, blob.size(), n});
-        blob.insert(blob.end(), q, q + n);
-        spos += n;
+    // codebooks and stream layout (compress.tpp:243-291), per field.  Only the
+    // header bytes travel to the device (a compact blob scattered into the
+    // zeroed stream buffer); payload bits are ORed in by the encoder.  Every
+    // field's segments, header bytes, codebooks and array descriptors go up in
+    // one upload.
+    struct FieldUp {
+        std::vector<uint8_t> blob;
+        std::vector<ByteSeg> bsegs;
+        std::vector<uint64_t> book;
+        uint64_t chunks = 0, o_seg = 0, o_blob = 0, o_book = 0, o_arr = 0, woff = 0, nwords = 0;
     };
-    {
-        uint8_t pre[9 + 2 * ASZ_MAX_LAYERS];
-        std::memcpy(pre, &mn, 4);
-        std::memcpy(pre + 4, &mx, 4);
-        pre[8] = (uint8_t)(L | 0x40 | (p.quintic ? 0x80 : 0));
-        for (uint64_t r = 0; r < L; r++) {
-            pre[9 + 2 * r] = (uint8_t)('0' + g.axis[r]);
-            pre[10 + 2 * r] = (uint8_t)('0' + g.order[r]);
-        }
-        emit(pre, 9 + 2 * L);
-    }
-    uint64_t book_total = 0, chunks = 0;
-    for (uint64_t k = 0; k < narr; k++) book_total += lay.arrays[k].count ? lay.arrays[k].range : 0;
-    std::vector<uint64_t> book(book_total + 1);
-    book_total = 0;
-    for (uint64_t k = 0; k < narr; k++) {
-        HuffArray& A = lay.arrays[k];
-        HuffArrayDev& D = lay.dev[k];
-        std::memset(&D, 0, sizeof(D));
-        uint8_t cnt8[8];
-        for (int b = 0; b < 8; b++) cnt8[b] = (uint8_t)(A.count >> (8 * b));
-        emit(cnt8, 8);
-        if (!A.count) continue;
-        emit(A.header.data(), A.header.size());
-        if (A.single) continue;
-        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
-        D.count = A.count;
-        D.offset = A.offset;
-        D.range = A.range;
-        D.book_off = book_total;
-        D.chunk0 = chunks;
-        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
-        chunks += D.nchunk;
-        D.payload_bit = spos * 8;
-        spos += (A.payload_bits + 7) >> 3;
-        int alen = 0;
-        for (uint64_t s2 = 0; s2 < A.range; s2++) {
-            if (A.len[s2] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
-            book[book_total + s2] = A.code[s2] | ((uint64_t)A.len[s2] << 56);
-            alen = std::max<int>(alen, A.len[s2]);
-        }
-        D.lds_book = (A.range <= kLdsBook && alen <= 24) ? 1 : 0;
-        lay.max_len = std::max(lay.max_len, alen);
-        book_total += A.range;
-    }
-    lay.total_bytes = spos;
-    S.stream_bytes = lay.total_bytes;
-    for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lay.arrays[k + 1].count;
-    // one upload: header segments, header bytes, codebooks, array descriptors
+    std::vector<FieldUp> ups(nf);
     auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
-    const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
-                   o_arr = al8(o_book + book_total * 8), up_bytes = o_arr + narr * sizeof(HuffArrayDev);
+    uint64_t up_bytes = 0, words_total = 0, max_chunks = 0;
+    int max_len = 0;
+    for (uint32_t f = 0; f < nf; f++) {
+        Layout& lay = lays[f];
+        FieldUp& U = ups[f];
+        uint64_t spos = 0;
+        auto emit = [&](const uint8_t* q, uint64_t n) {
+            if (!U.bsegs.empty() && U.bsegs.back().dst + U.bsegs.back().len == spos) U.bsegs.back().len += n;
+            else U.bsegs.push_back(ByteSeg{spos, U.blob.size(), n});
+            U.blob.insert(U.blob.end(), q, q + n);
+            spos += n;
+        };
+        {
+            uint8_t pre[9 + 2 * ASZ_MAX_LAYERS];
+            std::memcpy(pre, &fin[f].mn, 4);
+            std::memcpy(pre + 4, &fin[f].mx, 4);
+            pre[8] = (uint8_t)(L | 0x40 | (p.quintic ? 0x80 : 0));
+            for (uint64_t r = 0; r < L; r++) {
+                pre[9 + 2 * r] = (uint8_t)('0' + g.axis[r]);
+                pre[10 + 2 * r] = (uint8_t)('0' + g.order[r]);
+            }
+            emit(pre, 9 + 2 * L);
+        }
+        uint64_t book_total = 0, chunks = 0;
+        for (uint64_t k = 0; k < narr; k++) book_total += lay.arrays[k].count ? lay.arrays[k].range : 0;
+        U.book.assign(book_total + 1, 0);
+        book_total = 0;
+        for (uint64_t k = 0; k < narr; k++) {
+            HuffArray& A = lay.arrays[k];
+            HuffArrayDev& D = lay.dev[k];
+            std::memset(&D, 0, sizeof(D));
+            uint8_t cnt8[8];
+            for (int b = 0; b < 8; b++) cnt8[b] = (uint8_t)(A.count >> (8 * b));
+            emit(cnt8, 8);
+            if (!A.count) continue;
+            emit(A.header.data(), A.header.size());
+            if (A.single) continue;
+            D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+            D.count = A.count;
+            D.offset = A.offset;
+            D.range = A.range;
+            D.book_off = book_total;
+            D.chunk0 = chunks;
+            D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
+            chunks += D.nchunk;
+            D.payload_bit = spos * 8;
+            spos += (A.payload_bits + 7) >> 3;
+            int alen = 0;
+            for (uint64_t s2 = 0; s2 < A.range; s2++) {
+                if (A.len[s2] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
+                U.book[book_total + s2] = A.code[s2] | ((uint64_t)A.len[s2] << 56);
+                alen = std::max<int>(alen, A.len[s2]);
+            }
+            D.lds_book = (A.range <= kLdsBook && alen <= 24) ? 1 : 0;
+            lay.max_len = std::max(lay.max_len, alen);
+            book_total += A.range;
+        }
+        lay.total_bytes = spos;
+        U.chunks = chunks;
+        U.o_seg = up_bytes;
+        U.o_blob = al8(U.o_seg + U.bsegs.size() * sizeof(ByteSeg));
+        U.o_book = al8(U.o_blob + U.blob.size());
+        U.o_arr = al8(U.o_book + book_total * 8);
+        up_bytes = al8(U.o_arr + narr * sizeof(HuffArrayDev));
+        U.nwords = (lay.total_bytes + 3) / 4 + 4;
+        U.woff = words_total;
+        words_total += (U.nwords + 63) / 64 * 64;
+        max_chunks = std::max(max_chunks, chunks);
+        max_len = std::max(max_len, lay.max_len);
+        c->fstats[f].stream_bytes = lay.total_bytes;
+        for (uint64_t k = 0; k <= L; k++) c->fstats[f].qb_counts[k] = lay.arrays[k + 1].count;
+    }
+    S.stream_bytes = lays[0].total_bytes;
+    for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lays[0].arrays[k + 1].count;
     uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
-    std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
-    std::memcpy(hu + o_blob, blob.data(), blob.size());
-    std::memcpy(hu + o_book, book.data(), book_total * 8);
-    std::memcpy(hu + o_arr, lay.dev.data(), narr * sizeof(HuffArrayDev));
+    for (uint32_t f = 0; f < nf; f++) {
+        const FieldUp& U = ups[f];
+        std::memcpy(hu + U.o_seg, U.bsegs.data(), U.bsegs.size() * sizeof(ByteSeg));
+        std::memcpy(hu + U.o_blob, U.blob.data(), U.blob.size());
+        std::memcpy(hu + U.o_book, U.book.data(), (U.book.size() - 1) * 8);
+        std::memcpy(hu + U.o_arr, lays[f].dev.data(), narr * sizeof(HuffArrayDev));
+    }
     uint8_t* du = c->hmeta.get<uint8_t>(up_bytes);
     HIPCHECK(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, st));
     S.huffman_host_ms = host_ms + ms_since(hh0);
 
-    const uint64_t nwords = (lay.total_bytes + 3) / 4 + 4;
-    uint32_t* dwords = c->words.get<uint32_t>(nwords);
-    // chunk scratch: each chunk's bits at the start of its own slot
-    const uint64_t slot_words = (uint64_t)kEncChunk * std::max(lay.max_len, 1) / 32 + 2;
-    uint32_t* scratch = c->scratch.get<uint32_t>(chunks * slot_words + 1);
-    uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
-    uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
+    uint32_t* dwords = c->words.get<uint32_t>(words_total);
+    // chunk scratch: each chunk's bits at the start of its own slot; the
+    // fields are coded one after another on the stream and share it
+    const uint64_t slot_words = (uint64_t)kEncChunk * std::max(max_len, 1) / 32 + 2;
+    uint32_t* scratch = c->scratch.get<uint32_t>(max_chunks * slot_words + 1);
+    uint32_t* cbits = c->chunk_bits.get<uint32_t>(max_chunks + 1);
+    uint64_t* coffs = c->chunk_off.get<uint64_t>(max_chunks + 1);
     HIPCHECK(hipEventRecord(c->ev[4], st));
-    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), nwords, 0, st);
-    if (lay.max_len > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), chunks * slot_words, 0, st);
-    launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
-                         reinterpret_cast<uint8_t*>(dwords), st);
-    const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
+    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), words_total, 0, st);
     int ke = kev_begin(c, KEV_ENCODE, 0);
-    launch_encode_chunks(sym, darr, (uint32_t)narr, chunks, reinterpret_cast<const uint64_t*>(du + o_book), scratch,
-                         slot_words, cbits, lay.max_len, st);
-    launch_chunk_scan(darr, (uint32_t)narr, cbits, coffs, st);
-    launch_compact_bits(scratch, slot_words, cbits, coffs, chunks, dwords, st);
+    for (uint32_t f = 0; f < nf; f++) {
+        const FieldUp& U = ups[f];
+        const int ml = lays[f].max_len;
+        uint32_t* fw = dwords + U.woff;
+        if (ml > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), U.chunks * slot_words, 0, st);
+        launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + U.o_seg), (uint32_t)U.bsegs.size(),
+                             du + U.o_blob, reinterpret_cast<uint8_t*>(fw), st);
+        const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + U.o_arr);
+        launch_encode_chunks(sym_of(f), darr, (uint32_t)narr, U.chunks, reinterpret_cast<const uint64_t*>(du + U.o_book),
+                             scratch, slot_words, cbits, ml, st);
+        launch_chunk_scan(darr, (uint32_t)narr, cbits, coffs, st);
+        launch_compact_bits(scratch, slot_words, cbits, coffs, U.chunks, fw, st);
+    }
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
-    c->out_bytes = lay.total_bytes;
+    c->fouts.assign(nf, {nullptr, 0});
     if (c->stream_on_device) {
-        c->out_stream = reinterpret_cast<uint8_t*>(dwords);
+        for (uint32_t f = 0; f < nf; f++)
+            c->fouts[f] = {reinterpret_cast<uint8_t*>(dwords + ups[f].woff), lays[f].total_bytes};
     } else {
-        c->out_stream = c->pin_stream.get<uint8_t>(lay.total_bytes + 1);
-        HIPCHECK(hipMemcpyAsync(c->out_stream, dwords, lay.total_bytes, hipMemcpyDeviceToHost, st));
+        uint8_t* hs = c->pin_stream.get<uint8_t>(words_total * 4 + 1);
+        for (uint32_t f = 0; f < nf; f++) {
+            uint8_t* dst = hs + ups[f].woff * 4;
+            HIPCHECK(hipMemcpyAsync(dst, dwords + ups[f].woff, lays[f].total_bytes, hipMemcpyDeviceToHost, st));
+            c->fouts[f] = {dst, lays[f].total_bytes};
+        }
     }
+    c->out_stream = c->fouts[0].first;
+    c->out_bytes = c->fouts[0].second;
     if (qb_out) HIPCHECK(hipMemcpyAsync(qb_out, qb, N * 4, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> t32;
     if (t_out) {
@@ -681,6 +846,14 @@ static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], con
     S.tsel_ms = tsel_host_ms;
     S.total_gpu_ms = S.rounds_ms + S.huffman_ms;
     S.wall_ms = ms_since(wall0);
+}
+
+// One field (the plain drop-in).
+static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
+                          const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
+                          int32_t* qb_out, int64_t* t_out, int stop_round = -1) {
+    const FieldIn f{d_work, mn, mx, p.default_slope};
+    compress_fields(c, 1, &f, dims, plan, p, tsel_mode, t_inject, qb_out, t_out, stop_round);
 }
 
 // ------------------------------------------------------------------ decompress
@@ -846,12 +1019,25 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, st);
     kev_end(c, kp);
+    uint64_t stage_max = 0;
+    for (uint64_t r = 0; r < L; r++)
+        if (round_needs_staging(P.rounds[r])) stage_max = std::max(stage_max, P.rounds[r].count);
+    int32_t* stage = stage_max ? c->stage.get<int32_t>(stage_max + 1) : nullptr;
     for (uint64_t r = 0; r < L; r++) {
         RoundDesc R = P.rounds[r];
-        check_grid(R);
+        check_grid(R, 1);
         R.qb_off = 0;
         int kr = kev_begin(c, KEV_ROUND, (int)r);
-        launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, dsegs, st);
+        if (round_needs_staging(R)) {
+            // the round's QBs back into "012" order, then the round reads them there
+            uint32_t cmax[3];
+            round_cmax(P, R, cmax);
+            launch_qb_order(R, tabs, cmax, stage, sym + out_off[r + 2], 0, 0, 1, false, st);
+            R.order = 2;
+            launch_round_decompress(R, Q.quintic, d_out, stage, tabs, tblk + r * nbl, dsegs, st);
+        } else {
+            launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, dsegs, st);
+        }
         kev_end(c, kr);
     }
     HIPCHECK(hipGetLastError());
@@ -945,6 +1131,11 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
     return guarded(c, [&] {
         if (option == ASZ_OPT_STREAM_ON_DEVICE) {
             c->stream_on_device = value != 0;
+            return;
+        }
+        if (option == ASZ_OPT_BATCH_FIELDS) {
+            if (value < 0 || value > 65535) fail(ASZ_ERR_ARG, "fields per chunk must be in [0, 65535]");
+            c->batch_fields = (uint32_t)value;
             return;
         }
         if (option != ASZ_OPT_KERNEL_EVENTS && option != ASZ_OPT_TIMED_ROUND) fail(ASZ_ERR_ARG, "unknown option");
@@ -1120,7 +1311,46 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         if (tsel_mode == ASZ_TSEL_INJECT) fail(ASZ_ERR_ARG, "t injection is not available in batches");
         for (uint32_t i = 0; i < nfields; i++) streams[i] = nullptr;
         const uint32_t T = std::max<uint32_t>(1, std::min<uint32_t>(concurrency ? concurrency : 4, nfields));
-        while (c->kids.size() < T) {
+        // Fields of one shape and plan (time steps of a series, same-shape
+        // fields) are compressed together in chunks: every per-round kernel of
+        // a chunk covers all of its fields.  Chunks hold about a 2T-th of their
+        // group (so the worker contexts stay busy to the end), at most
+        // kMaxChunk (or ASZ_OPT_BATCH_FIELDS) fields and kMaxChunkPoints
+        // points; the largest go first.
+        const uint32_t kMaxChunk = c->batch_fields ? c->batch_fields : 16;
+        constexpr uint64_t kMaxChunkPoints = 1ull << 30;
+        const bool can_group = tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD;
+        std::vector<std::vector<uint32_t>> groups;
+        for (uint32_t i = 0; i < nfields; i++) {
+            bool placed = false;
+            for (auto& gr : groups) {
+                const uint32_t j = gr[0];
+                if (can_group && std::memcmp(dims + 3 * (uint64_t)i, dims + 3 * (uint64_t)j, 24) == 0 &&
+                    std::memcmp(&plans[i], &plans[j], sizeof(asz_plan)) == 0) {
+                    gr.push_back(i);
+                    placed = true;
+                    break;
+                }
+            }
+            if (!placed) groups.push_back({i});
+        }
+        std::vector<std::vector<uint32_t>> chunks;
+        for (const auto& gr : groups) {
+            const uint64_t* d = dims + 3 * (uint64_t)gr[0];
+            const uint64_t n = std::max<uint64_t>(1, d[0] * d[1] * d[2]);
+            uint64_t cs = (gr.size() + 2 * T - 1) / (2 * T);
+            cs = std::max<uint64_t>(1, std::min<uint64_t>({cs, kMaxChunk, kMaxChunkPoints / n}));
+            for (size_t i0 = 0; i0 < gr.size(); i0 += cs)
+                chunks.emplace_back(gr.begin() + i0, gr.begin() + std::min<size_t>(gr.size(), i0 + cs));
+        }
+        auto points = [&](const std::vector<uint32_t>& ch) {
+            const uint64_t* d = dims + 3 * (uint64_t)ch[0];
+            return d[0] * d[1] * d[2] * ch.size();
+        };
+        std::stable_sort(chunks.begin(), chunks.end(),
+                         [&](const auto& x, const auto& y) { return points(x) > points(y); });
+        const uint32_t W = std::min<uint32_t>(T, (uint32_t)chunks.size());
+        while (c->kids.size() < W) {
             asz_ctx* k = nullptr;
             const int rc = asz_ctx_create(c->device, &k);
             if (rc != ASZ_OK) fail(rc, "cannot create a batch worker context");
@@ -1131,58 +1361,79 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         std::mutex err_mu;
         std::string first_err;
         int first_code = ASZ_OK;
+        const uint32_t nchunks = (uint32_t)chunks.size();
         auto worker = [&](asz_ctx* k) {
             (void)hipSetDevice(k->device);
             for (;;) {
-                const uint32_t i = next.fetch_add(1);
-                if (i >= nfields) return;
+                const uint32_t ci = next.fetch_add(1);
+                if (ci >= nchunks) return;
+                const std::vector<uint32_t>& ch = chunks[ci];
+                const uint32_t nf = (uint32_t)ch.size();
                 const int rc = guarded(k, [&] {
-                    const uint64_t* d = dims + 3 * (uint64_t)i;
+                    const uint64_t* d = dims + 3 * (uint64_t)ch[0];
                     const uint64_t n = d[0] * d[1] * d[2];
-                    float mn, mx;
-                    asz::range_scan(k, d_fields[i], n, &mn, &mx);
-                    asz_params q = *p;
-                    asz::SlopeResult sr = asz::default_slope(k, d_fields[i], d, q, mn, mx);
-                    q.default_slope = sr.slope;
+                    std::vector<const float*> ptrs(nf);
+                    std::vector<uint64_t> ns(nf, n);
+                    std::vector<float> mn(nf), mx(nf);
+                    for (uint32_t j = 0; j < nf; j++) ptrs[j] = d_fields[ch[j]];
+                    asz::range_scan_fields(k, nf, ptrs.data(), ns.data(), mn.data(), mx.data(), ch.data());
+                    std::vector<asz::SlopeResult> sr(nf);
+                    asz::default_slope_fields(k, nf, ptrs.data(), d, *p, mn.data(), mx.data(), sr.data());
+                    std::vector<asz::FieldIn> fin(nf);
+                    for (uint32_t j = 0; j < nf; j++) fin[j] = asz::FieldIn{d_fields[ch[j]], mn[j], mx[j], sr[j].slope};
                     k->stream_on_device = on_device;
-                    asz::compress_impl(k, d_fields[i], d, plans[i], q, mn, mx, tsel_mode, nullptr, nullptr, nullptr);
-                    uint8_t* out = nullptr;
-                    if (on_device) {
-                        // a device buffer of the stream's size, filled device to device
-                        if (hipMalloc(&out, k->out_bytes ? k->out_bytes : 1) != hipSuccess)
-                            fail(ASZ_ERR_NOMEM, "device allocation failed");
-                        HIPCHECK(hipMemcpyAsync(out, k->out_stream, k->out_bytes, hipMemcpyDeviceToDevice, k->stream));
-                        HIPCHECK(hipStreamSynchronize(k->stream));
-                    } else {
-                        out = static_cast<uint8_t*>(std::malloc(k->out_bytes ? k->out_bytes : 1));
-                        if (!out) fail(ASZ_ERR_NOMEM, "host allocation failed");
-                        std::memcpy(out, k->out_stream, k->out_bytes);
+                    asz::compress_fields(k, nf, fin.data(), d, plans[ch[0]], *p, tsel_mode, nullptr, nullptr,
+                                         nullptr);
+                    for (uint32_t j = 0; j < nf; j++) {
+                        const uint32_t i = ch[j];
+                        const auto& o = k->fouts[j];
+                        uint8_t* out = nullptr;
+                        if (on_device) {
+                            // a device buffer of the stream's size, filled device to device
+                            if (hipMalloc(&out, o.second ? o.second : 1) != hipSuccess)
+                                fail(ASZ_ERR_NOMEM, "device allocation failed");
+                            streams[i] = out;
+                            HIPCHECK(hipMemcpyAsync(out, o.first, o.second, hipMemcpyDeviceToDevice, k->stream));
+                        } else {
+                            out = static_cast<uint8_t*>(std::malloc(o.second ? o.second : 1));
+                            if (!out) fail(ASZ_ERR_NOMEM, "host allocation failed");
+                            streams[i] = out;
+                            std::memcpy(out, o.first, o.second);
+                        }
+                        stream_bytes[i] = o.second;
+                        if (info) {
+                            const asz_stats& Sf = k->fstats[j];
+                            info[i].min_val = mn[j];
+                            info[i].max_val = mx[j];
+                            info[i].default_slope = sr[j].slope;
+                            info[i].pad = 0;
+                            info[i].max_err = Sf.max_err;
+                            info[i].sq_err_sum = Sf.sq_err_sum;
+                            info[i].t_fallbacks = Sf.t_fallbacks;
+                        }
                     }
-                    streams[i] = out;
-                    stream_bytes[i] = k->out_bytes;
-                    if (info) {
-                        info[i].min_val = mn;
-                        info[i].max_val = mx;
-                        info[i].default_slope = sr.slope;
-                        info[i].pad = 0;
-                        info[i].max_err = k->stats.max_err;
-                        info[i].sq_err_sum = k->stats.sq_err_sum;
-                        info[i].t_fallbacks = k->stats.t_fallbacks;
-                    }
+                    if (on_device) HIPCHECK(hipStreamSynchronize(k->stream));
                 });
                 if (rc != ASZ_OK) {
                     std::lock_guard<std::mutex> lk(err_mu);
                     if (first_code == ASZ_OK) {
                         first_code = rc;
-                        first_err = "field " + std::to_string(i) + ": " + k->err;
+                        // errors that concern one field name it; others name the chunk
+                        if (k->err.rfind("field ", 0) == 0) {
+                            first_err = k->err;
+                        } else {
+                            first_err = nf == 1 ? "field " : "fields ";
+                            for (uint32_t j = 0; j < nf; j++) first_err += (j ? ", " : "") + std::to_string(ch[j]);
+                            first_err += ": " + k->err;
+                        }
                     }
-                    next = nfields;  // stop handing out fields
+                    next = nchunks;  // stop handing out chunks
                     return;
                 }
             }
         };
         std::vector<std::thread> ts;
-        for (uint32_t t = 0; t < T; t++) ts.emplace_back(worker, c->kids[t]);
+        for (uint32_t t = 0; t < W; t++) ts.emplace_back(worker, c->kids[t]);
         for (auto& t : ts) t.join();
         if (first_code != ASZ_OK) {
             for (uint32_t i = 0; i < nfields; i++) {

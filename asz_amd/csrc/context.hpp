@@ -95,8 +95,8 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, cmeta, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream;
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc;
     hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
     // host result slots (pinned, written by the GPU): kHostSlotWords per array
     uint32_t* hres_host = nullptr;
@@ -106,8 +106,12 @@ struct asz_ctx {
     std::vector<asz_ctx*> kids;  // batch workers: one context (and HIP stream) each
     asz::PlanCache pc;
     bool stream_on_device = false;
+    uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;
+    // multi-field calls: each field's stream (in words or pin_stream) and statistics
+    std::vector<std::pair<uint8_t*, uint64_t>> fouts;
+    std::vector<asz_stats> fstats;
     hipEvent_t ev[8];
     // per-kernel event pairs (ASZ_OPT_KERNEL_EVENTS)
     bool kev_on = false;

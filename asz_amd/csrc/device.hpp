@@ -125,6 +125,16 @@ __device__ __forceinline__ void hist_add(uint32_t* lh, uint32_t win, uint32_t* g
     }
 }
 
+// Multi-field kernels: the layer scalars of one field replace the plan's.
+__device__ __forceinline__ void field_round(const FieldDesc& F, RoundDesc& R) {
+    const FieldRound& q = F.r[R.layer];
+    R.inv_half_eb = q.inv_half_eb;
+    R.eb_d = q.eb_d;
+    R.neg_two_eb = q.neg_two_eb;
+    R.eb = q.eb;
+    R.layer_slope = q.layer_slope;
+}
+
 using asz::ArrayAcc;
 using asz::ErrAcc;
 

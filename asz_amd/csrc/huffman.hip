@@ -151,9 +151,18 @@ __device__ Partial fold_partials(const Partial* __restrict__ part, uint64_t p0, 
 __global__ __launch_bounds__(256) void k_fold_rounds(const FoldDesc* __restrict__ descs, uint32_t r0,
                                                      const Partial* __restrict__ part,
                                                      const uint32_t* __restrict__ phist, Partial* fpart, DevAcc* acc,
-                                                     uint32_t* ghist, uint32_t* hres, uint32_t epoch) {
+                                                     uint32_t* ghist, uint32_t* hres, uint32_t epoch, FieldSet fs) {
     __shared__ uint32_t lsum[256];
     __shared__ int s_last;
+    if (fs.fd) {  // field blockIdx.z of a multi-field call
+        const uint32_t f = blockIdx.z;
+        part += f * fs.part;
+        phist += f * fs.phist;
+        fpart += f * fs.fpart;
+        acc += f;
+        ghist += f * fs.ghist;
+        hres += f * fs.hres;
+    }
     const uint32_t r = r0 + blockIdx.y;
     const FoldDesc F = descs[r];
     if (blockIdx.x >= F.groups) return;
@@ -498,10 +507,10 @@ void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64
 }
 void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_t max_groups, const Partial* part,
                         const uint32_t* phist, Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres,
-                        uint32_t epoch, hipStream_t st) {
+                        uint32_t epoch, hipStream_t st, const FieldSet& fs, uint32_t nfields) {
     if (!nr || !max_groups) return;
-    hipLaunchKernelGGL(dev::k_fold_rounds, dim3(max_groups, nr), dim3(256), 0, st, descs, r0, part, phist, fpart, acc,
-                       ghist, hres, epoch);
+    hipLaunchKernelGGL(dev::k_fold_rounds, dim3(max_groups, nr, nfields), dim3(256), 0, st, descs, r0, part, phist,
+                       fpart, acc, ghist, hres, epoch, fs);
 }
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
                       uint32_t* hres, uint32_t epoch, hipStream_t st) {

@@ -81,7 +81,38 @@ struct RoundDesc {
     uint64_t hist_off;  // first word of this round's per-workgroup histogram windows
     uint32_t hist_win;  // bins of each workgroup's stored (partial) window (power of 2)
     uint32_t lds_win;   // bins of each workgroup's LDS histogram (power of 2, >= hist_win)
+    uint32_t layer;     // index of this round in the plan
+    uint32_t pad;
     FastDiv tpb;        // targets per full block along the round axis, B[d] / (2h)
+};
+
+// Multi-field calls: F fields of one shape compressed with one plan, each
+// per-round kernel covering all of them with the field index as a grid
+// dimension.  The plan tables are shared; what depends on a field's value
+// range (pivot and layer error bounds, layer slopes, the range itself) comes
+// from its FieldDesc, and every per-call buffer holds one slice per field at
+// the FieldSet's stride (in elements of that buffer).  fd == nullptr: a
+// single field, described by the kernel's own arguments.
+struct FieldRound {
+    double inv_half_eb, eb_d, neg_two_eb;  // as RoundDesc
+    float eb, layer_slope;
+};
+struct FieldDesc {
+    float* work;               // the field, rewritten with its reconstruction
+    double inv_half_pe, two_pe;  // pivot quantizer (as PivotDesc)
+    float range;               // max_val - min_val
+    uint32_t pad;
+    FieldRound r[ASZ_MAX_LAYERS];
+};
+struct FieldSet {
+    const FieldDesc* fd;
+    uint64_t sym;    // t array + QB arrays (int32)
+    uint64_t tblk;   // per-block float t
+    uint64_t part;   // round partials
+    uint64_t phist;  // per-workgroup histogram windows (words)
+    uint64_t fpart;  // pivot partials then fold partials
+    uint64_t ghist;  // global histogram windows (words)
+    uint64_t hres;   // host result slots (words)
 };
 
 constexpr uint32_t kRoundRow = 0;     // round axis 2: waves walk lattice rows along DIM2

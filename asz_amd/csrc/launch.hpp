@@ -41,21 +41,38 @@ struct SlopeSampleDesc {
 void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st);
 void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st);
 // compress: one Partial per 256 pivots into part[0 ..]; decompress: part unused.
-void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st);
+// With fs.fd set, nfields fields (compress only; the pointers are field 0's slices).
+void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st,
+                   const FieldSet& fs = FieldSet{}, uint32_t nfields = 1);
 // part / phist: this round's per-workgroup partials and histogram windows (at
-// R.part_off); gwin: the round array's global histogram window.
+// R.part_off); gwin: the round array's global histogram window.  With fs.fd
+// set, one launch covers nfields fields.
 void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
                            const float* tblk, const SweepSeg* segs, Partial* part, uint32_t* phist, uint32_t* gwin,
-                           hipStream_t st);
+                           hipStream_t st, const FieldSet& fs = FieldSet{}, uint32_t nfields = 1);
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
                              const float* tblk, const SweepSeg* segs, hipStream_t st);
+// Rounds of at least kStageMinCount targets whose serialization order runs
+// along neither DIM2 nor the round axis are compressed with their QBs in
+// "012" order into a staging array (R.order = 2, R.qb_off = 0) and moved into
+// their order by launch_qb_order (to_order); decompression moves them back
+// first.  staged / ordered: the round array's first QB in each layout (field
+// f at f * stride); cmax: the largest per-axis block counts of the round.
+constexpr uint64_t kStageMinCount = 1ull << 18;
+bool round_needs_staging(const RoundDesc& R);
+void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cmax[3], int32_t* staged,
+                     int32_t* ordered, uint64_t staged_stride, uint64_t ordered_stride, uint32_t nfields,
+                     bool to_order, hipStream_t st);
 // Resets the accumulators: errors and fallbacks to 0, every QB range to empty.
 void launch_init_acc(DevAcc* acc, hipStream_t st);
 
 // grad: gradient-tuned candidates (projected Newton on the sampled residual
 // energy, snapped to the t grid) instead of the five block candidates.
-void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
-                 unsigned int* fallbacks, uint64_t nblocks, bool grad, hipStream_t st);
+// Fallback blocks are counted in acc->fallbacks.  With fs.fd set, one launch
+// covers nfields fields (acc: one DevAcc per field).
+void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int, DevAcc* acc,
+                 uint64_t nblocks, bool grad, hipStream_t st, const FieldSet& fs = FieldSet{},
+                 uint32_t nfields = 1);
 void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st);
 
 // Residual energy of round R per block (forward: E[b], FP64) or its adjoint
@@ -110,9 +127,10 @@ inline void make_fold_descs(const std::vector<RoundDesc>& rounds, std::vector<Fo
 }
 // Rounds r0 .. r0 + nr - 1 in one launch; max_groups = their largest groups.
 // hres: device address of the pinned host result slots (kHostSlotWords per array).
+// With fs.fd set, one launch covers nfields fields (acc: one DevAcc per field).
 void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_t max_groups, const Partial* part,
                         const uint32_t* phist, Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres,
-                        uint32_t epoch, hipStream_t st);
+                        uint32_t epoch, hipStream_t st, const FieldSet& fs = FieldSet{}, uint32_t nfields = 1);
 // Finishes the t array (k = 0, range and fallbacks from acc) or the pivot
 // array (k = 1, folding the pivot kernel's nparts Partials) into its host slot.
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,

@@ -66,6 +66,7 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
     for (int r = 0; r < g.L; r++) {
         const int d = g.axis[r];
         RoundDesc R{};
+        R.layer = (uint32_t)r;
         R.d = (uint32_t)d;
         R.order = g.order[r];
         R.h = (uint32_t)(s[d] / 2);

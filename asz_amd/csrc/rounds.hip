@@ -69,8 +69,12 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; k++) { mn = min(mn, s_mn[k]); mx = max(mx, s_mx[k]); bad |= s_bad[k]; }
-        atomicMin(&out[0], mn);
-        atomicMax(&out[1], mx);
+        // only a workgroup that improves on the value already there issues the
+        // atomic: thousands of atomics on one address serialise in L2 (a stale
+        // read only costs a redundant atomic)
+        const volatile uint32_t* vo = out;
+        if (mn < vo[0]) atomicMin(&out[0], mn);
+        if (mx > vo[1]) atomicMax(&out[1], mx);
         if (bad) atomicOr(&out[2], 1u);
     }
 }
@@ -99,11 +103,13 @@ __global__ __launch_bounds__(256) void k_zero_index(const float* __restrict__ x,
 // ------------------------------------------------------------------ partial reductions
 
 // Per-workgroup results of a compress round kernel, stored at the
-// workgroup's linear index: one Partial (max |x - x'|, sum (x - x')^2, QB
-// range) and the LDS histogram window.  Called by every thread after its last
-// QB; the barrier inside also completes the LDS histogram.
+// workgroup's linear index wgl within the round: one Partial (max |x - x'|,
+// sum (x - x')^2, QB range) and the LDS histogram window.  Called by every
+// thread after its last QB; the barrier inside also completes the LDS
+// histogram.
 __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, int32_t qmax, const RoundDesc& R,
-                                          Partial* part, const uint32_t* lh, uint32_t* phist, uint32_t* gwin) {
+                                          uint64_t wgl, Partial* part, const uint32_t* lh, uint32_t* phist,
+                                          uint32_t* gwin) {
     __shared__ double s_err[4], s_sq[4];
     __shared__ int32_t s_min[4], s_max[4];
     err = wave_max(err);
@@ -113,8 +119,7 @@ __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, i
     const int w = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
     __syncthreads();
-    const uint64_t wg =
-        R.part_off + blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
+    const uint64_t wg = R.part_off + wgl;
     if (threadIdx.x == 0) {
         for (int k = 1; k < 4; k++) {
             err = fmax(err, s_err[k]);
@@ -159,9 +164,18 @@ __device__ __forceinline__ void pivot_coords(const PivotDesc& P, uint64_t i, uin
 }
 
 __global__ __launch_bounds__(256) void k_pivots_compress(float* __restrict__ work, int32_t* __restrict__ qb,
-                                                         PivotDesc P, Partial* __restrict__ part) {
+                                                         PivotDesc P, Partial* __restrict__ part, FieldSet fs) {
     __shared__ double s_err[4], s_sq[4];
     __shared__ int32_t s_min[4], s_max[4];
+    if (fs.fd) {  // field blockIdx.y of a multi-field call
+        const uint32_t f = blockIdx.y;
+        const FieldDesc& F = fs.fd[f];
+        work = F.work;
+        qb += f * fs.sym;
+        part += f * fs.fpart;
+        P.inv_half_pe = F.inv_half_pe;
+        P.two_pe = F.two_pe;
+    }
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
@@ -225,8 +239,19 @@ template <bool QUIN, bool DEC, int ORDER>
 __global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
                                                    const uint32_t* __restrict__ tabs, const float* __restrict__ tblk,
                                                    Partial* part, uint32_t* __restrict__ phist,
-                                                   uint32_t* __restrict__ gwin) {
+                                                   uint32_t* __restrict__ gwin, FieldSet fs) {
     extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
+    if (!DEC && fs.fd) {  // field blockIdx.y of a multi-field call
+        const uint32_t f = blockIdx.y;
+        const FieldDesc& F = fs.fd[f];
+        work = F.work;
+        qb += f * fs.sym;
+        tblk += f * fs.tblk;
+        part += f * fs.part;
+        phist += f * fs.phist;
+        gwin += f * fs.ghist;
+        field_round(F, R);
+    }
     constexpr int RU = (int)kRowsPerWave;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t win = R.lds_win;
@@ -312,7 +337,7 @@ __global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restric
             }
         }
     }
-    if (!DEC) finish_wg(err, sq, qmin, qmax, R, part, lh, phist, gwin);
+    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist, gwin);
 }
 
 // ------------------------------------------------------------------ sweep rounds (axis 0 / 1)
@@ -325,8 +350,22 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                                                      const uint32_t* __restrict__ tabs,
                                                      const float* __restrict__ tblk, const SweepSeg* __restrict__ segs,
                                                      Partial* part, uint32_t* __restrict__ phist,
-                                                     uint32_t* __restrict__ gwin) {
+                                                     uint32_t* __restrict__ gwin, FieldSet fs) {
     constexpr int OA = 1 - AX;  // the non-round axis other than DIM2
+    // grid z: field f's segments at f * nseg .. (f + 1) * nseg - 1
+    uint32_t sz = blockIdx.z;
+    if (!DEC && fs.fd) {
+        const uint32_t f = blockIdx.z / R.nseg;
+        sz -= f * R.nseg;
+        const FieldDesc& F = fs.fd[f];
+        work = F.work;
+        qb += f * fs.sym;
+        tblk += f * fs.tblk;
+        part += f * fs.part;
+        phist += f * fs.phist;
+        gwin += f * fs.ghist;
+        field_round(F, R);
+    }
     __shared__ int32_t tile[WMODE == 1 ? 4 : 1][WMODE == 1 ? kSweepSeg : 1][65];
     __shared__ uint64_t lbase[WMODE == 1 ? 4 : 1][64];
     extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
@@ -335,7 +374,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
         for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) lh[i] = 0;
         if (WMODE != 1) __syncthreads();
     }
-    const SweepSeg S = segs[R.seg_off + blockIdx.z];
+    const SweepSeg S = segs[R.seg_off + sz];
     const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ko = blockIdx.y;
     const bool valid = k2 < R.n[2];
@@ -438,7 +477,79 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
                 if (kl < R.n[2]) qb[lbase[wave][jl] + c] = tile[wave][c][jl];
             }
         }
-        finish_wg(err, sq, qmin, qmax, R, part, lh, phist, gwin);
+        finish_wg(err, sq, qmin, qmax, R, blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * sz),
+                  part, lh, phist, gwin);
+    }
+}
+
+// ------------------------------------------------------------------ QB order moves
+
+// Rounds whose serialization order runs along neither DIM2 nor the round axis
+// ("120" / "021" on the row kernel, the other lattice axis on the sweep
+// kernel) would store their QBs 4 bytes at a time, one cache line per lane.
+// Their round kernels write the QBs in "012" order instead (coalesced, into a
+// staging array), and k_qb_order moves every block's array into its order;
+// before a decompression round it moves them back.  Within a block of counts
+// (c0, c1, c2) the "012" index of (j0, j1, j2) is (j0 c1 + j1) c2 + j2, so
+// order "120" is the transpose [c0][c1 c2] -> [c1 c2][c0] and order "021" the
+// transpose [c1][c2] -> [c2][c1] of each j0 slab: a = p Q + q  <->  b = q P + p
+// per slab.  A workgroup moves one kXT x kXT tile of one slab through LDS;
+// each side is accessed in its own row order, so whole rows (or, when the tile
+// spans the full row, the whole tile) are contiguous on both sides.
+constexpr uint32_t kXT = 64;
+
+template <bool TO_ORDER>
+__global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* __restrict__ tabs, int32_t* staged,
+                                                  int32_t* ordered, uint64_t staged_stride, uint64_t ordered_stride,
+                                                  uint32_t tiles) {
+    __shared__ int32_t tile[kXT][kXT + 1];
+    const uint32_t f = blockIdx.y;
+    const uint64_t bid = blockIdx.x / tiles;
+    const uint32_t t = (uint32_t)(blockIdx.x - bid * tiles);
+    const uint32_t b2 = (uint32_t)(bid % R.nb[2]), b1 = (uint32_t)((bid / R.nb[2]) % R.nb[1]),
+                   b0 = (uint32_t)(bid / ((uint64_t)R.nb[2] * R.nb[1]));
+    const uint32_t bb[3] = {b0, b1, b2};
+    uint32_t cnt[3], pre[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        cnt[a] = tabs[R.tab_off[a] + bb[a]];
+        pre[a] = tabs[R.tab_off[a] + R.nb[a] + bb[a]];
+    }
+    const uint64_t base = block_base(R, cnt, pre);
+    const bool o120 = R.order == 0;
+    const uint32_t slabs = o120 ? 1 : cnt[0], P = o120 ? cnt[0] : cnt[1];
+    const uint64_t Q = o120 ? (uint64_t)cnt[1] * cnt[2] : cnt[2];
+    if (slabs == 0 || P == 0 || Q == 0) return;  // a block without targets
+    const uint32_t tq_n = (uint32_t)((Q + kXT - 1) / kXT), tp_n = (P + kXT - 1) / kXT;
+    const uint32_t tq = t % tq_n, rest = t / tq_n, tp = rest % tp_n, slab = rest / tp_n;
+    if (slab >= slabs) return;
+    const uint32_t p0 = tp * kXT;
+    const uint64_t q0 = (uint64_t)tq * kXT;
+    const uint32_t np = min(kXT, P - p0), nq = (uint32_t)min<uint64_t>(kXT, Q - q0);
+    const uint64_t slab_off = base + (uint64_t)slab * P * Q;
+    int32_t* A = staged + f * staged_stride + slab_off;   // "012": p Q + q
+    int32_t* B = ordered + f * ordered_stride + slab_off;  // order:  q P + p
+    const uint32_t n = np * nq;
+    if (TO_ORDER) {
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t pl = e / nq, ql = e - pl * nq;
+            tile[pl][ql] = A[(uint64_t)(p0 + pl) * Q + q0 + ql];
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t ql = e / np, pl = e - ql * np;
+            B[(q0 + ql) * P + p0 + pl] = tile[pl][ql];
+        }
+    } else {
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t ql = e / np, pl = e - ql * np;
+            tile[pl][ql] = B[(q0 + ql) * P + p0 + pl];
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t pl = e / nq, ql = e - pl * nq;
+            A[(uint64_t)(p0 + pl) * Q + q0 + ql] = tile[pl][ql];
+        }
     }
 }
 
@@ -455,10 +566,11 @@ void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, h
     hipLaunchKernelGGL(dev::k_zero_index, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out2);
 }
 
-void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st) {
-    const dim3 grid((uint32_t)((P.count + 255) / 256));
+void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st,
+                   const FieldSet& fs, uint32_t nfields) {
+    const dim3 grid((uint32_t)((P.count + 255) / 256), compress ? nfields : 1);
     if (compress)
-        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, part);
+        hipLaunchKernelGGL(dev::k_pivots_compress, grid, dim3(256), 0, st, work, qb, P, part, fs);
     else
         hipLaunchKernelGGL(dev::k_pivots_decompress, grid, dim3(256), 0, st, work, (const int32_t*)qb, P);
 }
@@ -478,12 +590,14 @@ struct RoundArgs {
     Partial* part;
     uint32_t* phist;
     uint32_t* gwin;
+    FieldSet fs;
+    uint32_t nf;  // fields (grid dimension)
 };
 
 template <bool Q, bool DEC, int ORDER>
 void row(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((dev::k_round_row<Q, DEC, ORDER>), dim3((uint32_t)R.nwg), dim3(256), DEC ? 0 : R.lds_win * 4,
-                       st, R, a.w, a.qb, a.tabs, a.tblk, a.part, a.phist, a.gwin);
+    hipLaunchKernelGGL((dev::k_round_row<Q, DEC, ORDER>), dim3((uint32_t)R.nwg, a.nf), dim3(256),
+                       DEC ? 0 : R.lds_win * 4, st, R, a.w, a.qb, a.tabs, a.tblk, a.part, a.phist, a.gwin, a.fs);
 }
 
 template <bool Q, bool DEC>
@@ -495,9 +609,9 @@ void row_order(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
 
 template <int AX, bool Q, bool DEC, int WM>
 void sweep(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
-    dim3 grid((R.n[2] + 255) / 256, R.n[1 - AX], R.nseg);
+    dim3 grid((R.n[2] + 255) / 256, R.n[1 - AX], R.nseg * a.nf);
     hipLaunchKernelGGL((dev::k_round_sweep<AX, Q, DEC, WM>), grid, dim3(256), DEC ? 0 : R.lds_win * 4, st, R, a.w,
-                       a.qb, a.tabs, a.tblk, a.segs, a.part, a.phist, a.gwin);
+                       a.qb, a.tabs, a.tblk, a.segs, a.part, a.phist, a.gwin, a.fs);
 }
 
 template <bool DEC>
@@ -531,15 +645,46 @@ void dispatch(const RoundDesc& R, bool q, const RoundArgs& a, hipStream_t st) {
 
 void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_t* qb, const uint32_t* tabs,
                            const float* tblk, const SweepSeg* segs, Partial* part, uint32_t* phist, uint32_t* gwin,
-                           hipStream_t st) {
+                           hipStream_t st, const FieldSet& fs, uint32_t nfields) {
     if (R.count == 0) return;
-    dispatch<false>(R, quintic, RoundArgs{work, qb, tabs, tblk, segs, part, phist, gwin}, st);
+    dispatch<false>(R, quintic, RoundArgs{work, qb, tabs, tblk, segs, part, phist, gwin, fs, nfields}, st);
+}
+
+bool round_needs_staging(const RoundDesc& R) {
+    if (R.count < kStageMinCount || R.order == 2) return false;
+    if (R.kind == kRoundRow) return true;  // "120" / "021" along DIM2
+    const uint32_t fastest = R.order == 0 ? 0 : 1;
+    return fastest != R.d;  // the sweep's scattered write mode
+}
+
+uint32_t qb_order_tiles(const RoundDesc& R, const uint32_t cmax[3]) {
+    const bool o120 = R.order == 0;
+    const uint64_t slabs = o120 ? 1 : cmax[0], P = o120 ? cmax[0] : cmax[1];
+    const uint64_t Q = o120 ? (uint64_t)cmax[1] * cmax[2] : cmax[2];
+    return (uint32_t)(slabs * ((P + dev::kXT - 1) / dev::kXT) * ((Q + dev::kXT - 1) / dev::kXT));
+}
+
+void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cmax[3], int32_t* staged,
+                     int32_t* ordered, uint64_t staged_stride, uint64_t ordered_stride, uint32_t nfields,
+                     bool to_order, hipStream_t st) {
+    if (R.count == 0) return;
+    const uint32_t tiles = qb_order_tiles(R, cmax);
+    const uint64_t nbl = (uint64_t)R.nb[0] * R.nb[1] * R.nb[2];
+    if (nbl * tiles > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
+    const dim3 grid((uint32_t)(nbl * tiles), nfields);
+    if (to_order)
+        hipLaunchKernelGGL(dev::k_qb_order<true>, grid, dim3(256), 0, st, R, tabs, staged, ordered, staged_stride,
+                           ordered_stride, tiles);
+    else
+        hipLaunchKernelGGL(dev::k_qb_order<false>, grid, dim3(256), 0, st, R, tabs, staged, ordered, staged_stride,
+                           ordered_stride, tiles);
 }
 
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
                              const float* tblk, const SweepSeg* segs, hipStream_t st) {
     if (R.count == 0) return;
-    dispatch<true>(R, quintic, RoundArgs{out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, nullptr, nullptr},
+    dispatch<true>(R, quintic,
+                   RoundArgs{out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, nullptr, nullptr, FieldSet{}, 1},
                    st);
 }
 

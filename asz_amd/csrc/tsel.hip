@@ -96,11 +96,25 @@ __device__ __forceinline__ double entropy_term(uint32_t cnt, float rec) {
 template <bool GRAD>
 __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
                                                        const uint32_t* __restrict__ ss_tab, float* tblk,
-                                                       int32_t* t_int, unsigned int* fallbacks) {
+                                                       int32_t* t_int, DevAcc* acc, FieldSet fs) {
     __shared__ TselShared sh;
     __shared__ float s_cand[kNCand];
     __shared__ int s_nc;
     const RoundDesc& R = T.R;
+    // the field's layer error bound, layer slope and value range (kept out of
+    // T, so that T stays in kernel arguments)
+    float eb = R.eb, layer_slope = R.layer_slope, range = T.range;
+    if (fs.fd) {  // field blockIdx.y of a multi-field call
+        const uint32_t f = blockIdx.y;
+        const FieldDesc& F = fs.fd[f];
+        work = F.work;
+        tblk += f * fs.tblk;
+        t_int += f * fs.sym;
+        acc += f;
+        eb = F.r[R.layer].eb;
+        layer_slope = F.r[R.layer].layer_slope;
+        range = F.range;
+    }
     const uint32_t bid = blockIdx.x;
     const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
     const uint32_t bb[3] = {b0, b1, b2};
@@ -123,9 +137,8 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
             cnt[a] = (hi[a] - lo[a] + step[a] - 1) / step[a];
             ns *= cnt[a];
         }
-        const float stat_eb = T.stat_eb_ratio * R.eb;
+        const float stat_eb = T.stat_eb_ratio * eb;
         const float neg2 = 0.5f / stat_eb;
-        const float eb = R.eb;
         const float neg_2_eb = 0.5f / eb;
         const float two_eb = eb * 2.0f;
         for (int i = threadIdx.x; i < kNCand * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         __syncthreads();
         if (threadIdx.x == 0) {
             float h1[kNCand], psnr[kNCand];
-            const float r2 = T.range * T.range;
+            const float r2 = range * range;
             for (int k = 0; k < nc; k++) {
                 double hs = 0, ms = 0;
                 for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][kNCand + k]; }
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
             int best = GRAD ? 0 : 1;
             for (int k = 0; k < nc; k++) {
                 if (cand[k] == cand[best] || psnr[best] > FLT_MAX) continue;
-                const float sc = __fmaf_rn(h1[best] - h1[k], R.layer_slope, psnr[k] - psnr[best]);
+                const float sc = __fmaf_rn(h1[best] - h1[k], layer_slope, psnr[k] - psnr[best]);
                 if (sc > 0.0f || psnr[k] > FLT_MAX) best = k;
             }
             best_t = cand[best];
@@ -233,7 +246,7 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
     if (threadIdx.x == 0) {
         tblk[bid] = best_t;
         t_int[(uint64_t)bid * T.L + T.r] = (int32_t)lround_f64(best_t * T.t_precision);
-        if (fallback) atomicAdd(fallbacks, 1u);
+        if (fallback) atomicAdd(&acc->fallbacks, 1u);
     }
 }
 
@@ -305,14 +318,14 @@ __global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const f
 
 }  // namespace dev
 
-void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
-                 unsigned int* fallbacks, uint64_t nblocks, bool grad, hipStream_t st) {
+void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int, DevAcc* acc,
+                 uint64_t nblocks, bool grad, hipStream_t st, const FieldSet& fs, uint32_t nfields) {
+    const dim3 grid((uint32_t)nblocks, nfields);
     if (grad)
-        hipLaunchKernelGGL(dev::k_tsel<true>, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss,
-                           tblk, t_int, fallbacks);
+        hipLaunchKernelGGL(dev::k_tsel<true>, grid, dim3(dev::kTselThreads), 0, st, T, work, ss, tblk, t_int, acc, fs);
     else
-        hipLaunchKernelGGL(dev::k_tsel<false>, dim3((uint32_t)nblocks), dim3(dev::kTselThreads), 0, st, T, work, ss,
-                           tblk, t_int, fallbacks);
+        hipLaunchKernelGGL(dev::k_tsel<false>, grid, dim3(dev::kTselThreads), 0, st, T, work, ss, tblk, t_int, acc,
+                           fs);
 }
 
 void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,

@@ -100,6 +100,11 @@ typedef struct asz_stats {
  * kernel only (asz_stats.round_ms[value]), leaving every other launch without
  * events; a negative value turns kernel events off. */
 #define ASZ_OPT_TIMED_ROUND 3
+/* ASZ_OPT_BATCH_FIELDS: the most fields asz_compress_batch compresses
+ * together in one chunk of same-shape, same-plan fields (every per-round
+ * kernel of a chunk covers all of its fields).  0 (default): automatic;
+ * 1: every field on its own. */
+#define ASZ_OPT_BATCH_FIELDS 4
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 
 /* ------------------------------------------------------------------ context */
@@ -186,8 +191,11 @@ int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_byt
  * one field per s1_compress process, t1_start_compressors.py:482-542).  Each
  * field goes through load_input's range scan, get_default_slope and
  * param_c::compress as in asz_range_f32 + asz_default_slope + asz_compress,
+ * with the same result.  Fields of one shape and plan are compressed together
+ * in chunks (ASZ_OPT_BATCH_FIELDS; with ASZ_TSEL_GPU / ASZ_TSEL_GRAD), each
+ * per-round kernel of a chunk covering all of its fields, and the chunks run
  * on `concurrency` worker contexts (0: 4), each with its own HIP stream and
- * host thread, so the fields' pipelines overlap on the device.  d_fields[i]
+ * host thread, so their pipelines overlap on the device.  d_fields[i]
  * (device, dims[3i..3i+2]) is overwritten with its reconstruction; plans[i]
  * is that field's plan (time steps typically share one).  On success
  * streams[i] / stream_bytes[i] hold field i's pre-Zstd stream in host memory

@@ -1,7 +1,9 @@
 """Batched form: asz_compress_batch on fields of uneven shape (time steps
 sharing one plan, mixed shapes with their own) must give, field by field,
 exactly the stream the single-field path gives, and every stream decodes
-within the error bound."""
+within the error bound.  With one worker the five time steps go through the
+multi-field kernels in chunks of three and two fields; with three workers
+every chunk holds one field."""
 import sys
 
 import numpy as np
@@ -31,17 +33,17 @@ def _fields():
     return flds, plans
 
 
-@pytest.mark.parametrize("tsel", ["gpu", "grad"])
-def test_batch_equals_single_field_path(tsel):
+@pytest.mark.parametrize("tsel,concurrency", [("gpu", 3), ("grad", 3), ("gpu", 1), ("grad", 1)])
+def test_batch_equals_single_field_path(tsel, concurrency):
     flds, plans = _fields()
     originals = [f.clone() for f in flds]
     ctx = codec.Context(0)
     try:
-        print("batch of", len(flds), "fields, tsel", tsel, flush=True)
+        print("batch of", len(flds), "fields, tsel", tsel, "concurrency", concurrency, flush=True)
         sys.stdout.flush()
         work = [f.clone() for f in flds]
         streams, info = batch.compress_fields(ctx, work, [codec.make_plan(p) for p in plans], tsel=tsel,
-                                              concurrency=3)[:2]
+                                              concurrency=concurrency)[:2]
         params = codec.make_params(rel_eb=float(np.float32(1e-3)), block_pivot_ratio=3, family="cubic")
         for i, (f, pl) in enumerate(zip(originals, plans)):
             w = f.clone()
@@ -61,12 +63,14 @@ def test_batch_equals_single_field_path(tsel):
         ctx.close()
 
 
-def test_batch_error_names_the_field():
+@pytest.mark.parametrize("bad", [2, 3])
+def test_batch_error_names_the_field(bad):
+    # two workers: the time steps go in chunks [0, 1], [2, 3], [4]
     flds, plans = _fields()
-    flds[2].fill_(1.0)  # a constant field: max_val == min_val
+    flds[bad].fill_(1.0)  # a constant field: max_val == min_val
     ctx = codec.Context(0)
     try:
-        with pytest.raises(codec.AszError, match="field 2: max_val==min_val"):
+        with pytest.raises(codec.AszError, match=f"field {bad}: max_val==min_val"):
             batch.compress_fields(ctx, flds, [codec.make_plan(p) for p in plans], concurrency=2)
     finally:
         ctx.close()
