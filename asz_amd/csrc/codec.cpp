@@ -1313,10 +1313,9 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         const uint32_t T = std::max<uint32_t>(1, std::min<uint32_t>(concurrency ? concurrency : 4, nfields));
         // Fields of one shape and plan (time steps of a series, same-shape
         // fields) are compressed together in chunks: every per-round kernel of
-        // a chunk covers all of its fields.  Chunks hold about a 2T-th of their
-        // group (so the worker contexts stay busy to the end), at most
-        // kMaxChunk (or ASZ_OPT_BATCH_FIELDS) fields and kMaxChunkPoints
-        // points; the largest go first.
+        // a chunk covers all of its fields.  Chunks hold about a T-th of their
+        // group, at most kMaxChunk (or ASZ_OPT_BATCH_FIELDS) fields and
+        // kMaxChunkPoints points; the largest go first.
         const uint32_t kMaxChunk = c->batch_fields ? c->batch_fields : 16;
         constexpr uint64_t kMaxChunkPoints = 1ull << 30;
         const bool can_group = tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD;
@@ -1338,7 +1337,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         for (const auto& gr : groups) {
             const uint64_t* d = dims + 3 * (uint64_t)gr[0];
             const uint64_t n = std::max<uint64_t>(1, d[0] * d[1] * d[2]);
-            uint64_t cs = (gr.size() + 2 * T - 1) / (2 * T);
+            uint64_t cs = (gr.size() + T - 1) / T;
             cs = std::max<uint64_t>(1, std::min<uint64_t>({cs, kMaxChunk, kMaxChunkPoints / n}));
             for (size_t i0 = 0; i0 < gr.size(); i0 += cs)
                 chunks.emplace_back(gr.begin() + i0, gr.begin() + std::min<size_t>(gr.size(), i0 + cs));

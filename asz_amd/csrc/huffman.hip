@@ -81,13 +81,14 @@ __global__ __launch_bounds__(256) void k_hist_small(const int32_t* __restrict__ 
 }
 
 // Fold of rounds r0 .. r0 + gridDim.y - 1 (one launch after a round kernel
-// covers one round, or a batch of finished rounds).  Workgroup (g, y) owns
-// round r's per-workgroup results [g F.pper, (g+1) F.pper): it sums their
-// histogram windows (about 4096 words per workgroup, coalesced along the
-// bins), adds the nonzero sums to the array's global window, and folds their
-// Partials into fpart[F.fpart_off + g].  The last workgroup of the round to
-// finish (counted in acc->fold_count[r], which it resets) folds those into
-// acc->arr[2 + r] and the window's tail.
+// covers one round, or a batch of finished rounds; grid z: fields).
+// Workgroup (g, y), g = pg F.slices + slice, owns round r's per-workgroup
+// results [pg F.pper, (pg+1) F.pper) over one slice of at most 256 bins: it
+// sums their histogram windows (4096 words per workgroup, coalesced along the
+// bins) and adds the nonzero sums to the array's global window; the
+// first-slice workgroup folds their Partials into fpart[F.fpart_off + pg].
+// The last workgroup of the round to finish (counted in acc->fold_count[r],
+// which it resets) folds those into acc->arr[2 + r] and the window's tail.
 // Finishes array k for the host (whole workgroup): the tail, then the bins
 // of [qmin, qmax] from the global window gw when they lie inside it, into the
 // host slot; after a system-scope fence, the epoch into the slot's flag.
@@ -167,7 +168,9 @@ __global__ __launch_bounds__(256) void k_fold_rounds(const FoldDesc* __restrict_
     const FoldDesc F = descs[r];
     if (blockIdx.x >= F.groups) return;
     uint32_t* gw = ghist + (uint64_t)(2 + r) * kHistStride;
-    const uint64_t p0 = (uint64_t)blockIdx.x * F.pper, p1 = min<uint64_t>(F.nwg, p0 + F.pper);
+    // fold workgroup = (partial group pg, bin slice)
+    const uint32_t slice = blockIdx.x % F.slices, pg = blockIdx.x / F.slices;
+    const uint64_t p0 = (uint64_t)pg * F.pper, p1 = min<uint64_t>(F.nwg, p0 + F.pper);
     const uint32_t win = F.win;
     const uint32_t* src = phist + F.hist_off;
     uint32_t* g = gw + kHistCentre - win / 2;
@@ -184,24 +187,26 @@ __global__ __launch_bounds__(256) void k_fold_rounds(const FoldDesc* __restrict_
             if (s) atomicAdd(&g[bin], s);
         }
     } else {
-        for (uint32_t bin = threadIdx.x; bin < win; bin += blockDim.x) {
-            uint32_t s = 0;
-#pragma unroll 4
-            for (uint64_t p = p0; p < p1; p++) s += src[p * win + bin];
-            if (s) atomicAdd(&g[bin], s);
-        }
+        // one bin of this workgroup's 256-bin slice per thread
+        const uint32_t bin = slice * 256 + threadIdx.x;
+        uint32_t s = 0;
+#pragma unroll 8
+        for (uint64_t p = p0; p < p1; p++) s += src[p * win + bin];
+        if (s) atomicAdd(&g[bin], s);
     }
     Partial* fp = fpart + F.fpart_off;
-    const Partial mine = fold_partials(part + F.part_off, p0, p1);
+    if (slice == 0) {  // uniform per workgroup
+        const Partial mine = fold_partials(part + F.part_off, p0, p1);
+        if (threadIdx.x == 0) fp[pg] = mine;
+    }
     if (threadIdx.x == 0) {
-        fp[blockIdx.x] = mine;
         __threadfence();
         s_last = atomicAdd(&acc->fold_count[r], 1u) == F.groups - 1;
     }
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    const Partial all = fold_partials(fp, 0, F.groups);
+    const Partial all = fold_partials(fp, 0, F.groups / F.slices);
     if (threadIdx.x == 0) {
         acc->rerr[r] = ErrAcc{all.max_err, all.sq};
         acc->arr[2 + r].qmin = all.qmin;

@@ -129,12 +129,16 @@ constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave process
 // round kernel's main loop: gfx9 counts them in vmcnt with the loads, so a
 // slow one would stall every later load.  Rounds with few workgroups (the
 // coarse ones, whose QBs spread widest) get the widest partial windows, so the
-// partials stay near kHistPartialWords words per round.  Arrays whose QB
-// range leaves the global window are histogrammed again by a dedicated pass.
+// partials stay near a round's budget: a quarter word per target, between
+// kHistPartialMin and kHistPartialWords words (the partials are written once
+// and folded once, so a larger budget costs HBM traffic on small rounds,
+// while every QB outside the stored window is one more atomic at most).
+// Arrays whose QB range leaves the global window are histogrammed again by a
+// dedicated pass.
 constexpr int kHistWin = 128;                   // window of the t / pivot arrays
 constexpr uint32_t kHistWinMin = 128, kHistWinMax = 4096;
 constexpr uint32_t kLdsWinMin = 2048;
-constexpr uint64_t kHistPartialWords = 1 << 20;
+constexpr uint64_t kHistPartialWords = 1 << 20, kHistPartialMin = 1 << 15;
 constexpr uint32_t kHistGlobal = 65536;
 constexpr int32_t kHistCentre = 32768;
 

@@ -142,7 +142,8 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.part_off = P.total_partials;
         P.total_partials += R.nwg;
         uint32_t win = kHistWinMin;
-        while (win < kHistWinMax && (uint64_t)win * 2 * std::max<uint64_t>(R.nwg, 1) <= kHistPartialWords) win *= 2;
+        const uint64_t budget = std::min(kHistPartialWords, std::max(kHistPartialMin, R.count / 4));
+        while (win < kHistWinMax && (uint64_t)win * 2 * std::max<uint64_t>(R.nwg, 1) <= budget) win *= 2;
         R.hist_win = win;
         R.lds_win = std::max(win, kLdsWinMin);
         R.hist_off = P.total_hist_words;

@@ -45,12 +45,12 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
     const uint64_t n4 = n / 4;
     const float4* x4 = reinterpret_cast<const float4*>(x);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        float4 v = x4[i];
-        float e[4] = {v.x, v.y, v.z, v.w};
+        const float4 v = x4[i];
+        const float e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             bad |= !isfinite(e[k]);
-            uint32_t key = order_key(e[k]);
+            const uint32_t key = order_key(e[k]);
             mn = min(mn, key);
             mx = max(mx, key);
         }
