@@ -105,6 +105,13 @@ class Context:
         if rc != 0:
             raise AszError(rc, self._L.asz_last_error(self._c).decode())
 
+    def after_torch(self):
+        """Order the context's next work after what torch has enqueued on its
+        current stream (the inputs' producers), without a host wait."""
+        import torch
+        if torch.cuda.is_initialized():
+            self._check(self._L.asz_stream_wait(self._c, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
     def set_kernel_events(self, on: bool = True):
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_KERNEL_EVENTS, 1 if on else 0))
 
@@ -125,6 +132,7 @@ class Context:
     # ------------------------------------------------------------ device helpers
     def range(self, d_ptr: int, n: int):
         mn, mx = ctypes.c_float(), ctypes.c_float()
+        self.after_torch()
         self._check(self._L.asz_range_f32(self._c, ctypes.c_void_p(d_ptr), n, ctypes.byref(mn), ctypes.byref(mx)))
         return mn.value, mx.value
 
@@ -132,6 +140,7 @@ class Context:
         d = np.asarray(dims, np.uint64)
         s, inv = ctypes.c_float(), ctypes.c_float()
         fb, reason = ctypes.c_int(), ctypes.c_int()
+        self.after_torch()
         self._check(self._L.asz_default_slope(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(params), mn, mx,
                                               ctypes.byref(s), ctypes.byref(inv), ctypes.byref(fb),
                                               ctypes.byref(reason)))
@@ -145,6 +154,7 @@ class Context:
         winner's predictor name ("" when none had samples) and best_dist."""
         d = np.asarray(dims, np.uint64)
         plan, rep = Plan(), _lib.PlanReport()
+        self.after_torch()
         self._check(self._L.asz_generate_plan(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(params), mn, mx,
                                               max_layer_num, full_predictor_search_layer_num, sample_stride,
                                               ctypes.byref(plan), ctypes.byref(rep)))
@@ -169,6 +179,7 @@ class Context:
         o = _lib.FamilyOpts(suffix_layers, sample_blocks, sample_point_limit, float(np.float32(sample_fraction_limit)),
                             float(np.float32(fine_eb_threshold)), coarse_probe_bpr, fine_probe_bpr)
         rep = _lib.FamilyReport()
+        self.after_torch()
         self._check(self._L.asz_select_family(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
                                               ctypes.byref(params), mn, mx, ctypes.byref(o), ctypes.byref(rep)))
         return rep
@@ -194,6 +205,7 @@ class Context:
         qb = np.zeros(N, np.int32) if want_qb else None
         t = np.zeros(nt, np.int64) if want_t else None
         sp, sn = ctypes.c_void_p(), ctypes.c_uint64()
+        self.after_torch()
         self._check(self._L.asz_compress(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
                                          ctypes.byref(params), mn, mx, TSEL[tsel], _ptr(ti), ctypes.byref(sp),
                                          ctypes.byref(sn), _ptr(qb), _ptr(t)))
@@ -257,6 +269,7 @@ class Context:
         nbytes = (ctypes.c_uint64 * n)()
         info = (_lib.BatchField * n)()
         try:
+            self.after_torch()
             self._check(self._L.asz_compress_batch(self._c, n, ptrs, _ptr(dims), pl, ctypes.byref(params),
                                                    TSEL[tsel], concurrency, outs, nbytes, info))
         finally:
@@ -293,6 +306,7 @@ class Context:
         torch.cuda.synchronize()
         qb = np.zeros(N, np.int32) if want_qb else None
         t = np.zeros(nt, np.int64) if want_t else None
+        self.after_torch()
         self._check(self._L.asz_decompress(self._c, _ptr(buf), buf.size, _ptr(d), ctypes.byref(params),
                                            ctypes.c_void_p(dev.data_ptr()), _ptr(qb), _ptr(t)))
         out[...] = dev.cpu().numpy().reshape(out.shape)

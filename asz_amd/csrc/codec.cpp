@@ -1121,6 +1121,7 @@ void asz_ctx_destroy(asz_ctx* c) {
     if (c->hres_host) (void)hipHostFree(c->hres_host);
     if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
+    if (c->order_ev) (void)hipEventDestroy(c->order_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1153,6 +1154,15 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
             c->kev_tag.resize(256);
             for (auto& e : c->kev) HIPCHECK(hipEventCreate(&e));
         }
+    });
+}
+
+int asz_stream_wait(asz_ctx* c, void* stream) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        if (!c->order_ev) HIPCHECK(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(c->order_ev, static_cast<hipStream_t>(stream)));
+        HIPCHECK(hipStreamWaitEvent(c->stream, c->order_ev, 0));
     });
 }
 
@@ -1356,6 +1366,11 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
             c->kids.push_back(k);
         }
         const bool on_device = c->stream_on_device;
+        // the workers' streams start after everything enqueued on this context's
+        // stream (asz_stream_wait orders that after the caller's)
+        if (!c->order_ev) HIPCHECK(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(c->order_ev, c->stream));
+        for (uint32_t t = 0; t < W; t++) HIPCHECK(hipStreamWaitEvent(c->kids[t]->stream, c->order_ev, 0));
         std::atomic<uint32_t> next{0};
         std::mutex err_mu;
         std::string first_err;

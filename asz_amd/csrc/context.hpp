@@ -98,6 +98,7 @@ struct asz_ctx {
         chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc;
     hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
+    hipEvent_t order_ev = nullptr;  // asz_stream_wait / batch workers: cross-stream ordering
     // host result slots (pinned, written by the GPU): kHostSlotWords per array
     uint32_t* hres_host = nullptr;
     uint32_t* hres_dev = nullptr;

@@ -255,13 +255,14 @@ constexpr uint32_t kEncMaxArrays = ASZ_MAX_LAYERS + 2;
 // and the last partial word (shared with the next) are ORed into the LDS
 // buffer, the others are stored.  The chunk's bits go to the start of its
 // own scratch slot, its bit count to chunk_bits.  Codes longer than 32 bits
-// (short_codes false) are ORed straight into the zeroed scratch slot.
+// (short_codes false) are ORed straight into the zeroed scratch slot.  The
+// kernel is held to 128 VGPRs so that four workgroups share a CU.
 template <bool short_codes>
-__global__ __launch_bounds__(256) void k_encode_chunks(const int32_t* __restrict__ sym,
-                                                       const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                       uint64_t nchunks, const uint64_t* __restrict__ book,
-                                                       uint32_t* scratch, uint64_t slot_words,
-                                                       uint32_t* __restrict__ chunk_bits) {
+__global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restrict__ sym,
+                                                          const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                          uint64_t nchunks, const uint64_t* __restrict__ book,
+                                                          uint32_t* scratch, uint64_t slot_words,
+                                                          uint32_t* __restrict__ chunk_bits) {
     __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
     __shared__ uint32_t lbook[kLdsBook];
     __shared__ HuffArrayDev sA[kEncMaxArrays];
@@ -435,8 +436,15 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
     const uint64_t per = (A.nchunk + 1023) / 1024;
     const uint64_t r0 = min<uint64_t>(A.nchunk, threadIdx.x * per), r1 = min<uint64_t>(A.nchunk, r0 + per);
     const uint32_t* cb = chunk_bits + A.chunk0;
+    // a thread's range in batches of 8 loads in flight
     uint64_t mine = 0;
-    for (uint64_t i = r0; i < r1; i++) mine += cb[i];
+    for (uint64_t i = r0; i < r1; i += 8) {
+        uint32_t t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = i + u < r1 ? cb[i + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) mine += t[u];
+    }
     uint64_t incl = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -448,9 +456,15 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
     uint64_t off = A.payload_bit + incl - mine;
     for (uint32_t w = 0; w < wave; w++) off += wsum[w];
     uint64_t* co = chunk_off + A.chunk0;
-    for (uint64_t i = r0; i < r1; i++) {
-        co[i] = off;
-        off += cb[i];
+    for (uint64_t i = r0; i < r1; i += 8) {
+        uint32_t t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = i + u < r1 ? cb[i + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (i + u < r1) co[i + u] = off;
+            off += t[u];
+        }
     }
 }
 

@@ -48,6 +48,7 @@ class RoundState:
         mn, mx = ctx.range(self.work.data_ptr(), self.work.numel())
         ti = np.ascontiguousarray(t_int, np.int64)
         L = _lib.lib()
+        ctx.after_torch()
         ctx._check(L.asz_prepare_round(ctx._c, _p(self.work), self.dims.ctypes.data_as(ctypes.c_void_p),
                                        ctypes.byref(plan), ctypes.byref(params), mn, mx, self.round,
                                        ti.ctypes.data_as(ctypes.c_void_p)))
@@ -63,6 +64,7 @@ class RoundState:
         E = torch.empty(self.nblocks, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         L = _lib.lib()
+        self.ctx.after_torch()
         self.ctx._check(L.asz_round_energy(self.ctx._c, _p(self.work), self.dims.ctypes.data_as(ctypes.c_void_p),
                                            ctypes.byref(self.plan), ctypes.byref(self.params), self.round,
                                            int(self.quintic), _p(th), _p(E)))
@@ -74,6 +76,7 @@ class RoundState:
         g = torch.empty_like(th)
         torch.cuda.synchronize()
         L = _lib.lib()
+        self.ctx.after_torch()
         self.ctx._check(L.asz_round_energy_adjoint(
             self.ctx._c, _p(self.work), self.dims.ctypes.data_as(ctypes.c_void_p), ctypes.byref(self.plan),
             ctypes.byref(self.params), self.round, int(self.quintic), _p(th), _p(ge), _p(g)))

@@ -46,12 +46,17 @@ HBM_PEAK_GBPS = 8000.0
 
 
 def dom_kernel_name(plan, r):
-    """Template name of the round kernel that runs round r of a cubic plan."""
+    """Template name of the round kernel that runs round r of a cubic plan (a
+    large round whose order runs along neither DIM2 nor the round axis writes
+    its QBs in "012" order and k_qb_order moves them into place)."""
     axis, order = int(plan[r][5]), ("120", "021", "012").index(plan[r][7:])
     if axis == 2:
-        return f"k_round_row<false, false, {order}>"
+        return f"k_round_row<false, false, {order}>" if order == 2 else \
+            "k_round_row<false, false, 2> + k_qb_order<true>"
     fastest = 2 if order == 2 else (0 if order == 0 else 1)
     wm = 0 if fastest == 2 else (1 if fastest == axis else 2)
+    if wm == 2:
+        return f"k_round_sweep<{axis}, false, false, 0> + k_qb_order<true>"
     return f"k_round_sweep<{axis}, false, false, {wm}>"
 
 

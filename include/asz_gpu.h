@@ -106,6 +106,13 @@ typedef struct asz_stats {
  * 1: every field on its own. */
 #define ASZ_OPT_BATCH_FIELDS 4
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
+/* Orders the context's later work after everything already enqueued on
+ * `stream` (a hipStream_t of the same device; NULL: the null stream), without
+ * blocking the host: callers whose inputs are produced on their own stream
+ * call it before handing those device pointers over.  Every call returns with
+ * the context's own work finished.  (Host integration; no reference
+ * counterpart.) */
+int asz_stream_wait(asz_ctx *ctx, void *stream);
 
 /* ------------------------------------------------------------------ context */
 int asz_ctx_create(int device, asz_ctx **out);
