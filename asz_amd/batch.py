@@ -82,7 +82,7 @@ def c4_plans(flds):
             for f in flds]
 
 
-def config4_comparison(concurrency=4, reps=2):
+def config4_comparison(concurrency=4, reps=3):
     """Batched (gradient-tuned, concurrent) vs plain per-field GPU form on the
     60 config-4 fields at REL 1e-3: wall time, total compression ratio (host
     Zstd after the timed region) and max error over eb.  The plain form is
