@@ -45,7 +45,9 @@ class Stats(ctypes.Structure):
                 ("layers", ctypes.c_uint64), ("qb_counts", ctypes.c_uint64 * (MAX_LAYERS + 1)),
                 ("pivot_ms", ctypes.c_double), ("tsel_round_ms", ctypes.c_double * MAX_LAYERS),
                 ("round_ms", ctypes.c_double * MAX_LAYERS), ("hist_ms", ctypes.c_double),
-                ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double)]
+                ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double),
+                ("codebook_wait_ms", ctypes.c_double), ("codebook_max_ms", ctypes.c_double),
+                ("layout_ms", ctypes.c_double)]
 
 class BatchField(ctypes.Structure):
     _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("default_slope", ctypes.c_float),

@@ -199,11 +199,12 @@ class Context:
         returns the context's pinned buffer (valid until the next call)."""
         d = np.asarray(dims, np.uint64)
         N = int(np.prod(d))
-        _, _, bn = self.geometry(dims, plan, params.block_pivot_ratio)
-        nt = int(np.prod(bn)) * plan.layers
         ti = None if t_inject is None else np.ascontiguousarray(t_inject, np.int64)
         qb = np.zeros(N, np.int32) if want_qb else None
-        t = np.zeros(nt, np.int64) if want_t else None
+        t = None
+        if want_t:
+            _, _, bn = self.geometry(dims, plan, params.block_pivot_ratio)
+            t = np.zeros(int(np.prod(bn)) * plan.layers, np.int64)
         sp, sn = ctypes.c_void_p(), ctypes.c_uint64()
         self.after_torch()
         self._check(self._L.asz_compress(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),

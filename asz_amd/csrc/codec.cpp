@@ -143,11 +143,20 @@ struct Layout {
 // names (may be null): the fields' indices, named in the error messages.
 static void range_scan_fields(asz_ctx* c, uint32_t nf, const float* const* d_fields, const uint64_t* n, float* mn,
                               float* mx, const uint32_t* names = nullptr) {
-    uint32_t* r = c->misc.get<uint32_t>(4 * (uint64_t)nf + 4);
-    std::vector<uint32_t> init(4 * (uint64_t)nf, 0u);
+    // device: per-field result words (4 each), the zero-index scratch, then
+    // the field pointers of a one-launch scan
+    const uint64_t nr = 4 * (uint64_t)nf + 4;
+    uint32_t* r = c->misc.get<uint32_t>(nr + 2 * (uint64_t)nf);
+    std::vector<uint32_t> init(nr + 2 * (uint64_t)nf, 0u);
     for (uint32_t f = 0; f < nf; f++) init[4 * f] = 0xFFFFFFFFu;
+    bool same_n = nf > 1;
+    for (uint32_t f = 1; f < nf; f++) same_n = same_n && n[f] == n[0];
+    if (same_n) std::memcpy(init.data() + nr, d_fields, nf * sizeof(float*));
     HIPCHECK(hipMemcpyAsync(r, init.data(), init.size() * 4, hipMemcpyHostToDevice, c->stream));
-    for (uint32_t f = 0; f < nf; f++) launch_range(d_fields[f], n[f], r + 4 * f, c->stream);
+    if (same_n)
+        launch_range(d_fields[0], n[0], r, c->stream, reinterpret_cast<const float* const*>(r + nr), nf);
+    else
+        for (uint32_t f = 0; f < nf; f++) launch_range(d_fields[f], n[f], r + 4 * f, c->stream);
     std::vector<uint32_t> res(4 * (uint64_t)nf);
     HIPCHECK(hipMemcpyAsync(res.data(), r, res.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -417,7 +426,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
     auto gwin = [&](uint32_t f, uint64_t k) { return ghist + f * fs.ghist + k * kHistStride; };
     ensure_host_slots(c, narr * nf);
     const uint32_t epoch = ++c->epoch ? c->epoch : ++c->epoch;  // never 0 (the slots start zeroed)
-    for (uint32_t f = 0; f < nf; f++) launch_init_acc(acc + f, st);
+    launch_init_acc(acc, st, nf);
     launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * fs.ghist, 0, st);
 
     if (tsel_mode == ASZ_TSEL_INJECT) {
@@ -501,6 +510,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         if (q0 > q1 || A.range > (1ull << 24)) fail(ASZ_ERR_DATA, "QB alphabet wider than 2^24 symbols");
         if (q0 >= -kHistCentre && q1 < kHistCentre) {
             build_codebook(A, hs + kHostSlotBins);
+            if (!A.single) pack_book(A);
             return;
         }
         // the array's range leaves the global window: bin it again over its
@@ -527,6 +537,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         HIPCHECK(hipMemcpyAsync(hexact.data(), dh, A.range * 4, hipMemcpyDeviceToHost, c->side));
         HIPCHECK(hipStreamSynchronize(c->side));
         build_codebook(A, hexact.data());
+        if (!A.single) pack_book(A);
     };
     c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
@@ -534,10 +545,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
     int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
     kev_end(c, kp);
-    for (uint32_t f = 0; f < nf; f++) {
-        launch_hist_small(nullptr, 0, sym_of(f) + nt, P.pivot_count, nullptr, gwin(f, 1), st);
-        launch_hist_tail(acc + f, 1, ppart + f * fs.fpart, npp, gwin(f, 1), c->hres_dev + f * fs.hres, epoch, st);
-    }
+    launch_hist_small(nullptr, 0, sym + nt, P.pivot_count, nullptr, gwin(0, 1), st, nf, fs.sym, fs.ghist);
+    launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, st, nf, fs.fpart, fs.ghist, fs.hres);
     // Codebook workers: the host pool's threads take the arrays in `order`
     // (every field's copy of an array in turn) as their results arrive,
     // concurrently with the launches below and with the later rounds on the GPU.
@@ -622,11 +631,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         }
         if (r + 1 == L) {
             // every t is final: the t array's range, histogram and tail
-            for (uint32_t f = 0; f < nf; f++) {
-                launch_minmax_i32(sym_of(f), nt, (int32_t*)&acc[f].arr[0], st);
-                launch_hist_small(sym_of(f), nt, nullptr, 0, gwin(f, 0), nullptr, st);
-                launch_hist_tail(acc + f, 0, nullptr, 0, gwin(f, 0), c->hres_dev + f * fs.hres, epoch, st);
-            }
+            launch_minmax_i32(sym, nt, (int32_t*)&acc[0].arr[0], st, nf, fs.sym, sizeof(DevAcc) / 4);
+            launch_hist_small(sym, nt, nullptr, 0, gwin(0, 0), nullptr, st, nf, fs.sym, fs.ghist);
+            launch_hist_tail(acc, 0, nullptr, 0, gwin(0, 0), c->hres_dev, epoch, st, nf, 0, fs.ghist, fs.hres);
         }
         int kr = kev_begin(c, KEV_ROUND, (int)r);
         if (round_needs_staging(R)) {
@@ -664,9 +671,14 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
     HIPCHECK(hipEventRecord(c->ev[1], st));
     enqueued = true;
     joiner.armed = false;
+    const auto enq0 = Clock::now();
     pool.wait();
+    S.codebook_wait_ms = ms_since(enq0);
     double host_ms = 0;
-    for (double v : host_ms_w) host_ms += v;
+    for (double v : host_ms_w) {
+        host_ms += v;
+        S.codebook_max_ms = std::max(S.codebook_max_ms, v);
+    }
     host_ms /= std::max(1u, pool.workers());
     // errors and fallbacks from the slots
     c->fstats.assign(nf, asz_stats{});
@@ -700,7 +712,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
     struct FieldUp {
         std::vector<uint8_t> blob;
         std::vector<ByteSeg> bsegs;
-        std::vector<uint64_t> book;
+        uint64_t book_words = 0;
         uint64_t chunks = 0, o_seg = 0, o_blob = 0, o_book = 0, o_arr = 0, woff = 0, nwords = 0;
     };
     std::vector<FieldUp> ups(nf);
@@ -729,9 +741,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
             emit(pre, 9 + 2 * L);
         }
         uint64_t book_total = 0, chunks = 0;
-        for (uint64_t k = 0; k < narr; k++) book_total += lay.arrays[k].count ? lay.arrays[k].range : 0;
-        U.book.assign(book_total + 1, 0);
-        book_total = 0;
         for (uint64_t k = 0; k < narr; k++) {
             HuffArray& A = lay.arrays[k];
             HuffArrayDev& D = lay.dev[k];
@@ -752,16 +761,11 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
             chunks += D.nchunk;
             D.payload_bit = spos * 8;
             spos += (A.payload_bits + 7) >> 3;
-            int alen = 0;
-            for (uint64_t s2 = 0; s2 < A.range; s2++) {
-                if (A.len[s2] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
-                U.book[book_total + s2] = A.code[s2] | ((uint64_t)A.len[s2] << 56);
-                alen = std::max<int>(alen, A.len[s2]);
-            }
-            D.lds_book = (A.range <= kLdsBook && alen <= 24) ? 1 : 0;
-            lay.max_len = std::max(lay.max_len, alen);
+            D.lds_book = (A.range <= kLdsBook && A.max_len <= 24) ? 1 : 0;
+            lay.max_len = std::max(lay.max_len, A.max_len);
             book_total += A.range;
         }
+        U.book_words = book_total;
         lay.total_bytes = spos;
         U.chunks = chunks;
         U.o_seg = up_bytes;
@@ -784,12 +788,17 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         const FieldUp& U = ups[f];
         std::memcpy(hu + U.o_seg, U.bsegs.data(), U.bsegs.size() * sizeof(ByteSeg));
         std::memcpy(hu + U.o_blob, U.blob.data(), U.blob.size());
-        std::memcpy(hu + U.o_book, U.book.data(), (U.book.size() - 1) * 8);
+        for (uint64_t k = 0; k < narr; k++) {
+            const HuffArray& A = lays[f].arrays[k];
+            if (A.count && !A.single)
+                std::memcpy(hu + U.o_book + lays[f].dev[k].book_off * 8, A.book.data(), A.range * 8);
+        }
         std::memcpy(hu + U.o_arr, lays[f].dev.data(), narr * sizeof(HuffArrayDev));
     }
     uint8_t* du = c->hmeta.get<uint8_t>(up_bytes);
     HIPCHECK(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, st));
-    S.huffman_host_ms = host_ms + ms_since(hh0);
+    S.layout_ms = ms_since(hh0);
+    S.huffman_host_ms = host_ms + S.layout_ms;
 
     uint32_t* dwords = c->words.get<uint32_t>(words_total);
     // chunk scratch: each chunk's bits at the start of its own slot; the

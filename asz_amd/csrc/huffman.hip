@@ -65,12 +65,13 @@ __global__ __launch_bounds__(256) void k_hist(const int32_t* __restrict__ sym, c
 // symbols, LDS window flushed with atomics (these arrays are short).
 __global__ __launch_bounds__(256) void k_hist_small(const int32_t* __restrict__ s0, uint64_t n0,
                                                     const int32_t* __restrict__ s1, uint64_t n1, uint32_t* g0,
-                                                    uint32_t* g1) {
+                                                    uint32_t* g1, uint64_t s_stride, uint64_t g_stride) {
     __shared__ uint32_t lh[kHistWin];
     const bool second = blockIdx.y == 1;
-    const int32_t* s = second ? s1 : s0;
+    // field blockIdx.z: its arrays s_stride symbols and g_stride words further
+    const int32_t* s = (second ? s1 : s0) + blockIdx.z * s_stride;
     const uint64_t n = second ? n1 : n0;
-    uint32_t* gw = second ? g1 : g0;
+    uint32_t* gw = (second ? g1 : g0) + blockIdx.z * g_stride;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kHistWin; i += blockDim.x) lh[i] = 0;
     __syncthreads();
     const uint64_t first = (uint64_t)blockIdx.x * kHistChunk, last = min<uint64_t>(n, first + kHistChunk);
@@ -221,7 +222,13 @@ __global__ __launch_bounds__(256) void k_fold_rounds(const FoldDesc* __restrict_
 // pivot array (k = 1: fold of the pivot kernel's Partials) for the host.
 __global__ __launch_bounds__(256) void k_hist_tail(const DevAcc* __restrict__ acc, uint32_t k,
                                                    const Partial* __restrict__ part, uint64_t nparts,
-                                                   const uint32_t* gw, uint32_t* hres, uint32_t epoch) {
+                                                   const uint32_t* gw, uint32_t* hres, uint32_t epoch,
+                                                   uint64_t part_stride, uint64_t g_stride, uint64_t h_stride) {
+    // field blockIdx.x: its accumulators, partials, window and host slots
+    acc += blockIdx.x;
+    if (part) part += blockIdx.x * part_stride;
+    gw += blockIdx.x * g_stride;
+    hres += blockIdx.x * h_stride;
     HistTail t{};
     if (k == 0) {
         t = HistTail{acc->arr[0].qmin, acc->arr[0].qmax, acc->fallbacks, 0, 0.0, 0.0};
@@ -519,10 +526,11 @@ void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t*
     hipLaunchKernelGGL(dev::k_hist, dim3((uint32_t)nchunks), dim3(256), 0, st, sym, arrays, chunk_arr, hist);
 }
 void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64_t n1, uint32_t* g0, uint32_t* g1,
-                       hipStream_t st) {
+                       hipStream_t st, uint32_t nfields, uint64_t s_stride, uint64_t g_stride) {
     const uint64_t nc = (std::max(n0, n1) + kHistChunk - 1) / kHistChunk;
     if (!nc) return;
-    hipLaunchKernelGGL(dev::k_hist_small, dim3((uint32_t)nc, 2), dim3(256), 0, st, s0, n0, s1, n1, g0, g1);
+    hipLaunchKernelGGL(dev::k_hist_small, dim3((uint32_t)nc, 2, nfields), dim3(256), 0, st, s0, n0, s1, n1, g0, g1,
+                       s_stride, g_stride);
 }
 void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_t max_groups, const Partial* part,
                         const uint32_t* phist, Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres,
@@ -532,8 +540,10 @@ void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_
                        fpart, acc, ghist, hres, epoch, fs);
 }
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
-                      uint32_t* hres, uint32_t epoch, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_hist_tail, dim3(1), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch);
+                      uint32_t* hres, uint32_t epoch, hipStream_t st, uint32_t nfields, uint64_t part_stride,
+                      uint64_t g_stride, uint64_t h_stride) {
+    hipLaunchKernelGGL(dev::k_hist_tail, dim3(nfields), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch,
+                       part_stride, g_stride, h_stride);
 }
 void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
                           const uint64_t* book, uint32_t* scratch, uint64_t slot_words, uint32_t* chunk_bits,

@@ -234,9 +234,14 @@ struct HuffArray {
     std::vector<uint8_t> len;    // per symbol index: length, 0 = absent
     uint64_t payload_bits = 0;
     bool single = false;         // maxval == 1 path (huffman.hpp:193-210, 342-347)
+    // encoder table (pack_book): code in bits 0..55, length in bits 56..63
+    std::vector<uint64_t> book;
+    int max_len = 0;
 };
 // Build the reference tree codebook from a dense histogram (huffman.hpp:263-320, 650-691).
 void build_codebook(HuffArray& a, const uint32_t* hist);
+// The encoder's table of a built codebook (fails on codes over 56 bits).
+void pack_book(HuffArray& a);
 
 // Decode tables built from a serialized tree (huffman.hpp:693-783).
 struct DecodeInfo {

@@ -38,7 +38,10 @@ struct SlopeSampleDesc {
     float neg_2_stat_eb;
 };
 
-void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st);
+// d_xs (device array of nfields pointers, every field n elements): one launch
+// for all of them, results at d_out3 + 4 f.
+void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st, const float* const* d_xs = nullptr,
+                  uint32_t nfields = 1);
 void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st);
 // compress: one Partial per 256 pivots into part[0 ..]; decompress: part unused.
 // With fs.fd set, nfields fields (compress only; the pointers are field 0's slices).
@@ -64,7 +67,7 @@ void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cm
                      int32_t* ordered, uint64_t staged_stride, uint64_t ordered_stride, uint32_t nfields,
                      bool to_order, hipStream_t st);
 // Resets the accumulators: errors and fallbacks to 0, every QB range to empty.
-void launch_init_acc(DevAcc* acc, hipStream_t st);
+void launch_init_acc(DevAcc* acc, hipStream_t st, uint32_t nfields = 1);
 
 // grad: gradient-tuned candidates (projected Newton on the sampled residual
 // energy, snapped to the t grid) instead of the five block candidates.
@@ -104,8 +107,10 @@ constexpr uint64_t kEncGroups = 2048;    // workgroups of the persistent encoder
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
                  uint32_t* hist, hipStream_t st);
+// nfields > 1: field f's arrays s_stride symbols and its windows g_stride words
+// after field 0's.
 void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64_t n1, uint32_t* g0, uint32_t* g1,
-                       hipStream_t st);
+                       hipStream_t st, uint32_t nfields = 1, uint64_t s_stride = 0, uint64_t g_stride = 0);
 // Fold of a round's per-workgroup partials (error, QB range) and histogram
 // windows into the round array's global window (kHistStride words per array
 // in ghist: bins, then the HistTail) and acc->arr[2 + r].  One descriptor per
@@ -141,7 +146,8 @@ void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_
 // Finishes the t array (k = 0, range and fallbacks from acc) or the pivot
 // array (k = 1, folding the pivot kernel's nparts Partials) into its host slot.
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
-                      uint32_t* hres, uint32_t epoch, hipStream_t st);
+                      uint32_t* hres, uint32_t epoch, hipStream_t st, uint32_t nfields = 1, uint64_t part_stride = 0,
+                      uint64_t g_stride = 0, uint64_t h_stride = 0);
 // slot_words: scratch words per chunk (>= kEncChunk * max_len / 32 + 2); with
 // max_len > 32 the slots must be zeroed first.
 void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,

@@ -12,7 +12,9 @@ struct ByteSeg {
 };
 void launch_scatter_bytes(const ByteSeg* segs, uint32_t nsegs, const uint8_t* src, uint8_t* dst, hipStream_t st);
 
-void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st);
+// nfields > 1: field f's values v_stride and its result out_stride int32 further.
+void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st, uint32_t nfields = 1,
+                       uint64_t v_stride = 0, uint64_t out_stride = 0);
 void launch_fill_i32(int32_t* v, uint64_t n, int32_t val, hipStream_t st);
 void launch_t_float(const int32_t* t_int, uint64_t nblocks, uint32_t L, float prec, float* tblk, hipStream_t st);
 

@@ -5,7 +5,10 @@
 namespace asz {
 namespace dev {
 
-__global__ __launch_bounds__(256) void k_minmax_i32(const int32_t* __restrict__ v, uint64_t n, int32_t* out2) {
+__global__ __launch_bounds__(256) void k_minmax_i32(const int32_t* __restrict__ v, uint64_t n, int32_t* out2,
+                                                    uint64_t v_stride, uint64_t out_stride) {
+    v += blockIdx.y * v_stride;  // field blockIdx.y
+    out2 += blockIdx.y * out_stride;
     int32_t mn = INT32_MAX, mx = INT32_MIN;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         mn = min(mn, v[i]);
@@ -47,10 +50,12 @@ void launch_scatter_bytes(const ByteSeg* segs, uint32_t nsegs, const uint8_t* sr
     hipLaunchKernelGGL(dev::k_scatter_bytes, dim3(nsegs), dim3(256), 0, st, segs, src, dst);
 }
 
-void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st) {
+void launch_minmax_i32(const int32_t* v, uint64_t n, int32_t* out2, hipStream_t st, uint32_t nfields,
+                       uint64_t v_stride, uint64_t out_stride) {
     if (!n) return;
     uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(dev::k_minmax_i32, dim3((uint32_t)blocks), dim3(256), 0, st, v, n, out2);
+    hipLaunchKernelGGL(dev::k_minmax_i32, dim3((uint32_t)blocks, nfields), dim3(256), 0, st, v, n, out2, v_stride,
+                       out_stride);
 }
 void launch_fill_i32(int32_t* v, uint64_t n, int32_t val, hipStream_t st) {
     if (!n) return;

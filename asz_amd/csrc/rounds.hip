@@ -37,9 +37,15 @@ __device__ __forceinline__ uint32_t order_key(float x) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint64_t n, uint32_t* out) {
-    // out[0] = min key, out[1] = max key, out[2] = non-finite flag
+__global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint64_t n, uint32_t* out,
+                                               const float* const* __restrict__ xs) {
+    // out[0] = min key, out[1] = max key, out[2] = non-finite flag; with xs,
+    // field blockIdx.y of several (its array xs[y], its results at out + 4 y)
     __shared__ uint32_t s_mn[4], s_mx[4], s_bad[4];
+    if (xs) {
+        x = xs[blockIdx.y];
+        out += 4 * blockIdx.y;
+    }
     uint32_t mn = 0xFFFFFFFFu, mx = 0u, bad = 0u;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
@@ -140,6 +146,7 @@ __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, i
 }
 
 __global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
+    acc += blockIdx.x;  // one workgroup per field
     const uint32_t i = threadIdx.x;
     if (i == 0) {
         acc->err = ErrAcc{0.0, 0.0};
@@ -557,9 +564,13 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
 
 // ------------------------------------------------------------------ launchers
 
-void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st) {
-    uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, 2048);
-    hipLaunchKernelGGL(dev::k_range, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out3);
+void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st, const float* const* d_xs,
+                  uint32_t nfields) {
+    // about 2048 workgroups in all
+    const uint64_t per = std::max<uint64_t>(64, 2048 / std::max<uint32_t>(nfields, 1));
+    const uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, per);
+    hipLaunchKernelGGL(dev::k_range, dim3((uint32_t)blocks, d_xs ? nfields : 1), dim3(256), 0, st, x, n, d_out3,
+                       d_xs);
 }
 void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st) {
     uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
@@ -575,8 +586,8 @@ void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, 
         hipLaunchKernelGGL(dev::k_pivots_decompress, grid, dim3(256), 0, st, work, (const int32_t*)qb, P);
 }
 
-void launch_init_acc(DevAcc* acc, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_init_acc, dim3(1), dim3(128), 0, st, acc);
+void launch_init_acc(DevAcc* acc, hipStream_t st, uint32_t nfields) {
+    hipLaunchKernelGGL(dev::k_init_acc, dim3(nfields), dim3(128), 0, st, acc);
 }
 
 namespace {

@@ -85,6 +85,9 @@ typedef struct asz_stats {
     double tsel_round_ms[ASZ_MAX_LAYERS];
     double round_ms[ASZ_MAX_LAYERS];
     double hist_ms, chunk_ms, encode_ms, decode_ms;
+    /* compress host timeline: from the last round launch to every codebook
+     * built, the longest codebook build, then the stream layout and its upload */
+    double codebook_wait_ms, codebook_max_ms, layout_ms;
 } asz_stats;
 
 /* Options (asz_set_option).  ASZ_OPT_KERNEL_EVENTS: record a HIP event pair on
