@@ -31,12 +31,10 @@ struct TselShared {
     double red[kTselThreads / kWave][2 * kNCand];
 };
 
-// Keys (lroundf(A / 2 sigma), lroundf(B / 2 sigma)) of sample idx; A and B
-// themselves in Araw / Braw (the residual of the interior 1D4R predictor at
-// t is A t + B).
-__device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
+// The five values sample idx reads: a = x[-h], c = x[-3h], b = x[+h], e = x[+3h], x0.
+__device__ __forceinline__ void sample_load(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
                                             const uint32_t step[3], const uint32_t cnt[3], uint32_t idx,
-                                            float neg2, float& af, float& bf, float& Araw, float& Braw) {
+                                            float v[5]) {
     const RoundDesc& R = T.R;
     uint32_t q2 = idx % cnt[2];
     uint32_t q01 = idx / cnt[2];
@@ -45,8 +43,18 @@ __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __re
     uint64_t pos = ((uint64_t)(lo[0] + q0 * step[0]) * R.D[1] + (lo[1] + q1 * step[1])) * R.D[2] + (lo[2] + q2 * step[2]);
     const int64_t dist = (int64_t)R.dist;
     const float* p = work + pos;
-    float a = p[-dist], c = p[-3 * dist], b = p[dist], e = p[3 * dist];
-    float x0 = daz(p[0]);
+    v[0] = p[-dist];
+    v[1] = p[-3 * dist];
+    v[2] = p[dist];
+    v[3] = p[3 * dist];
+    v[4] = p[0];
+}
+
+// Keys and (A, B) of a sample from its five values.
+__device__ __forceinline__ void sample_from(const float v[5], float neg2, float& af, float& bf, float& Araw,
+                                            float& Braw) {
+    const float a = v[0], c = v[1], b = v[2], e = v[3];
+    const float x0 = daz(v[4]);
     float A = (a + b) - (c + e);
     float S = (b + e) + (a + c);
     float B = (float)__fma_rn((double)S, 0.25, -(double)x0);
@@ -54,6 +62,17 @@ __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __re
     bf = (float)lround_f64(neg2 * B);
     Araw = A;
     Braw = B;
+}
+
+// Keys (lroundf(A / 2 sigma), lroundf(B / 2 sigma)) of sample idx; A and B
+// themselves in Araw / Braw (the residual of the interior 1D4R predictor at
+// t is A t + B).
+__device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __restrict__ work, const uint32_t lo[3],
+                                            const uint32_t step[3], const uint32_t cnt[3], uint32_t idx,
+                                            float neg2, float& af, float& bf, float& Araw, float& Braw) {
+    float v[5];
+    sample_load(T, work, lo, step, cnt, idx, v);
+    sample_from(v, neg2, af, bf, Araw, Braw);
 }
 
 __device__ __forceinline__ void count_qb(TselShared& sh, int k, long long q) {
@@ -148,12 +167,24 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         }
         if (threadIdx.x < kNCand) sh.ovf[threadIdx.x] = 0;
         double saa = 0, sab = 0;  // GRAD: sum A^2, sum A B over the samples
-        for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
-            if (!GRAD && idx >= kMaxKeep) break;
-            float af, bf, A, B;
-            sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf, A, B);
-            if (idx < kMaxKeep) { sh.af[idx] = af; sh.bf[idx] = bf; }
-            if (GRAD) { saa += (double)A * (double)A; sab += (double)A * (double)B; }
+        // four samples per thread at a time: all twenty loads in flight before any use
+        const uint32_t ns_pass = GRAD ? ns : min(ns, (uint32_t)kMaxKeep);
+        for (uint32_t idx0 = threadIdx.x; idx0 < ns_pass; idx0 += 4 * kTselThreads) {
+            float v[4][5];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t idx = idx0 + u * kTselThreads;
+                if (idx < ns_pass) sample_load(T, work, lo, step, cnt, idx, v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t idx = idx0 + u * kTselThreads;
+                if (idx >= ns_pass) break;
+                float af, bf, A, B;
+                sample_from(v[u], neg2, af, bf, A, B);
+                if (idx < kMaxKeep) { sh.af[idx] = af; sh.bf[idx] = bf; }
+                if (GRAD) { saa += (double)A * (double)A; sab += (double)A * (double)B; }
+            }
         }
         if (GRAD) {
             const int w0 = threadIdx.x / kWave;
