@@ -48,11 +48,11 @@ def c4_fields(steps=48, seed=4, device="cuda"):
 
 
 def compress_fields(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, tsel="grad", concurrency=4,
-                    on_device=False, fields_per_chunk=0):
+                    on_device=False, fields_per_chunk=0, family="cubic"):
     """Batched form; returns (streams, per-field info, wall seconds).  With
     on_device the streams stay in HBM as (pointer, bytes) pairs;
     fields_per_chunk = 1 compresses one field at a time."""
-    params = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=block_pivot_ratio, family="cubic")
+    params = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=block_pivot_ratio, family=family)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     streams, info = ctx.compress_batch(flds, plans, params, tsel=tsel, concurrency=concurrency, on_device=on_device,

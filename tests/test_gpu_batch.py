@@ -33,18 +33,19 @@ def _fields():
     return flds, plans
 
 
-@pytest.mark.parametrize("tsel,concurrency", [("gpu", 3), ("grad", 3), ("gpu", 1), ("grad", 1)])
-def test_batch_equals_single_field_path(tsel, concurrency):
+@pytest.mark.parametrize("tsel,concurrency,family", [("gpu", 3, "cubic"), ("grad", 3, "cubic"), ("gpu", 1, "cubic"),
+                                                     ("grad", 1, "cubic"), ("gpu", 1, "quintic")])
+def test_batch_equals_single_field_path(tsel, concurrency, family):
     flds, plans = _fields()
     originals = [f.clone() for f in flds]
     ctx = codec.Context(0)
     try:
-        print("batch of", len(flds), "fields, tsel", tsel, "concurrency", concurrency, flush=True)
+        print("batch of", len(flds), "fields, tsel", tsel, "concurrency", concurrency, family, flush=True)
         sys.stdout.flush()
         work = [f.clone() for f in flds]
         streams, info = batch.compress_fields(ctx, work, [codec.make_plan(p) for p in plans], tsel=tsel,
-                                              concurrency=concurrency)[:2]
-        params = codec.make_params(rel_eb=float(np.float32(1e-3)), block_pivot_ratio=3, family="cubic")
+                                              concurrency=concurrency, family=family)[:2]
+        params = codec.make_params(rel_eb=float(np.float32(1e-3)), block_pivot_ratio=3, family=family)
         for i, (f, pl) in enumerate(zip(originals, plans)):
             w = f.clone()
             mn, mx = ctx.range(w.data_ptr(), w.numel())

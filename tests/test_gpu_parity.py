@@ -129,6 +129,12 @@ PLANS = {
                 "cubic1_021", "cubic2_120", "cubic0_012", "cubic1_012", "cubic2_021", "cubic0_120", "cubic1_012"],
     "z_last": ["cubic2_012", "cubic1_012", "cubic0_012", "cubic0_120", "cubic1_021", "cubic2_012", "cubic1_120",
                "cubic2_021", "cubic0_012", "cubic1_012", "cubic2_012"],
+    # the largest rounds in orders whose QBs are staged in "012" order and
+    # moved into place: axis 2 / "120" (row kernel) and axis 1 / "120"
+    # (sweep), both above the staging threshold at 96x128x160; axis 0 /
+    # "021" below it
+    "staged14": ["cubic0_012", "cubic1_012", "cubic2_012", "cubic0_012", "cubic1_012", "cubic2_012", "cubic0_012",
+                 "cubic1_012", "cubic2_012", "cubic0_021", "cubic1_120", "cubic0_021", "cubic2_120", "cubic1_120"],
 }
 
 
@@ -137,6 +143,8 @@ PLANS = {
     ((96, 128, 160), 1e-2, "mixed14", "quintic", 3),
     ((128, 96, 100), 1e-4, "z_last", "quintic", 96),
     ((70, 71, 90), 1e-5, "z_last", "cubic", 96),
+    ((96, 128, 160), 1e-3, "staged14", "cubic", 3),
+    ((96, 128, 160), 1e-4, "staged14", "quintic", 3),
 ])
 def test_generated_fields_against_oracle(ctx, shape, rel, plan, fam, bpr):
     _say("case", shape, rel, plan, fam)
