@@ -91,7 +91,7 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
            "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
-           "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_prepare_round", "asz_round_energy",
+           "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_compress_field", "asz_prepare_round", "asz_round_energy",
            "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
            "asz_select_family")
 
@@ -115,6 +115,9 @@ def lib():
     L.asz_get_stats.argtypes = [_P, ctypes.POINTER(Stats)]
     L.asz_set_option.argtypes = [_P, ctypes.c_int, ctypes.c_int64]
     L.asz_stream_wait.argtypes = [_P, ctypes.c_void_p]
+    L.asz_compress_field.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_int,
+                                     ctypes.POINTER(_P), _U64P, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(ctypes.c_float)]
     L.asz_device_alloc.argtypes = [_P, _U64, ctypes.POINTER(_P)]
     L.asz_device_free.argtypes = [_P, _P]
     L.asz_copy_to_device.argtypes = [_P, _P, _P, _U64]

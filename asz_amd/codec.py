@@ -218,6 +218,27 @@ class Context:
         st = self.stats()
         return CompressResult(stream, mn, mx, st, qb, t, None, [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
+    def compress_field(self, d_ptr: int, dims, plan: Plan, params: Params, tsel="gpu",
+                       copy_stream=True) -> CompressResult:
+        """Range scan + compression of the device field at d_ptr in one call
+        (asz_compress_field): the stream of range() + compress_device(), with
+        no host wait between the scan and the compression."""
+        d = np.asarray(dims, np.uint64)
+        sp, sn = ctypes.c_void_p(), ctypes.c_uint64()
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        self.after_torch()
+        self._check(self._L.asz_compress_field(self._c, ctypes.c_void_p(d_ptr), _ptr(d), ctypes.byref(plan),
+                                               ctypes.byref(params), TSEL[tsel], ctypes.byref(sp), ctypes.byref(sn),
+                                               ctypes.byref(mn), ctypes.byref(mx)))
+        self._last_stream = (sp.value, sn.value)
+        stream = None
+        if copy_stream:
+            stream = self.stream_view().tobytes() if getattr(self, "_stream_dev", False) else \
+                ctypes.string_at(sp, sn.value)
+        st = self.stats()
+        return CompressResult(stream, mn.value, mx.value, st, None, None, None,
+                              [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
+
     def stream_view(self) -> np.ndarray:
         """The last compressed stream: the context's pinned buffer (no copy), or
         a host copy when the stream was left in device memory."""

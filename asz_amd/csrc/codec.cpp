@@ -139,49 +139,62 @@ struct Layout {
 
 // ------------------------------------------------------------------ range
 
-// Range scans of nf fields: one launch each, one copy back, one synchronisation.
-// names (may be null): the fields' indices, named in the error messages.
+// Preset result words of nf range scans (launch_range).
+static std::vector<uint32_t> range_init(uint32_t nf) {
+    std::vector<uint32_t> init(kRangeWords * (uint64_t)nf, 0u);
+    for (uint32_t f = 0; f < nf; f++) {
+        init[kRangeWords * f] = 0xFFFFFFFFu;      // min key
+        init[kRangeWords * f + 4] = 0xFFFFFFFFu;  // first zero index (uint64 ~0)
+        init[kRangeWords * f + 5] = 0xFFFFFFFFu;
+    }
+    return init;
+}
+
+static float key_to_float(uint32_t k) {
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// Range scans of nf fields: one launch (one per field when the sizes
+// differ), one copy back, one synchronisation.  names (may be null): the
+// fields' indices, named in the error messages.
 static void range_scan_fields(asz_ctx* c, uint32_t nf, const float* const* d_fields, const uint64_t* n, float* mn,
                               float* mx, const uint32_t* names = nullptr) {
-    // device: per-field result words (4 each), the zero-index scratch, then
-    // the field pointers of a one-launch scan
-    const uint64_t nr = 4 * (uint64_t)nf + 4;
-    uint32_t* r = c->misc.get<uint32_t>(nr + 2 * (uint64_t)nf);
-    std::vector<uint32_t> init(nr + 2 * (uint64_t)nf, 0u);
-    for (uint32_t f = 0; f < nf; f++) init[4 * f] = 0xFFFFFFFFu;
+    // device: per-field result words, then the field pointers of a one-launch scan
+    const uint64_t nw = kRangeWords * (uint64_t)nf;
+    uint32_t* r = c->misc.get<uint32_t>(nw + 2 * (uint64_t)nf);
+    std::vector<uint32_t> init = range_init(nf);
     bool same_n = nf > 1;
     for (uint32_t f = 1; f < nf; f++) same_n = same_n && n[f] == n[0];
-    if (same_n) std::memcpy(init.data() + nr, d_fields, nf * sizeof(float*));
+    if (same_n) {
+        init.resize(nw + 2 * (uint64_t)nf);
+        std::memcpy(init.data() + nw, d_fields, nf * sizeof(float*));
+    }
     HIPCHECK(hipMemcpyAsync(r, init.data(), init.size() * 4, hipMemcpyHostToDevice, c->stream));
     if (same_n)
-        launch_range(d_fields[0], n[0], r, c->stream, reinterpret_cast<const float* const*>(r + nr), nf);
+        launch_range(d_fields[0], n[0], r, c->stream, reinterpret_cast<const float* const*>(r + nw), nf);
     else
-        for (uint32_t f = 0; f < nf; f++) launch_range(d_fields[f], n[f], r + 4 * f, c->stream);
-    std::vector<uint32_t> res(4 * (uint64_t)nf);
-    HIPCHECK(hipMemcpyAsync(res.data(), r, res.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        for (uint32_t f = 0; f < nf; f++) launch_range(d_fields[f], n[f], r + kRangeWords * f, c->stream);
+    std::vector<uint32_t> res(nw);
+    HIPCHECK(hipMemcpyAsync(res.data(), r, nw * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    auto key_to_float = [](uint32_t k) {
-        uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-        float f;
-        std::memcpy(&f, &b, 4);
-        return f;
-    };
     for (uint32_t f = 0; f < nf; f++) {
+        const uint32_t* w = res.data() + kRangeWords * f;
         const std::string who = names ? "field " + std::to_string(names[f]) + ": " : std::string();
-        if (res[4 * f + 2]) fail(ASZ_ERR_DATA, who + "input contains a nonfinite value");
-        float vmin = key_to_float(res[4 * f]), vmax = key_to_float(res[4 * f + 1]);
-        if (vmin == 0.0f || vmax == 0.0f) {
-            // ties among zeros / denormals under DAZ: minmax_element keeps the first
-            // smallest and the last largest element, with their own bits
-            unsigned long long* zi = reinterpret_cast<unsigned long long*>(r + 4 * (uint64_t)nf);
-            unsigned long long zinit[2] = {~0ull, 0ull};
-            HIPCHECK(hipMemcpyAsync(zi, zinit, sizeof(zinit), hipMemcpyHostToDevice, c->stream));
-            launch_zero_index(d_fields[f], n[f], zi, c->stream);
-            unsigned long long idx[2];
-            HIPCHECK(hipMemcpyAsync(idx, zi, sizeof(idx), hipMemcpyDeviceToHost, c->stream));
+        if (w[2]) fail(ASZ_ERR_DATA, who + "input contains a nonfinite value");
+        float vmin = key_to_float(w[0]), vmax = key_to_float(w[1]);
+        // ties among zeros / denormals under DAZ: minmax_element keeps the first
+        // smallest and the last largest element, with their own bits
+        uint64_t zi[2];
+        std::memcpy(zi, w + 4, 16);
+        if ((vmin == 0.0f || vmax == 0.0f) && zi[0] != ~0ull) {
+            if (vmin == 0.0f)
+                HIPCHECK(hipMemcpyAsync(&vmin, d_fields[f] + zi[0], 4, hipMemcpyDeviceToHost, c->stream));
+            if (vmax == 0.0f)
+                HIPCHECK(hipMemcpyAsync(&vmax, d_fields[f] + zi[1], 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHECK(hipStreamSynchronize(c->stream));
-            if (vmin == 0.0f) HIPCHECK(hipMemcpy(&vmin, d_fields[f] + idx[0], 4, hipMemcpyDeviceToHost));
-            if (vmax == 0.0f) HIPCHECK(hipMemcpy(&vmax, d_fields[f] + idx[1], 4, hipMemcpyDeviceToHost));
         }
         if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, who + "max_val==min_val");
         mn[f] = vmin;
@@ -312,21 +325,30 @@ struct FieldIn {
 // stop_round >= 0 (one field): run the pivots and rounds [0, stop_round) only
 // (no Huffman coding), leaving the field in the state the round stop_round
 // reads.
-static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const uint64_t dims[3],
+// dev_range (one field, GPU t selection, no optional outputs): fin's min /
+// max are not used; the range scan runs on the context's stream ahead of the
+// compression and the error bounds follow from it on the device
+// (k_field_desc), so the host never waits between the two.  The range comes
+// back with the rest (c->dev_mn / c->dev_mx), and its errors (non-finite
+// input, max == min) are reported once the rounds have run.
+static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, const uint64_t dims[3],
                             const asz_plan& plan, const asz_params& p, int tsel_mode, const int64_t* t_inject,
-                            int32_t* qb_out, int64_t* t_out, int stop_round = -1) {
+                            int32_t* qb_out, int64_t* t_out, int stop_round = -1, bool dev_range = false) {
     auto wall0 = Clock::now();
     validate_params(p);
     if (tsel_mode < ASZ_TSEL_GPU || tsel_mode > ASZ_TSEL_GRAD) fail(ASZ_ERR_ARG, "unknown t selection mode");
     if (nf == 0) fail(ASZ_ERR_ARG, "no field to compress");
     const bool multi = nf > 1;
-    if (multi && ((tsel_mode != ASZ_TSEL_GPU && tsel_mode != ASZ_TSEL_GRAD) || t_inject || qb_out || t_out ||
-                  stop_round >= 0))
-        fail(ASZ_ERR_ARG, "multi-field calls take the GPU t selection and no optional outputs");
+    if ((multi || dev_range) && ((tsel_mode != ASZ_TSEL_GPU && tsel_mode != ASZ_TSEL_GRAD) || t_inject || qb_out ||
+                                 t_out || stop_round >= 0))
+        fail(ASZ_ERR_ARG, "multi-field and device-range calls take the GPU t selection and no optional outputs");
+    if (dev_range && multi) fail(ASZ_ERR_ARG, "device-range calls take one field");
     hipStream_t st = c->stream;
+    std::vector<FieldIn> fin(fin_in, fin_in + nf);  // min / max filled in later with dev_range
     float* d_work = fin[0].work;
     const float mn = fin[0].mn, mx = fin[0].mx;
-    const float aeb = abs_eb_of(p.rel_eb, mn, mx);
+    // the plan tables do not depend on the bounds (any positive one will do)
+    const float aeb = dev_range ? 1.0f : abs_eb_of(p.rel_eb, mn, mx);
 
     // plan and its device tables, reused while dims, plan, block pivot ratio and
     // sampling count repeat (the error bounds and slopes are set per call)
@@ -359,7 +381,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
     // per-field scalars (field 0 last: the plan keeps field 0's bounds)
     FieldDesc* hfd = nullptr;
     if (multi) hfd = c->pin_fdesc.get<FieldDesc>(nf);
-    for (uint32_t f = nf; f-- > 0;) {
+    for (uint32_t f = nf; f-- > 0 && !dev_range;) {
         set_plan_bounds(pc.P, abs_eb_of(p.rel_eb, fin[f].mn, fin[f].mx), p.pivot_eb_ratio, p.eb_ratio,
                         p.slope_ratio, fin[f].default_slope);
         if (!multi) break;
@@ -414,6 +436,38 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         HIPCHECK(hipMemcpyAsync(dfd, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
         fs.fd = dfd;
     }
+    uint32_t* range_res = nullptr;  // dev_range: min bits, max bits, status (pinned)
+    if (dev_range) {
+        uint32_t* rng = c->misc.get<uint32_t>(kRangeWords + 4);
+        const std::vector<uint32_t> init = range_init(1);
+        HIPCHECK(hipMemcpyAsync(rng, init.data(), kRangeWords * 4, hipMemcpyHostToDevice, st));
+        launch_range(d_work, N, rng, st);
+        RangeSeed seed{};
+        seed.work = d_work;
+        seed.rel_eb = p.rel_eb;
+        seed.pivot_eb_ratio = p.pivot_eb_ratio;
+        seed.L = (uint32_t)L;
+        // as set_plan_bounds (compress.tpp:194-195)
+        for (uint64_t r = 0; r < L; r++) {
+            seed.eb_factor[r] = std::pow((double)p.eb_ratio, (double)(L - r - 1));
+            seed.layer_slope[r] =
+                (float)((double)fin[0].default_slope * std::pow((double)p.slope_ratio, (double)(L - r - 1)));
+        }
+        FieldDesc* dfd = c->fdesc.get<FieldDesc>(1);
+        launch_field_desc(rng, d_work, seed, dfd, rng + kRangeWords, st);
+        range_res = c->pin_range.get<uint32_t>(4);
+        HIPCHECK(hipMemcpyAsync(range_res, rng + kRangeWords, 16, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(c->ev[2], st));
+        fs.fd = dfd;
+    }
+    // dev_range: the range's verdict (from any thread; the copy is long done
+    // by the time a codebook is built)
+    auto check_range = [&] {
+        if (!dev_range) return;
+        HIPCHECK(hipEventSynchronize(c->ev[2]));
+        if (range_res[2] == 1) fail(ASZ_ERR_DATA, "input contains a nonfinite value");
+        if (range_res[2] == 2) fail(ASZ_ERR_DATA, "max_val==min_val");
+    };
     int32_t* sym = sym0;  // field 0
     int32_t* qb = sym + nt;
     // staging array of the rounds whose QBs are written in "012" order first
@@ -499,6 +553,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
         std::atomic_thread_fence(std::memory_order_acquire);
     };
     auto build = [&](uint32_t f, uint64_t k) {
+        check_range();
         HuffArray& A = lays[f].arrays[k];
         if (A.count == 0) return;
         const uint32_t* hs = slot(f, k);
@@ -697,6 +752,13 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin, const u
                 Sf.sq_err_sum += tail.sq;
             }
         }
+    }
+    check_range();
+    if (dev_range) {
+        std::memcpy(&fin[0].mn, range_res, 4);
+        std::memcpy(&fin[0].mx, range_res + 1, 4);
+        c->dev_mn = fin[0].mn;
+        c->dev_mx = fin[0].mx;
     }
     S.max_err = c->fstats[0].max_err;
     S.sq_err_sum = c->fstats[0].sq_err_sum;
@@ -1249,6 +1311,21 @@ int asz_compress(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_pl
         asz::compress_impl(c, d_work, dims, *plan, *p, mn, mx, tsel_mode, t_inject, qb_out, t_out);
         *stream = c->out_stream;
         *stream_bytes = c->out_bytes;
+    });
+}
+
+int asz_compress_field(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan* plan, const asz_params* p,
+                       int tsel_mode, const uint8_t** stream, uint64_t* stream_bytes, float* mn, float* mx) {
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        const uint64_t N = dims[0] * dims[1] * dims[2];
+        if (N == 0) fail(ASZ_ERR_ARG, "empty field");
+        const asz::FieldIn f{d_work, 0.0f, 0.0f, p->default_slope};
+        asz::compress_fields(c, 1, &f, dims, *plan, *p, tsel_mode, nullptr, nullptr, nullptr, -1, true);
+        *stream = c->out_stream;
+        *stream_bytes = c->out_bytes;
+        *mn = c->dev_mn;
+        *mx = c->dev_mx;
     });
 }
 

@@ -96,7 +96,7 @@ struct asz_ctx {
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, cmeta, phist, ghist, xfer, hmeta, scratch,
         chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range;
     hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
     hipEvent_t order_ev = nullptr;  // asz_stream_wait / batch workers: cross-stream ordering
     // host result slots (pinned, written by the GPU): kHostSlotWords per array
@@ -108,6 +108,7 @@ struct asz_ctx {
     asz::PlanCache pc;
     bool stream_on_device = false;
     uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
+    float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;
     // multi-field calls: each field's stream (in words or pin_stream) and statistics

@@ -38,11 +38,27 @@ struct SlopeSampleDesc {
     float neg_2_stat_eb;
 };
 
-// d_xs (device array of nfields pointers, every field n elements): one launch
-// for all of them, results at d_out3 + 4 f.
-void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st, const float* const* d_xs = nullptr,
+// Results: kRangeWords words at d_out (preset: min key ~0, max key 0, flag 0,
+// first zero index ~0, last 0).  d_xs (device array of nfields pointers,
+// every field n elements): one launch for all of them, field f's words at
+// d_out + kRangeWords f.
+void launch_range(const float* x, uint64_t n, uint32_t* d_out, hipStream_t st, const float* const* d_xs = nullptr,
                   uint32_t nfields = 1);
-void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st);
+// Range scan result words per field (k_range): min / max keys, non-finite
+// flag, first / last DAZ-zero index.
+constexpr uint32_t kRangeWords = 8;
+// Host-side inputs of k_field_desc: what does not depend on the range.
+struct RangeSeed {
+    float* work;
+    float rel_eb, pivot_eb_ratio;
+    uint32_t L;
+    double eb_factor[ASZ_MAX_LAYERS];   // (double) eb_ratio^(L-1-r), libm pow
+    float layer_slope[ASZ_MAX_LAYERS];  // compress.tpp:195
+};
+// The error-bound scalars of one field from its range words (rng, preset and
+// filled by launch_range), into fd; res: min bits, max bits, status.
+void launch_field_desc(const uint32_t* rng, const float* x, const RangeSeed& seed, FieldDesc* fd, uint32_t* res,
+                       hipStream_t st);
 // compress: one Partial per 256 pivots into part[0 ..]; decompress: part unused.
 // With fs.fd set, nfields fields (compress only; the pointers are field 0's slices).
 void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st,

@@ -37,16 +37,22 @@ __device__ __forceinline__ uint32_t order_key(float x) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// Range scan.  Per field, kRangeWords words at out: [0] min key, [1] max key,
+// [2] non-finite flag, [4:5] / [6:7] the first / last index of an element
+// whose DAZ value is zero (uint64; first = ~0 when there is none), which the
+// zero ties of min_element / max_element need (their own bits).  With xs,
+// field blockIdx.y of several (its array xs[y], its words at out + 8 y).
 __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint64_t n, uint32_t* out,
                                                const float* const* __restrict__ xs) {
-    // out[0] = min key, out[1] = max key, out[2] = non-finite flag; with xs,
-    // field blockIdx.y of several (its array xs[y], its results at out + 4 y)
     __shared__ uint32_t s_mn[4], s_mx[4], s_bad[4];
+    __shared__ unsigned long long s_zf[4], s_zl[4];
     if (xs) {
         x = xs[blockIdx.y];
-        out += 4 * blockIdx.y;
+        out += kRangeWords * blockIdx.y;
     }
+    constexpr uint32_t kZeroKey = 0x80000000u;  // order_key of every DAZ-zero value
     uint32_t mn = 0xFFFFFFFFu, mx = 0u, bad = 0u;
+    unsigned long long zf = ~0ull, zl = 0ull;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
     const float4* x4 = reinterpret_cast<const float4*>(x);
@@ -59,22 +65,44 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
             const uint32_t key = order_key(e[k]);
             mn = min(mn, key);
             mx = max(mx, key);
+            if (key == kZeroKey) {
+                zf = min(zf, (unsigned long long)(4 * i + k));
+                zl = max(zl, (unsigned long long)(4 * i + k));
+            }
         }
     }
     for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         bad |= !isfinite(x[i]);
-        uint32_t key = order_key(x[i]);
+        const uint32_t key = order_key(x[i]);
         mn = min(mn, key);
         mx = max(mx, key);
+        if (key == kZeroKey) {
+            zf = min(zf, (unsigned long long)i);
+            zl = max(zl, (unsigned long long)i);
+        }
     }
     mn = wave_min(mn);
     mx = wave_max(mx);
     bad = wave_max(bad);
+    zf = wave_min(zf);
+    zl = wave_max(zl);
     const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) { s_mn[w] = mn; s_mx[w] = mx; s_bad[w] = bad; }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_mn[w] = mn;
+        s_mx[w] = mx;
+        s_bad[w] = bad;
+        s_zf[w] = zf;
+        s_zl[w] = zl;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; k++) { mn = min(mn, s_mn[k]); mx = max(mx, s_mx[k]); bad |= s_bad[k]; }
+        for (int k = 1; k < 4; k++) {
+            mn = min(mn, s_mn[k]);
+            mx = max(mx, s_mx[k]);
+            bad |= s_bad[k];
+            zf = min(zf, s_zf[k]);
+            zl = max(zl, s_zl[k]);
+        }
         // only a workgroup that improves on the value already there issues the
         // atomic: thousands of atomics on one address serialise in L2 (a stale
         // read only costs a redundant atomic)
@@ -82,27 +110,55 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
         if (mn < vo[0]) atomicMin(&out[0], mn);
         if (mx > vo[1]) atomicMax(&out[1], mx);
         if (bad) atomicOr(&out[2], 1u);
+        unsigned long long* z = reinterpret_cast<unsigned long long*>(out + 4);
+        if (zf != ~0ull) {
+            atomicMin(&z[0], zf);
+            atomicMax(&z[1], zl);
+        }
     }
 }
 
-// first index whose DAZ value equals zero (min side) / last such index (max side)
-__global__ __launch_bounds__(256) void k_zero_index(const float* __restrict__ x, uint64_t n, unsigned long long* out) {
-    unsigned long long first = ~0ull, last = 0ull;
-    int any = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (daz(x[i]) == 0.0f) {
-            first = min(first, (unsigned long long)i);
-            last = max(last, (unsigned long long)i);
-            any = 1;
-        }
+// The field's error-bound scalars from its range scan, on the device, so that
+// compression can follow the scan without a host round trip: min / max as
+// load_input takes them (a DAZ-zero extreme keeps the bits of the first
+// smallest / last largest element), the absolute bound, the pivot bound
+// (compress.tpp:166) and every layer's bound (compress.tpp:194: the host
+// gives (double) eb_ratio^(L-r-1) and the layer slopes, which do not depend on
+// the range).  res[0..1] = min / max bits, res[2] = status (0 valid, 1
+// non-finite input, 2 max == min).  One workgroup of 64 threads.
+__global__ __launch_bounds__(64) void k_field_desc(const uint32_t* __restrict__ rng, const float* __restrict__ x,
+                                                   RangeSeed seed, FieldDesc* fd, uint32_t* res) {
+    auto key_to_float = [](uint32_t k) {
+        const uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        return __uint_as_float(b);
+    };
+    float vmin = key_to_float(rng[0]), vmax = key_to_float(rng[1]);
+    const unsigned long long* z = reinterpret_cast<const unsigned long long*>(rng + 4);
+    if (vmin == 0.0f && z[0] != ~0ull) vmin = x[z[0]];
+    if (vmax == 0.0f && z[0] != ~0ull) vmax = x[z[1]];
+    const float range = daz(daz(vmax) - daz(vmin));
+    const float aeb = daz(daz(seed.rel_eb) * range);
+    const float pe = daz(aeb) * daz(seed.pivot_eb_ratio);
+    const uint32_t r = threadIdx.x;
+    if (r < seed.L) {
+        FieldRound q;
+        q.eb = (float)((double)aeb * seed.eb_factor[r]);
+        q.layer_slope = seed.layer_slope[r];
+        q.inv_half_eb = 0.5 / (double)q.eb;
+        q.eb_d = (double)q.eb;
+        q.neg_two_eb = -(2.0 * (double)q.eb);
+        fd->r[r] = q;
     }
-    first = wave_min(first);
-    last = wave_max(last);
-    any = wave_max(any);
-    if ((threadIdx.x & (kWave - 1)) == 0 && any) {
-        atomicMin(&out[0], first);
-        atomicMax(&out[1], last);
+    if (r == 0) {
+        fd->work = seed.work;
+        fd->inv_half_pe = 0.5 / (double)pe;
+        fd->two_pe = (double)pe * 2.0;
+        fd->range = daz(vmax) - daz(vmin);
+        fd->pad = 0;
+        res[0] = __float_as_uint(vmin);
+        res[1] = __float_as_uint(vmax);
+        res[2] = rng[2] ? 1u : (daz(vmin) == daz(vmax) ? 2u : 0u);
+        res[3] = 0;
     }
 }
 
@@ -566,15 +622,15 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
 
 void launch_range(const float* x, uint64_t n, uint32_t* d_out3, hipStream_t st, const float* const* d_xs,
                   uint32_t nfields) {
-    // about 2048 workgroups in all
+    // about 2048 workgroups in all (the result words are preset by the caller)
     const uint64_t per = std::max<uint64_t>(64, 2048 / std::max<uint32_t>(nfields, 1));
     const uint64_t blocks = std::min<uint64_t>((n / 4 + 255) / 256 + 1, per);
     hipLaunchKernelGGL(dev::k_range, dim3((uint32_t)blocks, d_xs ? nfields : 1), dim3(256), 0, st, x, n, d_out3,
                        d_xs);
 }
-void launch_zero_index(const float* x, uint64_t n, unsigned long long* d_out2, hipStream_t st) {
-    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(dev::k_zero_index, dim3((uint32_t)blocks), dim3(256), 0, st, x, n, d_out2);
+void launch_field_desc(const uint32_t* rng, const float* x, const RangeSeed& seed, FieldDesc* fd, uint32_t* res,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_field_desc, dim3(1), dim3(64), 0, st, rng, x, seed, fd, res);
 }
 
 void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, Partial* part, hipStream_t st,

@@ -162,6 +162,10 @@ def main():
     torch.cuda.synchronize()
 
     def step(w):
+        if a.tsel == "gpu":
+            # range scan + compression in one call (asz_compress_field): the
+            # range and the bounds derived from it stay on the device
+            return ctx.compress_field(w.data_ptr(), shape, pl, params, tsel=a.tsel, copy_stream=False)
         m0, m1 = ctx.range(w.data_ptr(), N)
         return ctx.compress_device(w.data_ptr(), shape, pl, params, m0, m1, tsel=a.tsel, copy_stream=False)
 

@@ -180,6 +180,17 @@ int asz_compress(asz_ctx *ctx, float *d_work, const uint64_t dims[3], const asz_
                  float min_val, float max_val, int tsel_mode, const int64_t *t_inject, const uint8_t **stream,
                  uint64_t *stream_bytes, int32_t *qb_out, int64_t *t_out);
 
+/* Range scan + compression of a resident field in one call, as load_input
+ * followed by param_c::compress (s1_compress.cpp:101-140, compress.tpp:146-299):
+ * the same stream as asz_range_f32 + asz_compress with the range it finds.
+ * The range and the error bounds derived from it stay on the device, so the
+ * host does not wait between the scan and the compression; *mn / *mx receive
+ * the range.  tsel_mode: ASZ_TSEL_GPU or ASZ_TSEL_GRAD; p->default_slope is
+ * used as given.  Input errors (non-finite values, max == min) are reported
+ * as asz_range_f32 reports them.  (No reference counterpart as one call.) */
+int asz_compress_field(asz_ctx *ctx, float *d_work, const uint64_t dims[3], const asz_plan *plan,
+                       const asz_params *p, int tsel_mode, const uint8_t **stream, uint64_t *stream_bytes,
+                       float *mn, float *mx);
 /* Host-buffer convenience: copies field to the device, runs asz_range_f32 +
  * asz_compress, and optionally copies the reconstruction back (recon may be NULL). */
 int asz_compress_host(asz_ctx *ctx, const float *field, const uint64_t dims[3], const asz_plan *plan,

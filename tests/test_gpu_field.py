@@ -1,0 +1,99 @@
+"""asz_compress_field: the range scan and the compression in one call, with
+the error bounds derived from the range on the device.  It must give the
+stream, reconstruction and range that asz_range_f32 + asz_compress give.  The
+range itself follows std::minmax_element under DAZ (s1_compress.cpp:101-140:
+first smallest, last largest, each with its own bits), checked against a
+numpy restatement on fields whose extremes are zero ties (+0, -0 and
+denormals compare equal)."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from asz_amd import codec, fields
+
+pytestmark = pytest.mark.gpu
+
+PLAN = ["cubic0_012", "cubic1_012", "cubic2_012", "cubic0_021", "cubic1_120", "cubic2_012", "cubic0_012",
+        "cubic1_021", "cubic2_120", "cubic0_012", "cubic1_012", "cubic2_021", "cubic0_120", "cubic1_012"]
+TINY = np.finfo(np.float32).tiny
+
+
+def _daz_range(a):
+    """(min, max) as minmax_element picks them in a DAZ process."""
+    flat = a.reshape(-1)
+    d = np.where(np.abs(flat) < TINY, np.float32(0), flat).astype(np.float32)
+    mn, mx = d.min(), d.max()
+    return flat[np.flatnonzero(d == mn)[0]], flat[np.flatnonzero(d == mx)[-1]]
+
+
+def _field(kind, shape=(48, 56, 80)):
+    f = fields.to_numpy(fields.config2_field(shape, seed=21)).astype(np.float32)
+    if kind == "grf":
+        return f
+    g = ((f - f.min()) / (f.max() - f.min())).astype(np.float32)  # [0, 1], exact 0 somewhere
+    flat = g.reshape(-1)
+    z = int(np.flatnonzero(flat == 0)[0])
+    if kind == "min_tie":
+        flat[min(z, 7) // 2] = np.float32(1e-41)     # a denormal before the exact zero: the first tie
+        flat[-3] = np.float32(-0.0)
+        return g
+    # max_tie: every value <= 0, ties at the top
+    g = -g
+    flat = g.reshape(-1)
+    flat[1] = np.float32(-0.0)
+    flat[-5] = np.float32(-1e-42)                     # the last tie
+    return g
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = codec.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("kind,rel,fam,tsel", [
+    ("grf", 1e-3, "cubic", "gpu"),
+    ("min_tie", 1e-3, "cubic", "gpu"),
+    ("max_tie", 1e-4, "quintic", "gpu"),
+    ("grf", 1e-2, "quintic", "grad"),
+])
+def test_one_call_equals_range_then_compress(ctx, kind, rel, fam, tsel):
+    print("case", kind, rel, fam, tsel, flush=True)
+    sys.stdout.flush()
+    a = _field(kind)
+    want_mn, want_mx = _daz_range(a)
+    t = torch.from_numpy(a).cuda()
+    p = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=3, family=fam)
+    w1, w2 = t.clone(), t.clone()
+    mn, mx = ctx.range(w1.data_ptr(), w1.numel())
+    assert np.float32(mn).view(np.uint32) == want_mn.view(np.uint32)
+    assert np.float32(mx).view(np.uint32) == want_mx.view(np.uint32)
+    slope, *_ = ctx.default_slope(w1.data_ptr(), a.shape, p, mn, mx)
+    p.default_slope = slope
+    pl = codec.make_plan(PLAN)
+    ref = ctx.compress_device(w1.data_ptr(), a.shape, pl, p, mn, mx, tsel=tsel)
+    res = ctx.compress_field(w2.data_ptr(), a.shape, pl, p, tsel=tsel)
+    assert np.float32(res.min_val).view(np.uint32) == want_mn.view(np.uint32)
+    assert np.float32(res.max_val).view(np.uint32) == want_mx.view(np.uint32)
+    assert res.stream == ref.stream
+    assert torch.equal(w1.view(torch.int32), w2.view(torch.int32))
+    assert res.counts == ref.counts
+
+
+@pytest.mark.parametrize("kind,msg", [("constant", "max_val==min_val"), ("nan", "nonfinite")])
+def test_one_call_reports_input_errors(ctx, kind, msg):
+    a = _field("grf")
+    if kind == "constant":
+        a[...] = np.float32(3.5)
+    else:
+        a.reshape(-1)[1234] = np.float32("nan")
+    t = torch.from_numpy(a).cuda()
+    p = codec.make_params(rel_eb=1e-3, block_pivot_ratio=3)
+    with pytest.raises(codec.AszError, match=msg):
+        ctx.compress_field(t.data_ptr(), a.shape, codec.make_plan(PLAN), p)
+    # the context stays usable
+    g = torch.from_numpy(_field("grf")).cuda()
+    ctx.compress_field(g.data_ptr(), tuple(g.shape), codec.make_plan(PLAN), p)
