@@ -531,7 +531,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     std::mutex side_mu;  // the exact-histogram path uses the side stream and shared scratch
     std::vector<double> host_ms_w;
     auto slot = [&](uint32_t f, uint64_t k) { return c->hres_host + (f * narr + k) * kHostSlotWords; };
-    auto wait_slot = [&](uint32_t f, uint64_t k) {
+    // spin: never sleep (the calling thread has nothing else to do)
+    auto wait_slot = [&](uint32_t f, uint64_t k, bool spin) {
         volatile uint32_t* flag = slot(f, k);
         uint64_t spins = 0;
         while (*flag != epoch) {
@@ -546,7 +547,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                 }
             }
             // spin briefly, then give the core away (batches run many of these waits at once)
-            if (spins < 4096) _mm_pause();
+            if (spin || spins < 4096) _mm_pause();
             else if (spins < 16384) std::this_thread::yield();
             else std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
@@ -605,20 +606,22 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // Codebook workers: the host pool's threads take the arrays in `order`
     // (every field's copy of an array in turn) as their results arrive,
     // concurrently with the launches below and with the later rounds on the GPU.
+    // With one field, this thread builds the last round's codebook itself once
+    // its launches are done (no hand-over to a sleeping worker at the end).
     const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
-    const uint64_t ntask = order.size() * nf;
-    host_ms_w.assign(ntask, 0.0);
+    const uint64_t ntask_all = order.size() * nf, ntask = ntask_all - (nf == 1 ? 1 : 0);
+    host_ms_w.assign(ntask_all, 0.0);
+    auto task = [&](size_t i, bool spin) {
+        const uint64_t k = order[i / nf];
+        const uint32_t f = (uint32_t)(i % nf);
+        if (lays[f].arrays[k].count == 0) return;  // an empty round has no slot
+        wait_slot(f, k, spin);
+        auto h0 = Clock::now();
+        build(f, k);
+        host_ms_w[i] = ms_since(h0);
+    };
     TaskPool& pool = host_pool(c);
-    if (stop_round < 0)
-        pool.start(ntask, [&](size_t i) {
-            const uint64_t k = order[i / nf];
-            const uint32_t f = (uint32_t)(i % nf);
-            if (lays[f].arrays[k].count == 0) return;  // an empty round has no slot
-            wait_slot(f, k);
-            auto h0 = Clock::now();
-            build(f, k);
-            host_ms_w[i] = ms_since(h0);
-        });
+    if (stop_round < 0) pool.start(ntask, [&](size_t i) { task(i, false); });
     // stops and joins the workers on every exit path below
     struct Joiner {
         TaskPool& pool;
@@ -727,6 +730,16 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     enqueued = true;
     joiner.armed = false;
     const auto enq0 = Clock::now();
+    try {
+        for (size_t i = ntask; i < ntask_all; i++) task(i, true);
+    } catch (...) {
+        abort_all = true;
+        try {
+            pool.wait();
+        } catch (...) {
+        }
+        throw;
+    }
     pool.wait();
     S.codebook_wait_ms = ms_since(enq0);
     double host_ms = 0;

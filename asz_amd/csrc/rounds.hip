@@ -59,15 +59,24 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
         const float4 v = x4[i];
         const float e[4] = {v.x, v.y, v.z, v.w};
+        uint32_t key[4];
+        bool z = false;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             bad |= !isfinite(e[k]);
-            const uint32_t key = order_key(e[k]);
-            mn = min(mn, key);
-            mx = max(mx, key);
-            if (key == kZeroKey) {
-                zf = min(zf, (unsigned long long)(4 * i + k));
-                zl = max(zl, (unsigned long long)(4 * i + k));
+            key[k] = order_key(e[k]);
+            mn = min(mn, key[k]);
+            mx = max(mx, key[k]);
+            z |= key[k] == kZeroKey;
+        }
+        // zero ties are rare: the index bookkeeping runs only in waves that meet one
+        if (__ballot(z)) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (key[k] == kZeroKey) {
+                    zf = min(zf, (unsigned long long)(4 * i + k));
+                    zl = max(zl, (unsigned long long)(4 * i + k));
+                }
             }
         }
     }
