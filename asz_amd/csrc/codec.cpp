@@ -996,6 +996,12 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     hipStream_t st = c->stream;
     asz_stats& S = c->stats;
     S = asz_stats{};
+    // the stream goes up while the host parses its headers and builds the
+    // decode tables
+    const uint64_t nwords = (n + 3) / 4 + 4;
+    uint32_t* dwords = c->words.get<uint32_t>(nwords);
+    launch_fill_i32(reinterpret_cast<int32_t*>(dwords + n / 4), nwords - n / 4, 0, st);
+    HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
     Parsed Q = parse_stream(s, n, p);
     Geometry g = make_geometry(dims, Q.plan, p.block_pivot_ratio);
     const float aeb = abs_eb_of(p.rel_eb, Q.mn, Q.mx);
@@ -1046,10 +1052,6 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
         da.push_back(D);
     }
     const uint64_t nwg = wg_arr.size(), nsub = nwg * 256;
-    const uint64_t nwords = (n + 3) / 4 + 4;
-    uint32_t* dwords = c->words.get<uint32_t>(nwords);
-    HIPCHECK(hipMemsetAsync(dwords, 0, nwords * 4, st));
-    HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
     uint32_t* dtab = c->dec_tab.get<uint32_t>(tables.size() + 1);
     HIPCHECK(hipMemcpyAsync(dtab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
     DecArrayDev* dda = c->dec_arrays.get<DecArrayDev>(da.size() + 1);
@@ -1061,30 +1063,36 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     uint64_t* dex1 = c->exit1.get<uint64_t>(nsub + 1);
     uint32_t* dcnt = c->count.get<uint32_t>(nsub + 1);
     uint64_t* dwgs = c->out_pos.get<uint64_t>(2 * nwg + 2);
-    unsigned int* flags = c->misc.get<unsigned int>(8);
-    HIPCHECK(hipMemsetAsync(flags, 0, 8 * 4, st));
+    // flags[0]: error bits of the write pass; flags[8 + i]: "changed" of fix pass i of a group
+    constexpr uint32_t kFixGroup = 6;
+    unsigned int* flags = c->misc.get<unsigned int>(8 + kFixGroup);
+    launch_fill_i32(reinterpret_cast<int32_t*>(flags), 8 + kFixGroup, 0, st);
     c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
     int kd = kev_begin(c, KEV_DECODE, 0);
     launch_dec_sync(dwords, dda, dwg, nwg, dtab, dstart, dex0, dcnt, st);
+    // fix passes in groups, one synchronisation per group: a pass after
+    // convergence changes nothing (and its workgroups skip the table load)
     uint64_t iters = 0;
     for (;;) {
-        HIPCHECK(hipMemsetAsync(flags, 0, 4, st));
-        launch_dec_fix(dwords, dda, dwg, nwg, dtab, dstart, dex0, dex1, dcnt, flags, st);
-        std::swap(dex0, dex1);
-        unsigned int changed = 0;
-        HIPCHECK(hipMemcpyAsync(&changed, flags, 4, hipMemcpyDeviceToHost, st));
+        if (iters) launch_fill_i32(reinterpret_cast<int32_t*>(flags + 8), kFixGroup, 0, st);
+        for (uint32_t i = 0; i < kFixGroup; i++) {
+            launch_dec_fix(dwords, dda, dwg, nwg, dtab, dstart, dex0, dex1, dcnt, flags + 8 + i, st);
+            std::swap(dex0, dex1);
+        }
+        unsigned int changed[kFixGroup];
+        HIPCHECK(hipMemcpyAsync(changed, flags + 8, sizeof(changed), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        iters++;
-        if (!changed) break;
+        iters += kFixGroup;
+        if (!changed[kFixGroup - 1]) break;
         if (iters > nsub + 2) fail(ASZ_ERR_DATA, "Huffman decoder did not synchronise");
     }
-    launch_dec_offsets(dda, (uint32_t)da.size(), nwg, dcnt, dwgs, dwgs + nwg + 1, flags + 1, st);
-    launch_dec_write(dwords, dda, dwg, nwg, dtab, dstart, dcnt, dwgs + nwg + 1, sym, flags + 1, st);
+    launch_dec_offsets(dda, (uint32_t)da.size(), nwg, dcnt, dwgs, dwgs + nwg + 1, flags, st);
+    launch_dec_write(dwords, dda, dwg, nwg, dtab, dstart, dcnt, dwgs + nwg + 1, sym, flags, st);
     kev_end(c, kd);
     HIPCHECK(hipEventRecord(c->ev[1], st));
     unsigned int derr = 0;
-    HIPCHECK(hipMemcpyAsync(&derr, flags + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&derr, flags, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     if (derr & 1) fail(ASZ_ERR_DATA, "Huffman payload length does not match its value count");
     if (derr & 2) fail(ASZ_ERR_DATA, "Huffman payload contains too many values");
@@ -1561,15 +1569,18 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
 
 int asz_decompress(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,
                    float* d_out, int32_t* qb_out, int64_t* t_out) {
-    return guarded(c, [&] {
+    const int rc = guarded(c, [&] {
         HIPCHECK(hipSetDevice(c->device));
         asz::decompress_impl(c, stream, n, dims, *p, d_out, qb_out, t_out);
     });
+    // an error can leave the stream's upload running: it reads the caller's buffer
+    if (rc != ASZ_OK) (void)hipStreamSynchronize(c->stream);
+    return rc;
 }
 
 int asz_decompress_host(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t dims[3], const asz_params* p,
                         float* out) {
-    return guarded(c, [&] {
+    const int rc = guarded(c, [&] {
         HIPCHECK(hipSetDevice(c->device));
         const uint64_t N = dims[0] * dims[1] * dims[2];
         float* w = c->work.get<float>(N);
@@ -1577,6 +1588,8 @@ int asz_decompress_host(asz_ctx* c, const uint8_t* stream, uint64_t n, const uin
         HIPCHECK(hipMemcpyAsync(out, w, N * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
+    if (rc != ASZ_OK) (void)hipStreamSynchronize(c->stream);
+    return rc;
 }
 
 }  // extern "C"

@@ -174,10 +174,16 @@ __global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ wo
                                                  uint32_t* count, unsigned int* changed) {
     __shared__ uint32_t lds[kRootMax];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    load_root(lds, A, tables);
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t u = j - A.sub0;
-    if (u >= A.nsub || u == 0 || start[j] == exit_in[j - 1]) {
+    const bool settled = u >= A.nsub || u == 0 || start[j] == exit_in[j - 1];
+    // the table only when some lane re-parses (after convergence: none)
+    if (!__syncthreads_or(!settled)) {
+        exit_out[j] = exit_in[j];
+        return;
+    }
+    load_root(lds, A, tables);
+    if (settled) {
         exit_out[j] = exit_in[j];
         return;
     }
