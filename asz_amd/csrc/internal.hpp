@@ -137,7 +137,7 @@ constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave process
 // dedicated pass.
 constexpr int kHistWin = 128;                   // window of the t / pivot arrays
 constexpr uint32_t kHistWinMin = 128, kHistWinMax = 4096;
-constexpr uint32_t kLdsWinMin = 2048;
+constexpr uint32_t kLdsWinMin = 1024;  // 4 KB: the axis-0/1 sweep then fits five workgroups per CU
 constexpr uint64_t kHistPartialWords = 1 << 20, kHistPartialMin = 1 << 15;
 constexpr uint32_t kHistGlobal = 65536;
 constexpr int32_t kHistCentre = 32768;
