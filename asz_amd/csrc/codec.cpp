@@ -640,15 +640,32 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     double tsel_host_ms = 0;
     std::vector<int64_t> hkeys;
     std::vector<uint64_t> koff;
+    // With the GPU t selection, a round's fold runs in the same launch as the
+    // next round's t selection (its time is then part of that round's
+    // tsel_round_ms); fold_r0 / fold_nr: the rounds waiting for their fold.
+    const bool gpu_tsel = tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD;
+    uint64_t fold_r0 = 0, fold_nr = 0;
+    auto fold_groups = [&](uint64_t r0, uint64_t nr) {
+        uint32_t mg = 0;
+        for (uint64_t q = r0; q < r0 + nr; q++) mg = std::max(mg, pc.folds[q].groups);
+        return mg;
+    };
     for (uint64_t r = 0; r < Lrun; r++) {
         const RoundDesc& R = P.rounds[r];
         check_grid(R, nf);
         float* tb = tblk + r * nbl;
-        if (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) {
+        if (gpu_tsel) {
             TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
                        p.t_min, p.t_max};
             int kt = kev_begin(c, KEV_TSEL, (int)r);
-            launch_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, st, fs, nf);
+            if (fold_nr) {
+                launch_fold_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, dfolds,
+                                 (uint32_t)fold_r0, (uint32_t)fold_nr, fold_groups(fold_r0, fold_nr), part, phist,
+                                 fpart, ghist, c->hres_dev, epoch, st, fs, nf);
+                fold_nr = 0;
+            } else {
+                launch_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, st, fs, nf);
+            }
             kev_end(c, kt);
         } else if (tsel_mode == ASZ_TSEL_EXACT) {
             auto h0 = Clock::now();
@@ -712,12 +729,15 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         kev_end(c, kr);
         if (r + 1 >= batch_end) {
             const uint64_t f0 = r + 1 == batch_end ? 0 : r;
-            int kh = kev_begin(c, KEV_HIST, (int)r);
-            uint32_t mg = 0;
-            for (uint64_t q = f0; q <= r; q++) mg = std::max(mg, pc.folds[q].groups);
-            launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), mg, part, phist, fpart, acc, ghist,
-                               c->hres_dev, epoch, st, fs, nf);
-            kev_end(c, kh);
+            if (gpu_tsel && r + 1 < Lrun) {
+                fold_r0 = f0;  // with the next round's t selection
+                fold_nr = r + 1 - f0;
+            } else {
+                int kh = kev_begin(c, KEV_HIST, (int)r);
+                launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), fold_groups(f0, r + 1 - f0), part,
+                                   phist, fpart, acc, ghist, c->hres_dev, epoch, st, fs, nf);
+                kev_end(c, kh);
+            }
         }
     }
     HIPCHECK(hipGetLastError());

@@ -93,6 +93,14 @@ void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float
                  uint64_t nblocks, bool grad, hipStream_t st, const FieldSet& fs = FieldSet{},
                  uint32_t nfields = 1);
 void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st);
+// launch_tsel for round T.r together with launch_fold_rounds for rounds
+// r0 .. r0 + nr - 1 (finished before T.r), in one launch.
+struct FoldDesc;
+void launch_fold_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
+                      DevAcc* acc, uint64_t nblocks, bool grad, const FoldDesc* descs, uint32_t r0, uint32_t nr,
+                      uint32_t max_groups, const Partial* part, const uint32_t* phist, Partial* fpart,
+                      uint32_t* ghist, uint32_t* hres, uint32_t epoch, hipStream_t st, const FieldSet& fs = FieldSet{},
+                      uint32_t nfields = 1);
 
 // Residual energy of round R per block (forward: E[b], FP64) or its adjoint
 // (g[b * np + j] = gE[b] * dE_b/dtheta_j, FP32), np = 1 (cubic t) or 2 (quintic a, b).

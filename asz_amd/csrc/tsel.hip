@@ -11,6 +11,7 @@
 // so H1 and PSNR agree with the reference to float rounding and the chosen t
 // agrees except where two candidates score within that rounding of each other.
 #include "device.hpp"
+#include "fold.hpp"
 #include "launch.hpp"
 
 namespace asz {
@@ -112,10 +113,11 @@ __device__ __forceinline__ double entropy_term(uint32_t cnt, float rec) {
 // neighbouring points of the 1/t_precision grid, and {floor, 1/4, ceil}
 // are judged by the exact proxy.  Any t on the grid decodes with the
 // reference decoder (decompress.tpp:185).
+// Block bid of field f (f = 0 unless fs.fd is set).
 template <bool GRAD>
-__global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
-                                                       const uint32_t* __restrict__ ss_tab, float* tblk,
-                                                       int32_t* t_int, DevAcc* acc, FieldSet fs) {
+__device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __restrict__ work,
+                                           const uint32_t* __restrict__ ss_tab, float* tblk, int32_t* t_int,
+                                           DevAcc* acc, const FieldSet& fs, uint32_t bid, uint32_t f) {
     __shared__ TselShared sh;
     __shared__ float s_cand[kNCand];
     __shared__ int s_nc;
@@ -123,8 +125,7 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
     // the field's layer error bound, layer slope and value range (kept out of
     // T, so that T stays in kernel arguments)
     float eb = R.eb, layer_slope = R.layer_slope, range = T.range;
-    if (fs.fd) {  // field blockIdx.y of a multi-field call
-        const uint32_t f = blockIdx.y;
+    if (fs.fd) {  // field f of a multi-field call
         const FieldDesc& F = fs.fd[f];
         work = F.work;
         tblk += f * fs.tblk;
@@ -134,7 +135,6 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
         layer_slope = F.r[R.layer].layer_slope;
         range = F.range;
     }
-    const uint32_t bid = blockIdx.x;
     const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
     const uint32_t bb[3] = {b0, b1, b2};
     const uint32_t ss = ss_tab[bid];
@@ -281,6 +281,43 @@ __global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* 
     }
 }
 
+template <bool GRAD>
+__global__ __launch_bounds__(kTselThreads) void k_tsel(TselDesc T, const float* __restrict__ work,
+                                                       const uint32_t* __restrict__ ss_tab, float* tblk,
+                                                       int32_t* t_int, DevAcc* acc, FieldSet fs) {
+    tsel_block<GRAD>(T, work, ss_tab, tblk, t_int, acc, fs, blockIdx.x, blockIdx.y);
+}
+
+// The fold of rounds r0 .. r0 + nr - 1 beside round T.r's t selection, in one
+// launch: the two are independent (the fold reads the finished rounds'
+// partials, the selection reads the field), so the fold no longer adds a
+// launch of its own to the chain of rounds.  Grid x: the nbl blocks' t
+// selection, then mg x nr fold workgroups; grid y: fields.
+template <bool GRAD>
+__global__ __launch_bounds__(kTselThreads) void k_fold_tsel(TselDesc T, const float* __restrict__ work,
+                                                            const uint32_t* __restrict__ ss_tab, float* tblk,
+                                                            int32_t* t_int, DevAcc* acc, FieldSet fs, uint32_t nbl,
+                                                            const FoldDesc* __restrict__ descs, uint32_t r0,
+                                                            uint32_t mg, const Partial* __restrict__ part,
+                                                            const uint32_t* __restrict__ phist, Partial* fpart,
+                                                            uint32_t* ghist, uint32_t* hres, uint32_t epoch) {
+    const uint32_t f = blockIdx.y;
+    if (blockIdx.x < nbl) {
+        tsel_block<GRAD>(T, work, ss_tab, tblk, t_int, acc, fs, blockIdx.x, f);
+        return;
+    }
+    const uint32_t g = blockIdx.x - nbl;
+    if (fs.fd) {
+        part += f * fs.part;
+        phist += f * fs.phist;
+        fpart += f * fs.fpart;
+        acc += f;
+        ghist += f * fs.ghist;
+        hres += f * fs.hres;
+    }
+    fold_group(descs, r0 + g / mg, g % mg, part, phist, fpart, acc, ghist, hres, epoch);
+}
+
 // Keys of every sample of every block, in the reference's loop order, for
 // the host scoring of ASZ_TSEL_EXACT (compress.tpp:102-120).
 __global__ __launch_bounds__(256) void k_tsel_keys(TselDesc T, const float* __restrict__ work,
@@ -357,6 +394,24 @@ void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float
     else
         hipLaunchKernelGGL(dev::k_tsel<false>, grid, dim3(dev::kTselThreads), 0, st, T, work, ss, tblk, t_int, acc,
                            fs);
+}
+
+void launch_fold_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float* tblk, int32_t* t_int,
+                      DevAcc* acc, uint64_t nblocks, bool grad, const FoldDesc* descs, uint32_t r0, uint32_t nr,
+                      uint32_t max_groups, const Partial* part, const uint32_t* phist, Partial* fpart,
+                      uint32_t* ghist, uint32_t* hres, uint32_t epoch, hipStream_t st, const FieldSet& fs,
+                      uint32_t nfields) {
+    const uint64_t nx = nblocks + (uint64_t)max_groups * nr;
+    if (nx > 0x7FFFFFFFull) throw Error(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
+    const dim3 grid((uint32_t)nx, nfields);
+    if (grad)
+        hipLaunchKernelGGL(dev::k_fold_tsel<true>, grid, dim3(dev::kTselThreads), 0, st, T, work, ss, tblk, t_int,
+                           acc, fs, (uint32_t)nblocks, descs, r0, max_groups, part, phist, fpart, ghist, hres,
+                           epoch);
+    else
+        hipLaunchKernelGGL(dev::k_fold_tsel<false>, grid, dim3(dev::kTselThreads), 0, st, T, work, ss, tblk, t_int,
+                           acc, fs, (uint32_t)nblocks, descs, r0, max_groups, part, phist, fpart, ghist, hres,
+                           epoch);
 }
 
 void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,
