@@ -47,7 +47,7 @@ def c4_fields(steps=48, seed=4, device="cuda"):
     return out
 
 
-def compress_fields(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, tsel="grad", concurrency=4,
+def compress_fields(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, tsel="grad", concurrency=6,
                     on_device=False, fields_per_chunk=0, family="cubic"):
     """Batched form; returns (streams, per-field info, wall seconds).  With
     on_device the streams stay in HBM as (pointer, bytes) pairs;
@@ -82,7 +82,7 @@ def c4_plans(flds):
             for f in flds]
 
 
-def config4_comparison(concurrency=4, reps=3):
+def config4_comparison(concurrency=6, reps=3):
     """Batched (gradient-tuned, concurrent) vs plain per-field GPU form on the
     60 config-4 fields at REL 1e-3: wall time, total compression ratio (host
     Zstd after the timed region) and max error over eb.  The plain form is
@@ -144,4 +144,4 @@ def config4_comparison(concurrency=4, reps=3):
 if __name__ == "__main__":
     import json
     import sys
-    print(json.dumps(config4_comparison(concurrency=int(sys.argv[1]) if len(sys.argv) > 1 else 4), indent=1))
+    print(json.dumps(config4_comparison(concurrency=int(sys.argv[1]) if len(sys.argv) > 1 else 6), indent=1))

@@ -269,7 +269,7 @@ class Context:
         return CompressResult(stream, float(mn), float(mx), st, qb, t, recon,
                               [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
-    def compress_batch(self, fields, plans, params: Params, tsel="grad", concurrency=4, on_device=False,
+    def compress_batch(self, fields, plans, params: Params, tsel="grad", concurrency=6, on_device=False,
                        fields_per_chunk=0):
         """Compress many device fields (CUDA float32 tensors, overwritten with
         their reconstructions) in one call; plans: one Plan per field (or one

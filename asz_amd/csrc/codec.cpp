@@ -1447,7 +1447,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         if (nfields == 0) return;
         if (tsel_mode == ASZ_TSEL_INJECT) fail(ASZ_ERR_ARG, "t injection is not available in batches");
         for (uint32_t i = 0; i < nfields; i++) streams[i] = nullptr;
-        const uint32_t T = std::max<uint32_t>(1, std::min<uint32_t>(concurrency ? concurrency : 4, nfields));
+        const uint32_t T = std::max<uint32_t>(1, std::min<uint32_t>(concurrency ? concurrency : 6, nfields));
         // Fields of one shape and plan (time steps of a series, same-shape
         // fields) are compressed together in chunks: every per-round kernel of
         // a chunk covers all of its fields.  Chunks hold about a T-th of their

@@ -215,7 +215,7 @@ int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_byt
  * with the same result.  Fields of one shape and plan are compressed together
  * in chunks (ASZ_OPT_BATCH_FIELDS; with ASZ_TSEL_GPU / ASZ_TSEL_GRAD), each
  * per-round kernel of a chunk covering all of its fields, and the chunks run
- * on `concurrency` worker contexts (0: 4), each with its own HIP stream and
+ * on `concurrency` worker contexts (0: 6), each with its own HIP stream and
  * host thread, so their pipelines overlap on the device.  d_fields[i]
  * (device, dims[3i..3i+2]) is overwritten with its reconstruction; plans[i]
  * is that field's plan (time steps typically share one).  On success
