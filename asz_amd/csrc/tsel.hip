@@ -23,13 +23,16 @@ constexpr int kDirect = 1024;      // direct LDS bins per candidate: QB in [-512
 constexpr int kHashSlots = 512;    // per candidate, for QBs outside the direct bins
 constexpr int kMaxKeep = 2048;     // samples whose keys stay in LDS; more are recomputed
 
+// NC: candidates scored per block (5 for the discrete scan, 3 for the
+// gradient-tuned form, whose smaller footprint lets more workgroups share a CU).
+template <int NC>
 struct TselShared {
     float af[kMaxKeep], bf[kMaxKeep];
-    uint32_t bins[kNCand][kDirect];
-    long long hkey[kNCand][kHashSlots];
-    uint32_t hcnt[kNCand][kHashSlots];
-    uint32_t ovf[kNCand];          // QBs that found no hash slot (counted as distinct symbols)
-    double red[kTselThreads / kWave][2 * kNCand];
+    uint32_t bins[NC][kDirect];
+    long long hkey[NC][kHashSlots];
+    uint32_t hcnt[NC][kHashSlots];
+    uint32_t ovf[NC];              // QBs that found no hash slot (counted as distinct symbols)
+    double red[kTselThreads / kWave][2 * NC];
 };
 
 // The five values sample idx reads: a = x[-h], c = x[-3h], b = x[+h], e = x[+3h], x0.
@@ -76,7 +79,8 @@ __device__ __forceinline__ void sample_keys(const TselDesc& T, const float* __re
     sample_from(v, neg2, af, bf, Araw, Braw);
 }
 
-__device__ __forceinline__ void count_qb(TselShared& sh, int k, long long q) {
+template <int NC>
+__device__ __forceinline__ void count_qb(TselShared<NC>& sh, int k, long long q) {
     if (q >= -kDirect / 2 && q < kDirect / 2) {
         atomicAdd(&sh.bins[k][(int)q + kDirect / 2], 1u);
         return;
@@ -118,8 +122,9 @@ template <bool GRAD>
 __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __restrict__ work,
                                            const uint32_t* __restrict__ ss_tab, float* tblk, int32_t* t_int,
                                            DevAcc* acc, const FieldSet& fs, uint32_t bid, uint32_t f) {
-    __shared__ TselShared sh;
-    __shared__ float s_cand[kNCand];
+    constexpr int NC = GRAD ? 3 : kNCand;  // GRAD: 1/4 and the two grid points around the minimiser
+    __shared__ TselShared<NC> sh;
+    __shared__ float s_cand[NC];
     __shared__ int s_nc;
     const RoundDesc& R = T.R;
     // the field's layer error bound, layer slope and value range (kept out of
@@ -160,12 +165,12 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         const float neg2 = 0.5f / stat_eb;
         const float neg_2_eb = 0.5f / eb;
         const float two_eb = eb * 2.0f;
-        for (int i = threadIdx.x; i < kNCand * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
-        for (int i = threadIdx.x; i < kNCand * kHashSlots; i += kTselThreads) {
+        for (int i = threadIdx.x; i < NC * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
+        for (int i = threadIdx.x; i < NC * kHashSlots; i += kTselThreads) {
             (&sh.hkey[0][0])[i] = LLONG_MIN;
             (&sh.hcnt[0][0])[i] = 0;
         }
-        if (threadIdx.x < kNCand) sh.ovf[threadIdx.x] = 0;
+        if (threadIdx.x < NC) sh.ovf[threadIdx.x] = 0;
         double saa = 0, sab = 0;  // GRAD: sum A^2, sum A B over the samples
         // four samples per thread at a time: all twenty loads in flight before any use
         const uint32_t ns_pass = GRAD ? ns : min(ns, (uint32_t)kMaxKeep);
@@ -213,21 +218,21 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
             }
         } else if (threadIdx.x == 0) {
             const float c5[kNCand] = {1.0f / 16, 1.0f / 4, 9.0f / 32, 5.0f / 16, 11.0f / 32};
-            for (int k = 0; k < kNCand; k++) s_cand[k] = c5[k];
-            s_nc = kNCand;
+            for (int k = 0; k < NC; k++) s_cand[k] = c5[k];
+            s_nc = NC;
         }
         __syncthreads();
         const int nc = s_nc;
-        float cand[kNCand];
+        float cand[NC];
 #pragma unroll
-        for (int k = 0; k < kNCand; k++) cand[k] = k < nc ? s_cand[k] : 0.0f;
-        double mse[kNCand] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < NC; k++) cand[k] = k < nc ? s_cand[k] : 0.0f;
+        double mse[NC] = {};
         for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
             float af, bf, A, B;
             if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
             else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf, A, B);
 #pragma unroll
-            for (int k = 0; k < kNCand; k++) {
+            for (int k = 0; k < NC; k++) {
                 if (k >= nc) break;
                 const float err = __fmaf_rn(bf, 2.0f, af * (cand[k] * 2.0f)) * stat_eb;
                 const long long q = lround_f64(err * neg_2_eb);
@@ -239,9 +244,9 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         __syncthreads();
         const float dn = (float)ns;
         const float rec = 1.0f / dn;
-        double h[kNCand] = {0, 0, 0, 0, 0};
+        double h[NC] = {};
 #pragma unroll
-        for (int k = 0; k < kNCand; k++) {
+        for (int k = 0; k < NC; k++) {
             for (int i = threadIdx.x; i < kDirect; i += kTselThreads)
                 if (sh.bins[k][i]) h[k] -= entropy_term(sh.bins[k][i], rec);
             for (int i = threadIdx.x; i < kHashSlots; i += kTselThreads)
@@ -250,17 +255,17 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         }
         const int w = threadIdx.x / kWave;
 #pragma unroll
-        for (int k = 0; k < kNCand; k++) {
+        for (int k = 0; k < NC; k++) {
             const double hs = wave_sum(h[k]), ms = wave_sum(mse[k]);
-            if ((threadIdx.x & (kWave - 1)) == 0) { sh.red[w][k] = hs; sh.red[w][kNCand + k] = ms; }
+            if ((threadIdx.x & (kWave - 1)) == 0) { sh.red[w][k] = hs; sh.red[w][NC + k] = ms; }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            float h1[kNCand], psnr[kNCand];
+            float h1[NC], psnr[NC];
             const float r2 = range * range;
             for (int k = 0; k < nc; k++) {
                 double hs = 0, ms = 0;
-                for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][kNCand + k]; }
+                for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][NC + k]; }
                 h1[k] = (float)hs;
                 psnr[k] = log10f((r2 / (float)ms) * dn) * 10.0f;
             }
