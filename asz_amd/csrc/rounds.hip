@@ -602,24 +602,42 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
     int32_t* A = staged + f * staged_stride + slab_off;   // "012": p Q + q
     int32_t* B = ordered + f * ordered_stride + slab_off;  // order:  q P + p
     const uint32_t n = np * nq;
+    // e / d for e < kXT^2 and d <= kXT through a float reciprocal: (e + 1/2) / d
+    // lies at least 1/(2d) from an integer and the float error stays far
+    // below that, so the floor is exact
+    const float inv_nq = 1.0f / (float)nq, inv_np = 1.0f / (float)np;
+    auto qdiv = [](uint32_t e, float inv) { return (uint32_t)(((float)e + 0.5f) * inv); };
+    constexpr uint32_t kPer = kXT * kXT / 256;  // elements per thread when the tile is full
     if (TO_ORDER) {
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t pl = e / nq, ql = e - pl * nq;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e >= n) break;
+            const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
             tile[pl][ql] = A[(uint64_t)(p0 + pl) * Q + q0 + ql];
         }
         __syncthreads();
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t ql = e / np, pl = e - ql * np;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e >= n) break;
+            const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
             B[(q0 + ql) * P + p0 + pl] = tile[pl][ql];
         }
     } else {
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t ql = e / np, pl = e - ql * np;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e >= n) break;
+            const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
             tile[pl][ql] = B[(q0 + ql) * P + p0 + pl];
         }
         __syncthreads();
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t pl = e / nq, ql = e - pl * nq;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e >= n) break;
+            const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
             A[(uint64_t)(p0 + pl) * Q + q0 + ql] = tile[pl][ql];
         }
     }
