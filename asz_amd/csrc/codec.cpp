@@ -804,24 +804,25 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // zeroed stream buffer); payload bits are ORed in by the encoder.  Every
     // field's segments, header bytes, codebooks and array descriptors go up in
     // one upload.
-    struct FieldUp {
-        std::vector<uint8_t> blob;
-        std::vector<ByteSeg> bsegs;
-        uint64_t book_words = 0;
-        uint64_t chunks = 0, o_seg = 0, o_blob = 0, o_book = 0, o_arr = 0, woff = 0, nwords = 0;
-    };
-    std::vector<FieldUp> ups(nf);
-    auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
-    uint64_t up_bytes = 0, words_total = 0, max_chunks = 0;
+    // Every field's header segments, header bytes, codebooks and array
+    // descriptors in one upload, with offsets into the combined symbol
+    // buffer, codebook table and stream buffer (field f's stream starts at
+    // word woff[f]), so that one launch of each encoder kernel codes them all.
+    std::vector<uint8_t> blob;
+    std::vector<ByteSeg> bsegs;
+    std::vector<HuffArrayDev> darrs;
+    std::vector<uint64_t> woff(nf);
+    uint64_t words_total = 0, book_total = 0, chunks = 0;
     int max_len = 0;
     for (uint32_t f = 0; f < nf; f++) {
         Layout& lay = lays[f];
-        FieldUp& U = ups[f];
+        woff[f] = words_total;
+        const uint64_t sbase = woff[f] * 4;  // the field's first stream byte
         uint64_t spos = 0;
         auto emit = [&](const uint8_t* q, uint64_t n) {
-            if (!U.bsegs.empty() && U.bsegs.back().dst + U.bsegs.back().len == spos) U.bsegs.back().len += n;
-            else U.bsegs.push_back(ByteSeg{spos, U.blob.size(), n});
-            U.blob.insert(U.blob.end(), q, q + n);
+            if (!bsegs.empty() && bsegs.back().dst + bsegs.back().len == sbase + spos) bsegs.back().len += n;
+            else bsegs.push_back(ByteSeg{sbase + spos, blob.size(), n});
+            blob.insert(blob.end(), q, q + n);
             spos += n;
         };
         {
@@ -835,7 +836,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             }
             emit(pre, 9 + 2 * L);
         }
-        uint64_t book_total = 0, chunks = 0;
         for (uint64_t k = 0; k < narr; k++) {
             HuffArray& A = lay.arrays[k];
             HuffArrayDev& D = lay.dev[k];
@@ -846,7 +846,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             if (!A.count) continue;
             emit(A.header.data(), A.header.size());
             if (A.single) continue;
-            D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+            D.src_off = f * fs.sym + (k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off));
             D.count = A.count;
             D.offset = A.offset;
             D.range = A.range;
@@ -854,82 +854,71 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             D.chunk0 = chunks;
             D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
             chunks += D.nchunk;
-            D.payload_bit = spos * 8;
+            D.payload_bit = (sbase + spos) * 8;
             spos += (A.payload_bits + 7) >> 3;
             D.lds_book = (A.range <= kLdsBook && A.max_len <= 24) ? 1 : 0;
             lay.max_len = std::max(lay.max_len, A.max_len);
             book_total += A.range;
+            darrs.push_back(D);
         }
-        U.book_words = book_total;
         lay.total_bytes = spos;
-        U.chunks = chunks;
-        U.o_seg = up_bytes;
-        U.o_blob = al8(U.o_seg + U.bsegs.size() * sizeof(ByteSeg));
-        U.o_book = al8(U.o_blob + U.blob.size());
-        U.o_arr = al8(U.o_book + book_total * 8);
-        up_bytes = al8(U.o_arr + narr * sizeof(HuffArrayDev));
-        U.nwords = (lay.total_bytes + 3) / 4 + 4;
-        U.woff = words_total;
-        words_total += (U.nwords + 63) / 64 * 64;
-        max_chunks = std::max(max_chunks, chunks);
+        words_total += ((lay.total_bytes + 3) / 4 + 4 + 63) / 64 * 64;
         max_len = std::max(max_len, lay.max_len);
         c->fstats[f].stream_bytes = lay.total_bytes;
         for (uint64_t k = 0; k <= L; k++) c->fstats[f].qb_counts[k] = lay.arrays[k + 1].count;
     }
     S.stream_bytes = lays[0].total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lays[0].arrays[k + 1].count;
+    auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
+    const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
+                   o_arr = al8(o_book + book_total * 8), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
     uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
-    for (uint32_t f = 0; f < nf; f++) {
-        const FieldUp& U = ups[f];
-        std::memcpy(hu + U.o_seg, U.bsegs.data(), U.bsegs.size() * sizeof(ByteSeg));
-        std::memcpy(hu + U.o_blob, U.blob.data(), U.blob.size());
-        for (uint64_t k = 0; k < narr; k++) {
-            const HuffArray& A = lays[f].arrays[k];
-            if (A.count && !A.single)
-                std::memcpy(hu + U.o_book + lays[f].dev[k].book_off * 8, A.book.data(), A.range * 8);
-        }
-        std::memcpy(hu + U.o_arr, lays[f].dev.data(), narr * sizeof(HuffArrayDev));
+    std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
+    std::memcpy(hu + o_blob, blob.data(), blob.size());
+    {
+        size_t i = 0;
+        for (uint32_t f = 0; f < nf; f++)
+            for (uint64_t k = 0; k < narr; k++) {
+                const HuffArray& A = lays[f].arrays[k];
+                if (!A.count || A.single) continue;
+                std::memcpy(hu + o_book + darrs[i++].book_off * 8, A.book.data(), A.range * 8);
+            }
     }
+    std::memcpy(hu + o_arr, darrs.data(), darrs.size() * sizeof(HuffArrayDev));
     uint8_t* du = c->hmeta.get<uint8_t>(up_bytes);
     HIPCHECK(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, st));
     S.layout_ms = ms_since(hh0);
     S.huffman_host_ms = host_ms + S.layout_ms;
 
     uint32_t* dwords = c->words.get<uint32_t>(words_total);
-    // chunk scratch: each chunk's bits at the start of its own slot; the
-    // fields are coded one after another on the stream and share it
+    // chunk scratch: each chunk's bits at the start of its own slot
     const uint64_t slot_words = (uint64_t)kEncChunk * std::max(max_len, 1) / 32 + 2;
-    uint32_t* scratch = c->scratch.get<uint32_t>(max_chunks * slot_words + 1);
-    uint32_t* cbits = c->chunk_bits.get<uint32_t>(max_chunks + 1);
-    uint64_t* coffs = c->chunk_off.get<uint64_t>(max_chunks + 1);
+    uint32_t* scratch = c->scratch.get<uint32_t>(chunks * slot_words + 1);
+    uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
+    uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
     HIPCHECK(hipEventRecord(c->ev[4], st));
     launch_fill_i32(reinterpret_cast<int32_t*>(dwords), words_total, 0, st);
+    if (max_len > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), chunks * slot_words, 0, st);
     int ke = kev_begin(c, KEV_ENCODE, 0);
-    for (uint32_t f = 0; f < nf; f++) {
-        const FieldUp& U = ups[f];
-        const int ml = lays[f].max_len;
-        uint32_t* fw = dwords + U.woff;
-        if (ml > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), U.chunks * slot_words, 0, st);
-        launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + U.o_seg), (uint32_t)U.bsegs.size(),
-                             du + U.o_blob, reinterpret_cast<uint8_t*>(fw), st);
-        const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + U.o_arr);
-        launch_encode_chunks(sym_of(f), darr, (uint32_t)narr, U.chunks, reinterpret_cast<const uint64_t*>(du + U.o_book),
-                             scratch, slot_words, cbits, ml, st);
-        launch_chunk_scan(darr, (uint32_t)narr, cbits, coffs, st);
-        launch_compact_bits(scratch, slot_words, cbits, coffs, U.chunks, fw, st);
-    }
+    launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
+                         reinterpret_cast<uint8_t*>(dwords), st);
+    const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
+    launch_encode_chunks(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
+                         scratch, slot_words, cbits, max_len, st);
+    launch_chunk_scan(darr, (uint32_t)darrs.size(), cbits, coffs, st);
+    launch_compact_bits(scratch, slot_words, cbits, coffs, chunks, dwords, st);
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
     c->fouts.assign(nf, {nullptr, 0});
     if (c->stream_on_device) {
         for (uint32_t f = 0; f < nf; f++)
-            c->fouts[f] = {reinterpret_cast<uint8_t*>(dwords + ups[f].woff), lays[f].total_bytes};
+            c->fouts[f] = {reinterpret_cast<uint8_t*>(dwords + woff[f]), lays[f].total_bytes};
     } else {
         uint8_t* hs = c->pin_stream.get<uint8_t>(words_total * 4 + 1);
         for (uint32_t f = 0; f < nf; f++) {
-            uint8_t* dst = hs + ups[f].woff * 4;
-            HIPCHECK(hipMemcpyAsync(dst, dwords + ups[f].woff, lays[f].total_bytes, hipMemcpyDeviceToHost, st));
+            uint8_t* dst = hs + woff[f] * 4;
+            HIPCHECK(hipMemcpyAsync(dst, dwords + woff[f], lays[f].total_bytes, hipMemcpyDeviceToHost, st));
             c->fouts[f] = {dst, lays[f].total_bytes};
         }
     }

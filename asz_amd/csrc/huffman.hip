@@ -130,12 +130,11 @@ __device__ __forceinline__ uint64_t book_code(uint64_t e) { return e & ((1ull <<
 constexpr int kEncPerThread = kEncChunk / 256;  // 16
 constexpr int kEncPad = kEncChunk + kEncChunk / kEncPerThread;  // staging with one pad word per 16
 constexpr int kEncWords = kEncChunk + 2;        // LDS bit buffer for codes up to 32 bits
-constexpr uint32_t kEncMaxArrays = ASZ_MAX_LAYERS + 2;
 
 // Persistent encoder: workgroup g codes chunks g, g + G, g + 2G, ... of
-// kEncChunk symbols (G = gridDim.x), with the array table in LDS and the
-// codebook of the current array in LDS when it is small.  The symbols of the
-// next chunk are loaded (coalesced: lane l of step j reads symbol 256 j + l)
+// kEncChunk symbols (G = gridDim.x), with the codebook of the current array
+// in LDS when it is small.  The symbols of the next chunk are loaded
+// (coalesced: lane l of step j reads symbol 256 j + l)
 // while the current one is coded.  Per chunk: codebook entries are staged in
 // LDS so that thread t owns the 16 consecutive symbols 16 t .. 16 t + 15
 // (padded rows, no bank conflicts); after a workgroup scan of the per-thread
@@ -154,23 +153,17 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
                                                           uint32_t* __restrict__ chunk_bits) {
     __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
     __shared__ uint32_t lbook[kLdsBook];
-    __shared__ HuffArrayDev sA[kEncMaxArrays];
     __shared__ uint32_t s_scan[4];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    {
-        const uint32_t nw = narr * (uint32_t)(sizeof(HuffArrayDev) / 4);
-        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x)
-            reinterpret_cast<uint32_t*>(sA)[i] = reinterpret_cast<const uint32_t*>(arrays)[i];
-    }
-    __syncthreads();
     const uint64_t G = gridDim.x;
     // chunks of one workgroup increase, so its array index only moves forward
+    // (the table stays in global memory: one call may code many fields' arrays)
     auto array_of = [&](uint64_t c, uint32_t k) {
-        while (k + 1 < narr && c >= sA[k].chunk0 + sA[k].nchunk) k++;
+        while (k + 1 < narr && c >= arrays[k].chunk0 + arrays[k].nchunk) k++;
         return k;
     };
     auto load_chunk = [&](uint64_t c, uint32_t k, int32_t v[kEncPerThread]) {
-        const HuffArrayDev& A = sA[k];
+        const HuffArrayDev A = arrays[k];
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
         const int32_t* s = sym + A.src_off + first;
@@ -187,7 +180,7 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
     if (short_codes) load_chunk(c, k, v);
     int book_k = -1;
     while (true) {
-        const HuffArrayDev& A = sA[k];
+        const HuffArrayDev A = arrays[k];
         const uint64_t* b = book + A.book_off;
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
@@ -431,7 +424,6 @@ void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32
                           const uint64_t* book, uint32_t* scratch, uint64_t slot_words, uint32_t* chunk_bits,
                           int max_len, hipStream_t st) {
     if (!nchunks) return;
-    if (narr > dev::kEncMaxArrays) throw Error(ASZ_ERR_ARG, "too many Huffman arrays");
     // one resident wave of workgroups: CUs x workgroups that fit on a CU
     int dev_id = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev_id);
