@@ -209,11 +209,14 @@ static void range_scan(asz_ctx* c, const float* d_field, uint64_t n, float* mn, 
 // ------------------------------------------------------------------ default slope
 
 // Default slopes of nf fields of shape dims: every field's AB-key samples in
-// one buffer, one copy back and one synchronisation, then the fields' host
-// scoring in parallel on the context's pool.
+// one buffer (one launch per kSlopeMaxFields fields), one copy back and one
+// synchronisation, then the fields' host scoring in parallel on the
+// context's pool.  gpu_hold, when given, is released once the GPU part is
+// done (the batch's workers take turns at it).
 static void default_slope_fields(asz_ctx* c, uint32_t nf, const float* const* d_fields, const uint64_t dims[3],
-                                 const asz_params& p, const float* mn, const float* mx, SlopeResult* out) {
-    SlopeSampleDesc SD[3];
+                                 const asz_params& p, const float* mn, const float* mx, SlopeResult* out,
+                                 std::unique_lock<std::mutex>* gpu_hold = nullptr) {
+    SlopeFieldsDesc B{};
     uint64_t off[4] = {0, 0, 0, 0};  // key offsets of the three axes in one field's slice
     for (int d = 0; d < 3; d++) {
         SlopeSampleDesc S{};
@@ -235,22 +238,25 @@ static void default_slope_fields(asz_ctx* c, uint32_t nf, const float* const* d_
             S.cnt[a] = (uint32_t)((hi[a] - lo[a] + S.step[a] - 1) / S.step[a]);
         }
         S.dist = dist;
-        SD[d] = S;
+        B.S[d] = S;
+        B.off[d] = off[d];
         off[d + 1] = off[d] + 2 * (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
     }
     const uint64_t per = off[3];
+    B.per = per;
     int64_t* dk = c->keys.get<int64_t>(per * nf + 2);
-    for (uint32_t f = 0; f < nf; f++) {
-        const float base = abs_eb_of(p.rel_eb, mn[f], mx[f]) * p.stat_eb_ratio;
-        for (int d = 0; d < 3; d++) {
-            SlopeSampleDesc S = SD[d];
-            S.neg_2_stat_eb = 1.0f / base;
-            launch_slope_sample(S, d_fields[f], dk + f * per + off[d], c->stream);
+    for (uint32_t f0 = 0; f0 < nf; f0 += kSlopeMaxFields) {
+        B.nf = std::min<uint32_t>(kSlopeMaxFields, nf - f0);
+        for (uint32_t j = 0; j < B.nf; j++) {
+            B.f[j] = d_fields[f0 + j];
+            B.scale[j] = 1.0f / (abs_eb_of(p.rel_eb, mn[f0 + j], mx[f0 + j]) * p.stat_eb_ratio);
         }
+        launch_slope_sample_fields(B, dk + f0 * per, c->stream);
     }
     int64_t* hk = c->pin_xfer.get<int64_t>(per * nf + 2);
     HIPCHECK(hipMemcpyAsync(hk, dk, per * nf * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    if (gpu_hold) gpu_hold->unlock();
     host_pool(c).run(nf, [&](size_t f) {
         std::vector<std::vector<int64_t>> keys(3);
         for (int d = 0; d < 3; d++) keys[d].assign(hk + f * per + off[d], hk + f * per + off[d + 1]);
@@ -1488,7 +1494,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         HIPCHECK(hipEventRecord(c->order_ev, c->stream));
         for (uint32_t t = 0; t < W; t++) HIPCHECK(hipStreamWaitEvent(c->kids[t]->stream, c->order_ev, 0));
         std::atomic<uint32_t> next{0};
-        std::mutex err_mu;
+        std::mutex err_mu, pre_mu;
         std::string first_err;
         int first_code = ASZ_OK;
         const uint32_t nchunks = (uint32_t)chunks.size();
@@ -1506,9 +1512,13 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
                     std::vector<uint64_t> ns(nf, n);
                     std::vector<float> mn(nf), mx(nf);
                     for (uint32_t j = 0; j < nf; j++) ptrs[j] = d_fields[ch[j]];
+                    // the workers take turns at the range scan and the slope
+                    // samples, so the first chunk's rounds start after its own
+                    // scan instead of after every worker's
+                    std::unique_lock<std::mutex> hold(pre_mu);
                     asz::range_scan_fields(k, nf, ptrs.data(), ns.data(), mn.data(), mx.data(), ch.data());
                     std::vector<asz::SlopeResult> sr(nf);
-                    asz::default_slope_fields(k, nf, ptrs.data(), d, *p, mn.data(), mx.data(), sr.data());
+                    asz::default_slope_fields(k, nf, ptrs.data(), d, *p, mn.data(), mx.data(), sr.data(), &hold);
                     std::vector<asz::FieldIn> fin(nf);
                     for (uint32_t j = 0; j < nf; j++) fin[j] = asz::FieldIn{d_fields[ch[j]], mn[j], mx[j], sr[j].slope};
                     k->stream_on_device = on_device;

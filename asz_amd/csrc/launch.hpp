@@ -38,6 +38,17 @@ struct SlopeSampleDesc {
     float neg_2_stat_eb;
 };
 
+// The default slope's samples of several fields of one shape in one launch.
+constexpr uint32_t kSlopeMaxFields = 16;
+struct SlopeFieldsDesc {
+    SlopeSampleDesc S[3];           // one per axis (neg_2_stat_eb unused)
+    const float* f[kSlopeMaxFields];
+    float scale[kSlopeMaxFields];   // 1 / (abs_eb stat_eb_ratio) per field
+    uint64_t off[3];                // key offsets of the axes in one field's slice
+    uint64_t per;                   // keys per field
+    uint32_t nf;
+};
+
 // Results: kRangeWords words at d_out (preset: min key ~0, max key 0, flag 0,
 // first zero index ~0, last 0).  d_xs (device array of nfields pointers,
 // every field n elements): one launch for all of them, field f's words at
@@ -93,6 +104,7 @@ void launch_tsel(const TselDesc& T, const float* work, const uint32_t* ss, float
                  uint64_t nblocks, bool grad, hipStream_t st, const FieldSet& fs = FieldSet{},
                  uint32_t nfields = 1);
 void launch_slope_sample(const SlopeSampleDesc& S, const float* field, int64_t* keys, hipStream_t st);
+void launch_slope_sample_fields(const SlopeFieldsDesc& B, int64_t* keys, hipStream_t st);
 // launch_tsel for round T.r together with launch_fold_rounds for rounds
 // r0 .. r0 + nr - 1 (finished before T.r), in one launch.
 struct FoldDesc;

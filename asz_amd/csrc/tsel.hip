@@ -10,6 +10,8 @@
 // here the sums run over samples in double (the same terms, count-weighted),
 // so H1 and PSNR agree with the reference to float rounding and the chosen t
 // agrees except where two candidates score within that rounding of each other.
+#include <algorithm>
+
 #include "device.hpp"
 #include "fold.hpp"
 #include "launch.hpp"
@@ -366,12 +368,9 @@ __global__ __launch_bounds__(256) void k_tsel_keys(TselDesc T, const float* __re
     }
 }
 
-// Keys (A_key, B_key) of every sample, in the reference's loop order (i0, i1, i2).
-__global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const float* __restrict__ f,
-                                                      long long* keys) {
-    const uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= ns) return;
+// Keys (A_key, B_key) of sample idx, in the reference's loop order (i0, i1, i2).
+__device__ __forceinline__ void slope_key(const SlopeSampleDesc& S, float scale, const float* __restrict__ f,
+                                          uint64_t idx, long long* keys) {
     uint32_t q2 = idx % S.cnt[2];
     uint64_t q01 = idx / S.cnt[2];
     uint32_t q1 = q01 % S.cnt[1];
@@ -385,8 +384,27 @@ __global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const f
     float A = (a + b) - (c + e);
     float Ssum = (b + e) + (a + c);
     float B = (float)__fma_rn((double)Ssum, 0.25, -(double)x0);
-    keys[2 * idx] = lround_f64(S.neg_2_stat_eb * A);
-    keys[2 * idx + 1] = lround_f64(S.neg_2_stat_eb * B);
+    keys[2 * idx] = lround_f64(scale * A);
+    keys[2 * idx + 1] = lround_f64(scale * B);
+}
+
+__global__ __launch_bounds__(256) void k_slope_sample(SlopeSampleDesc S, const float* __restrict__ f,
+                                                      long long* keys) {
+    const uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ns) return;
+    slope_key(S, S.neg_2_stat_eb, f, idx, keys);
+}
+
+// The three axes' samples of up to kSlopeMaxFields fields of one shape: grid
+// y = axis, z = field; field f's keys of axis d at keys + f per + off[d].
+__global__ __launch_bounds__(256) void k_slope_sample_fields(SlopeFieldsDesc B, long long* keys) {
+    const uint32_t d = blockIdx.y, f = blockIdx.z;
+    const SlopeSampleDesc& S = B.S[d];
+    const uint64_t ns = (uint64_t)S.cnt[0] * S.cnt[1] * S.cnt[2];
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ns) return;
+    slope_key(S, B.scale[f], B.f[f], idx, keys + f * B.per + B.off[d]);
 }
 
 }  // namespace dev
@@ -422,6 +440,15 @@ void launch_fold_tsel(const TselDesc& T, const float* work, const uint32_t* ss, 
 void launch_tsel_keys(const TselDesc& T, const float* work, const uint32_t* ss, const uint64_t* key_off,
                       int64_t* keys, uint64_t nblocks, hipStream_t st) {
     hipLaunchKernelGGL(dev::k_tsel_keys, dim3((uint32_t)nblocks), dim3(256), 0, st, T, work, ss, key_off,
+                       (long long*)keys);
+}
+
+void launch_slope_sample_fields(const SlopeFieldsDesc& B, int64_t* keys, hipStream_t st) {
+    uint64_t nx = 0;
+    for (int d = 0; d < 3; d++) nx = std::max<uint64_t>(nx, (uint64_t)B.S[d].cnt[0] * B.S[d].cnt[1] * B.S[d].cnt[2]);
+    if (!nx || !B.nf) return;
+    if (B.nf > kSlopeMaxFields) throw Error(ASZ_ERR_ARG, "too many fields for one slope-sample launch");
+    hipLaunchKernelGGL(dev::k_slope_sample_fields, dim3((uint32_t)((nx + 255) / 256), 3, B.nf), dim3(256), 0, st, B,
                        (long long*)keys);
 }
 
