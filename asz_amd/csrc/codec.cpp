@@ -356,16 +356,27 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // the plan tables do not depend on the bounds (any positive one will do)
     const float aeb = dev_range ? 1.0f : abs_eb_of(p.rel_eb, mn, mx);
 
-    // plan and its device tables, reused while dims, plan, block pivot ratio and
-    // sampling count repeat (the error bounds and slopes are set per call)
+    // plan and its device tables, kept per (dims, plan, block pivot ratio,
+    // sampling count) for the last few keys (the error bounds and slopes are
+    // set per call)
     PlanKey key;
     std::memset(&key, 0, sizeof(key));
     for (int a = 0; a < 3; a++) key.dims[a] = dims[a];
     key.plan = plan;
     key.bpr = p.block_pivot_ratio;
     key.h1 = p.h1_block_sampling_num;
-    PlanCache& pc = c->pc;
-    if (!pc.valid || std::memcmp(&key, &pc.key, sizeof(key)) != 0) {
+    PlanCache* pslot = nullptr;
+    for (PlanCache& e : c->pcs)
+        if (e.valid && std::memcmp(&key, &e.key, sizeof(key)) == 0) pslot = &e;
+    const bool hit = pslot != nullptr;
+    if (!hit) {
+        pslot = &c->pcs[0];
+        for (PlanCache& e : c->pcs)
+            if ((e.valid ? e.used : 0) < (pslot->valid ? pslot->used : 0)) pslot = &e;
+    }
+    PlanCache& pc = *pslot;
+    pc.used = ++c->pc_tick;
+    if (!hit) {
         pc.valid = false;
         Geometry g0 = make_geometry(dims, plan, p.block_pivot_ratio);
         pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, fin[0].default_slope,
@@ -378,7 +389,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         std::memcpy(hm + u.segs, P0.segs.data(), P0.segs.size() * sizeof(SweepSeg));
         make_fold_descs(P0.rounds, pc.folds, pc.fpart_total);
         std::memcpy(hm + u.folds, pc.folds.data(), pc.folds.size() * sizeof(FoldDesc));
-        uint8_t* dm = c->cmeta.get<uint8_t>(u.bytes);
+        uint8_t* dm = pc.meta.get<uint8_t>(u.bytes);
         HIPCHECK(hipMemcpyAsync(dm, hm, u.bytes, hipMemcpyHostToDevice, st));
         pc.up = u;
         pc.key = key;
@@ -404,7 +415,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     }
     const Plan& P = pc.P;
     const Geometry& g = P.g;
-    uint8_t* dm = static_cast<uint8_t*>(c->cmeta.p);
+    uint8_t* dm = static_cast<uint8_t*>(pc.meta.p);
     const uint32_t* tabs = reinterpret_cast<const uint32_t*>(dm + pc.up.tabs);
     const uint32_t* ss = reinterpret_cast<const uint32_t*>(dm + pc.up.ss);
     const SweepSeg* segs = reinterpret_cast<const SweepSeg*>(dm + pc.up.segs);

@@ -77,7 +77,9 @@ struct PlanKey {
 
 struct PlanCache {
     bool valid = false;
+    uint64_t used = 0;             // context tick of the last call that used it
     PlanKey key;
+    DevBuf meta;                   // the device tables (layout in up)
     Plan P;
     std::vector<FoldDesc> folds;   // per round (make_fold_descs)
     uint64_t fpart_total = 0;      // fold scratch Partials over all rounds
@@ -94,7 +96,7 @@ struct asz_ctx {
     std::string err;
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
-        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, cmeta, phist, ghist, xfer, hmeta, scratch,
+        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
         chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range;
     hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
@@ -105,7 +107,11 @@ struct asz_ctx {
     uint64_t hres_arrays = 0;
     uint32_t epoch = 0;
     std::vector<asz_ctx*> kids;  // batch workers: one context (and HIP stream) each
-    asz::PlanCache pc;
+    // plans of the last kPlanSlots (dims, plan, pivot ratio, sampling count)
+    // keys, least recently used replaced first
+    static constexpr int kPlanSlots = 4;
+    asz::PlanCache pcs[kPlanSlots];
+    uint64_t pc_tick = 0;
     bool stream_on_device = false;
     uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
     float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field

@@ -156,8 +156,12 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             HIPCHECK(hipStreamSynchronize(st));
             // the (h1, psnr) scores of every (axis, t) on the host workers
             ABMap maps[3];
+            FlatAB flats[3];
             for (int d = 0; d < 3; d++)
-                if (has[d]) build_ab_map(hk + koff[d], (koff[d + 1] - koff[d]) / 2, maps[d]);
+                if (has[d]) {
+                    build_ab_map(hk + koff[d], (koff[d + 1] - koff[d]) / 2, maps[d]);
+                    flatten_ab(maps[d], flats[d]);
+                }
             const size_t nt = cand_t.size() + 1;
             std::vector<std::pair<float, float>> sc[3];
             std::vector<std::pair<int, size_t>> tasks;
@@ -171,7 +175,7 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
                 const int d = tasks[i].first;
                 const size_t k = tasks[i].second;
                 const float t = k == 0 ? 0.25f : cand_t[k - 1];
-                h1_psnr(maps[d], t, layer_eb, stat_eb, mn, mx, sc[d][k].first, sc[d][k].second);
+                h1_psnr(flats[d], t, layer_eb, stat_eb, mn, mx, sc[d][k].first, sc[d][k].second);
             });
             for (int d = 0; d < 3; d++)
                 if (!sc[d].empty()) best_t[d] = pick_t(sc[d], slope, cand_t);

@@ -16,7 +16,13 @@ struct KeyHash {
 using ABMap = std::unordered_map<std::tuple<int64_t, int64_t>, uint64_t, KeyHash>;
 
 void build_ab_map(const int64_t* keys, uint64_t n, ABMap& m);
-void h1_psnr(const ABMap& m, float t, float abs_eb, float stat_eb, float min_val, float max_val, float& h1,
+// A map's entries in its iteration order, keys as the scoring loop converts them.
+struct FlatAB {
+    std::vector<float> a, b;
+    std::vector<uint64_t> n;
+};
+void flatten_ab(const ABMap& m, FlatAB& f);
+void h1_psnr(const FlatAB& f, float t, float abs_eb, float stat_eb, float min_val, float max_val, float& h1,
              float& psnr);
 float select_t_exact(const int64_t* keys, uint64_t n, float layer_eb, float stat_eb, float layer_slope, float min_val,
                      float max_val);
