@@ -565,16 +565,21 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
 // (c0, c1, c2) the "012" index of (j0, j1, j2) is (j0 c1 + j1) c2 + j2, so
 // order "120" is the transpose [c0][c1 c2] -> [c1 c2][c0] and order "021" the
 // transpose [c1][c2] -> [c2][c1] of each j0 slab: a = p Q + q  <->  b = q P + p
-// per slab.  A workgroup moves one kXT x kXT tile of one slab through LDS;
-// each side is accessed in its own row order, so whole rows (or, when the tile
-// spans the full row, the whole tile) are contiguous on both sides.
+// per slab.  A workgroup moves one tile of kXT^2 elements of one slab through
+// LDS: TP rows along p (the next power of two of the round's longest p, at
+// most kXT) by kXT^2 / TP along q, so a block with short runs is still moved
+// by few workgroups.  Each side is accessed in its own row order, so whole
+// rows (or, when the tile spans the full row, the whole tile) are contiguous
+// on both sides.
 constexpr uint32_t kXT = 64;
 
 template <bool TO_ORDER>
 __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* __restrict__ tabs, int32_t* staged,
                                                   int32_t* ordered, uint64_t staged_stride, uint64_t ordered_stride,
-                                                  uint32_t tiles) {
-    __shared__ int32_t tile[kXT][kXT + 1];
+                                                  uint32_t tiles, uint32_t tp_shift) {
+    // a tile of TP x TQ = kXT^2 elements, TP = 2^tp_shift <= kXT (rows: p)
+    __shared__ int32_t tile[kXT * (kXT + 1)];
+    const uint32_t TP = 1u << tp_shift, TQ = (kXT * kXT) >> tp_shift, ld = TQ + 1;
     const uint32_t f = blockIdx.y;
     const uint64_t bid = blockIdx.x / tiles;
     const uint32_t t = (uint32_t)(blockIdx.x - bid * tiles);
@@ -592,29 +597,40 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
     const uint32_t slabs = o120 ? 1 : cnt[0], P = o120 ? cnt[0] : cnt[1];
     const uint64_t Q = o120 ? (uint64_t)cnt[1] * cnt[2] : cnt[2];
     if (slabs == 0 || P == 0 || Q == 0) return;  // a block without targets
-    const uint32_t tq_n = (uint32_t)((Q + kXT - 1) / kXT), tp_n = (P + kXT - 1) / kXT;
+    const uint32_t tq_n = (uint32_t)((Q + TQ - 1) / TQ), tp_n = (P + TP - 1) >> tp_shift;
     const uint32_t tq = t % tq_n, rest = t / tq_n, tp = rest % tp_n, slab = rest / tp_n;
     if (slab >= slabs) return;
-    const uint32_t p0 = tp * kXT;
-    const uint64_t q0 = (uint64_t)tq * kXT;
-    const uint32_t np = min(kXT, P - p0), nq = (uint32_t)min<uint64_t>(kXT, Q - q0);
+    const uint32_t p0 = tp * TP;
+    const uint64_t q0 = (uint64_t)tq * TQ;
+    const uint32_t np = min(TP, P - p0), nq = (uint32_t)min<uint64_t>(TQ, Q - q0);
     const uint64_t slab_off = base + (uint64_t)slab * P * Q;
     int32_t* A = staged + f * staged_stride + slab_off;   // "012": p Q + q
     int32_t* B = ordered + f * ordered_stride + slab_off;  // order:  q P + p
     const uint32_t n = np * nq;
-    // e / d for e < kXT^2 and d <= kXT through a float reciprocal: (e + 1/2) / d
-    // lies at least 1/(2d) from an integer and the float error stays far
-    // below that, so the floor is exact
+    // e / d for e < kXT^2 and d <= kXT^2 through a float reciprocal: (e + 1/2) / d
+    // lies at least 1/(2d) from an integer and the float error (below
+    // 2^-22 (e / d)) stays far below that, so the floor is exact
     const float inv_nq = 1.0f / (float)nq, inv_np = 1.0f / (float)np;
     auto qdiv = [](uint32_t e, float inv) { return (uint32_t)(((float)e + 0.5f) * inv); };
     constexpr uint32_t kPer = kXT * kXT / 256;  // elements per thread when the tile is full
+    // every load of a thread is issued before the first LDS store waits for one
+    int32_t v[kPer];
     if (TO_ORDER) {
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
             const uint32_t e = threadIdx.x + 256 * k;
-            if (e >= n) break;
-            const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
-            tile[pl][ql] = A[(uint64_t)(p0 + pl) * Q + q0 + ql];
+            if (e < n) {
+                const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
+                v[k] = A[(uint64_t)(p0 + pl) * Q + q0 + ql];
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e < n) {
+                const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
+                tile[pl * ld + ql] = v[k];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -622,15 +638,24 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
             const uint32_t e = threadIdx.x + 256 * k;
             if (e >= n) break;
             const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
-            B[(q0 + ql) * P + p0 + pl] = tile[pl][ql];
+            B[(q0 + ql) * P + p0 + pl] = tile[pl * ld + ql];
         }
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
             const uint32_t e = threadIdx.x + 256 * k;
-            if (e >= n) break;
-            const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
-            tile[pl][ql] = B[(q0 + ql) * P + p0 + pl];
+            if (e < n) {
+                const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
+                v[k] = B[(q0 + ql) * P + p0 + pl];
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + 256 * k;
+            if (e < n) {
+                const uint32_t ql = qdiv(e, inv_np), pl = e - ql * np;
+                tile[pl * ld + ql] = v[k];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -638,7 +663,7 @@ __global__ __launch_bounds__(256) void k_qb_order(RoundDesc R, const uint32_t* _
             const uint32_t e = threadIdx.x + 256 * k;
             if (e >= n) break;
             const uint32_t pl = qdiv(e, inv_nq), ql = e - pl * nq;
-            A[(uint64_t)(p0 + pl) * Q + q0 + ql] = tile[pl][ql];
+            A[(uint64_t)(p0 + pl) * Q + q0 + ql] = tile[pl * ld + ql];
         }
     }
 }
@@ -751,11 +776,22 @@ bool round_needs_staging(const RoundDesc& R) {
     return fastest != R.d;  // the sweep's scattered write mode
 }
 
+// Tile rows (along p, the order's run axis) for a round: the next power of two
+// of its largest run count, at most kXT; the tile holds kXT^2 elements.
+static uint32_t qb_order_tp_shift(const RoundDesc& R, const uint32_t cmax[3]) {
+    const uint64_t P = R.order == 0 ? cmax[0] : cmax[1];
+    uint32_t sh = 0;
+    while ((1ull << sh) < P && (1u << sh) < dev::kXT) sh++;
+    return sh;
+}
+
 uint32_t qb_order_tiles(const RoundDesc& R, const uint32_t cmax[3]) {
     const bool o120 = R.order == 0;
     const uint64_t slabs = o120 ? 1 : cmax[0], P = o120 ? cmax[0] : cmax[1];
     const uint64_t Q = o120 ? (uint64_t)cmax[1] * cmax[2] : cmax[2];
-    return (uint32_t)(slabs * ((P + dev::kXT - 1) / dev::kXT) * ((Q + dev::kXT - 1) / dev::kXT));
+    const uint32_t sh = qb_order_tp_shift(R, cmax);
+    const uint64_t TP = 1ull << sh, TQ = ((uint64_t)dev::kXT * dev::kXT) >> sh;
+    return (uint32_t)(slabs * ((P + TP - 1) / TP) * ((Q + TQ - 1) / TQ));
 }
 
 void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cmax[3], int32_t* staged,
@@ -763,15 +799,16 @@ void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cm
                      bool to_order, hipStream_t st) {
     if (R.count == 0) return;
     const uint32_t tiles = qb_order_tiles(R, cmax);
+    const uint32_t sh = qb_order_tp_shift(R, cmax);
     const uint64_t nbl = (uint64_t)R.nb[0] * R.nb[1] * R.nb[2];
     if (nbl * tiles > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "field dimensions exceed the launch grid");
     const dim3 grid((uint32_t)(nbl * tiles), nfields);
     if (to_order)
         hipLaunchKernelGGL(dev::k_qb_order<true>, grid, dim3(256), 0, st, R, tabs, staged, ordered, staged_stride,
-                           ordered_stride, tiles);
+                           ordered_stride, tiles, sh);
     else
         hipLaunchKernelGGL(dev::k_qb_order<false>, grid, dim3(256), 0, st, R, tabs, staged, ordered, staged_stride,
-                           ordered_stride, tiles);
+                           ordered_stride, tiles, sh);
 }
 
 void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const int32_t* qb, const uint32_t* tabs,
