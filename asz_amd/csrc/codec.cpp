@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <immintrin.h>
@@ -1458,7 +1459,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         // fields) are compressed together in chunks: every per-round kernel of
         // a chunk covers all of its fields.  Chunks hold about a T-th of their
         // group, at most kMaxChunk (or ASZ_OPT_BATCH_FIELDS) fields and
-        // kMaxChunkPoints points; the largest go first.
+        // kMaxChunkPoints points.
         const uint32_t kMaxChunk = c->batch_fields ? c->batch_fields : 16;
         constexpr uint64_t kMaxChunkPoints = 1ull << 30;
         const bool can_group = tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD;
@@ -1491,6 +1492,17 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         };
         std::stable_sort(chunks.begin(), chunks.end(),
                          [&](const auto& x, const auto& y) { return points(x) > points(y); });
+        // largest, smallest, second largest, ...: the small chunks' latency-bound
+        // rounds overlap the large chunks' early on, and the batch ends on large
+        // chunks, which keep the GPU busy on fewer streams
+        {
+            std::vector<std::vector<uint32_t>> mixed;
+            for (size_t lo = 0, hi = chunks.size(); lo < hi;) {
+                mixed.push_back(std::move(chunks[lo++]));
+                if (lo < hi) mixed.push_back(std::move(chunks[--hi]));
+            }
+            chunks.swap(mixed);
+        }
         const uint32_t W = std::min<uint32_t>(T, (uint32_t)chunks.size());
         while (c->kids.size() < W) {
             asz_ctx* k = nullptr;
