@@ -350,45 +350,58 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
     }
 }
 
-// One wave per chunk (four chunks per workgroup): the chunk's bits from its
-// scratch slot to bit position chunk_off in the stream.  Each pass a lane
-// loads 4 consecutive source words (all loads of the pass in flight at
-// once); the word below a lane's first comes from the neighbouring lane.
-// Interior output words are stored, the two boundary words shared with the
-// neighbouring chunks are ORed in.
+// One wave per kCompactPerWave consecutive chunks (their bit counts and
+// offsets loaded together first): each chunk's bits from its scratch slot to
+// bit position chunk_off in the stream.  Each pass a lane loads 4 consecutive
+// source words (all loads of the pass in flight at once); the word below a
+// lane's first comes from the neighbouring lane.  Interior output words are
+// stored, the two boundary words shared with the neighbouring chunks are ORed
+// in.
+constexpr uint32_t kCompactPerWave = 4;
 __global__ __launch_bounds__(256) void k_compact_bits(const uint32_t* __restrict__ scratch, uint64_t slot_words,
                                                       const uint32_t* __restrict__ chunk_bits,
                                                       const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
                                                       uint32_t* words) {
     constexpr uint32_t kW = 4;  // words per lane per pass
-    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t g0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kCompactPerWave;
     const uint32_t lane = threadIdx.x & 63;
-    if (g >= nchunks) return;
-    const uint32_t nb = chunk_bits[g];
-    if (nb == 0) return;
-    const uint64_t o = chunk_off[g];
-    const uint32_t sh = (uint32_t)(o & 31);
-    const uint32_t nsrc = (nb + 31) / 32, nout = (sh + nb + 31) / 32;
-    const uint32_t* src = scratch + g * slot_words;
-    uint32_t* dst = words + (o >> 5);
-    uint32_t carry = 0;  // source word below this pass's first word
-    for (uint32_t base = 0; base < nout; base += 64 * kW) {
-        const uint32_t i0 = base + lane * kW;
-        uint32_t cur[kW];
+    if (g0 >= nchunks) return;
+    uint32_t nbs[kCompactPerWave];
+    uint64_t offs[kCompactPerWave];
 #pragma unroll
-        for (uint32_t q = 0; q < kW; q++) cur[q] = i0 + q < nsrc ? src[i0 + q] : 0u;
-        uint32_t below = __shfl_up(cur[kW - 1], 1, 64);
-        if (lane == 0) below = carry;
-        carry = __shfl(cur[kW - 1], 63, 64);
+    for (uint32_t j = 0; j < kCompactPerWave; j++) {
+        const bool in = g0 + j < nchunks;
+        nbs[j] = in ? chunk_bits[g0 + j] : 0u;
+        offs[j] = in ? chunk_off[g0 + j] : 0ull;
+    }
+    for (uint32_t j = 0; j < kCompactPerWave; j++) {
+        const uint32_t nb = nbs[j];
+        if (nb == 0) continue;
+        const uint64_t g = g0 + j;
+        const uint64_t o = offs[j];
+        const uint32_t sh = (uint32_t)(o & 31);
+        const uint32_t nsrc = (nb + 31) / 32, nout = (sh + nb + 31) / 32;
+        const uint32_t* src = scratch + g * slot_words;
+        uint32_t* dst = words + (o >> 5);
+        uint32_t carry = 0;  // source word below this pass's first word
+        for (uint32_t base = 0; base < nout; base += 64 * kW) {
+            const uint32_t i0 = base + lane * kW;
+            uint32_t cur[kW];
 #pragma unroll
-        for (uint32_t q = 0; q < kW; q++) {
-            const uint32_t i = i0 + q;
-            if (i >= nout) break;
-            const uint32_t prev = q ? cur[q - 1] : below;
-            uint32_t v = cur[q] << sh;
-            if (sh && i > 0) v |= prev >> (32 - sh);
-            if (i == 0 || i == nout - 1) atomicOr(&dst[i], v);
-            else dst[i] = v;
+            for (uint32_t q = 0; q < kW; q++) cur[q] = i0 + q < nsrc ? src[i0 + q] : 0u;
+            uint32_t below = __shfl_up(cur[kW - 1], 1, 64);
+            if (lane == 0) below = carry;
+            carry = __shfl(cur[kW - 1], 63, 64);
+#pragma unroll
+            for (uint32_t q = 0; q < kW; q++) {
+                const uint32_t i = i0 + q;
+                if (i >= nout) break;
+                const uint32_t prev = q ? cur[q - 1] : below;
+                uint32_t v = cur[q] << sh;
+                if (sh && i > 0) v |= prev >> (32 - sh);
+                if (i == 0 || i == nout - 1) atomicOr(&dst[i], v);
+                else dst[i] = v;
+            }
         }
     }
 }
@@ -448,7 +461,8 @@ void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint3
 void launch_compact_bits(const uint32_t* scratch, uint64_t slot_words, const uint32_t* chunk_bits,
                          const uint64_t* chunk_off, uint64_t nchunks, uint32_t* stream_words, hipStream_t st) {
     if (!nchunks) return;
-    hipLaunchKernelGGL(dev::k_compact_bits, dim3((uint32_t)((nchunks + 3) / 4)), dim3(256), 0, st, scratch,
+    const uint64_t per_wg = 4ull * dev::kCompactPerWave;
+    hipLaunchKernelGGL(dev::k_compact_bits, dim3((uint32_t)((nchunks + per_wg - 1) / per_wg)), dim3(256), 0, st, scratch,
                        slot_words, chunk_bits, chunk_off, nchunks, stream_words);
 }
 
