@@ -97,3 +97,26 @@ def test_one_call_reports_input_errors(ctx, kind, msg):
     # the context stays usable
     g = torch.from_numpy(_field("grf")).cuda()
     ctx.compress_field(g.data_ptr(), tuple(g.shape), codec.make_plan(PLAN), p)
+
+
+def test_plan_cache_across_more_shapes_than_slots(ctx):
+    # six shapes twice through one context (a context keeps four plans, least
+    # recently used replaced first): every stream equals a fresh context's
+    shapes = [(24, 40, 48), (40, 24, 48), (48, 40, 24), (30, 30, 30), (25, 33, 65), (64, 26, 36)]
+    p = codec.make_params(rel_eb=1e-3, block_pivot_ratio=3)
+    pl = codec.make_plan(PLAN[:5])
+    fresh = {}
+    for rep in range(2):
+        for shp in shapes:
+            print("plan cache case", rep, shp, flush=True)
+            t = torch.from_numpy(_field("grf", shp)).cuda()
+            w = t.clone()
+            got = ctx.compress_field(w.data_ptr(), shp, pl, p).stream
+            if shp not in fresh:
+                w2 = t.clone()
+                c2 = codec.Context(0)
+                try:
+                    fresh[shp] = c2.compress_field(w2.data_ptr(), shp, pl, p).stream
+                finally:
+                    c2.close()
+            assert got == fresh[shp], f"shape {shp}, pass {rep}"
