@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("round_ms", ctypes.c_double * MAX_LAYERS), ("hist_ms", ctypes.c_double),
                 ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double),
                 ("codebook_wait_ms", ctypes.c_double), ("codebook_max_ms", ctypes.c_double),
-                ("layout_ms", ctypes.c_double)]
+                ("layout_ms", ctypes.c_double), ("plan_cache_hit", ctypes.c_uint64)]
 
 class BatchField(ctypes.Structure):
     _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("default_slope", ctypes.c_float),
@@ -90,7 +90,8 @@ ASZ_OPT_BATCH_FIELDS = 4
 EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_default", "asz_get_stats",
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
-           "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_decompress", "asz_free",
+           "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_compress_mt", "asz_zstd_decompress",
+           "asz_free",
            "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_compress_field", "asz_prepare_round", "asz_round_energy",
            "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
            "asz_select_family")
@@ -136,6 +137,8 @@ def lib():
     L.asz_huffman_codebook.argtypes = [_P, _U64, ctypes.c_int64, _U64, ctypes.c_int, _P, _U64, _U64P, _P, _P,
                                        _U64P]
     L.asz_zstd_compress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
+    L.asz_zstd_compress_mt.argtypes = [_P, _U64, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(_P), _U64P,
+                                       ctypes.POINTER(ctypes.c_uint32)]
     L.asz_zstd_decompress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]
     L.asz_prepare_round.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_float,
                                     ctypes.c_float, ctypes.c_int, _P]

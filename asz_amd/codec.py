@@ -374,11 +374,19 @@ def huffman_codebook(hist, offset: int, count: int, is_int64=False):
     return hdr[:hn.value].tobytes(), codes, lens, bits.value
 
 
-def zstd_compress(payload: bytes) -> bytes:
+def zstd_compress(payload: bytes, level: int = 3, threads: int = 1) -> bytes:
+    """One Zstd frame around the pre-Zstd stream (compress.tpp:294-299: level 3,
+    one thread).  threads > 1 compresses block-aligned slices concurrently and
+    splices them into one frame (asz_zstd_compress_mt); 0 picks the thread count."""
     L = _lib.lib()
     buf = np.frombuffer(payload, np.uint8)
     out, n = ctypes.c_void_p(), ctypes.c_uint64()
-    if L.asz_zstd_compress(_ptr(buf), buf.size, ctypes.byref(out), ctypes.byref(n)) != 0:
+    if threads == 1 and level == 3:
+        rc = L.asz_zstd_compress(_ptr(buf), buf.size, ctypes.byref(out), ctypes.byref(n))
+    else:
+        rc = L.asz_zstd_compress_mt(_ptr(buf), buf.size, int(level), int(threads), ctypes.byref(out),
+                                    ctypes.byref(n), None)
+    if rc != 0:
         raise AszError(-2, "ZSTD_compress failed")
     try:
         return ctypes.string_at(out, n.value)

@@ -26,8 +26,10 @@ struct CliState {
     float pf_sample_fraction_limit = 0.02f;
     uint64_t pf_coarse_bpr = 3, pf_fine_bpr = 4;
     float pf_fine_eb_threshold = 1e-4f;
-    int tsel = ASZ_TSEL_GPU;
+    int tsel = ASZ_TSEL_EXACT;  // the reference's t, block for block (compress.tpp:90-144)
     int device = 0;
+    int zstd_level = 3;         // compress.tpp:297
+    uint32_t zstd_threads = 1;
 };
 
 // s1_compress.cpp:5-28

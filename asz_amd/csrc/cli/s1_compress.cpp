@@ -1,9 +1,13 @@
 // Drop-in s1_compress (reference: c_codes/s1_compress.cpp).  Same command
 // line, validation, stdout lines and .asz stream; the core runs on the GPU
 // through libasz_gpu.so.  Additional flags (not in the reference):
-//   --tsel gpu|exact|grad  per-block t from the GPU scorer (default), from GPU
-//                      samples scored on the host in the reference's order, or
-//                      from the gradient-tuned form
+//   --tsel exact|gpu|grad  per-block t from GPU samples scored on the host in
+//                      the reference's order (default: the reference's t and
+//                      stream), from the GPU scorer (faster; t may differ from
+//                      the reference's where two candidates score within float
+//                      rounding of each other), or from the gradient-tuned form
+//   --zstd-level L     Zstd level of the .asz frame (default 3, the reference's)
+//   --zstd-threads T   host threads for the Zstd frame (default 1; 0: automatic)
 //   --device N         HIP device
 #include <chrono>
 #include <cstring>
@@ -115,6 +119,8 @@ int main(int argc, char* argv[]) try {
             else throw invalid_argument("tsel must be gpu, exact or grad");
         }
         else if (a == "--device") s.device = stoi(next());
+        else if (a == "--zstd-level") s.zstd_level = stoi(next());
+        else if (a == "--zstd-threads") s.zstd_threads = (uint32_t)stoul(next());
         else throw invalid_argument("unknown argument: " + a);
     }
     validate(s);
@@ -246,7 +252,8 @@ int main(int argc, char* argv[]) try {
     uint8_t* zs = nullptr;
     uint64_t zn = 0;
     auto z0 = Clock::now();
-    if (asz_zstd_compress(hs, hn, &zs, &zn) != ASZ_OK) throw runtime_error("ZSTD_compress failed");
+    if (asz_zstd_compress_mt(hs, hn, s.zstd_level, s.zstd_threads, &zs, &zn, nullptr) != ASZ_OK)
+        throw runtime_error("ZSTD_compress failed");
     auto z1 = Clock::now();
     double core = secs(c0, z1);
     printf("Huffman compression time: %f s\n", (st.huffman_ms + st.huffman_host_ms) / 1e3);

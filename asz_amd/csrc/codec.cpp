@@ -363,7 +363,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     PlanKey key;
     std::memset(&key, 0, sizeof(key));
     for (int a = 0; a < 3; a++) key.dims[a] = dims[a];
-    key.plan = plan;
+    key.plan = canonical_plan(plan);
     key.bpr = p.block_pivot_ratio;
     key.h1 = p.h1_block_sampling_num;
     PlanCache* pslot = nullptr;
@@ -427,6 +427,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S = asz_stats{};
     S.layers = L;
     S.t_count = nt;
+    S.plan_cache_hit = hit ? 1 : 0;
 
     // Per-call buffers, one slice per field (multiples of 64 elements apart).
     // symbol buffer: t array (nt) then pivot + round QB arrays (N)
@@ -656,6 +657,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     } joiner{pool, abort_all};
 
     double tsel_host_ms = 0;
+    // ASZ_TSEL_EXACT: every round's block t (per round) and the block-major t
+    // array, alive until the call returns (their uploads are asynchronous)
+    std::vector<float> tf_all(tsel_mode == ASZ_TSEL_EXACT ? L * nbl : 0);
+    std::vector<int32_t> t_host(tsel_mode == ASZ_TSEL_EXACT ? nt : 0);
     std::vector<int64_t> hkeys;
     std::vector<uint64_t> koff;
     // With the GPU t selection, a round's fold runs in the same launch as the
@@ -710,16 +715,14 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                 float prod = t * (float)p.t_precision;
                 ti[b] = (int32_t)(prod + std::copysign(0.49999997f, prod));
             }
-            HIPCHECK(hipMemcpyAsync(tb, tf.data(), nbl * 4, hipMemcpyHostToDevice, st));
-            // scatter into the block-major t array
-            std::vector<int32_t> cur(nt);
-            if (r > 0) {
-                HIPCHECK(hipMemcpyAsync(cur.data(), sym, nt * 4, hipMemcpyDeviceToHost, st));
-                HIPCHECK(hipStreamSynchronize(st));
-            }
-            for (uint64_t b = 0; b < nbl; b++) cur[b * L + r] = ti[b];
-            HIPCHECK(hipMemcpyAsync(sym, cur.data(), nt * 4, hipMemcpyHostToDevice, st));
-            HIPCHECK(hipStreamSynchronize(st));
+            // this round's t for the round kernel; the block-major t array
+            // (compress.tpp:201) is kept on the host and goes up once, after
+            // the last round's selection (the only host wait of a round is the
+            // key download above)
+            std::memcpy(tf_all.data() + r * nbl, tf.data(), nbl * 4);
+            HIPCHECK(hipMemcpyAsync(tb, tf_all.data() + r * nbl, nbl * 4, hipMemcpyHostToDevice, st));
+            for (uint64_t b = 0; b < nbl; b++) t_host[b * L + r] = ti[b];
+            if (r + 1 == Lrun) HIPCHECK(hipMemcpyAsync(sym, t_host.data(), nt * 4, hipMemcpyHostToDevice, st));
             tsel_host_ms += ms_since(h0);
         }
         if (r + 1 == L) {
@@ -1469,7 +1472,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
             for (auto& gr : groups) {
                 const uint32_t j = gr[0];
                 if (can_group && std::memcmp(dims + 3 * (uint64_t)i, dims + 3 * (uint64_t)j, 24) == 0 &&
-                    std::memcmp(&plans[i], &plans[j], sizeof(asz_plan)) == 0) {
+                    asz::same_plan(plans[i], plans[j])) {
                     gr.push_back(i);
                     placed = true;
                     break;
@@ -1492,9 +1495,10 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         };
         std::stable_sort(chunks.begin(), chunks.end(),
                          [&](const auto& x, const auto& y) { return points(x) > points(y); });
-        // largest, smallest, second largest, ...: the small chunks' latency-bound
-        // rounds overlap the large chunks' early on, and the batch ends on large
-        // chunks, which keep the GPU busy on fewer streams
+        // largest, smallest, second largest, second smallest, ...: large and
+        // small chunks interleave, so the small chunks' latency-bound rounds
+        // overlap the large chunks' bandwidth-bound ones; the middle-sized
+        // chunks come last
         {
             std::vector<std::vector<uint32_t>> mixed;
             for (size_t lo = 0, hi = chunks.size(); lo < hi;) {

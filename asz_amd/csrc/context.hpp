@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -67,8 +68,27 @@ struct PinnedBuf {
     }
 };
 
+// A plan with every entry past its `layers` rounds (and the padding) zeroed,
+// so that two plans compare bytewise equal exactly when their rounds match
+// (a caller may leave the unused entries of its struct uninitialised).
+inline asz_plan canonical_plan(const asz_plan& p) {
+    asz_plan c;
+    std::memset(&c, 0, sizeof(c));
+    c.layers = p.layers;
+    const int n = std::max(0, std::min<int>(p.layers, ASZ_MAX_LAYERS));
+    for (int r = 0; r < n; r++) {
+        c.axis[r] = p.axis[r];
+        c.order[r] = p.order[r];
+    }
+    return c;
+}
+inline bool same_plan(const asz_plan& a, const asz_plan& b) {
+    const asz_plan x = canonical_plan(a), y = canonical_plan(b);
+    return std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+
 // Inputs that shape a compression plan and its device tables.  Compared
-// bytewise (the struct is zeroed before it is filled).
+// bytewise (the struct is zeroed before it is filled, the plan canonical).
 struct PlanKey {
     uint64_t dims[3];
     asz_plan plan;

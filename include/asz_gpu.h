@@ -88,6 +88,9 @@ typedef struct asz_stats {
     /* compress host timeline: from the last round launch to every codebook
      * built, the longest codebook build, then the stream layout and its upload */
     double codebook_wait_ms, codebook_max_ms, layout_ms;
+    /* 1 when the call found its plan tables among the context's cached plans
+     * (the last four (dims, plan, block pivot ratio, sampling count) keys) */
+    uint64_t plan_cache_hit;
 } asz_stats;
 
 /* Options (asz_set_option).  ASZ_OPT_KERNEL_EVENTS: record a HIP event pair on
@@ -338,6 +341,19 @@ int asz_huffman_codebook(const uint64_t *hist, uint64_t range, int64_t offset, u
 /* Compression / decompression of a whole .asz frame with host Zstd (level 3),
  * compress.tpp:294-299 / decompress.tpp:83-86.  Host only. */
 int asz_zstd_compress(const uint8_t *src, uint64_t n, uint8_t **dst, uint64_t *dst_bytes);
+/* The same frame at compression level `level` with `threads` host threads
+ * (0: the hardware thread count, at most 16).  No reference counterpart: it
+ * replaces the single ZSTD_compress call of compress.tpp:294-299 for callers
+ * that want it faster.  The stream is cut into slices of whole 128 KiB
+ * blocks; slice 0 is compressed with the whole stream's size pledged (so the
+ * frame header is ZSTD_compress's) and every later slice on its own, and
+ * their blocks are spliced into one frame when every block after slice 0 is
+ * raw or RLE (stateless; Huffman output is stored raw).  Otherwise the
+ * stream is compressed again in one call.  The frame carries the content
+ * size decompress.tpp:83-86 reads.  *threads_used (may be NULL) receives the
+ * slices used (1: one call).  Host only; release *dst with asz_free. */
+int asz_zstd_compress_mt(const uint8_t *src, uint64_t n, int level, uint32_t threads, uint8_t **dst,
+                         uint64_t *dst_bytes, uint32_t *threads_used);
 int asz_zstd_decompress(const uint8_t *src, uint64_t n, uint8_t **dst, uint64_t *dst_bytes);
 void asz_free(void *p);
 

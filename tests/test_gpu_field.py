@@ -99,19 +99,26 @@ def test_one_call_reports_input_errors(ctx, kind, msg):
     ctx.compress_field(g.data_ptr(), tuple(g.shape), codec.make_plan(PLAN), p)
 
 
-def test_plan_cache_across_more_shapes_than_slots(ctx):
-    # six shapes twice through one context (a context keeps four plans, least
-    # recently used replaced first): every stream equals a fresh context's
-    shapes = [(24, 40, 48), (40, 24, 48), (48, 40, 24), (30, 30, 30), (25, 33, 65), (64, 26, 36)]
+def test_plan_cache_hits_and_evictions():
+    # A context keeps its last four plans, least recently used replaced first.
+    # The order A B C D A E B A reuses a slot that is not the most recent one
+    # (the second A), evicts the least recently used plan (E replaces B) and
+    # then misses on the evicted one (B replaces C) before hitting A again.
+    # Every stream equals a fresh context's.
+    A, B, C, D, E = (24, 40, 48), (40, 24, 48), (48, 40, 24), (30, 30, 30), (25, 33, 65)
+    seq = [A, B, C, D, A, E, B, A]
+    hits = [0, 0, 0, 0, 1, 0, 0, 1]
     p = codec.make_params(rel_eb=1e-3, block_pivot_ratio=3)
     pl = codec.make_plan(PLAN[:5])
+    ctx = codec.Context(0)  # its own context: no earlier plan in its slots
     fresh = {}
-    for rep in range(2):
-        for shp in shapes:
-            print("plan cache case", rep, shp, flush=True)
+    try:
+        for i, shp in enumerate(seq):
+            print("plan cache case", i, shp, flush=True)
             t = torch.from_numpy(_field("grf", shp)).cuda()
             w = t.clone()
             got = ctx.compress_field(w.data_ptr(), shp, pl, p).stream
+            assert ctx.stats().plan_cache_hit == hits[i], f"call {i} shape {shp}"
             if shp not in fresh:
                 w2 = t.clone()
                 c2 = codec.Context(0)
@@ -119,4 +126,6 @@ def test_plan_cache_across_more_shapes_than_slots(ctx):
                     fresh[shp] = c2.compress_field(w2.data_ptr(), shp, pl, p).stream
                 finally:
                     c2.close()
-            assert got == fresh[shp], f"shape {shp}, pass {rep}"
+            assert got == fresh[shp], f"call {i} shape {shp}"
+    finally:
+        ctx.close()

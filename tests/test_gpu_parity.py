@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from asz_amd import codec, fields, zstd
+from asz_amd import codec, fields
 from oracle import api
 from tests import golden_util as gu
 
@@ -65,7 +65,7 @@ def _slope_on_device(ctx, data, params):
 def test_inject_t_stream_is_byte_identical(ctx, name):
     info, data, asz = gu.load(name)
     _say("case", name, data.shape)
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     st = api.parse_payload(payload)
     p = _params(info)
     res = ctx.compress(data, codec.make_plan(codec.plan_strings(st.axis, st.order)), p, tsel="inject",
@@ -80,7 +80,7 @@ def test_inject_t_stream_is_byte_identical(ctx, name):
 def test_gpu_decodes_reference_stream(ctx, name):
     info, data, asz = gu.load(name)
     _say("case", name, data.shape)
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     st = api.parse_payload(payload)
     out, qb, t = ctx.decompress(payload, data.shape, _params(info), want_qb=True, want_t=True, nt=st.t.size)
     assert np.array_equal(t, st.t)
@@ -92,7 +92,7 @@ def test_gpu_decodes_reference_stream(ctx, name):
 def test_default_slope_and_exact_t_selection(ctx, name):
     info, data, asz = gu.load(name)
     _say("case", name, data.shape)
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     st = api.parse_payload(payload)
     p = _params(info)
     (slope, inv, fb, reason), _ = _slope_on_device(ctx, data, p)
@@ -109,7 +109,7 @@ def test_gpu_t_selection_agreement(ctx):
     agree = total = 0
     for name in NAMES:
         info, data, asz = gu.load(name)
-        st = api.parse_payload(zstd.decompress(asz))
+        st = api.parse_payload(codec.zstd_decompress(asz))
         p = _params(info)
         (slope, *_), _ = _slope_on_device(ctx, data, p)
         p.default_slope = slope
@@ -176,6 +176,6 @@ def test_errors_are_reported(ctx):
     with pytest.raises(codec.AszError, match="nonfinite"):
         ctx.compress(f, codec.make_plan(["cubic0_012"]), p)
     info, data, asz = gu.load("small_9x13x17")
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     with pytest.raises(codec.AszError):
         ctx.decompress(payload[: len(payload) // 2], data.shape, _params(info))

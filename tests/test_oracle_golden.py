@@ -12,7 +12,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from asz_amd import zstd
+from asz_amd import codec
 from oracle import api
 from tests import golden_util as gu
 
@@ -30,7 +30,7 @@ def _params(info, slope=1000.0):
 @pytest.mark.parametrize("name", NAMES)
 def test_parse_consumes_whole_payload(name):
     info, data, asz = gu.load(name)
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     st = api.parse_payload(payload)
     assert st.nbytes == len(payload)
     assert int(st.counts.sum()) == data.size  # compress.tpp:230-231
@@ -43,7 +43,7 @@ def test_parse_consumes_whole_payload(name):
 @pytest.mark.parametrize("name", NAMES)
 def test_compress_with_stream_t_is_bit_exact(name):
     info, data, asz = gu.load(name)
-    payload = zstd.decompress(asz)
+    payload = codec.zstd_decompress(asz)
     st = api.parse_payload(payload)
     p = _params(info)
     t, counts, qb, recon, _ = api.compress(data, st.axis, st.order, st.quintic, p, t_inject=st.t,
@@ -58,7 +58,7 @@ def test_compress_with_stream_t_is_bit_exact(name):
 @pytest.mark.parametrize("name", NAMES)
 def test_decompress_is_bit_exact(name):
     info, data, asz = gu.load(name)
-    st = api.parse_payload(zstd.decompress(asz))
+    st = api.parse_payload(codec.zstd_decompress(asz))
     recon = api.decompress(st, data.shape, _params(info))
     assert hashlib.sha256(recon.tobytes()).hexdigest() == info["recon_sha256"]
     if "recon_file" in info:
@@ -69,7 +69,7 @@ def test_decompress_is_bit_exact(name):
 @pytest.mark.parametrize("name", NAMES)
 def test_default_slope_and_t_selection(name):
     info, data, asz = gu.load(name)
-    st = api.parse_payload(zstd.decompress(asz))
+    st = api.parse_payload(codec.zstd_decompress(asz))
     s = gu.settings(info)
     p = _params(info)
     slope, inv, fb, reason = api.default_slope(data, p, t_min=s["t_min"], t_max=s["t_max"])
@@ -89,7 +89,7 @@ def test_error_bound_with_float_rounding_slack():
     """|x - x'| <= eb + ulp_f32(x')/2 on every fixture (the reference's own reconstruction)."""
     for name in NAMES:
         info, data, asz = gu.load(name)
-        st = api.parse_payload(zstd.decompress(asz))
+        st = api.parse_payload(codec.zstd_decompress(asz))
         recon = api.decompress(st, data.shape, _params(info))
         eb = float(api.abs_eb(gu.settings(info)["rel_eb"], st.min_val, st.max_val))
         err = np.abs(data.astype(np.float64) - recon.astype(np.float64))
