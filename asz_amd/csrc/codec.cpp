@@ -911,26 +911,25 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.layout_ms = ms_since(hh0);
     S.huffman_host_ms = host_ms + S.layout_ms;
 
-    uint32_t* dwords = c->words.get<uint32_t>(words_total);
-    // chunk scratch: each chunk's bits at the start of its own slot
-    const uint64_t slot_words = (uint64_t)kEncChunk * std::max(max_len, 1) / 32 + 2;
-    uint32_t* scratch = c->scratch.get<uint32_t>(chunks * slot_words + 1);
-    uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
-    uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
+    // the stream buffer, then the encoder's ticket counter, error flag and
+    // look-back status words (all zeroed by one fill)
+    const uint64_t lbw = encode_lb_words(chunks);
+    uint32_t* dwords = c->words.get<uint32_t>(words_total + lbw);
+    uint32_t* lbwords = dwords + words_total;
     HIPCHECK(hipEventRecord(c->ev[4], st));
-    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), words_total, 0, st);
-    if (max_len > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), chunks * slot_words, 0, st);
+    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), words_total + lbw, 0, st);
     int ke = kev_begin(c, KEV_ENCODE, 0);
     launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                          reinterpret_cast<uint8_t*>(dwords), st);
     const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-    launch_encode_chunks(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
-                         scratch, slot_words, cbits, max_len, st);
-    launch_chunk_scan(darr, (uint32_t)darrs.size(), cbits, coffs, st);
-    launch_compact_bits(scratch, slot_words, cbits, coffs, chunks, dwords, st);
+    launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book), dwords,
+                  lbwords, max_len, st);
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
+    uint32_t* enc_err = c->pin_flag.get<uint32_t>(1);
+    *enc_err = 0;
+    HIPCHECK(hipMemcpyAsync(enc_err, lbwords + 2, 4, hipMemcpyDeviceToHost, st));
     c->fouts.assign(nf, {nullptr, 0});
     if (c->stream_on_device) {
         for (uint32_t f = 0; f < nf; f++)
@@ -952,6 +951,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         HIPCHECK(hipMemcpyAsync(t32.data(), sym, nt * 4, hipMemcpyDeviceToHost, st));
     }
     HIPCHECK(hipStreamSynchronize(st));
+    if (*enc_err) fail(ASZ_ERR_HIP, "Huffman encoder: a chunk never received its bit position");
     if (t_out)
         for (uint64_t i = 0; i < nt; i++) t_out[i] = t32[i];
     kev_collect(c);

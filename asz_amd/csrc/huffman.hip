@@ -12,15 +12,13 @@
 // rounds run, without a copy or an API call in between.  An array whose range
 // leaves the global window is binned again by k_hist over its exact range.
 //
-// Encoding.  The host builds the reference-tree codebooks.  k_encode_chunks
-// codes each chunk of kEncChunk symbols (one workgroup) from an LDS copy of
-// the codebook when it is small, packs the codes LSB-first in an LDS bit
-// buffer and stores it at the start of the chunk's own scratch slot, with the
-// chunk's bit count.  k_chunk_scan turns the counts into absolute bit
-// positions per array, and k_compact_bits shifts every chunk's bits into
-// place in the stream buffer (which already holds the header bytes): interior
-// words are stored, the two boundary words shared with the neighbours are
-// ORed in.
+// Encoding.  The host builds the reference-tree codebooks.  k_encode codes
+// the arrays in one pass: chunks of kEncChunk symbols are taken by ticket,
+// coded from an LDS copy of the codebook when it is small, and each chunk
+// finds its bit position in the stream by a decoupled look-back over the bit
+// counts its predecessors publish, then writes its codes there directly
+// (the stream buffer already holds the header bytes): interior words are
+// stored, the two boundary words shared with the neighbours are ORed in.
 #include "device.hpp"
 #include "fold.hpp"
 #include "launch.hpp"
@@ -131,32 +129,100 @@ constexpr int kEncPerThread = kEncChunk / 256;  // 16
 constexpr int kEncPad = kEncChunk + kEncChunk / kEncPerThread;  // staging with one pad word per 16
 constexpr int kEncWords = kEncChunk + 2;        // LDS bit buffer for codes up to 32 bits
 
-// Persistent encoder: workgroup g codes chunks g, g + G, g + 2G, ... of
-// kEncChunk symbols (G = gridDim.x), with the codebook of the current array
-// in LDS when it is small.  The symbols of the next chunk are loaded
-// (coalesced: lane l of step j reads symbol 256 j + l)
-// while the current one is coded.  Per chunk: codebook entries are staged in
-// LDS so that thread t owns the 16 consecutive symbols 16 t .. 16 t + 15
-// (padded rows, no bank conflicts); after a workgroup scan of the per-thread
-// bit counts each thread packs its codes into a 64-bit accumulator and emits
-// whole 32-bit words: only the first word (shared with the previous thread)
-// and the last partial word (shared with the next) are ORed into the LDS
-// buffer, the others are stored.  The chunk's bits go to the start of its
-// own scratch slot, its bit count to chunk_bits.  Codes longer than 32 bits
-// (short_codes false) are ORed straight into the zeroed scratch slot.  The
-// kernel is held to 128 VGPRs so that four workgroups share a CU.
+// Look-back status of a chunk: flag in bits 62..63 (0 not yet published,
+// 1 the chunk's own bit count, 2 its inclusive bit prefix within its array),
+// the count in bits 0..61.
+constexpr uint64_t kLbAggregate = 1ull << 62, kLbPrefix = 2ull << 62, kLbValue = (1ull << 62) - 1;
+// Polls of one predecessor's status before the encoder gives up and raises
+// its error flag (far beyond any real wait: a predecessor publishes its count
+// right after its own scan).
+constexpr uint32_t kLbSpinLimit = 1u << 22;
+
+__device__ __forceinline__ void lb_publish(unsigned long long* status, uint64_t c, uint64_t w) {
+    __hip_atomic_store(status + c, (unsigned long long)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_load(unsigned long long* status, uint64_t c) {
+    return __hip_atomic_load(status + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decoupled look-back (one wave): publishes chunk c's bit count, sums the
+// counts of its predecessors within its array (first = the array's first
+// chunk) back to the nearest one that has published its inclusive prefix,
+// publishes c's own inclusive prefix and returns its exclusive one.  Lane l
+// inspects predecessor j - l, 64 predecessors per step.  Every predecessor
+// holds a smaller ticket, so it belongs to a workgroup that is already
+// running and publishes its count before it looks back itself: the wait
+// always ends.
+__device__ uint64_t lookback(unsigned long long* status, uint64_t c, uint64_t first, uint64_t total, uint32_t lane,
+                             unsigned int* err) {
+    if (c == first) {
+        if (lane == 0) lb_publish(status, c, kLbPrefix | total);
+        return 0;
+    }
+    if (lane == 0) lb_publish(status, c, kLbAggregate | total);
+    uint64_t excl = 0;
+    int64_t j = (int64_t)c - 1;
+    while (true) {
+        const int64_t idx = j - (int64_t)lane;
+        uint64_t w = kLbPrefix;  // before the array's first chunk: an inclusive prefix of 0
+        if (idx >= (int64_t)first) {
+            w = lb_load(status, (uint64_t)idx);
+            for (uint32_t it = 0; (w >> 62) == 0; it++) {
+                if (it >= kLbSpinLimit) {
+                    atomicOr(err, 1u);
+                    w = kLbPrefix;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                w = lb_load(status, (uint64_t)idx);
+            }
+        }
+        const uint64_t pm = __ballot((w >> 62) == 2);
+        const uint64_t val = w & kLbValue;
+        if (pm) {
+            const uint32_t lp = (uint32_t)__ffsll((unsigned long long)pm) - 1;
+            excl += wave_sum<uint64_t>(lane <= lp ? val : 0ull);
+            break;
+        }
+        excl += wave_sum<uint64_t>(val);
+        j -= kWave;
+    }
+    if (lane == 0) lb_publish(status, c, kLbPrefix | (excl + total));
+    return excl;
+}
+
+// Single-pass encoder.  Each workgroup takes chunks of kEncChunk symbols by
+// ticket (an atomic counter, so a chunk's predecessors always belong to
+// running workgroups) until the tickets run out, with the codebook of the
+// current array in LDS when it is small; the symbols of its next chunk are
+// loaded (coalesced: lane l of step j reads symbol 256 j + l) while the
+// current one is coded.  Per chunk: codebook entries are staged in LDS so
+// that thread t owns the 16 consecutive symbols 16 t .. 16 t + 15 (padded
+// rows, no bank conflicts); a workgroup scan of the per-thread bit counts
+// gives the chunk's bit count, and the look-back its bit position in the
+// stream.  Each thread then packs its codes into a 64-bit accumulator at
+// that position's offset within a 32-bit word and emits whole words into an
+// LDS bit buffer (only the words it shares with its neighbours are ORed in);
+// the buffer goes to the stream buffer, interior words stored and the two
+// boundary words (shared with the neighbouring chunks or the array's header
+// bytes) ORed in.  Codes longer than 32 bits (short_codes false) are ORed
+// straight into the stream.  The kernel is held to 128 VGPRs so that four
+// workgroups share a CU.  lb: [0] ticket counter, [1] error flag, [2 + c]
+// chunk c's status; zeroed, like the stream buffer, before the launch.
 template <bool short_codes>
-__global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restrict__ sym,
-                                                          const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                          uint64_t nchunks, const uint64_t* __restrict__ book,
-                                                          uint32_t* scratch, uint64_t slot_words,
-                                                          uint32_t* __restrict__ chunk_bits) {
+__global__ __launch_bounds__(256, 4) void k_encode(const int32_t* __restrict__ sym,
+                                                   const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                   uint64_t nchunks, const uint64_t* __restrict__ book,
+                                                   uint32_t* words, unsigned long long* lb) {
     __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
     __shared__ uint32_t lbook[kLdsBook];
     __shared__ uint32_t s_scan[4];
+    __shared__ unsigned long long s_tk[2];
+    __shared__ unsigned long long s_excl;
+    unsigned long long* status = lb + 2;
+    unsigned int* err = reinterpret_cast<unsigned int*>(lb + 1);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t G = gridDim.x;
-    // chunks of one workgroup increase, so its array index only moves forward
+    // tickets of one workgroup increase, so its array index only moves forward
     // (the table stays in global memory: one call may code many fields' arrays)
     auto array_of = [&](uint64_t c, uint32_t k) {
         while (k + 1 < narr && c >= arrays[k].chunk0 + arrays[k].nchunk) k++;
@@ -173,21 +239,24 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
             v[j] = i < n ? s[i] : 0;
         }
     };
-    uint64_t c = blockIdx.x;
+    if (threadIdx.x == 0) s_tk[0] = atomicAdd(lb, 1ull);
+    __syncthreads();
+    uint64_t c = s_tk[0];
     if (c >= nchunks) return;
     uint32_t k = array_of(c, 0);
     int32_t v[kEncPerThread];
     if (short_codes) load_chunk(c, k, v);
     int book_k = -1;
     while (true) {
+        if (threadIdx.x == 0) s_tk[1] = atomicAdd(lb, 1ull);  // the next chunk, read after the next barrier
         const HuffArrayDev A = arrays[k];
         const uint64_t* b = book + A.book_off;
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
-        const uint64_t cn = c + G;
-        const uint32_t kn = cn < nchunks ? array_of(cn, k) : k;
         uint64_t e[kEncPerThread];
         uint32_t mybits = 0;
+        uint64_t cn;
+        uint32_t kn;
         if (short_codes) {
             if (A.lds_book && book_k != (int)k) {
                 for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) {
@@ -196,7 +265,9 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
                 }
                 book_k = (int)k;
             }
-            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free
+            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free; next ticket in LDS
+            cn = s_tk[1];
+            kn = cn < nchunks ? array_of(cn, k) : k;
 #pragma unroll
             for (int j = 0; j < kEncPerThread; j++) {
                 const uint32_t i = j * 256 + threadIdx.x;
@@ -230,6 +301,8 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
             }
         } else {
             lds_barrier();
+            cn = s_tk[1];
+            kn = cn < nchunks ? array_of(cn, k) : k;
             const uint32_t my0 = threadIdx.x * kEncPerThread;
             const int32_t* s = sym + A.src_off + first;
 #pragma unroll
@@ -254,15 +327,22 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
         for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
         const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
         const uint32_t my_off = wave_off + incl - mybits;
-        if (threadIdx.x == 0) chunk_bits[c] = total;
-        uint32_t* out = scratch + c * slot_words;
+        if (wave == 0) {
+            const uint64_t ex = lookback(status, c, A.chunk0, total, lane, err);
+            if (lane == 0) s_excl = ex;
+        }
+        lds_barrier();
+        const uint64_t gbit = A.payload_bit + s_excl;  // the chunk's first bit in the stream
+        const uint32_t sh = (uint32_t)(gbit & 31);
+        uint32_t* out = words + (gbit >> 5);
         if (short_codes) {
-            const uint32_t nw = (total + 31) / 32;
+            const uint32_t nw = (sh + total + 31) / 32;
             for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lbuf[w] = 0;
             lds_barrier();
             if (mybits) {
-                uint32_t word = my_off >> 5;
-                uint32_t nbits = my_off & 31;
+                const uint32_t p0 = sh + my_off;
+                uint32_t word = p0 >> 5;
+                uint32_t nbits = p0 & 31;
                 const bool shared_first = nbits != 0;
                 bool first_word = true;
                 uint64_t accb = 0;
@@ -283,18 +363,22 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
                 if (nbits) atomicOr(&lbuf[word], (uint32_t)accb);
             }
             lds_barrier();
-            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) out[w] = lbuf[w];
+            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+                const uint32_t val = lbuf[w];
+                if (w == 0 || w + 1 == nw) atomicOr(&out[w], val);
+                else out[w] = val;
+            }
         } else {
-            uint64_t p = my_off;
+            uint64_t p = (uint64_t)sh + my_off;
             for (int j = 0; j < kEncPerThread; j++) {
                 const uint32_t len = book_len(e[j]);
                 if (!len) continue;
                 const uint64_t code = book_code(e[j]);
-                const uint32_t sh = (uint32_t)(p & 31);
+                const uint32_t s2 = (uint32_t)(p & 31);
                 uint32_t* w = out + (p >> 5);
-                atomicOr(&w[0], (uint32_t)(code << sh));
-                if (sh + len > 32) atomicOr(&w[1], (uint32_t)(code >> (32 - sh)));
-                if (sh + len > 64) atomicOr(&w[2], (uint32_t)(code >> (64 - sh)));
+                atomicOr(&w[0], (uint32_t)(code << s2));
+                if (s2 + len > 32) atomicOr(&w[1], (uint32_t)(code >> (32 - s2)));
+                if (s2 + len > 64) atomicOr(&w[2], (uint32_t)(code >> (64 - s2)));
                 p += len;
             }
         }
@@ -303,109 +387,6 @@ __global__ __launch_bounds__(256, 4) void k_encode_chunks(const int32_t* __restr
         k = kn;
     }
 }
-
-// One workgroup (1024 threads) per array: chunk_off[c] = payload start +
-// exclusive scan of the array's chunk bit counts.  Thread t owns a
-// contiguous run of chunks: it sums the run, the workgroup scans the 1024
-// sums (wave shuffles, then the 16 wave totals), and the thread writes its
-// run's offsets.
-__global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restrict__ arrays,
-                                                     const uint32_t* __restrict__ chunk_bits, uint64_t* chunk_off) {
-    __shared__ uint64_t wsum[16];
-    const HuffArrayDev A = arrays[blockIdx.x];
-    if (A.nchunk == 0) return;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t per = (A.nchunk + 1023) / 1024;
-    const uint64_t r0 = min<uint64_t>(A.nchunk, threadIdx.x * per), r1 = min<uint64_t>(A.nchunk, r0 + per);
-    const uint32_t* cb = chunk_bits + A.chunk0;
-    // a thread's range in batches of 8 loads in flight
-    uint64_t mine = 0;
-    for (uint64_t i = r0; i < r1; i += 8) {
-        uint32_t t[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = i + u < r1 ? cb[i + u] : 0u;
-#pragma unroll
-        for (int u = 0; u < 8; u++) mine += t[u];
-    }
-    uint64_t incl = mine;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint64_t t = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o) incl += t;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    uint64_t off = A.payload_bit + incl - mine;
-    for (uint32_t w = 0; w < wave; w++) off += wsum[w];
-    uint64_t* co = chunk_off + A.chunk0;
-    for (uint64_t i = r0; i < r1; i += 8) {
-        uint32_t t[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = i + u < r1 ? cb[i + u] : 0u;
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (i + u < r1) co[i + u] = off;
-            off += t[u];
-        }
-    }
-}
-
-// One wave per kCompactPerWave consecutive chunks (their bit counts and
-// offsets loaded together first): each chunk's bits from its scratch slot to
-// bit position chunk_off in the stream.  Each pass a lane loads 4 consecutive
-// source words (all loads of the pass in flight at once); the word below a
-// lane's first comes from the neighbouring lane.  Interior output words are
-// stored, the two boundary words shared with the neighbouring chunks are ORed
-// in.
-constexpr uint32_t kCompactPerWave = 4;
-__global__ __launch_bounds__(256) void k_compact_bits(const uint32_t* __restrict__ scratch, uint64_t slot_words,
-                                                      const uint32_t* __restrict__ chunk_bits,
-                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
-                                                      uint32_t* words) {
-    constexpr uint32_t kW = 4;  // words per lane per pass
-    const uint64_t g0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kCompactPerWave;
-    const uint32_t lane = threadIdx.x & 63;
-    if (g0 >= nchunks) return;
-    uint32_t nbs[kCompactPerWave];
-    uint64_t offs[kCompactPerWave];
-#pragma unroll
-    for (uint32_t j = 0; j < kCompactPerWave; j++) {
-        const bool in = g0 + j < nchunks;
-        nbs[j] = in ? chunk_bits[g0 + j] : 0u;
-        offs[j] = in ? chunk_off[g0 + j] : 0ull;
-    }
-    for (uint32_t j = 0; j < kCompactPerWave; j++) {
-        const uint32_t nb = nbs[j];
-        if (nb == 0) continue;
-        const uint64_t g = g0 + j;
-        const uint64_t o = offs[j];
-        const uint32_t sh = (uint32_t)(o & 31);
-        const uint32_t nsrc = (nb + 31) / 32, nout = (sh + nb + 31) / 32;
-        const uint32_t* src = scratch + g * slot_words;
-        uint32_t* dst = words + (o >> 5);
-        uint32_t carry = 0;  // source word below this pass's first word
-        for (uint32_t base = 0; base < nout; base += 64 * kW) {
-            const uint32_t i0 = base + lane * kW;
-            uint32_t cur[kW];
-#pragma unroll
-            for (uint32_t q = 0; q < kW; q++) cur[q] = i0 + q < nsrc ? src[i0 + q] : 0u;
-            uint32_t below = __shfl_up(cur[kW - 1], 1, 64);
-            if (lane == 0) below = carry;
-            carry = __shfl(cur[kW - 1], 63, 64);
-#pragma unroll
-            for (uint32_t q = 0; q < kW; q++) {
-                const uint32_t i = i0 + q;
-                if (i >= nout) break;
-                const uint32_t prev = q ? cur[q - 1] : below;
-                uint32_t v = cur[q] << sh;
-                if (sh && i > 0) v |= prev >> (32 - sh);
-                if (i == 0 || i == nout - 1) atomicOr(&dst[i], v);
-                else dst[i] = v;
-            }
-        }
-    }
-}
-
 }  // namespace dev
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
@@ -433,37 +414,28 @@ void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64
     hipLaunchKernelGGL(dev::k_hist_tail, dim3(nfields), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch,
                        part_stride, g_stride, h_stride);
 }
-void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
-                          const uint64_t* book, uint32_t* scratch, uint64_t slot_words, uint32_t* chunk_bits,
-                          int max_len, hipStream_t st) {
+uint64_t encode_lb_words(uint64_t nchunks) { return 2 * (nchunks + 2); }
+
+void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
+                   const uint64_t* book, uint32_t* stream_words, uint32_t* lb_words, int max_len, hipStream_t st) {
     if (!nchunks) return;
     // one resident wave of workgroups: CUs x workgroups that fit on a CU
     int dev_id = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
     if (max_len <= 32)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_chunks<true>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode<true>, 256, 0);
     else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_chunks<false>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode<false>, 256, 0);
     const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
     const uint64_t groups = std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
+    unsigned long long* lb = reinterpret_cast<unsigned long long*>(lb_words);
     if (max_len <= 32)
-        hipLaunchKernelGGL(dev::k_encode_chunks<true>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr,
-                           nchunks, book, scratch, slot_words, chunk_bits);
+        hipLaunchKernelGGL(dev::k_encode<true>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr, nchunks,
+                           book, stream_words, lb);
     else
-        hipLaunchKernelGGL(dev::k_encode_chunks<false>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr,
-                           nchunks, book, scratch, slot_words, chunk_bits);
-}
-void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint32_t* chunk_bits, uint64_t* chunk_off,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_chunk_scan, dim3(narrays), dim3(1024), 0, st, arrays, chunk_bits, chunk_off);
-}
-void launch_compact_bits(const uint32_t* scratch, uint64_t slot_words, const uint32_t* chunk_bits,
-                         const uint64_t* chunk_off, uint64_t nchunks, uint32_t* stream_words, hipStream_t st) {
-    if (!nchunks) return;
-    const uint64_t per_wg = 4ull * dev::kCompactPerWave;
-    hipLaunchKernelGGL(dev::k_compact_bits, dim3((uint32_t)((nchunks + per_wg - 1) / per_wg)), dim3(256), 0, st, scratch,
-                       slot_words, chunk_bits, chunk_off, nchunks, stream_words);
+        hipLaunchKernelGGL(dev::k_encode<false>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr, nchunks,
+                           book, stream_words, lb);
 }
 
 }  // namespace asz

@@ -184,15 +184,14 @@ void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
                       uint32_t* hres, uint32_t epoch, hipStream_t st, uint32_t nfields = 1, uint64_t part_stride = 0,
                       uint64_t g_stride = 0, uint64_t h_stride = 0);
-// slot_words: scratch words per chunk (>= kEncChunk * max_len / 32 + 2); with
-// max_len > 32 the slots must be zeroed first.
-void launch_encode_chunks(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
-                          const uint64_t* book, uint32_t* scratch, uint64_t slot_words, uint32_t* chunk_bits,
-                          int max_len, hipStream_t st);
-void launch_chunk_scan(const HuffArrayDev* arrays, uint32_t narrays, const uint32_t* chunk_bits, uint64_t* chunk_off,
-                       hipStream_t st);
-void launch_compact_bits(const uint32_t* scratch, uint64_t slot_words, const uint32_t* chunk_bits,
-                         const uint64_t* chunk_off, uint64_t nchunks, uint32_t* stream_words, hipStream_t st);
+// Single-pass encoder of the arrays' chunks straight into the stream buffer
+// (which holds the header bytes and is zero elsewhere).  lb_words: the
+// encoder's ticket counter, error flag and per-chunk look-back status,
+// encode_lb_words(nchunks) 32-bit words that must be zero at the launch; a
+// nonzero error flag (word 2) afterwards means a look-back wait gave up.
+uint64_t encode_lb_words(uint64_t nchunks);
+void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
+                   const uint64_t* book, uint32_t* stream_words, uint32_t* lb_words, int max_len, hipStream_t st);
 
 // Decoder (self-synchronising parallel decode of one bit stream per array).
 struct DecArrayDev {

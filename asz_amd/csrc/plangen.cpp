@@ -328,27 +328,25 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             std::memcpy(hu + o_arr, H.data(), ncand * sizeof(HuffArrayDev));
             uint8_t* du = c->hmeta.get<uint8_t>(up);
             HIPCHECK(hipMemcpyAsync(du, hu, up, hipMemcpyHostToDevice, st));
-            const uint64_t nwords = (total_bytes + 3) / 4 + 4;
-            uint32_t* dwords = c->words.get<uint32_t>(nwords);
-            launch_fill_i32(reinterpret_cast<int32_t*>(dwords), nwords, 0, st);
+            // the stream words, then the encoder's look-back words (one fill zeroes both)
+            const uint64_t nwords = ((total_bytes + 3) / 4 + 4 + 63) / 64 * 64;
+            uint32_t* dwords = c->words.get<uint32_t>(nwords + encode_lb_words(chunks));
+            launch_fill_i32(reinterpret_cast<int32_t*>(dwords), nwords + encode_lb_words(chunks), 0, st);
             launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                                  reinterpret_cast<uint8_t*>(dwords), st);
             if (chunks) {
-                const uint64_t slot_words = (uint64_t)kEncChunk * std::max(max_len, 1) / 32 + 2;
-                uint32_t* scratch = c->scratch.get<uint32_t>(chunks * slot_words + 1);
-                uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
-                uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
-                if (max_len > 32) launch_fill_i32(reinterpret_cast<int32_t*>(scratch), chunks * slot_words, 0, st);
                 const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-                launch_encode_chunks(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book), scratch,
-                                     slot_words, cbits, max_len, st);
-                launch_chunk_scan(darr, ncand, cbits, coffs, st);
-                launch_compact_bits(scratch, slot_words, cbits, coffs, chunks, dwords, st);
+                launch_encode(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book), dwords,
+                              dwords + nwords, max_len, st);
             }
             HIPCHECK(hipGetLastError());
             hbytes = c->pin_stream.get<uint8_t>(total_bytes + 1);
             HIPCHECK(hipMemcpyAsync(hbytes, dwords, total_bytes, hipMemcpyDeviceToHost, st));
+            uint32_t* enc_err = c->pin_flag.get<uint32_t>(1);
+            *enc_err = 0;
+            if (chunks) HIPCHECK(hipMemcpyAsync(enc_err, dwords + nwords + 2, 4, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
+            if (*enc_err) fail(ASZ_ERR_HIP, "Huffman encoder: a chunk never received its bit position");
         }
         // Zstd level 3 of each candidate's bytes (generate_config.tpp:639-641), one host thread each
         std::vector<uint64_t> zlen(ncand, 0);
