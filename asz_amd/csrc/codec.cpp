@@ -430,12 +430,14 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.plan_cache_hit = hit ? 1 : 0;
 
     // Per-call buffers, one slice per field (multiples of 64 elements apart).
-    // symbol buffer: t array (nt) then pivot + round QB arrays (N)
+    // symbol buffer: t array (nt), then the pivot and round QB arrays, each
+    // starting on a kSymAlign boundary
     auto al64 = [](uint64_t v) { return (v + 63) / 64 * 64; };
     const uint64_t npp = (P.pivot_count + 255) / 256;  // pivot kernel workgroups
     const uint64_t narr = L + 2;                       // t array, pivots, rounds
     FieldSet fs{};
-    fs.sym = al64(nt + N + 16);
+    const uint64_t ntp = sym_align(nt);                // the QB arrays start after the padded t array
+    fs.sym = al64(ntp + P.qb_total + 16);
     fs.tblk = al64(nt + 1);
     fs.part = al64(P.total_partials + 1);
     fs.phist = al64(P.total_hist_words + 1);
@@ -488,7 +490,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         if (range_res[2] == 2) fail(ASZ_ERR_DATA, "max_val==min_val");
     };
     int32_t* sym = sym0;  // field 0
-    int32_t* qb = sym + nt;
+    int32_t* qb = sym + ntp;
     // staging array of the rounds whose QBs are written in "012" order first
     uint64_t stage_max = 0;
     for (uint64_t r = 0; r < L; r++)
@@ -596,7 +598,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         // (several contexts then spread over the device's hardware queues)
         if (!c->side) HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HuffArrayDev D{};
-        D.src_off = k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off);
+        D.src_off = k == 0 ? 0 : (k == 1 ? ntp : ntp + P.rounds[k - 2].qb_off);
         D.count = A.count;
         D.offset = A.offset;
         D.range = A.range;
@@ -620,7 +622,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
     kev_end(c, kp);
-    launch_hist_small(nullptr, 0, sym + nt, P.pivot_count, nullptr, gwin(0, 1), st, nf, fs.sym, fs.ghist);
+    launch_hist_small(nullptr, 0, sym + ntp, P.pivot_count, nullptr, gwin(0, 1), st, nf, fs.sym, fs.ghist);
     launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, st, nf, fs.fpart, fs.ghist, fs.hres);
     // Codebook workers: the host pool's threads take the arrays in `order`
     // (every field's copy of an array in turn) as their results arrive,
@@ -867,7 +869,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             if (!A.count) continue;
             emit(A.header.data(), A.header.size());
             if (A.single) continue;
-            D.src_off = f * fs.sym + (k == 0 ? 0 : (k == 1 ? nt : nt + P.rounds[k - 2].qb_off));
+            D.src_off = f * fs.sym + (k == 0 ? 0 : (k == 1 ? ntp : ntp + P.rounds[k - 2].qb_off));
             D.count = A.count;
             D.offset = A.offset;
             D.range = A.range;
@@ -877,7 +879,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             chunks += D.nchunk;
             D.payload_bit = (sbase + spos) * 8;
             spos += (A.payload_bits + 7) >> 3;
-            D.lds_book = (A.range <= kLdsBook && A.max_len <= 24) ? 1 : 0;
+            D.lds_book = A.range <= kLdsBook ? 1 : 0;
             lay.max_len = std::max(lay.max_len, A.max_len);
             book_total += A.range;
             darrs.push_back(D);
@@ -891,8 +893,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.stream_bytes = lays[0].total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lays[0].arrays[k + 1].count;
     auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
+    // codebooks as three parallel tables (64-bit entries, 32-bit codes, byte lengths)
     const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
-                   o_arr = al8(o_book + book_total * 8), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
+                   o_code = al8(o_book + book_total * 8), o_len = al8(o_code + book_total * 4),
+                   o_arr = al8(o_len + book_total), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
     uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
     std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
     std::memcpy(hu + o_blob, blob.data(), blob.size());
@@ -902,7 +906,14 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             for (uint64_t k = 0; k < narr; k++) {
                 const HuffArray& A = lays[f].arrays[k];
                 if (!A.count || A.single) continue;
-                std::memcpy(hu + o_book + darrs[i++].book_off * 8, A.book.data(), A.range * 8);
+                const uint64_t bo = darrs[i++].book_off;
+                std::memcpy(hu + o_book + bo * 8, A.book.data(), A.range * 8);
+                uint32_t* c32 = reinterpret_cast<uint32_t*>(hu + o_code) + bo;
+                uint8_t* l8 = hu + o_len + bo;
+                for (uint64_t e = 0; e < A.range; e++) {
+                    c32[e] = (uint32_t)A.book[e];
+                    l8[e] = (uint8_t)(A.book[e] >> 56);
+                }
             }
     }
     std::memcpy(hu + o_arr, darrs.data(), darrs.size() * sizeof(HuffArrayDev));
@@ -911,25 +922,22 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.layout_ms = ms_since(hh0);
     S.huffman_host_ms = host_ms + S.layout_ms;
 
-    // the stream buffer, then the encoder's ticket counter, error flag and
-    // look-back status words (all zeroed by one fill)
-    const uint64_t lbw = encode_lb_words(chunks);
-    uint32_t* dwords = c->words.get<uint32_t>(words_total + lbw);
-    uint32_t* lbwords = dwords + words_total;
+    // the stream buffer: the encoder stores whole payload words and ORs the
+    // words it shares (which its scan zeroes first); the header bytes are
+    // scattered in afterwards (they never share a byte with payload bits)
+    uint32_t* dwords = c->words.get<uint32_t>(words_total);
+    uint32_t* cbits = c->chunk_bits.get<uint32_t>(chunks + 1);
+    uint64_t* coffs = c->chunk_off.get<uint64_t>(chunks + 1);
     HIPCHECK(hipEventRecord(c->ev[4], st));
-    launch_fill_i32(reinterpret_cast<int32_t*>(dwords), words_total + lbw, 0, st);
     int ke = kev_begin(c, KEV_ENCODE, 0);
+    const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
+    launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
+                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, max_len, st);
     launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                          reinterpret_cast<uint8_t*>(dwords), st);
-    const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-    launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book), dwords,
-                  lbwords, max_len, st);
     kev_end(c, ke);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[5], st));
-    uint32_t* enc_err = c->pin_flag.get<uint32_t>(1);
-    *enc_err = 0;
-    HIPCHECK(hipMemcpyAsync(enc_err, lbwords + 2, 4, hipMemcpyDeviceToHost, st));
     c->fouts.assign(nf, {nullptr, 0});
     if (c->stream_on_device) {
         for (uint32_t f = 0; f < nf; f++)
@@ -944,14 +952,21 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     }
     c->out_stream = c->fouts[0].first;
     c->out_bytes = c->fouts[0].second;
-    if (qb_out) HIPCHECK(hipMemcpyAsync(qb_out, qb, N * 4, hipMemcpyDeviceToHost, st));
+    if (qb_out) {  // the pivot and round arrays, contiguous on the host
+        HIPCHECK(hipMemcpyAsync(qb_out, qb, P.pivot_count * 4, hipMemcpyDeviceToHost, st));
+        uint64_t o = P.pivot_count;
+        for (uint64_t r = 0; r < L; r++) {
+            const RoundDesc& R = P.rounds[r];
+            if (R.count) HIPCHECK(hipMemcpyAsync(qb_out + o, qb + R.qb_off, R.count * 4, hipMemcpyDeviceToHost, st));
+            o += R.count;
+        }
+    }
     std::vector<int32_t> t32;
     if (t_out) {
         t32.resize(nt);
         HIPCHECK(hipMemcpyAsync(t32.data(), sym, nt * 4, hipMemcpyDeviceToHost, st));
     }
     HIPCHECK(hipStreamSynchronize(st));
-    if (*enc_err) fail(ASZ_ERR_HIP, "Huffman encoder: a chunk never received its bit position");
     if (t_out)
         for (uint64_t i = 0; i < nt; i++) t_out[i] = t32[i];
     kev_collect(c);

@@ -12,13 +12,12 @@
 // rounds run, without a copy or an API call in between.  An array whose range
 // leaves the global window is binned again by k_hist over its exact range.
 //
-// Encoding.  The host builds the reference-tree codebooks.  k_encode codes
-// the arrays in one pass: chunks of kEncChunk symbols are taken by ticket,
-// coded from an LDS copy of the codebook when it is small, and each chunk
-// finds its bit position in the stream by a decoupled look-back over the bit
-// counts its predecessors publish, then writes its codes there directly
-// (the stream buffer already holds the header bytes): interior words are
-// stored, the two boundary words shared with the neighbours are ORed in.
+// Encoding.  The host builds the reference-tree codebooks.  The arrays are
+// cut into chunks of kEncChunk symbols; k_chunk_bits sums each chunk's code
+// bits, k_chunk_scan turns the sums into bit positions per array, and
+// k_encode_at writes each chunk's codes straight to its position in the
+// stream buffer (which holds the header bytes): interior words are stored,
+// the two boundary words shared with the neighbours are ORed in.
 #include "device.hpp"
 #include "fold.hpp"
 #include "launch.hpp"
@@ -128,190 +127,237 @@ __device__ __forceinline__ uint64_t book_code(uint64_t e) { return e & ((1ull <<
 constexpr int kEncPerThread = kEncChunk / 256;  // 16
 constexpr int kEncPad = kEncChunk + kEncChunk / kEncPerThread;  // staging with one pad word per 16
 constexpr int kEncWords = kEncChunk + 2;        // LDS bit buffer for codes up to 32 bits
+constexpr uint32_t kLdsLens = 8192;             // LDS code-length table of the bit-count pass
 
-// Look-back status of a chunk: flag in bits 62..63 (0 not yet published,
-// 1 the chunk's own bit count, 2 its inclusive bit prefix within its array),
-// the count in bits 0..61.
-constexpr uint64_t kLbAggregate = 1ull << 62, kLbPrefix = 2ull << 62, kLbValue = (1ull << 62) - 1;
-// Polls of one predecessor's status before the encoder gives up and raises
-// its error flag (far beyond any real wait: a predecessor publishes its count
-// right after its own scan).
-constexpr uint32_t kLbSpinLimit = 1u << 22;
-
-__device__ __forceinline__ void lb_publish(unsigned long long* status, uint64_t c, uint64_t w) {
-    __hip_atomic_store(status + c, (unsigned long long)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t lb_load(unsigned long long* status, uint64_t c) {
-    return __hip_atomic_load(status + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// The array of chunk c, from array k on (a workgroup's chunks increase, so
+// its array index only moves forward; the table stays in global memory: one
+// call may code many fields' arrays).
+__device__ __forceinline__ uint32_t chunk_array(const HuffArrayDev* __restrict__ arrays, uint32_t narr, uint64_t c,
+                                                uint32_t k) {
+    while (k + 1 < narr && c >= arrays[k].chunk0 + arrays[k].nchunk) k++;
+    return k;
 }
 
-// Decoupled look-back (one wave): publishes chunk c's bit count, sums the
-// counts of its predecessors within its array (first = the array's first
-// chunk) back to the nearest one that has published its inclusive prefix,
-// publishes c's own inclusive prefix and returns its exclusive one.  Lane l
-// inspects predecessor j - l, 64 predecessors per step.  Every predecessor
-// holds a smaller ticket, so it belongs to a workgroup that is already
-// running and publishes its count before it looks back itself: the wait
-// always ends.
-__device__ uint64_t lookback(unsigned long long* status, uint64_t c, uint64_t first, uint64_t total, uint32_t lane,
-                             unsigned int* err) {
-    if (c == first) {
-        if (lane == 0) lb_publish(status, c, kLbPrefix | total);
-        return 0;
+// Symbols of chunk c of array A, coalesced: lane l of step j reads symbol
+// 256 j + l (0 past the array's end).
+__device__ __forceinline__ void load_chunk(const int32_t* __restrict__ sym, const HuffArrayDev& A, uint64_t c,
+                                           int32_t v[kEncPerThread]) {
+    const uint64_t first = (c - A.chunk0) * kEncChunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
+    const int32_t* s = sym + A.src_off + first;
+#pragma unroll
+    for (int j = 0; j < kEncPerThread; j++) {
+        const uint32_t i = j * 256 + threadIdx.x;
+        v[j] = i < n ? s[i] : 0;
     }
-    if (lane == 0) lb_publish(status, c, kLbAggregate | total);
-    uint64_t excl = 0;
-    int64_t j = (int64_t)c - 1;
-    while (true) {
-        const int64_t idx = j - (int64_t)lane;
-        uint64_t w = kLbPrefix;  // before the array's first chunk: an inclusive prefix of 0
-        if (idx >= (int64_t)first) {
-            w = lb_load(status, (uint64_t)idx);
-            for (uint32_t it = 0; (w >> 62) == 0; it++) {
-                if (it >= kLbSpinLimit) {
-                    atomicOr(err, 1u);
-                    w = kLbPrefix;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                w = lb_load(status, (uint64_t)idx);
-            }
-        }
-        const uint64_t pm = __ballot((w >> 62) == 2);
-        const uint64_t val = w & kLbValue;
-        if (pm) {
-            const uint32_t lp = (uint32_t)__ffsll((unsigned long long)pm) - 1;
-            excl += wave_sum<uint64_t>(lane <= lp ? val : 0ull);
-            break;
-        }
-        excl += wave_sum<uint64_t>(val);
-        j -= kWave;
-    }
-    if (lane == 0) lb_publish(status, c, kLbPrefix | (excl + total));
-    return excl;
 }
 
-// Single-pass encoder.  Each workgroup takes chunks of kEncChunk symbols by
-// ticket (an atomic counter, so a chunk's predecessors always belong to
-// running workgroups) until the tickets run out, with the codebook of the
-// current array in LDS when it is small; the symbols of its next chunk are
-// loaded (coalesced: lane l of step j reads symbol 256 j + l) while the
-// current one is coded.  Per chunk: codebook entries are staged in LDS so
-// that thread t owns the 16 consecutive symbols 16 t .. 16 t + 15 (padded
-// rows, no bank conflicts); a workgroup scan of the per-thread bit counts
-// gives the chunk's bit count, and the look-back its bit position in the
-// stream.  Each thread then packs its codes into a 64-bit accumulator at
-// that position's offset within a 32-bit word and emits whole words into an
-// LDS bit buffer (only the words it shares with its neighbours are ORed in);
-// the buffer goes to the stream buffer, interior words stored and the two
-// boundary words (shared with the neighbouring chunks or the array's header
-// bytes) ORed in.  Codes longer than 32 bits (short_codes false) are ORed
-// straight into the stream.  The kernel is held to 128 VGPRs so that four
-// workgroups share a CU.  lb: [0] ticket counter, [1] error flag, [2 + c]
-// chunk c's status; zeroed, like the stream buffer, before the launch.
-template <bool short_codes>
-__global__ __launch_bounds__(256, 4) void k_encode(const int32_t* __restrict__ sym,
-                                                   const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                   uint64_t nchunks, const uint64_t* __restrict__ book,
-                                                   uint32_t* words, unsigned long long* lb) {
-    __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
-    __shared__ uint32_t lbook[kLdsBook];
-    __shared__ uint32_t s_scan[4];
-    __shared__ unsigned long long s_tk[2];
-    __shared__ unsigned long long s_excl;
-    unsigned long long* status = lb + 2;
-    unsigned int* err = reinterpret_cast<unsigned int*>(lb + 1);
+// Bit-count pass: chunk_bits[c] = the code bits of chunk c.  Persistent:
+// workgroup g takes chunks g, g + G, ... (G = gridDim.x), the next chunk's
+// symbols in flight while the current one is summed, with the array's code
+// lengths (len8: one byte per symbol of every array, at its book offset) in
+// LDS when its alphabet fits.
+__global__ __launch_bounds__(256) void k_chunk_bits(const int32_t* __restrict__ sym,
+                                                    const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                    uint64_t nchunks, const uint8_t* __restrict__ len8,
+                                                    uint32_t* __restrict__ chunk_bits) {
+    __shared__ uint8_t llen[kLdsLens];
+    __shared__ uint32_t s_sum[4];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // tickets of one workgroup increase, so its array index only moves forward
-    // (the table stays in global memory: one call may code many fields' arrays)
-    auto array_of = [&](uint64_t c, uint32_t k) {
-        while (k + 1 < narr && c >= arrays[k].chunk0 + arrays[k].nchunk) k++;
-        return k;
-    };
-    auto load_chunk = [&](uint64_t c, uint32_t k, int32_t v[kEncPerThread]) {
+    const uint64_t G = gridDim.x;
+    uint64_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    uint32_t k = chunk_array(arrays, narr, c, 0);
+    int32_t v[kEncPerThread];
+    load_chunk(sym, arrays[k], c, v);
+    int tab_k = -1;
+    while (true) {
         const HuffArrayDev A = arrays[k];
+        const uint8_t* L = len8 + A.book_off;
+        const bool lds = A.range <= kLdsLens;
+        if (lds && tab_k != (int)k) {
+            for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) llen[i] = L[i];
+            tab_k = (int)k;
+        }
+        lds_barrier();  // table complete; the previous chunk's sums read
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
-        const int32_t* s = sym + A.src_off + first;
+        const uint64_t cn = c + G;
+        const uint32_t kn = cn < nchunks ? chunk_array(arrays, narr, cn, k) : k;
+        uint32_t bits = 0;
 #pragma unroll
         for (int j = 0; j < kEncPerThread; j++) {
             const uint32_t i = j * 256 + threadIdx.x;
-            v[j] = i < n ? s[i] : 0;
+            if (i < n) {
+                const int64_t idx = (int64_t)v[j] - A.offset;
+                bits += lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+            }
         }
+        if (cn < nchunks) load_chunk(sym, arrays[kn], cn, v);
+        bits = wave_sum(bits);
+        if (lane == 0) s_sum[wave] = bits;
+        lds_barrier();
+        if (threadIdx.x == 0) chunk_bits[c] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+        if (cn >= nchunks) break;
+        c = cn;
+        k = kn;
+    }
+}
+
+// chunk_off[c] = the bit position of chunk c in the stream: its array's
+// payload start plus the sum of the bit counts of the array's earlier chunks.
+// One workgroup (1024 threads) per 1024 consecutive chunks, thread t owning
+// chunk 1024 g + t: a workgroup scan of the group's counts (wave shuffles,
+// then the 16 wave totals), rebased at every array that starts inside the
+// group; the array that started before the group adds the sum of its earlier
+// chunks, which the workgroup reads coalesced.  It also zeroes the stream
+// words that hold a chunk's first or last bit: the encoder ORs those (they
+// are shared with a neighbouring chunk or with header bytes, which are
+// scattered in after the encoder) and stores every other payload word whole.
+__global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                     uint64_t nchunks, const uint32_t* __restrict__ chunk_bits,
+                                                     uint64_t* chunk_off, uint32_t* words) {
+    __shared__ uint64_t ex[1024];
+    __shared__ uint64_t wsum[16];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t g0 = (uint64_t)blockIdx.x * 1024, c = g0 + threadIdx.x;
+    const bool live = c < nchunks;
+    const uint32_t bits = live ? chunk_bits[c] : 0u;
+    // the array of a chunk: the last one whose first chunk is <= it (an array
+    // without chunks has its successor's first chunk)
+    auto array_of = [&](uint64_t x) {
+        uint32_t lo = 0, hi = narr - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (arrays[mid].chunk0 <= x) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
     };
-    if (threadIdx.x == 0) s_tk[0] = atomicAdd(lb, 1ull);
+    // the sum of the first array's chunks before the group
+    const uint64_t a0 = arrays[array_of(g0)].chunk0;
+    uint64_t pre = 0;
+    for (uint64_t x = a0 + threadIdx.x; x < g0; x += 1024) pre += chunk_bits[x];
+    pre = wave_sum(pre);
+    if (lane == 0) wsum[wave] = pre;
     __syncthreads();
-    uint64_t c = s_tk[0];
+    pre = 0;
+    for (int w = 0; w < 16; w++) pre += wsum[w];
+    __syncthreads();
+    // exclusive scan of the group's counts
+    uint64_t incl = bits;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t t = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint64_t excl = incl - bits;
+    for (uint32_t w = 0; w < wave; w++) excl += wsum[w];
+    ex[threadIdx.x] = excl;
+    __syncthreads();
+    if (!live) return;
+    const HuffArrayDev A = arrays[array_of(c)];
+    const uint64_t start = A.chunk0 > g0 ? A.chunk0 - g0 : 0;  // the array's first thread in the group
+    const uint64_t off = A.payload_bit + excl - ex[start] + (A.chunk0 < g0 ? pre : 0);
+    chunk_off[c] = off;
+    words[off >> 5] = 0;
+    if (bits) words[(off + bits - 1) >> 5] = 0;
+}
+
+// Encode pass: every chunk's codes at its bit position chunk_off[c] in the
+// stream buffer.  Persistent: workgroup g codes chunks g, g + G, ..., the
+// next chunk's symbols and position in flight while the current one is
+// coded.  short_codes (every code of the call fits 32 bits): the codes come
+// from code32 / len8 (a 32-bit code and a byte length per symbol of every
+// array, at its book offset), in LDS when the array's alphabet is at most
+// kLdsBook symbols.  Per chunk: the symbol indices are staged in LDS so that
+// thread t owns the 16 consecutive symbols 16 t .. 16 t + 15 (padded rows,
+// no bank conflicts); a workgroup scan of the per-thread bit counts gives
+// each thread's first bit; each code is then ORed into an LDS bit buffer at
+// its bit position (one or two LDS atomics per code, no branches on the
+// running bit count), and the buffer goes to the stream: interior words
+// stored, the two boundary words (zeroed by k_chunk_scan, shared with the
+// neighbouring chunks or header bytes) ORed in.  Codes longer than 32 bits
+// (short_codes false) come from the 64-bit book (code in bits 0..55, length
+// in bits 56..63) and are ORed straight into the stream.  Held to 128 VGPRs
+// so that four workgroups share a CU.
+template <bool short_codes>
+__global__ __launch_bounds__(256, 4) void k_encode_at(const int32_t* __restrict__ sym,
+                                                      const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                      uint64_t nchunks, const uint64_t* __restrict__ book,
+                                                      const uint32_t* __restrict__ code32,
+                                                      const uint8_t* __restrict__ len8,
+                                                      const uint64_t* __restrict__ chunk_off, uint32_t* words) {
+    __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
+    __shared__ uint32_t lcode[kLdsBook];
+    __shared__ uint8_t llen[kLdsBook];
+    __shared__ uint32_t s_scan[4];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t G = gridDim.x;
+    uint64_t c = blockIdx.x;
     if (c >= nchunks) return;
-    uint32_t k = array_of(c, 0);
+    uint32_t k = chunk_array(arrays, narr, c, 0);
     int32_t v[kEncPerThread];
-    if (short_codes) load_chunk(c, k, v);
+    if (short_codes) load_chunk(sym, arrays[k], c, v);
+    uint64_t gbit = chunk_off[c];
     int book_k = -1;
     while (true) {
-        if (threadIdx.x == 0) s_tk[1] = atomicAdd(lb, 1ull);  // the next chunk, read after the next barrier
         const HuffArrayDev A = arrays[k];
-        const uint64_t* b = book + A.book_off;
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
-        uint64_t e[kEncPerThread];
+        const uint64_t cn = c + G;
+        const uint32_t kn = cn < nchunks ? chunk_array(arrays, narr, cn, k) : k;
+        uint32_t code[kEncPerThread], len[kEncPerThread];
+        uint64_t e64[short_codes ? 1 : kEncPerThread];
         uint32_t mybits = 0;
-        uint64_t cn;
-        uint32_t kn;
+        const uint64_t gbit_next = cn < nchunks ? chunk_off[cn] : 0;
         if (short_codes) {
-            if (A.lds_book && book_k != (int)k) {
+            const uint32_t* C = code32 + A.book_off;
+            const uint8_t* L = len8 + A.book_off;
+            const bool lds = A.lds_book != 0;
+            if (lds && book_k != (int)k) {
                 for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) {
-                    const uint64_t be = b[i];
-                    lbook[i] = (uint32_t)(be & 0xFFFFFFu) | (book_len(be) << 24);
+                    lcode[i] = C[i];
+                    llen[i] = L[i];
                 }
                 book_k = (int)k;
             }
-            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free; next ticket in LDS
-            cn = s_tk[1];
-            kn = cn < nchunks ? array_of(cn, k) : k;
+            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free
 #pragma unroll
             for (int j = 0; j < kEncPerThread; j++) {
                 const uint32_t i = j * 256 + threadIdx.x;
-                uint32_t ent = 0;
-                if (i < n) {
-                    const int64_t idx = (int64_t)v[j] - A.offset;
-                    if (A.lds_book) {
-                        ent = lbook[idx];
-                    } else {
-                        // codes of 25..32 bits keep the symbol index, resolved after staging
-                        const uint64_t be = b[idx];
-                        ent = book_len(be) <= 24 ? ((uint32_t)(be & 0xFFFFFFu) | (book_len(be) << 24))
-                                                 : (0x80000000u | (uint32_t)idx);
-                    }
-                }
-                lbuf[i + i / kEncPerThread] = ent;
+                lbuf[i + i / kEncPerThread] = i < n ? (uint32_t)((int64_t)v[j] - A.offset) : 0xFFFFFFFFu;
             }
             // the next chunk's symbols stay in flight while this one is packed
             // (LDS-only barriers below do not wait for them)
-            if (cn < nchunks) load_chunk(cn, kn, v);
+            if (cn < nchunks) load_chunk(sym, arrays[kn], cn, v);
             lds_barrier();
 #pragma unroll
             for (int j = 0; j < kEncPerThread; j++) {
                 const uint32_t i = threadIdx.x * kEncPerThread + j;
-                const uint32_t ent = lbuf[i + i / kEncPerThread];
-                uint64_t v64;
-                if (ent & 0x80000000u) v64 = b[ent & 0x7FFFFFFFu];
-                else v64 = (uint64_t)(ent & 0xFFFFFFu) | ((uint64_t)(ent >> 24) << 56);
-                e[j] = v64;
-                mybits += book_len(v64);
+                const uint32_t idx = lbuf[i + i / kEncPerThread];
+                code[j] = 0;
+                len[j] = 0;
+                if (idx != 0xFFFFFFFFu) {
+                    code[j] = lds ? lcode[idx] : C[idx];
+                    len[j] = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+                }
+                mybits += len[j];
             }
         } else {
             lds_barrier();
-            cn = s_tk[1];
-            kn = cn < nchunks ? array_of(cn, k) : k;
+            const uint64_t* b = book + A.book_off;
             const uint32_t my0 = threadIdx.x * kEncPerThread;
             const int32_t* s = sym + A.src_off + first;
 #pragma unroll
             for (int j = 0; j < kEncPerThread; j++) {
                 const uint32_t i = my0 + j;
-                uint64_t v64 = 0;
-                if (i < n) v64 = b[(int64_t)s[i] - A.offset];
-                e[j] = v64;
-                mybits += book_len(v64);
+                uint64_t ent = 0;
+                if (i < n) ent = b[(int64_t)s[i] - A.offset];
+                e64[short_codes ? 0 : j] = ent;
+                mybits += book_len(ent);
             }
         }
         // exclusive scan of the per-thread bit counts
@@ -327,40 +373,19 @@ __global__ __launch_bounds__(256, 4) void k_encode(const int32_t* __restrict__ s
         for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
         const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
         const uint32_t my_off = wave_off + incl - mybits;
-        if (wave == 0) {
-            const uint64_t ex = lookback(status, c, A.chunk0, total, lane, err);
-            if (lane == 0) s_excl = ex;
-        }
-        lds_barrier();
-        const uint64_t gbit = A.payload_bit + s_excl;  // the chunk's first bit in the stream
         const uint32_t sh = (uint32_t)(gbit & 31);
         uint32_t* out = words + (gbit >> 5);
         if (short_codes) {
             const uint32_t nw = (sh + total + 31) / 32;
             for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lbuf[w] = 0;
             lds_barrier();
-            if (mybits) {
-                const uint32_t p0 = sh + my_off;
-                uint32_t word = p0 >> 5;
-                uint32_t nbits = p0 & 31;
-                const bool shared_first = nbits != 0;
-                bool first_word = true;
-                uint64_t accb = 0;
+            uint32_t p = sh + my_off;
 #pragma unroll
-                for (int j = 0; j < kEncPerThread; j++) {
-                    accb |= book_code(e[j]) << nbits;
-                    nbits += book_len(e[j]);
-                    if (nbits >= 32) {  // at most one whole word per code (nbits < 64)
-                        const uint32_t lo = (uint32_t)accb;
-                        if (first_word && shared_first) atomicOr(&lbuf[word], lo);
-                        else lbuf[word] = lo;
-                        first_word = false;
-                        word++;
-                        accb >>= 32;
-                        nbits -= 32;
-                    }
-                }
-                if (nbits) atomicOr(&lbuf[word], (uint32_t)accb);
+            for (int j = 0; j < kEncPerThread; j++) {
+                const uint32_t w = p >> 5, s2 = p & 31;
+                if (len[j]) atomicOr(&lbuf[w], code[j] << s2);
+                if (s2 + len[j] > 32) atomicOr(&lbuf[w + 1], code[j] >> (32 - s2));
+                p += len[j];
             }
             lds_barrier();
             for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
@@ -371,22 +396,25 @@ __global__ __launch_bounds__(256, 4) void k_encode(const int32_t* __restrict__ s
         } else {
             uint64_t p = (uint64_t)sh + my_off;
             for (int j = 0; j < kEncPerThread; j++) {
-                const uint32_t len = book_len(e[j]);
-                if (!len) continue;
-                const uint64_t code = book_code(e[j]);
+                const uint64_t ent = e64[short_codes ? 0 : j];
+                const uint32_t l = book_len(ent);
+                if (!l) continue;
+                const uint64_t cd = book_code(ent);
                 const uint32_t s2 = (uint32_t)(p & 31);
                 uint32_t* w = out + (p >> 5);
-                atomicOr(&w[0], (uint32_t)(code << s2));
-                if (s2 + len > 32) atomicOr(&w[1], (uint32_t)(code >> (32 - s2)));
-                if (s2 + len > 64) atomicOr(&w[2], (uint32_t)(code >> (64 - s2)));
-                p += len;
+                atomicOr(&w[0], (uint32_t)(cd << s2));
+                if (s2 + l > 32) atomicOr(&w[1], (uint32_t)(cd >> (32 - s2)));
+                if (s2 + l > 64) atomicOr(&w[2], (uint32_t)(cd >> (64 - s2)));
+                p += l;
             }
         }
         if (cn >= nchunks) break;
         c = cn;
         k = kn;
+        gbit = gbit_next;
     }
 }
+
 }  // namespace dev
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
@@ -414,28 +442,33 @@ void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64
     hipLaunchKernelGGL(dev::k_hist_tail, dim3(nfields), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch,
                        part_stride, g_stride, h_stride);
 }
-uint64_t encode_lb_words(uint64_t nchunks) { return 2 * (nchunks + 2); }
-
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
-                   const uint64_t* book, uint32_t* stream_words, uint32_t* lb_words, int max_len, hipStream_t st) {
-    if (!nchunks) return;
-    // one resident wave of workgroups: CUs x workgroups that fit on a CU
-    int dev_id = 0, cus = 0, per_cu = 0;
+                   const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
+                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, hipStream_t st) {
+    if (!nchunks || !narr) return;
+    // one resident wave of workgroups for each persistent pass: CUs x workgroups that fit on a CU
+    int dev_id = 0, cus = 0, per_cu = 0, per_cu_bits = 0;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, dev::k_chunk_bits, 256, 0);
     if (max_len <= 32)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode<true>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_at<true>, 256, 0);
     else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode<false>, 256, 0);
-    const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
-    const uint64_t groups = std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
-    unsigned long long* lb = reinterpret_cast<unsigned long long*>(lb_words);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_at<false>, 256, 0);
+    auto groups = [&](int pc) {
+        const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(pc, 1);
+        return (uint32_t)std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
+    };
+    hipLaunchKernelGGL(dev::k_chunk_bits, dim3(groups(per_cu_bits)), dim3(256), 0, st, sym, arrays, narr, nchunks,
+                       len8, chunk_bits);
+    hipLaunchKernelGGL(dev::k_chunk_scan, dim3((uint32_t)((nchunks + 1023) / 1024)), dim3(1024), 0, st, arrays, narr,
+                       nchunks, chunk_bits, chunk_off, stream_words);
     if (max_len <= 32)
-        hipLaunchKernelGGL(dev::k_encode<true>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr, nchunks,
-                           book, stream_words, lb);
+        hipLaunchKernelGGL(dev::k_encode_at<true>, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks,
+                           book, code32, len8, chunk_off, stream_words);
     else
-        hipLaunchKernelGGL(dev::k_encode<false>, dim3((uint32_t)groups), dim3(256), 0, st, sym, arrays, narr, nchunks,
-                           book, stream_words, lb);
+        hipLaunchKernelGGL(dev::k_encode_at<false>, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr,
+                           nchunks, book, code32, len8, chunk_off, stream_words);
 }
 
 }  // namespace asz

@@ -65,7 +65,7 @@ struct RoundDesc {
     uint32_t tab_off[3];// offsets of the cnt/pre tables of each axis in the table buffer
     uint32_t d, order;
     uint64_t dist;      // h * prod_{a > d} D[a]
-    uint64_t qb_off;    // offset of this round's array in the concatenated QB buffer
+    uint64_t qb_off;    // offset of this round's array in the QB buffer (a multiple of kSymAlign)
     uint64_t count;     // targets of this round (= C0*C1*C2)
     float eb;           // layer error bound
     float layer_slope;  // layer default slope (compress.tpp:195)
@@ -175,6 +175,11 @@ struct Partial {
     int32_t qmin, qmax;
 };
 
+// Arrays of the compression symbol buffer (t array, pivots, each round)
+// start at multiples of kSymAlign elements.
+constexpr uint64_t kSymAlign = 64;
+inline uint64_t sym_align(uint64_t n) { return (n + kSymAlign - 1) / kSymAlign * kSymAlign; }
+
 struct Plan {
     Geometry g;
     std::vector<RoundDesc> rounds;
@@ -185,6 +190,7 @@ struct Plan {
     uint64_t total_partials = 0;   // per-workgroup partials over all rounds
     uint64_t total_hist_words = 0; // per-workgroup histogram windows over all rounds
     uint64_t pivot_count;
+    uint64_t qb_total;             // QB buffer elements: pivots and rounds, each array aligned (kSymAlign)
     uint32_t pn[3];                // pivot lattice counts per axis
     float pivot_eb;
 };

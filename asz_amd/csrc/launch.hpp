@@ -133,13 +133,12 @@ struct HuffArrayDev {
     uint64_t chunk0;      // first encode chunk of this array
     uint64_t nchunk;      // encode chunks (0 for an empty or single-symbol array)
     uint64_t payload_bit; // absolute bit position of the payload in the stream
-    uint32_t lds_book;    // codebook copied to LDS by the encoder (range <= kLdsBook, codes <= 24 bits)
+    uint32_t lds_book;    // codebook copied to LDS by the encoder (range <= kLdsBook)
     uint32_t pad;
 };
 constexpr uint32_t kHistChunk = 16384;   // symbols per histogram workgroup
 constexpr uint32_t kEncChunk = 4096;     // symbols per encode workgroup (256 threads x 16)
 constexpr uint32_t kLdsBook = 1024;      // LDS codebook entries of the encoder
-constexpr uint64_t kEncGroups = 2048;    // workgroups of the persistent encoder
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
                  uint32_t* hist, hipStream_t st);
@@ -184,14 +183,17 @@ void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
                       uint32_t* hres, uint32_t epoch, hipStream_t st, uint32_t nfields = 1, uint64_t part_stride = 0,
                       uint64_t g_stride = 0, uint64_t h_stride = 0);
-// Single-pass encoder of the arrays' chunks straight into the stream buffer
-// (which holds the header bytes and is zero elsewhere).  lb_words: the
-// encoder's ticket counter, error flag and per-chunk look-back status,
-// encode_lb_words(nchunks) 32-bit words that must be zero at the launch; a
-// nonzero error flag (word 2) afterwards means a look-back wait gave up.
-uint64_t encode_lb_words(uint64_t nchunks);
+// Huffman encoder of the arrays' chunks straight into the stream buffer
+// (header bytes are scattered in after it has run, with the same stream):
+// a bit-count pass into chunk_bits, a scan into chunk_off (which also zeroes
+// the payload words the encoder ORs), and the encode pass.  The codebook of
+// array k starts at its book_off in three parallel tables: book (code in bits
+// 0..55, length in bits 56..63), code32 (the code, when max_len <= 32) and
+// len8 (the length).  chunk_bits / chunk_off: nchunks entries.
+constexpr uint64_t kEncGroups = 2048;    // most workgroups of a persistent encoder pass
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
-                   const uint64_t* book, uint32_t* stream_words, uint32_t* lb_words, int max_len, hipStream_t st);
+                   const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
+                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, hipStream_t st);
 
 // Decoder (self-synchronising parallel decode of one bit stream per array).
 struct DecArrayDev {

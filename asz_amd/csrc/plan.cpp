@@ -62,7 +62,9 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
     P.ss.assign((uint64_t)g.L * nbl, 0);
     P.ns.assign((uint64_t)g.L * nbl, 0);
     uint64_t s[3] = {g.pivot_stride[0], g.pivot_stride[1], g.pivot_stride[2]};
-    uint64_t qb_off = P.pivot_count;
+    // every array starts on a 256-byte boundary of the symbol buffer (whole
+    // cache lines for the kernels that write and read it)
+    uint64_t qb_off = sym_align(P.pivot_count), qb_sum = P.pivot_count;
     for (int r = 0; r < g.L; r++) {
         const int d = g.axis[r];
         RoundDesc R{};
@@ -102,7 +104,8 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         }
         R.count = count;
         R.qb_off = qb_off;
-        qb_off += count;
+        qb_off = sym_align(qb_off + count);
+        qb_sum += count;
         // kernel shape: register-window sweep for rounds along axis 0/1, row walk along DIM2 otherwise
         R.kind = d == 2 ? kRoundRow : kRoundSweep;
         // targets per block along d; any value above the target count gives block 0
@@ -183,7 +186,8 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         P.rounds.push_back(R);
         s[d] /= 2;
     }
-    if (qb_off != g.N()) fail(ASZ_ERR_DATA, "total_qb_num!=data.size()");
+    if (qb_sum != g.N()) fail(ASZ_ERR_DATA, "total_qb_num!=data.size()");
+    P.qb_total = qb_off;
     set_plan_bounds(P, abs_eb, pivot_eb_ratio, eb_ratio, slope_ratio, default_slope);
     return P;
 }

@@ -314,39 +314,42 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
                 book[H[k].book_off + s] = A[k].code[s] | ((uint64_t)A[k].len[s] << 56);
                 alen = std::max<int>(alen, A[k].len[s]);
             }
-            H[k].lds_book = (A[k].range <= kLdsBook && alen <= 24) ? 1 : 0;
+            H[k].lds_book = A[k].range <= kLdsBook ? 1 : 0;
             max_len = std::max(max_len, alen);
         }
         uint8_t* hbytes = nullptr;
         if (total_bytes) {
+            // codebooks as three parallel tables (64-bit entries, 32-bit codes, byte lengths)
             const uint64_t o_seg = 0, o_blob = al64(bsegs.size() * sizeof(ByteSeg)), o_book = al64(o_blob + blob.size()),
-                           o_arr = al64(o_book + book_total * 8), up = o_arr + ncand * sizeof(HuffArrayDev);
+                           o_code = al64(o_book + book_total * 8), o_len = al64(o_code + book_total * 4),
+                           o_arr = al64(o_len + book_total), up = o_arr + ncand * sizeof(HuffArrayDev);
             uint8_t* hu = c->pin_huff.get<uint8_t>(up);
             std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
             std::memcpy(hu + o_blob, blob.data(), blob.size());
             std::memcpy(hu + o_book, book.data(), book_total * 8);
+            for (uint64_t e = 0; e < book_total; e++) {
+                reinterpret_cast<uint32_t*>(hu + o_code)[e] = (uint32_t)book[e];
+                hu[o_len + e] = (uint8_t)(book[e] >> 56);
+            }
             std::memcpy(hu + o_arr, H.data(), ncand * sizeof(HuffArrayDev));
             uint8_t* du = c->hmeta.get<uint8_t>(up);
             HIPCHECK(hipMemcpyAsync(du, hu, up, hipMemcpyHostToDevice, st));
-            // the stream words, then the encoder's look-back words (one fill zeroes both)
-            const uint64_t nwords = ((total_bytes + 3) / 4 + 4 + 63) / 64 * 64;
-            uint32_t* dwords = c->words.get<uint32_t>(nwords + encode_lb_words(chunks));
-            launch_fill_i32(reinterpret_cast<int32_t*>(dwords), nwords + encode_lb_words(chunks), 0, st);
-            launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
-                                 reinterpret_cast<uint8_t*>(dwords), st);
+            // payload words written (or zeroed and ORed) by the encoder, then the header bytes
+            const uint64_t nwords = (total_bytes + 3) / 4 + 4;
+            uint32_t* dwords = c->words.get<uint32_t>(nwords);
             if (chunks) {
                 const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-                launch_encode(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book), dwords,
-                              dwords + nwords, max_len, st);
+                launch_encode(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book),
+                              reinterpret_cast<const uint32_t*>(du + o_code), du + o_len,
+                              c->chunk_bits.get<uint32_t>(chunks + 1), c->chunk_off.get<uint64_t>(chunks + 1), dwords,
+                              max_len, st);
             }
+            launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
+                                 reinterpret_cast<uint8_t*>(dwords), st);
             HIPCHECK(hipGetLastError());
             hbytes = c->pin_stream.get<uint8_t>(total_bytes + 1);
             HIPCHECK(hipMemcpyAsync(hbytes, dwords, total_bytes, hipMemcpyDeviceToHost, st));
-            uint32_t* enc_err = c->pin_flag.get<uint32_t>(1);
-            *enc_err = 0;
-            if (chunks) HIPCHECK(hipMemcpyAsync(enc_err, dwords + nwords + 2, 4, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
-            if (*enc_err) fail(ASZ_ERR_HIP, "Huffman encoder: a chunk never received its bit position");
         }
         // Zstd level 3 of each candidate's bytes (generate_config.tpp:639-641), one host thread each
         std::vector<uint64_t> zlen(ncand, 0);
