@@ -1043,7 +1043,9 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     S = asz_stats{};
     // the stream goes up while the host parses its headers and builds the
     // decode tables
-    const uint64_t nwords = (n + 3) / 4 + 4;
+    // zero words after the stream: a write-pass thread parses up to 31
+    // codewords of at most 64 bits past its own symbols
+    const uint64_t nwords = (n + 3) / 4 + 128;
     uint32_t* dwords = c->words.get<uint32_t>(nwords);
     launch_fill_i32(reinterpret_cast<int32_t*>(dwords + n / 4), nwords - n / 4, 0, st);
     HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
@@ -1060,10 +1062,14 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     S.t_count = Q.nt;
     S.stream_bytes = n;
 
-    // symbols: t array (Q.nt) then the L+1 QB arrays as stored
+    // symbols: t array (Q.nt) then the L+1 QB arrays as stored, each array
+    // starting on a kSymAlign boundary (the decoder stores whole cache lines)
     uint64_t total_sym = 0;
     std::vector<uint64_t> out_off(L + 2);
-    for (uint64_t k = 0; k < L + 2; k++) { out_off[k] = total_sym; total_sym += Q.counts[k]; }
+    for (uint64_t k = 0; k < L + 2; k++) {
+        out_off[k] = total_sym;
+        total_sym = sym_align(total_sym + Q.counts[k]);
+    }
     int32_t* sym = c->sym.get<int32_t>(total_sym + 16);
     // decode tables and descriptors
     std::vector<uint32_t> tables;
@@ -1111,76 +1117,99 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     // flags[0]: error bits of the write pass; flags[8 + i]: "changed" of fix pass i of a group
     constexpr uint32_t kFixGroup = 6;
     unsigned int* flags = c->misc.get<unsigned int>(8 + kFixGroup);
-    launch_fill_i32(reinterpret_cast<int32_t*>(flags), 8 + kFixGroup, 0, st);
-    c->kev_n = 0;
-    HIPCHECK(hipEventRecord(c->ev[0], st));
-    int kd = kev_begin(c, KEV_DECODE, 0);
-    launch_dec_sync(dwords, dda, dwg, nwg, dtab, dstart, dex0, dcnt, st);
-    // fix passes in groups, one synchronisation per group: a pass after
-    // convergence changes nothing (and its workgroups skip the table load)
-    uint64_t iters = 0;
-    for (;;) {
-        if (iters) launch_fill_i32(reinterpret_cast<int32_t*>(flags + 8), kFixGroup, 0, st);
-        for (uint32_t i = 0; i < kFixGroup; i++) {
-            launch_dec_fix(dwords, dda, dwg, nwg, dtab, dstart, dex0, dex1, dcnt, flags + 8 + i, st);
-            std::swap(dex0, dex1);
-        }
-        unsigned int changed[kFixGroup];
-        HIPCHECK(hipMemcpyAsync(changed, flags + 8, sizeof(changed), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        iters += kFixGroup;
-        if (!changed[kFixGroup - 1]) break;
-        if (iters > nsub + 2) fail(ASZ_ERR_DATA, "Huffman decoder did not synchronise");
-    }
-    launch_dec_offsets(dda, (uint32_t)da.size(), nwg, dcnt, dwgs, dwgs + nwg + 1, flags, st);
-    launch_dec_write(dwords, dda, dwg, nwg, dtab, dstart, dcnt, dwgs + nwg + 1, sym, flags, st);
-    kev_end(c, kd);
-    HIPCHECK(hipEventRecord(c->ev[1], st));
-    unsigned int derr = 0;
-    HIPCHECK(hipMemcpyAsync(&derr, flags, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (derr & 1) fail(ASZ_ERR_DATA, "Huffman payload length does not match its value count");
-    if (derr & 2) fail(ASZ_ERR_DATA, "Huffman payload contains too many values");
-    if (derr & 4) fail(ASZ_ERR_DATA, "Huffman payload ends inside a code");
-
-    // reconstruction: t floats, pivots, rounds (block-major QB order within each round array)
+    // reconstruction inputs (independent of the decoded symbols)
     uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
     HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
     float* tblk = c->tblk.get<float>(nt + 1);
     SweepSeg* dsegs = c->segs.get<SweepSeg>(P.segs.size() + 1);
     if (!P.segs.empty())
         HIPCHECK(hipMemcpyAsync(dsegs, P.segs.data(), P.segs.size() * sizeof(SweepSeg), hipMemcpyHostToDevice, st));
-    HIPCHECK(hipEventRecord(c->ev[2], st));
-    launch_t_float(sym, nbl, (uint32_t)L, (float)p.t_precision, tblk, st);
-    PivotDesc PD = pivot_desc(P);
-    int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, st);
-    kev_end(c, kp);
     uint64_t stage_max = 0;
     for (uint64_t r = 0; r < L; r++)
         if (round_needs_staging(P.rounds[r])) stage_max = std::max(stage_max, P.rounds[r].count);
     int32_t* stage = stage_max ? c->stage.get<int32_t>(stage_max + 1) : nullptr;
-    for (uint64_t r = 0; r < L; r++) {
-        RoundDesc R = P.rounds[r];
-        check_grid(R, 1);
-        R.qb_off = 0;
-        int kr = kev_begin(c, KEV_ROUND, (int)r);
-        if (round_needs_staging(R)) {
-            // the round's QBs back into "012" order, then the round reads them there
-            uint32_t cmax[3];
-            round_cmax(P, R, cmax);
-            launch_qb_order(R, tabs, cmax, stage, sym + out_off[r + 2], 0, 0, 1, false, st);
-            R.order = 2;
-            launch_round_decompress(R, Q.quintic, d_out, stage, tabs, tblk + r * nbl, dsegs, st);
-        } else {
-            launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, dsegs, st);
+    PivotDesc PD = pivot_desc(P);
+    // host slots: the last fix pass's "changed" flag and the decoder's error bits
+    uint32_t* hflag = c->pin_range.get<uint32_t>(2);
+    launch_fill_i32(reinterpret_cast<int32_t*>(flags), 8 + kFixGroup, 0, st);
+    HIPCHECK(hipEventRecord(c->ev[0], st));
+    launch_dec_sync(dwords, dda, dwg, nwg, dtab, dstart, dex0, dcnt, st);
+    // The whole pipeline is enqueued behind one group of fix passes and the
+    // host waits once, at the end: parses from different starts merge within
+    // a few codewords, so a group nearly always reaches the fixed point (a
+    // pass after convergence changes nothing and its workgroups skip the table
+    // load).  Every parse stays inside its subsequence's bits, so a pipeline
+    // run before convergence reads and writes in bounds; if the group's last
+    // pass still changed an exit, further groups run (one host wait each)
+    // until none does, and then the pipeline runs again.
+    auto fix_group = [&] {
+        launch_fill_i32(reinterpret_cast<int32_t*>(flags + 8), kFixGroup, 0, st);
+        for (uint32_t i = 0; i < kFixGroup; i++) {
+            launch_dec_fix(dwords, dda, dwg, nwg, dtab, dstart, dex0, dex1, dcnt, flags + 8 + i, st);
+            std::swap(dex0, dex1);
         }
-        kev_end(c, kr);
+        HIPCHECK(hipMemcpyAsync(hflag, flags + 8 + kFixGroup - 1, 4, hipMemcpyDeviceToHost, st));
+    };
+    auto pipeline = [&] {
+        c->kev_n = 0;
+        int kd = kev_begin(c, KEV_DECODE, 0);
+        launch_fill_i32(reinterpret_cast<int32_t*>(flags), 1, 0, st);
+        launch_dec_offsets(dda, (uint32_t)da.size(), dwg, nwg, dcnt, dwgs, dwgs + nwg + 1, flags, st);
+        launch_dec_write(dwords, dda, dwg, nwg, dtab, dstart, dcnt, dwgs + nwg + 1, sym, flags, st);
+        kev_end(c, kd);
+        HIPCHECK(hipMemcpyAsync(hflag + 1, flags, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(c->ev[1], st));
+        // reconstruction: t floats, pivots, rounds (block-major QB order within each round array)
+        HIPCHECK(hipEventRecord(c->ev[2], st));
+        launch_t_float(sym, nbl, (uint32_t)L, (float)p.t_precision, tblk, st);
+        int kp = kev_begin(c, KEV_PIVOT, 0);
+        launch_pivots(false, d_out, sym + out_off[1], PD, nullptr, st);
+        kev_end(c, kp);
+        for (uint64_t r = 0; r < L; r++) {
+            RoundDesc R = P.rounds[r];
+            check_grid(R, 1);
+            R.qb_off = 0;
+            int kr = kev_begin(c, KEV_ROUND, (int)r);
+            if (round_needs_staging(R)) {
+                // the round's QBs back into "012" order, then the round reads them there
+                uint32_t cmax[3];
+                round_cmax(P, R, cmax);
+                launch_qb_order(R, tabs, cmax, stage, sym + out_off[r + 2], 0, 0, 1, false, st);
+                R.order = 2;
+                launch_round_decompress(R, Q.quintic, d_out, stage, tabs, tblk + r * nbl, dsegs, st);
+            } else {
+                launch_round_decompress(R, Q.quintic, d_out, sym + out_off[r + 2], tabs, tblk + r * nbl, dsegs, st);
+            }
+            kev_end(c, kr);
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(c->ev[3], st));
+        HIPCHECK(hipStreamSynchronize(st));
+    };
+    fix_group();
+    pipeline();
+    if (hflag[0]) {
+        uint64_t iters = kFixGroup;
+        do {
+            iters += kFixGroup;
+            if (iters > nsub + 2) fail(ASZ_ERR_DATA, "Huffman decoder did not synchronise");
+            fix_group();
+            HIPCHECK(hipStreamSynchronize(st));
+        } while (hflag[0]);
+        pipeline();
     }
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(c->ev[3], st));
-    if (qb_out)
-        HIPCHECK(hipMemcpyAsync(qb_out, sym + out_off[1], (total_sym - out_off[1]) * 4, hipMemcpyDeviceToHost, st));
+    const unsigned int derr = hflag[1];
+    if (derr & 1) fail(ASZ_ERR_DATA, "Huffman payload length does not match its value count");
+    if (derr & 2) fail(ASZ_ERR_DATA, "Huffman payload contains too many values");
+    if (derr & 4) fail(ASZ_ERR_DATA, "Huffman payload ends inside a code");
+    if (qb_out) {  // the pivot and round arrays, contiguous on the host
+        uint64_t o = 0;
+        for (uint64_t k = 1; k < L + 2; k++) {
+            if (Q.counts[k]) HIPCHECK(hipMemcpyAsync(qb_out + o, sym + out_off[k], Q.counts[k] * 4,
+                                                     hipMemcpyDeviceToHost, st));
+            o += Q.counts[k];
+        }
+    }
     std::vector<int32_t> t32;
     if (t_out) {
         t32.resize(Q.nt);

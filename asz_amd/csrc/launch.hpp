@@ -217,8 +217,8 @@ void launch_dec_sync(const uint32_t* words, const DecArrayDev* arrays, const uin
 void launch_dec_fix(const uint32_t* words, const DecArrayDev* arrays, const uint32_t* wg_arr, uint64_t nwg,
                     const uint32_t* tables, uint64_t* start, const uint64_t* exit_in, uint64_t* exit_out,
                     uint32_t* count, unsigned int* changed, hipStream_t st);
-void launch_dec_offsets(const DecArrayDev* arrays, uint32_t narrays, uint64_t nwg, const uint32_t* count,
-                        uint64_t* wg_sum, uint64_t* wg_off, unsigned int* err, hipStream_t st);
+void launch_dec_offsets(const DecArrayDev* arrays, uint32_t narrays, const uint32_t* wg_arr, uint64_t nwg,
+                        const uint32_t* count, uint64_t* wg_sum, uint64_t* wg_off, unsigned int* err, hipStream_t st);
 void launch_dec_write(const uint32_t* words, const DecArrayDev* arrays, const uint32_t* wg_arr, uint64_t nwg,
                       const uint32_t* tables, const uint64_t* start, const uint32_t* count, const uint64_t* wg_off,
                       int32_t* out, unsigned int* err, hipStream_t st);
