@@ -94,7 +94,7 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_free",
            "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_compress_field", "asz_prepare_round", "asz_round_energy",
            "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
-           "asz_select_family")
+           "asz_select_family", "asz_range_f32_noalloc")
 
 _lib = None
 
@@ -125,6 +125,8 @@ def lib():
     L.asz_copy_to_host.argtypes = [_P, _P, _P, _U64]
     L.asz_geometry.argtypes = [_P, ctypes.POINTER(Plan), _U64, _P, _P, _P]
     L.asz_range_f32.argtypes = [_P, _P, _U64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.asz_range_f32_noalloc.argtypes = [_P, _P, _U64, ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float)]
     L.asz_default_slope.argtypes = [_P, _P, _P, ctypes.POINTER(Params), ctypes.c_float, ctypes.c_float,
                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]

@@ -158,6 +158,8 @@ static float key_to_float(uint32_t k) {
     return f;
 }
 
+void range_result(asz_ctx* c, const uint32_t* w, const float* d_field, float* mn, float* mx, const std::string& who);
+
 // Range scans of nf fields: one launch (one per field when the sizes
 // differ), one copy back, one synchronisation.  names (may be null): the
 // fields' indices, named in the error messages.
@@ -182,25 +184,29 @@ static void range_scan_fields(asz_ctx* c, uint32_t nf, const float* const* d_fie
     HIPCHECK(hipMemcpyAsync(res.data(), r, nw * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     for (uint32_t f = 0; f < nf; f++) {
-        const uint32_t* w = res.data() + kRangeWords * f;
         const std::string who = names ? "field " + std::to_string(names[f]) + ": " : std::string();
-        if (w[2]) fail(ASZ_ERR_DATA, who + "input contains a nonfinite value");
-        float vmin = key_to_float(w[0]), vmax = key_to_float(w[1]);
-        // ties among zeros / denormals under DAZ: minmax_element keeps the first
-        // smallest and the last largest element, with their own bits
-        uint64_t zi[2];
-        std::memcpy(zi, w + 4, 16);
-        if ((vmin == 0.0f || vmax == 0.0f) && zi[0] != ~0ull) {
-            if (vmin == 0.0f)
-                HIPCHECK(hipMemcpyAsync(&vmin, d_fields[f] + zi[0], 4, hipMemcpyDeviceToHost, c->stream));
-            if (vmax == 0.0f)
-                HIPCHECK(hipMemcpyAsync(&vmax, d_fields[f] + zi[1], 4, hipMemcpyDeviceToHost, c->stream));
-            HIPCHECK(hipStreamSynchronize(c->stream));
-        }
-        if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, who + "max_val==min_val");
-        mn[f] = vmin;
-        mx[f] = vmax;
+        range_result(c, res.data() + kRangeWords * f, d_fields[f], mn + f, mx + f, who);
     }
+}
+
+// The range of one field from its kRangeWords result words (launch_range),
+// as load_input finds it (s1_compress.cpp:128-137); who prefixes the error
+// messages.
+void range_result(asz_ctx* c, const uint32_t* w, const float* d_field, float* mn, float* mx, const std::string& who) {
+    if (w[2]) fail(ASZ_ERR_DATA, who + "input contains a nonfinite value");
+    float vmin = key_to_float(w[0]), vmax = key_to_float(w[1]);
+    // ties among zeros / denormals under DAZ: minmax_element keeps the first
+    // smallest and the last largest element, with their own bits
+    uint64_t zi[2];
+    std::memcpy(zi, w + 4, 16);
+    if ((vmin == 0.0f || vmax == 0.0f) && zi[0] != ~0ull) {
+        if (vmin == 0.0f) HIPCHECK(hipMemcpyAsync(&vmin, d_field + zi[0], 4, hipMemcpyDeviceToHost, c->stream));
+        if (vmax == 0.0f) HIPCHECK(hipMemcpyAsync(&vmax, d_field + zi[1], 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    if (daz(vmin) == daz(vmax)) fail(ASZ_ERR_DATA, who + "max_val==min_val");
+    *mn = vmin;
+    *mx = vmax;
 }
 
 static void range_scan(asz_ctx* c, const float* d_field, uint64_t n, float* mn, float* mx) {
@@ -1263,6 +1269,7 @@ int asz_ctx_create(int device, asz_ctx** out) {
         if (device < 0 || device >= n) fail(ASZ_ERR_ARG, "no such HIP device");
         HIPCHECK(hipSetDevice(device));
         c->device = device;
+        asz::noalloc_setup(device);
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
     });

@@ -157,6 +157,12 @@ inline TaskPool& host_pool(asz_ctx* c) {
     return *c->pool;
 }
 
+// The range of one field from its launch_range result words (codec.cpp).
+void range_result(asz_ctx* c, const uint32_t* w, const float* d_field, float* mn, float* mx, const std::string& who);
+// Resolves, once per device (current), the address of the allocation-free
+// range scan's scratch array (noalloc.hip).
+void noalloc_setup(int device);
+
 inline float abs_eb_of(float rel_eb, float mn, float mx) {
     float range = daz(daz(mx) - daz(mn));
     return daz(daz(rel_eb) * range);

@@ -157,6 +157,16 @@ int asz_round_layout(const uint64_t dims[3], const asz_plan *plan, uint64_t bloc
  * Fails with ASZ_ERR_DATA on a non-finite value or max == min. */
 int asz_range_f32(asz_ctx *ctx, const float *d_field, uint64_t n, float *min_val, float *max_val);
 
+/* Allocation-free form of asz_range_f32.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc).  The same scan and results; it calls no hipMalloc,
+ * hipFree or other allocating routine.  Its only device memory beyond
+ * d_field is the scan's result words (32 bytes), kept in a device array of
+ * the library on every GPU it runs on until the process ends; their size
+ * does not depend on n, so no field is too long for them.  Calls that
+ * overlap on one GPU (from several contexts or threads) race on that array. */
+int asz_range_f32_noalloc(asz_ctx *ctx, const float *d_field, uint64_t n, float *min_val, float *max_val);
+
 /* ------------------------------------------------------------------ default slope
  * param_c::get_default_slope (get_default_slope.tpp:5-83): AB keys sampled on
  * the GPU, scored on the host in the reference's order.  Writes the slope and
