@@ -82,7 +82,7 @@ struct RoundDesc {
     uint32_t hist_win;  // bins of each workgroup's stored (partial) window (power of 2)
     uint32_t lds_win;   // bins of each workgroup's LDS histogram (power of 2, >= hist_win)
     uint32_t layer;     // index of this round in the plan
-    uint32_t pad;
+    uint32_t xsh;       // sweep: log2 of a workgroup's lanes along DIM2 (6..8); 256 >> xsh lattice rows each
     FastDiv tpb;        // targets per full block along the round axis, B[d] / (2h)
 };
 

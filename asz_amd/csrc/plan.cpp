@@ -129,7 +129,12 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             R.nseg = (uint32_t)P.segs.size() - R.seg_off;
         }
         if (R.kind == kRoundSweep) {
-            R.nwg = (uint64_t)((R.n[2] + 255) / 256) * R.n[1 - d] * R.nseg;
+            // a workgroup spans the next power of two of the DIM2 targets (64 to 256
+            // lanes) and as many lattice rows of the other axis as fill its 256 lanes
+            R.xsh = 6;
+            while (R.xsh < 8 && (1u << R.xsh) < R.n[2]) R.xsh++;
+            const uint32_t rows = 256u >> R.xsh;
+            R.nwg = (uint64_t)((R.n[2] + (1u << R.xsh) - 1) >> R.xsh) * ((R.n[1 - d] + rows - 1) / rows) * R.nseg;
         } else {
             // about 4096 targets per workgroup, but at least ~2048 workgroups when the round has the rows
             const uint64_t rows = (uint64_t)R.n[0] * R.n[1];

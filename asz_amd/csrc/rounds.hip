@@ -447,10 +447,11 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
         if (WMODE != 1) __syncthreads();
     }
     const SweepSeg S = segs[R.seg_off + sz];
-    const uint32_t k2 = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t ko = blockIdx.y;
-    const bool valid = k2 < R.n[2];
-    const uint32_t kk2 = valid ? k2 : 0;
+    // lanes: 1 << xsh consecutive DIM2 columns (whole waves) by 256 >> xsh rows of axis OA
+    const uint32_t k2 = (blockIdx.x << R.xsh) + (threadIdx.x & ((1u << R.xsh) - 1));
+    const uint32_t ko = blockIdx.y * (256u >> R.xsh) + (threadIdx.x >> R.xsh);
+    const bool valid = k2 < R.n[2] && ko < R.n[OA];
+    const uint32_t kk2 = valid ? k2 : 0, kko = valid ? ko : 0;
     const uint32_t h = R.h;
 
     // block coordinates and local indices of this lane's column
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     b[AX] = S.b;
     j[AX] = S.k0 - S.b * (R.B[AX] / (2 * h));
     side_loc(R.s[2] * kk2, kk2, R.B[2], R.s[2], b[2], j[2]);
-    side_loc(R.s[OA] * ko, ko, R.B[OA], R.s[OA], b[OA], j[OA]);
+    side_loc(R.s[OA] * kko, kko, R.B[OA], R.s[OA], b[OA], j[OA]);
     uint32_t cnt[3], pre[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     const float t = tblk[((uint64_t)b[0] * R.nb[1] + b[1]) * R.nb[2] + b[2]];
     const uint64_t sd = AX == 0 ? (uint64_t)R.D[1] * R.D[2] : (uint64_t)R.D[2];
     const uint64_t so = OA == 0 ? (uint64_t)R.D[1] * R.D[2] : (uint64_t)R.D[2];
-    const uint64_t col = (uint64_t)(R.s[OA] * ko) * so + (uint64_t)R.s[2] * kk2;
+    const uint64_t col = (uint64_t)(R.s[OA] * kko) * so + (uint64_t)R.s[2] * kk2;
     const int64_t Dd = R.D[AX];
 
     // QB slot of target c of the segment
@@ -479,14 +480,14 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
         jj[AX] += c;
         return base + local_slot(R.order, jj, cnt);
     };
+    constexpr uint64_t kNoLane = ~0ull;  // lbase of a lane past the round's targets
     if (WMODE == 1) {
-        lbase[wave][lane] = slot_of(0);
+        lbase[wave][lane] = valid ? slot_of(0) : kNoLane;
         if (DEC) {
             __syncthreads();
             for (uint32_t e = lane; e < 64 * S.cnt; e += 64) {
                 const uint32_t jl = e / S.cnt, c = e - jl * S.cnt;
-                const uint32_t kl = blockIdx.x * blockDim.x + wave * 64 + jl;
-                if (kl < R.n[2]) tile[wave][c][jl] = qb[lbase[wave][jl] + c];
+                if (lbase[wave][jl] != kNoLane) tile[wave][c][jl] = qb[lbase[wave][jl] + c];
             }
         }
         __syncthreads();
@@ -545,8 +546,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
             __syncthreads();
             for (uint32_t e = lane; e < 64 * S.cnt; e += 64) {
                 const uint32_t jl = e / S.cnt, c = e - jl * S.cnt;
-                const uint32_t kl = blockIdx.x * blockDim.x + wave * 64 + jl;
-                if (kl < R.n[2]) qb[lbase[wave][jl] + c] = tile[wave][c][jl];
+                if (lbase[wave][jl] != kNoLane) qb[lbase[wave][jl] + c] = tile[wave][c][jl];
             }
         }
         finish_wg(err, sq, qmin, qmax, R, blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * sz),
@@ -728,7 +728,8 @@ void row_order(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
 
 template <int AX, bool Q, bool DEC, int WM>
 void sweep(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
-    dim3 grid((R.n[2] + 255) / 256, R.n[1 - AX], R.nseg * a.nf);
+    const uint32_t rows = 256u >> R.xsh;
+    dim3 grid((R.n[2] + (1u << R.xsh) - 1) >> R.xsh, (R.n[1 - AX] + rows - 1) / rows, R.nseg * a.nf);
     hipLaunchKernelGGL((dev::k_round_sweep<AX, Q, DEC, WM>), grid, dim3(256), DEC ? 0 : R.lds_win * 4, st, R, a.w,
                        a.qb, a.tabs, a.tblk, a.segs, a.part, a.phist, a.gwin, a.fs);
 }
