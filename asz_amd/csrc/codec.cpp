@@ -937,8 +937,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     HIPCHECK(hipEventRecord(c->ev[4], st));
     int ke = kev_begin(c, KEV_ENCODE, 0);
     const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
+    uint32_t* estage = max_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(max_len)) : nullptr;
     launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
-                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, max_len, st);
+                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, max_len, estage,
+                  st);
     launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                          reinterpret_cast<uint8_t*>(dwords), st);
     kev_end(c, ke);

@@ -117,7 +117,7 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage;
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range;
     hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
     hipEvent_t order_ev = nullptr;  // asz_stream_wait / batch workers: cross-stream ordering

@@ -13,11 +13,14 @@
 // leaves the global window is binned again by k_hist over its exact range.
 //
 // Encoding.  The host builds the reference-tree codebooks.  The arrays are
-// cut into chunks of kEncChunk symbols; k_chunk_bits sums each chunk's code
-// bits, k_chunk_scan turns the sums into bit positions per array, and
-// k_encode_at writes each chunk's codes straight to its position in the
-// stream buffer (which holds the header bytes): interior words are stored,
-// the two boundary words shared with the neighbours are ORed in.
+// cut into chunks of kEncChunk symbols.  With every code at most 32 bits,
+// k_encode_pack reads the symbols once and packs each chunk from bit 0 into a
+// staging slot, counting its bits; k_chunk_scan turns the counts into bit
+// positions per array, and k_encode_place shifts each slot into its position
+// in the stream buffer (which holds the header bytes): interior words are
+// stored, the two boundary words shared with the neighbours are ORed in.
+// Longer codes: k_chunk_bits counts, k_chunk_scan places, k_encode_long ORs
+// the codes into the stream.
 #include "device.hpp"
 #include "fold.hpp"
 #include "launch.hpp"
@@ -125,8 +128,6 @@ __device__ __forceinline__ uint32_t book_len(uint64_t e) { return (uint32_t)(e >
 __device__ __forceinline__ uint64_t book_code(uint64_t e) { return e & ((1ull << 56) - 1); }
 
 constexpr int kEncPerThread = kEncChunk / 256;  // 16
-constexpr int kEncPad = kEncChunk + kEncChunk / kEncPerThread;  // staging with one pad word per 16
-constexpr int kEncWords = kEncChunk + 2;        // LDS bit buffer for codes up to 32 bits
 constexpr uint32_t kLdsLens = 8192;             // LDS code-length table of the bit-count pass
 
 // The array of chunk c, from array k on (a workgroup's chunks increase, so
@@ -266,42 +267,53 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
     if (bits) words[(off + bits - 1) >> 5] = 0;
 }
 
-// Encode pass: every chunk's codes at its bit position chunk_off[c] in the
-// stream buffer.  Persistent: workgroup g codes chunks g, g + G, ..., the
-// next chunk's symbols and position in flight while the current one is
-// coded.  short_codes (every code of the call fits 32 bits): the codes come
+// Pack pass (every code at most 32 bits): workgroup g packs chunks g, g + G,
+// ... (persistent, the next chunk's symbols in flight while the current one
+// is packed), each into its slot of the staging buffer (slot c at stage +
+// c * slot_words, the chunk's first code at bit 0), and leaves the chunk's
+// bit count in chunk_bits[c].  A chunk is read as 1024 int4 groups of four
+// consecutive symbols, thread t owning groups t, t + 256, t + 512 and t + 768
+// (one in each quarter of the chunk), so every load is a coalesced 16-byte
+// load.  A thread joins its group's four codes into one 64-bit value (first
+// symbol in the low bits); four workgroup scans of the group bit counts (two
+// per 32-bit word: a quarter holds at most 32768 bits) place each group in
+// the chunk, and each group is ORed into an LDS bit buffer with one to three
+// LDS atomics.  A group longer than 64 bits (four codes averaging more than
+// 16 bits) is ORed code by code from its symbols, read again.  The codes come
 // from code32 / len8 (a 32-bit code and a byte length per symbol of every
 // array, at its book offset), in LDS when the array's alphabet is at most
-// kLdsBook symbols.  Per chunk: the symbol indices are staged in LDS so that
-// thread t owns the 16 consecutive symbols 16 t .. 16 t + 15 (padded rows,
-// no bank conflicts); a workgroup scan of the per-thread bit counts gives
-// each thread's first bit; each code is then ORed into an LDS bit buffer at
-// its bit position (one or two LDS atomics per code, no branches on the
-// running bit count), and the buffer goes to the stream: interior words
-// stored, the two boundary words (zeroed by k_chunk_scan, shared with the
-// neighbouring chunks or header bytes) ORed in.  Codes longer than 32 bits
-// (short_codes false) come from the 64-bit book (code in bits 0..55, length
-// in bits 56..63) and are ORed straight into the stream.  Held to 128 VGPRs
-// so that four workgroups share a CU.
-template <bool short_codes>
-__global__ __launch_bounds__(256, 4) void k_encode_at(const int32_t* __restrict__ sym,
-                                                      const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                      uint64_t nchunks, const uint64_t* __restrict__ book,
-                                                      const uint32_t* __restrict__ code32,
-                                                      const uint8_t* __restrict__ len8,
-                                                      const uint64_t* __restrict__ chunk_off, uint32_t* words) {
-    __shared__ uint32_t lbuf[kEncPad > kEncWords ? kEncPad : kEncWords];  // staging, then the bit buffer
+// kLdsBook symbols.
+__device__ __forceinline__ void load_groups(const int32_t* __restrict__ sym, const HuffArrayDev& A, uint64_t c,
+                                            int4 v[4]) {
+    const uint64_t first = (c - A.chunk0) * kEncChunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
+    // 16-byte aligned: arrays start on 64-symbol boundaries, chunks on kEncChunk
+    // symbols; a group that starts before n ends inside the array's padding
+    const int4* s = reinterpret_cast<const int4*>(sym + A.src_off + first);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t g = 256 * j + threadIdx.x;
+        v[j] = 4 * g < n ? s[g] : make_int4(0, 0, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__ sym,
+                                                     const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                     uint64_t nchunks, const uint32_t* __restrict__ code32,
+                                                     const uint8_t* __restrict__ len8,
+                                                     uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage,
+                                                     uint64_t slot_words) {
+    __shared__ __align__(16) uint32_t lbuf[kEncChunk + 4];  // bit buffer: 32 bits per symbol at most
     __shared__ uint32_t lcode[kLdsBook];
     __shared__ uint8_t llen[kLdsBook];
-    __shared__ uint32_t s_scan[4];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ uint32_t s_tot[4][2];
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint64_t G = gridDim.x;
     uint64_t c = blockIdx.x;
     if (c >= nchunks) return;
     uint32_t k = chunk_array(arrays, narr, c, 0);
-    int32_t v[kEncPerThread];
-    if (short_codes) load_chunk(sym, arrays[k], c, v);
-    uint64_t gbit = chunk_off[c];
+    int4 v[4];
+    load_groups(sym, arrays[k], c, v);
     int book_k = -1;
     while (true) {
         const HuffArrayDev A = arrays[k];
@@ -309,112 +321,202 @@ __global__ __launch_bounds__(256, 4) void k_encode_at(const int32_t* __restrict_
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
         const uint64_t cn = c + G;
         const uint32_t kn = cn < nchunks ? chunk_array(arrays, narr, cn, k) : k;
-        uint32_t code[kEncPerThread], len[kEncPerThread];
-        uint64_t e64[short_codes ? 1 : kEncPerThread];
-        uint32_t mybits = 0;
-        const uint64_t gbit_next = cn < nchunks ? chunk_off[cn] : 0;
-        if (short_codes) {
-            const uint32_t* C = code32 + A.book_off;
-            const uint8_t* L = len8 + A.book_off;
-            const bool lds = A.lds_book != 0;
-            if (lds && book_k != (int)k) {
-                for (uint32_t i = threadIdx.x; i < A.range; i += blockDim.x) {
-                    lcode[i] = C[i];
-                    llen[i] = L[i];
-                }
-                book_k = (int)k;
+        const uint32_t* C = code32 + A.book_off;
+        const uint8_t* L = len8 + A.book_off;
+        const bool lds = A.lds_book != 0;
+        if (lds && book_k != (int)k) {
+            for (uint32_t i = t; i < A.range; i += blockDim.x) {
+                lcode[i] = C[i];
+                llen[i] = L[i];
             }
-            lds_barrier();  // LDS codebook complete; the previous chunk's buffer is free
-#pragma unroll
-            for (int j = 0; j < kEncPerThread; j++) {
-                const uint32_t i = j * 256 + threadIdx.x;
-                lbuf[i + i / kEncPerThread] = i < n ? (uint32_t)((int64_t)v[j] - A.offset) : 0xFFFFFFFFu;
-            }
-            // the next chunk's symbols stay in flight while this one is packed
-            // (LDS-only barriers below do not wait for them)
-            if (cn < nchunks) load_chunk(sym, arrays[kn], cn, v);
-            lds_barrier();
-#pragma unroll
-            for (int j = 0; j < kEncPerThread; j++) {
-                const uint32_t i = threadIdx.x * kEncPerThread + j;
-                const uint32_t idx = lbuf[i + i / kEncPerThread];
-                code[j] = 0;
-                len[j] = 0;
-                if (idx != 0xFFFFFFFFu) {
-                    code[j] = lds ? lcode[idx] : C[idx];
-                    len[j] = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
-                }
-                mybits += len[j];
-            }
-        } else {
-            lds_barrier();
-            const uint64_t* b = book + A.book_off;
-            const uint32_t my0 = threadIdx.x * kEncPerThread;
-            const int32_t* s = sym + A.src_off + first;
-#pragma unroll
-            for (int j = 0; j < kEncPerThread; j++) {
-                const uint32_t i = my0 + j;
-                uint64_t ent = 0;
-                if (i < n) ent = b[(int64_t)s[i] - A.offset];
-                e64[short_codes ? 0 : j] = ent;
-                mybits += book_len(ent);
-            }
+            book_k = (int)k;
         }
-        // exclusive scan of the per-thread bit counts
-        uint32_t incl = mybits;
+        lds_barrier();  // LDS codebook complete; the previous chunk's buffer written out
+        uint64_t acc[4];
+        uint32_t gb[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i0 = 1024 * j + 4 * t;
+            const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+            uint64_t a = 0;
+            uint32_t sh = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (i0 + e < n) {
+                    const uint32_t idx = (uint32_t)((int64_t)s[e] - A.offset);
+                    const uint32_t cd = lds ? lcode[idx] : C[idx];
+                    const uint32_t ln = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+                    if (sh < 64) a |= (uint64_t)cd << sh;
+                    sh += ln;
+                }
+            }
+            acc[j] = a;
+            gb[j] = sh;
+        }
+        // the next chunk's symbols stay in flight while this one is packed
+        // (the LDS-only barriers below do not wait for them)
+        if (cn < nchunks) load_groups(sym, arrays[kn], cn, v);
+        // group positions: inclusive wave scans of the packed counts, then the wave totals
+        const uint32_t p01 = gb[0] | (gb[1] << 16), p23 = gb[2] | (gb[3] << 16);
+        uint32_t i01 = p01, i23 = p23;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(incl, o, 64);
-            if (lane >= (uint32_t)o) incl += t;
+            const uint32_t u01 = __shfl_up(i01, o, 64), u23 = __shfl_up(i23, o, 64);
+            if (lane >= (uint32_t)o) {
+                i01 += u01;
+                i23 += u23;
+            }
         }
-        if (lane == 63) s_scan[wave] = incl;
-        lds_barrier();  // also: every thread has read its staged entries
-        uint32_t wave_off = 0;
-        for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
-        const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
-        const uint32_t my_off = wave_off + incl - mybits;
-        const uint32_t sh = (uint32_t)(gbit & 31);
-        uint32_t* out = words + (gbit >> 5);
-        if (short_codes) {
-            const uint32_t nw = (sh + total + 31) / 32;
-            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lbuf[w] = 0;
-            lds_barrier();
-            uint32_t p = sh + my_off;
+        if (lane == 63) {
+            s_tot[wave][0] = i01;
+            s_tot[wave][1] = i23;
+        }
+        lds_barrier();
+        uint32_t a01 = 0, a23 = 0, w01 = 0, w23 = 0;
 #pragma unroll
-            for (int j = 0; j < kEncPerThread; j++) {
-                const uint32_t w = p >> 5, s2 = p & 31;
-                if (len[j]) atomicOr(&lbuf[w], code[j] << s2);
-                if (s2 + len[j] > 32) atomicOr(&lbuf[w + 1], code[j] >> (32 - s2));
-                p += len[j];
-            }
-            lds_barrier();
-            for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
-                const uint32_t val = lbuf[w];
-                if (w == 0 || w + 1 == nw) atomicOr(&out[w], val);
-                else out[w] = val;
-            }
-        } else {
-            uint64_t p = (uint64_t)sh + my_off;
-            for (int j = 0; j < kEncPerThread; j++) {
-                const uint64_t ent = e64[short_codes ? 0 : j];
-                const uint32_t l = book_len(ent);
-                if (!l) continue;
-                const uint64_t cd = book_code(ent);
-                const uint32_t s2 = (uint32_t)(p & 31);
-                uint32_t* w = out + (p >> 5);
-                atomicOr(&w[0], (uint32_t)(cd << s2));
-                if (s2 + l > 32) atomicOr(&w[1], (uint32_t)(cd >> (32 - s2)));
-                if (s2 + l > 64) atomicOr(&w[2], (uint32_t)(cd >> (64 - s2)));
-                p += l;
+        for (uint32_t w = 0; w < 4; w++) {
+            const uint32_t x01 = s_tot[w][0], x23 = s_tot[w][1];
+            a01 += x01;
+            a23 += x23;
+            if (w < wave) {
+                w01 += x01;
+                w23 += x23;
             }
         }
+        const uint32_t T0 = a01 & 0xFFFFu, T1 = a01 >> 16, T2 = a23 & 0xFFFFu, T3 = a23 >> 16;
+        const uint32_t e01 = w01 + i01 - p01, e23 = w23 + i23 - p23;
+        const uint32_t pos[4] = {e01 & 0xFFFFu, T0 + (e01 >> 16), T0 + T1 + (e23 & 0xFFFFu),
+                                 T0 + T1 + T2 + (e23 >> 16)};
+        const uint32_t total = T0 + T1 + T2 + T3;
+        const uint32_t nw4 = (total + 127) / 128;  // 16-byte units of the buffer
+        for (uint32_t w = t; w < 4 * nw4; w += blockDim.x) lbuf[w] = 0;
+        lds_barrier();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t g = gb[j];
+            if (!g) continue;
+            uint32_t p = pos[j];
+            if (g <= 64) {
+                const uint32_t w = p >> 5, s2 = p & 31;
+                const uint64_t lo = acc[j] << s2;
+                atomicOr(&lbuf[w], (uint32_t)lo);
+                if (s2 + g > 32) atomicOr(&lbuf[w + 1], (uint32_t)(lo >> 32));
+                if (s2 + g > 64) atomicOr(&lbuf[w + 2], (uint32_t)(acc[j] >> (64 - s2)));
+            } else {
+                const int32_t* s = sym + A.src_off + first;
+                const uint32_t i0 = 1024 * j + 4 * t;
+                for (int e = 0; e < 4; e++) {
+                    if (i0 + e >= n) break;
+                    const uint32_t idx = (uint32_t)((int64_t)s[i0 + e] - A.offset);
+                    const uint32_t cd = lds ? lcode[idx] : C[idx];
+                    const uint32_t ln = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+                    const uint32_t w = p >> 5, s2 = p & 31;
+                    if (ln) atomicOr(&lbuf[w], cd << s2);
+                    if (s2 + ln > 32) atomicOr(&lbuf[w + 1], cd >> (32 - s2));
+                    p += ln;
+                }
+            }
+        }
+        lds_barrier();
+        uint4* dst = reinterpret_cast<uint4*>(stage + c * slot_words);
+        const uint4* src = reinterpret_cast<const uint4*>(lbuf);
+        for (uint32_t w = t; w < nw4; w += blockDim.x) dst[w] = src[w];
+        if (t == 0) chunk_bits[c] = total;
         if (cn >= nchunks) break;
         c = cn;
         k = kn;
-        gbit = gbit_next;
     }
 }
 
+// Place pass: the staged bits of chunk c at bit chunk_off[c] of the stream,
+// one wave per chunk.  Every stream word but the chunk's first and last
+// (zeroed by k_chunk_scan, shared with the neighbouring chunks or header
+// bytes) is stored whole; those two are ORed in.
+__global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
+                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
+                                                      const uint32_t* __restrict__ stage, uint64_t slot_words,
+                                                      uint32_t* words) {
+    const uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= nchunks) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t bits = chunk_bits[c];
+    if (!bits) return;
+    const uint64_t off = chunk_off[c];
+    const uint32_t sh = (uint32_t)(off & 31);
+    const uint32_t nsw = (bits + 31) >> 5, ndw = (sh + bits + 31) >> 5;  // ndw <= nsw + 1
+    const uint32_t* src = stage + c * slot_words;
+    uint32_t* dst = words + (off >> 5);
+    for (uint32_t i = lane; i < ndw; i += 64) {
+        uint32_t val = i < nsw ? src[i] << sh : 0u;
+        if (sh && i > 0) val |= src[i - 1] >> (32 - sh);
+        if (i == 0 || i + 1 == ndw) atomicOr(&dst[i], val);
+        else dst[i] = val;
+    }
+}
+
+// Long-code encode pass (some code longer than 32 bits): every chunk's codes
+// ORed straight into the stream at its bit position chunk_off[c], from the
+// 64-bit book (code in bits 0..55, length in bits 56..63).  Persistent:
+// workgroup g codes chunks g, g + G, ...; thread t owns the 16 consecutive
+// symbols 16 t .. 16 t + 15 of the chunk, and a workgroup scan of the
+// per-thread bit counts gives each thread's first bit.
+__global__ __launch_bounds__(256) void k_encode_long(const int32_t* __restrict__ sym,
+                                                     const HuffArrayDev* __restrict__ arrays, uint32_t narr,
+                                                     uint64_t nchunks, const uint64_t* __restrict__ book,
+                                                     const uint64_t* __restrict__ chunk_off, uint32_t* words) {
+    __shared__ uint32_t s_scan[4];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t G = gridDim.x;
+    uint64_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    uint32_t k = chunk_array(arrays, narr, c, 0);
+    while (true) {
+        const HuffArrayDev A = arrays[k];
+        const uint64_t first = (c - A.chunk0) * kEncChunk;
+        const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
+        const uint64_t* b = book + A.book_off;
+        const uint32_t my0 = threadIdx.x * kEncPerThread;
+        const int32_t* s = sym + A.src_off + first;
+        uint64_t e64[kEncPerThread];
+        uint32_t mybits = 0;
+#pragma unroll
+        for (int j = 0; j < kEncPerThread; j++) {
+            const uint32_t i = my0 + j;
+            uint64_t ent = 0;
+            if (i < n) ent = b[(int64_t)s[i] - A.offset];
+            e64[j] = ent;
+            mybits += book_len(ent);
+        }
+        uint32_t incl = mybits;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += u;
+        }
+        if (lane == 63) s_scan[wave] = incl;
+        __syncthreads();
+        uint32_t wave_off = 0;
+        for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
+        uint64_t p = chunk_off[c] + wave_off + incl - mybits;
+        for (int j = 0; j < kEncPerThread; j++) {
+            const uint64_t ent = e64[j];
+            const uint32_t l = book_len(ent);
+            if (!l) continue;
+            const uint64_t cd = book_code(ent);
+            const uint32_t s2 = (uint32_t)(p & 31);
+            uint32_t* w = words + (p >> 5);
+            atomicOr(&w[0], (uint32_t)(cd << s2));
+            if (s2 + l > 32) atomicOr(&w[1], (uint32_t)(cd >> (32 - s2)));
+            if (s2 + l > 64) atomicOr(&w[2], (uint32_t)(cd >> (64 - s2)));
+            p += l;
+        }
+        __syncthreads();  // s_scan free for the next chunk
+        const uint64_t cn = c + G;
+        if (cn >= nchunks) break;
+        k = chunk_array(arrays, narr, cn, k);
+        c = cn;
+    }
+}
 }  // namespace dev
 
 void launch_hist(const int32_t* sym, const HuffArrayDev* arrays, const uint32_t* chunk_arr, uint64_t nchunks,
@@ -442,33 +544,45 @@ void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64
     hipLaunchKernelGGL(dev::k_hist_tail, dim3(nfields), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch,
                        part_stride, g_stride, h_stride);
 }
+uint64_t enc_slot_words(int max_len) {
+    const uint64_t w = ((uint64_t)kEncChunk * (uint64_t)std::max(max_len, 1) + 31) / 32;
+    return (w + 63) / 64 * 64;  // slots on 256-byte boundaries
+}
+
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
                    const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
-                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, hipStream_t st) {
+                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, uint32_t* stage, hipStream_t st) {
     if (!nchunks || !narr) return;
+    const bool packed = max_len <= 32;
+    if (packed && !stage) fail(ASZ_ERR_ARG, "encoder staging buffer missing");
     // one resident wave of workgroups for each persistent pass: CUs x workgroups that fit on a CU
     int dev_id = 0, cus = 0, per_cu = 0, per_cu_bits = 0;
     (void)hipGetDevice(&dev_id);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, dev::k_chunk_bits, 256, 0);
-    if (max_len <= 32)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_at<true>, 256, 0);
-    else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_at<false>, 256, 0);
     auto groups = [&](int pc) {
         const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(pc, 1);
         return (uint32_t)std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
     };
+    const dim3 scan_grid((uint32_t)((nchunks + 1023) / 1024));
+    if (packed) {
+        const uint64_t slot = enc_slot_words(max_len);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_pack, 256, 0);
+        hipLaunchKernelGGL(dev::k_encode_pack, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks,
+                           code32, len8, chunk_bits, stage, slot);
+        hipLaunchKernelGGL(dev::k_chunk_scan, scan_grid, dim3(1024), 0, st, arrays, narr, nchunks, chunk_bits,
+                           chunk_off, stream_words);
+        hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)((nchunks + 3) / 4)), dim3(256), 0, st, chunk_bits,
+                           chunk_off, nchunks, stage, slot, stream_words);
+        return;
+    }
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, dev::k_chunk_bits, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_long, 256, 0);
     hipLaunchKernelGGL(dev::k_chunk_bits, dim3(groups(per_cu_bits)), dim3(256), 0, st, sym, arrays, narr, nchunks,
                        len8, chunk_bits);
-    hipLaunchKernelGGL(dev::k_chunk_scan, dim3((uint32_t)((nchunks + 1023) / 1024)), dim3(1024), 0, st, arrays, narr,
-                       nchunks, chunk_bits, chunk_off, stream_words);
-    if (max_len <= 32)
-        hipLaunchKernelGGL(dev::k_encode_at<true>, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks,
-                           book, code32, len8, chunk_off, stream_words);
-    else
-        hipLaunchKernelGGL(dev::k_encode_at<false>, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr,
-                           nchunks, book, code32, len8, chunk_off, stream_words);
+    hipLaunchKernelGGL(dev::k_chunk_scan, scan_grid, dim3(1024), 0, st, arrays, narr, nchunks, chunk_bits, chunk_off,
+                       stream_words);
+    hipLaunchKernelGGL(dev::k_encode_long, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks, book,
+                       chunk_off, stream_words);
 }
 
 }  // namespace asz

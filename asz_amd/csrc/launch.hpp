@@ -183,17 +183,23 @@ void launch_fold_rounds(const FoldDesc* descs, uint32_t r0, uint32_t nr, uint32_
 void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64_t nparts, const uint32_t* gw,
                       uint32_t* hres, uint32_t epoch, hipStream_t st, uint32_t nfields = 1, uint64_t part_stride = 0,
                       uint64_t g_stride = 0, uint64_t h_stride = 0);
-// Huffman encoder of the arrays' chunks straight into the stream buffer
-// (header bytes are scattered in after it has run, with the same stream):
-// a bit-count pass into chunk_bits, a scan into chunk_off (which also zeroes
-// the payload words the encoder ORs), and the encode pass.  The codebook of
-// array k starts at its book_off in three parallel tables: book (code in bits
-// 0..55, length in bits 56..63), code32 (the code, when max_len <= 32) and
-// len8 (the length).  chunk_bits / chunk_off: nchunks entries.
+// Huffman encoder of the arrays' chunks into the stream buffer (header bytes
+// are scattered in after it has run, with the same stream).  Every code at
+// most 32 bits (max_len <= 32): a pack pass codes each chunk from bit 0 into
+// its slot of the staging buffer (enc_slot_words(max_len) words per chunk)
+// and counts its bits into chunk_bits, a scan turns the counts into
+// chunk_off (and zeroes the payload words that chunks share), and a place
+// pass copies each slot to its bit position.  Longer codes: a bit-count pass,
+// the scan, and an encode pass that ORs the codes straight into the stream
+// (stage unused).  The codebook of array k starts at its book_off in three
+// parallel tables: book (code in bits 0..55, length in bits 56..63), code32
+// (the code, when max_len <= 32) and len8 (the length).  chunk_bits /
+// chunk_off: nchunks entries.
 constexpr uint64_t kEncGroups = 2048;    // most workgroups of a persistent encoder pass
+uint64_t enc_slot_words(int max_len);
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
                    const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
-                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, hipStream_t st);
+                   uint64_t* chunk_off, uint32_t* stream_words, int max_len, uint32_t* stage, hipStream_t st);
 
 // Decoder (self-synchronising parallel decode of one bit stream per array).
 struct DecArrayDev {
