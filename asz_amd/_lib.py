@@ -84,6 +84,7 @@ ASZ_OPT_KERNEL_EVENTS = 1
 ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
 ASZ_OPT_BATCH_FIELDS = 4
+ASZ_OPT_WIDE_CODES = 5
 
 
 # exported entry points (must match include/asz_gpu.h)

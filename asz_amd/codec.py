@@ -119,6 +119,10 @@ class Context:
         """Time round r's kernel only (HIP events around that launch); r < 0: no events."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_TIMED_ROUND, int(r)))
 
+    def set_wide_codes(self, on: bool = True):
+        """Route the Huffman encoder through its path for codes longer than 32 bits."""
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_WIDE_CODES, 1 if on else 0))
+
     def set_stream_on_device(self, on: bool = True):
         """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))

@@ -937,9 +937,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     HIPCHECK(hipEventRecord(c->ev[4], st));
     int ke = kev_begin(c, KEV_ENCODE, 0);
     const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-    uint32_t* estage = max_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(max_len)) : nullptr;
+    const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
+    uint32_t* estage = enc_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(enc_len)) : nullptr;
     launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
-                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, max_len, estage,
+                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, enc_len, estage,
                   st);
     launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                          reinterpret_cast<uint8_t*>(dwords), st);
@@ -1309,6 +1310,10 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
             c->stream_on_device = value != 0;
             return;
         }
+        if (option == ASZ_OPT_WIDE_CODES) {
+            c->wide_codes = value != 0;
+            return;
+        }
         if (option == ASZ_OPT_BATCH_FIELDS) {
             if (value < 0 || value > 65535) fail(ASZ_ERR_ARG, "fields per chunk must be in [0, 65535]");
             c->batch_fields = (uint32_t)value;
@@ -1567,7 +1572,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
             if (rc != ASZ_OK) fail(rc, "cannot create a batch worker context");
             c->kids.push_back(k);
         }
-        const bool on_device = c->stream_on_device;
+        const bool on_device = c->stream_on_device, wide = c->wide_codes;
         // the workers' streams start after everything enqueued on this context's
         // stream (asz_stream_wait orders that after the caller's)
         if (!c->order_ev) HIPCHECK(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
@@ -1602,6 +1607,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
                     std::vector<asz::FieldIn> fin(nf);
                     for (uint32_t j = 0; j < nf; j++) fin[j] = asz::FieldIn{d_fields[ch[j]], mn[j], mx[j], sr[j].slope};
                     k->stream_on_device = on_device;
+                    k->wide_codes = wide;
                     asz::compress_fields(k, nf, fin.data(), d, plans[ch[0]], *p, tsel_mode, nullptr, nullptr,
                                          nullptr);
                     for (uint32_t j = 0; j < nf; j++) {

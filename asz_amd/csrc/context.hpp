@@ -134,6 +134,7 @@ struct asz_ctx {
     uint64_t pc_tick = 0;
     bool stream_on_device = false;
     uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
+    bool wide_codes = false;  // ASZ_OPT_WIDE_CODES
     float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;

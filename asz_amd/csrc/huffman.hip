@@ -456,10 +456,11 @@ __global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict
 
 // Long-code encode pass (some code longer than 32 bits): every chunk's codes
 // ORed straight into the stream at its bit position chunk_off[c], from the
-// 64-bit book (code in bits 0..55, length in bits 56..63).  Persistent:
-// workgroup g codes chunks g, g + G, ...; thread t owns the 16 consecutive
-// symbols 16 t .. 16 t + 15 of the chunk, and a workgroup scan of the
-// per-thread bit counts gives each thread's first bit.
+// 64-bit book (code in bits 0..55, length in bits 56..63), after the
+// workgroup has zeroed the words that lie wholly inside the chunk.
+// Persistent: workgroup g codes chunks g, g + G, ...; thread t owns the 16
+// consecutive symbols 16 t .. 16 t + 15 of the chunk, and a workgroup scan of
+// the per-thread bit counts gives each thread's first bit.
 __global__ __launch_bounds__(256) void k_encode_long(const int32_t* __restrict__ sym,
                                                      const HuffArrayDev* __restrict__ arrays, uint32_t narr,
                                                      uint64_t nchunks, const uint64_t* __restrict__ book,
@@ -497,7 +498,14 @@ __global__ __launch_bounds__(256) void k_encode_long(const int32_t* __restrict__
         __syncthreads();
         uint32_t wave_off = 0;
         for (uint32_t w = 0; w < wave; w++) wave_off += s_scan[w];
-        uint64_t p = chunk_off[c] + wave_off + incl - mybits;
+        // the words strictly inside the chunk are its own: zeroed here before
+        // the codes are ORed in (k_chunk_scan zeroed the first and last)
+        const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
+        const uint64_t off = chunk_off[c];
+        if (total)
+            for (uint64_t w = (off >> 5) + 1 + threadIdx.x; w < ((off + total - 1) >> 5); w += blockDim.x) words[w] = 0;
+        __syncthreads();
+        uint64_t p = off + wave_off + incl - mybits;
         for (int j = 0; j < kEncPerThread; j++) {
             const uint64_t ent = e64[j];
             const uint32_t l = book_len(ent);

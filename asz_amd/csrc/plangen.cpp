@@ -339,11 +339,12 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             uint32_t* dwords = c->words.get<uint32_t>(nwords);
             if (chunks) {
                 const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-                uint32_t* estage = max_len <= 32 ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(max_len)) : nullptr;
+                const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
+                uint32_t* estage = enc_len <= 32 ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(enc_len)) : nullptr;
                 launch_encode(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book),
                               reinterpret_cast<const uint32_t*>(du + o_code), du + o_len,
                               c->chunk_bits.get<uint32_t>(chunks + 1), c->chunk_off.get<uint64_t>(chunks + 1), dwords,
-                              max_len, estage, st);
+                              enc_len, estage, st);
             }
             launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                                  reinterpret_cast<uint8_t*>(dwords), st);

@@ -111,6 +111,11 @@ typedef struct asz_stats {
  * kernel of a chunk covers all of its fields).  0 (default): automatic;
  * 1: every field on its own. */
 #define ASZ_OPT_BATCH_FIELDS 4
+/* ASZ_OPT_WIDE_CODES: when nonzero, the Huffman encoder takes the path it
+ * keeps for codes longer than 32 bits (64-bit codebook entries ORed straight
+ * into the stream) whatever the longest code is.  The stream is the same;
+ * the option exists to exercise that path. */
+#define ASZ_OPT_WIDE_CODES 5
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 /* Orders the context's later work after everything already enqueued on
  * `stream` (a hipStream_t of the same device; NULL: the null stream), without

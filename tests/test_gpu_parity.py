@@ -77,6 +77,44 @@ def test_inject_t_stream_is_byte_identical(ctx, name):
 
 
 @pytest.mark.parametrize("name", NAMES)
+def test_wide_code_encoder_path_writes_the_reference_stream(ctx, name):
+    """The encoder's path for codes longer than 32 bits (taken here whatever
+    the longest code is) writes the same bytes as its default path."""
+    info, data, asz = gu.load(name)
+    _say("case", name, data.shape, "wide codes")
+    payload = codec.zstd_decompress(asz)
+    st = api.parse_payload(payload)
+    p = _params(info)
+    ctx.set_wide_codes(True)
+    try:
+        res = ctx.compress(data, codec.make_plan(codec.plan_strings(st.axis, st.order)), p, tsel="inject",
+                           t_inject=st.t)
+    finally:
+        ctx.set_wide_codes(False)
+    assert res.stream == payload
+
+
+def test_wide_code_encoder_path_on_a_generated_field(ctx):
+    """Both encoder paths on a field of many chunks (192 x 160 x 224, every
+    array cut into several 4096-symbol chunks): the same stream."""
+    shape = (192, 160, 224)
+    data = fields.to_numpy(fields.config2_field(shape, seed=5)).astype(np.float32)
+    plan = codec.make_plan(["cubic0_012", "cubic1_120", "cubic2_021", "cubic0_012", "cubic1_012", "cubic2_012",
+                            "cubic0_120", "cubic1_021", "cubic2_012", "cubic0_012", "cubic1_012", "cubic2_012"])
+    p = codec.make_params(rel_eb=1e-4, block_pivot_ratio=3, family="cubic", default_slope=1000.0)
+    _say("case generated", shape, "default encoder path")
+    base = ctx.compress(data, plan, p, tsel="gpu")
+    ctx.set_wide_codes(True)
+    try:
+        _say("case generated", shape, "wide-code encoder path")
+        wide = ctx.compress(data, plan, p, tsel="gpu")
+    finally:
+        ctx.set_wide_codes(False)
+    assert len(base.stream) > 64 * 4096
+    assert wide.stream == base.stream
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_gpu_decodes_reference_stream(ctx, name):
     info, data, asz = gu.load(name)
     _say("case", name, data.shape)
