@@ -15,6 +15,17 @@ namespace dev {
 
 constexpr int kWave = 64;
 
+// Global-memory view of a pointer whose address space the compiler cannot
+// infer (one loaded from a descriptor in memory, or merged with such a load):
+// accesses through it compile to global_* instead of flat_* instructions,
+// which the LDS wait counter would also track.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
 __device__ __forceinline__ float daz(float x) { return fabsf(x) < FLT_MIN ? copysignf(0.0f, x) : x; }
 
 // GCC -Ofast expands llround(double) inline: trunc(y + copysign(pred(0.5), y)),

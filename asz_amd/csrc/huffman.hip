@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256) void k_chunk_bits(const int32_t* __restrict__ 
             const uint32_t i = j * 256 + threadIdx.x;
             if (i < n) {
                 const int64_t idx = (int64_t)v[j] - A.offset;
-                bits += lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+                if (lds) bits += llen[idx];
+                else bits += L[idx];
             }
         }
         if (cn < nchunks) load_chunk(sym, arrays[kn], cn, v);
@@ -268,8 +269,8 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
 }
 
 // Pack pass (every code at most 32 bits): workgroup g packs chunks g, g + G,
-// ... (persistent, the next chunk's symbols in flight while the current one
-// is packed), each into its slot of the staging buffer (slot c at stage +
+// ... (persistent, the symbols of the next two chunks in flight while one is
+// packed), each into its slot of the staging buffer (slot c at stage +
 // c * slot_words, the chunk's first code at bit 0), and leaves the chunk's
 // bit count in chunk_bits[c].  A chunk is read as 1024 int4 groups of four
 // consecutive symbols, thread t owning groups t, t + 256, t + 512 and t + 768
@@ -309,18 +310,14 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
     __shared__ uint32_t s_tot[4][2];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint64_t G = gridDim.x;
-    uint64_t c = blockIdx.x;
-    if (c >= nchunks) return;
-    uint32_t k = chunk_array(arrays, narr, c, 0);
-    int4 v[4];
-    load_groups(sym, arrays[k], c, v);
     int book_k = -1;
-    while (true) {
+
+    // Packs chunk c of array k from the symbols in v, then starts the load of
+    // chunk c + 2G into v (two chunks in flight while this one is packed).
+    auto pack = [&](int4 (&v)[4], uint64_t c, uint32_t k) {
         const HuffArrayDev A = arrays[k];
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
-        const uint64_t cn = c + G;
-        const uint32_t kn = cn < nchunks ? chunk_array(arrays, narr, cn, k) : k;
         const uint32_t* C = code32 + A.book_off;
         const uint8_t* L = len8 + A.book_off;
         const bool lds = A.lds_book != 0;
@@ -334,28 +331,41 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
         lds_barrier();  // LDS codebook complete; the previous chunk's buffer written out
         uint64_t acc[4];
         uint32_t gb[4];
+        // the table lookups, LDS or global, chosen at compile time: a run-time
+        // choice would merge the two into flat (generic) loads
+        auto join = [&](auto from_lds) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t i0 = 1024 * j + 4 * t;
-            const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-            uint64_t a = 0;
-            uint32_t sh = 0;
+            for (int j = 0; j < 4; j++) {
+                const uint32_t i0 = 1024 * j + 4 * t;
+                const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+                uint64_t a = 0;
+                uint32_t sh = 0;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                if (i0 + e < n) {
-                    const uint32_t idx = (uint32_t)((int64_t)s[e] - A.offset);
-                    const uint32_t cd = lds ? lcode[idx] : C[idx];
-                    const uint32_t ln = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
-                    if (sh < 64) a |= (uint64_t)cd << sh;
-                    sh += ln;
+                for (int e = 0; e < 4; e++) {
+                    if (i0 + e < n) {
+                        const uint32_t idx = (uint32_t)((int64_t)s[e] - A.offset);
+                        uint32_t cd, ln;
+                        if constexpr (decltype(from_lds)::value) {
+                            cd = lcode[idx];
+                            ln = llen[idx];
+                        } else {
+                            cd = C[idx];
+                            ln = L[idx];
+                        }
+                        if (sh < 64) a |= (uint64_t)cd << sh;
+                        sh += ln;
+                    }
                 }
+                acc[j] = a;
+                gb[j] = sh;
             }
-            acc[j] = a;
-            gb[j] = sh;
-        }
-        // the next chunk's symbols stay in flight while this one is packed
-        // (the LDS-only barriers below do not wait for them)
-        if (cn < nchunks) load_groups(sym, arrays[kn], cn, v);
+        };
+        if (lds) join(std::true_type{});
+        else join(std::false_type{});
+        // the load of chunk c + 2G stays in flight while this one is packed
+        // (the LDS-only barriers below do not wait for it)
+        const uint64_t c2 = c + 2 * G;
+        if (c2 < nchunks) load_groups(sym, arrays[chunk_array(arrays, narr, c2, k)], c2, v);
         // group positions: inclusive wave scans of the packed counts, then the wave totals
         const uint32_t p01 = gb[0] | (gb[1] << 16), p23 = gb[2] | (gb[3] << 16);
         uint32_t i01 = p01, i23 = p23;
@@ -408,8 +418,14 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
                 for (int e = 0; e < 4; e++) {
                     if (i0 + e >= n) break;
                     const uint32_t idx = (uint32_t)((int64_t)s[i0 + e] - A.offset);
-                    const uint32_t cd = lds ? lcode[idx] : C[idx];
-                    const uint32_t ln = lds ? (uint32_t)llen[idx] : (uint32_t)L[idx];
+                    uint32_t cd, ln;
+                    if (lds) {
+                        cd = lcode[idx];
+                        ln = llen[idx];
+                    } else {
+                        cd = C[idx];
+                        ln = L[idx];
+                    }
                     const uint32_t w = p >> 5, s2 = p & 31;
                     if (ln) atomicOr(&lbuf[w], cd << s2);
                     if (s2 + ln > 32) atomicOr(&lbuf[w + 1], cd >> (32 - s2));
@@ -422,35 +438,104 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
         const uint4* src = reinterpret_cast<const uint4*>(lbuf);
         for (uint32_t w = t; w < nw4; w += blockDim.x) dst[w] = src[w];
         if (t == 0) chunk_bits[c] = total;
-        if (cn >= nchunks) break;
-        c = cn;
-        k = kn;
+    };
+
+    uint64_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    uint32_t k = chunk_array(arrays, narr, c, 0);
+    int4 va[4], vb[4];
+    load_groups(sym, arrays[k], c, va);
+    uint32_t kb = k;
+    if (c + G < nchunks) {
+        kb = chunk_array(arrays, narr, c + G, k);
+        load_groups(sym, arrays[kb], c + G, vb);
+    }
+    // chunks alternate between the two register sets: va holds c, vb c + G
+    while (true) {
+        pack(va, c, k);
+        if (c + G >= nchunks) break;
+        pack(vb, c + G, kb);
+        c += 2 * G;
+        if (c >= nchunks) break;
+        k = chunk_array(arrays, narr, c, kb);
+        if (c + G < nchunks) kb = chunk_array(arrays, narr, c + G, k);
     }
 }
 
-// Place pass: the staged bits of chunk c at bit chunk_off[c] of the stream,
-// one wave per chunk.  Every stream word but the chunk's first and last
-// (zeroed by k_chunk_scan, shared with the neighbouring chunks or header
-// bytes) is stored whole; those two are ORed in.
-__global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
-                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
-                                                      const uint32_t* __restrict__ stage, uint64_t slot_words,
-                                                      uint32_t* words) {
-    const uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+// Place pass: the staged bits of chunk c (workgroup c, one wave) at bit
+// chunk_off[c] of the stream.  The wave writes the chunk's aligned 16-byte
+// quads of stream words, lane l taking quads q0 + l, q0 + l + 64, ... four at
+// a time (their loads all issued before the stores).  Stream word i of the
+// chunk (i = 0 at word chunk_off[c] / 32) is (s[i] << sh) | (s[i-1] >> (32 -
+// sh)) over the staged words s (zero outside the chunk's nsw words), so a
+// quad needs the five staged words from i0 - 1 on, which lie in two aligned
+// staged quads: two 16-byte loads.  A quad wholly inside the chunk is stored
+// with one 16-byte store; the chunk's first and last words (zeroed by
+// k_chunk_scan, shared with the neighbouring chunks or header bytes) are ORed
+// in; words of a quad outside the chunk are left alone.
+__global__ __launch_bounds__(64) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
+                                                     const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
+                                                     const uint32_t* __restrict__ stage, uint64_t slot_words,
+                                                     uint32_t* words) {
+    const uint64_t c = blockIdx.x;
     if (c >= nchunks) return;
-    const uint32_t lane = threadIdx.x & 63;
     const uint32_t bits = chunk_bits[c];
     if (!bits) return;
     const uint64_t off = chunk_off[c];
     const uint32_t sh = (uint32_t)(off & 31);
-    const uint32_t nsw = (bits + 31) >> 5, ndw = (sh + bits + 31) >> 5;  // ndw <= nsw + 1
-    const uint32_t* src = stage + c * slot_words;
-    uint32_t* dst = words + (off >> 5);
-    for (uint32_t i = lane; i < ndw; i += 64) {
-        uint32_t val = i < nsw ? src[i] << sh : 0u;
-        if (sh && i > 0) val |= src[i - 1] >> (32 - sh);
-        if (i == 0 || i + 1 == ndw) atomicOr(&dst[i], val);
-        else dst[i] = val;
+    const int64_t nsw = (bits + 31) >> 5, ndw = (sh + bits + 31) >> 5;  // ndw <= nsw + 1
+    const uint4* src4 = reinterpret_cast<const uint4*>(stage + c * slot_words);  // 256-byte aligned slot
+    const uint64_t g0 = off >> 5;  // the chunk's first stream word
+    const uint64_t q0 = g0 >> 2, q1 = (g0 + (uint64_t)ndw - 1) >> 2;
+    // staged word i0 - 1 of every quad sits at position rem of staged quad m
+    // (i0 - 1 = 4 m + rem; rem is the same for every quad of the chunk)
+    const int64_t a0 = (int64_t)(4 * q0) - (int64_t)g0 - 1;  // >= -4
+    const uint32_t rem = (uint32_t)(a0 & 3);
+    auto quad = [&](int64_t m) -> uint4 {  // staged quad m, words past nsw (or before 0) as 0
+        if (m < 0 || 4 * m >= nsw) return make_uint4(0, 0, 0, 0);
+        uint4 v = src4[m];
+        const int64_t j = 4 * m;
+        if (j + 1 >= nsw) v.y = 0;
+        if (j + 2 >= nsw) v.z = 0;
+        if (j + 3 >= nsw) v.w = 0;
+        return v;
+    };
+    for (uint64_t qb = q0; qb <= q1; qb += 4 * 64) {
+        uint4 A[4], B[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t q = qb + threadIdx.x + 64 * u;
+            const int64_t m = (a0 + 4 * (int64_t)(q - q0) - (int64_t)rem) / 4;  // exact: the numerator is 4 m
+            A[u] = q <= q1 ? quad(m) : make_uint4(0, 0, 0, 0);
+            B[u] = q <= q1 ? quad(m + 1) : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t q = qb + threadIdx.x + 64 * u;
+            if (q > q1) break;
+            // w[k]: staged word i0 - 1 + k
+            const uint32_t x[8] = {A[u].x, A[u].y, A[u].z, A[u].w, B[u].x, B[u].y, B[u].z, B[u].w};
+            uint32_t w[5];
+            if (rem == 0) { w[0] = x[0]; w[1] = x[1]; w[2] = x[2]; w[3] = x[3]; w[4] = x[4]; }
+            else if (rem == 1) { w[0] = x[1]; w[1] = x[2]; w[2] = x[3]; w[3] = x[4]; w[4] = x[5]; }
+            else if (rem == 2) { w[0] = x[2]; w[1] = x[3]; w[2] = x[4]; w[3] = x[5]; w[4] = x[6]; }
+            else { w[0] = x[3]; w[1] = x[4]; w[2] = x[5]; w[3] = x[6]; w[4] = x[7]; }
+            uint32_t val[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) val[e] = sh ? (w[e + 1] << sh) | (w[e] >> (32 - sh)) : w[e + 1];
+            const int64_t i0 = (int64_t)(4 * q) - (int64_t)g0;
+            if (i0 >= 1 && i0 + 3 <= ndw - 2) {
+                *reinterpret_cast<uint4*>(words + 4 * q) = make_uint4(val[0], val[1], val[2], val[3]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int64_t i = i0 + e;
+                    if (i < 0 || i >= ndw) continue;
+                    if (i == 0 || i == ndw - 1) atomicOr(&words[4 * q + e], val[e]);
+                    else words[4 * q + e] = val[e];
+                }
+            }
+        }
     }
 }
 
@@ -561,6 +646,7 @@ void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr
                    const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
                    uint64_t* chunk_off, uint32_t* stream_words, int max_len, uint32_t* stage, hipStream_t st) {
     if (!nchunks || !narr) return;
+    if (nchunks > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "symbol arrays exceed the encoder's launch grid");
     const bool packed = max_len <= 32;
     if (packed && !stage) fail(ASZ_ERR_ARG, "encoder staging buffer missing");
     // one resident wave of workgroups for each persistent pass: CUs x workgroups that fit on a CU
@@ -579,8 +665,8 @@ void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr
                            code32, len8, chunk_bits, stage, slot);
         hipLaunchKernelGGL(dev::k_chunk_scan, scan_grid, dim3(1024), 0, st, arrays, narr, nchunks, chunk_bits,
                            chunk_off, stream_words);
-        hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)((nchunks + 3) / 4)), dim3(256), 0, st, chunk_bits,
-                           chunk_off, nchunks, stage, slot, stream_words);
+        hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)nchunks), dim3(64), 0, st, chunk_bits, chunk_off,
+                           nchunks, stage, slot, stream_words);
         return;
     }
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, dev::k_chunk_bits, 256, 0);

@@ -48,7 +48,7 @@ __device__ __forceinline__ void sample_load(const TselDesc& T, const float* __re
     uint32_t q0 = q01 / cnt[1];
     uint64_t pos = ((uint64_t)(lo[0] + q0 * step[0]) * R.D[1] + (lo[1] + q1 * step[1])) * R.D[2] + (lo[2] + q2 * step[2]);
     const int64_t dist = (int64_t)R.dist;
-    const float* p = work + pos;
+    const gptr<const float> p = to_global(work + pos);
     v[0] = p[-dist];
     v[1] = p[-3 * dist];
     v[2] = p[dist];
@@ -133,14 +133,14 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
     // T, so that T stays in kernel arguments)
     float eb = R.eb, layer_slope = R.layer_slope, range = T.range;
     if (fs.fd) {  // field f of a multi-field call
-        const FieldDesc& F = fs.fd[f];
-        work = F.work;
+        const gptr<const FieldDesc> F = to_global(fs.fd + f);
+        work = F->work;
         tblk += f * fs.tblk;
         t_int += f * fs.sym;
         acc += f;
-        eb = F.r[R.layer].eb;
-        layer_slope = F.r[R.layer].layer_slope;
-        range = F.range;
+        eb = F->r[R.layer].eb;
+        layer_slope = F->r[R.layer].layer_slope;
+        range = F->range;
     }
     const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
     const uint32_t bb[3] = {b0, b1, b2};
