@@ -600,9 +600,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         // exact range on the side stream (its symbols are final: the flag is up)
         std::lock_guard<std::mutex> lk(side_mu);
         HIPCHECK(hipSetDevice(c->device));
-        // created on first use, so that a context normally owns one HIP stream
-        // (several contexts then spread over the device's hardware queues)
-        if (!c->side) HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HuffArrayDev D{};
         D.src_off = k == 0 ? 0 : (k == 1 ? ntp : ntp + P.rounds[k - 2].qb_off);
         D.count = A.count;
@@ -628,8 +625,13 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     int kp = kev_begin(c, KEV_PIVOT, 0);
     launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
     kev_end(c, kp);
-    launch_hist_small(nullptr, 0, sym + ntp, P.pivot_count, nullptr, gwin(0, 1), st, nf, fs.sym, fs.ghist);
-    launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, st, nf, fs.fpart, fs.ghist, fs.hres);
+    // the pivot array's histogram on the side stream: no round waits for it
+    ensure_side(c);
+    HIPCHECK(hipEventRecord(c->side_ev[0], st));
+    HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
+    launch_hist_small(nullptr, 0, sym + ntp, P.pivot_count, nullptr, gwin(0, 1), c->side, nf, fs.sym, fs.ghist);
+    launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, c->side, nf, fs.fpart, fs.ghist, fs.hres);
+    HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
     // Codebook workers: the host pool's threads take the arrays in `order`
     // (every field's copy of an array in turn) as their results arrive,
     // concurrently with the launches below and with the later rounds on the GPU.
@@ -734,10 +736,16 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             tsel_host_ms += ms_since(h0);
         }
         if (r + 1 == L) {
-            // every t is final: the t array's range, histogram and tail
-            launch_minmax_i32(sym, nt, (int32_t*)&acc[0].arr[0], st, nf, fs.sym, sizeof(DevAcc) / 4);
-            launch_hist_small(sym, nt, nullptr, 0, gwin(0, 0), nullptr, st, nf, fs.sym, fs.ghist);
-            launch_hist_tail(acc, 0, nullptr, 0, gwin(0, 0), c->hres_dev, epoch, st, nf, 0, fs.ghist, fs.hres);
+            // every t is final: the t array's range, histogram and tail, on the
+            // side stream beside the last round
+            std::lock_guard<std::mutex> lk(side_mu);
+            HIPCHECK(hipEventRecord(c->side_ev[0], st));
+            HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
+            launch_minmax_i32(sym, nt, (int32_t*)&acc[0].arr[0], c->side, nf, fs.sym, sizeof(DevAcc) / 4);
+            launch_hist_small(sym, nt, nullptr, 0, gwin(0, 0), nullptr, c->side, nf, fs.sym, fs.ghist);
+            launch_hist_tail(acc, 0, nullptr, 0, gwin(0, 0), c->hres_dev, epoch, c->side, nf, 0, fs.ghist,
+                             fs.hres);
+            HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
         }
         int kr = kev_begin(c, KEV_ROUND, (int)r);
         if (round_needs_staging(R)) {
@@ -770,6 +778,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         }
     }
     HIPCHECK(hipGetLastError());
+    // the main stream resumes after the side stream's histograms (its last work)
+    HIPCHECK(hipStreamWaitEvent(st, c->side_ev[1], 0));
     if (stop_round >= 0) {
         HIPCHECK(hipStreamSynchronize(st));
         S.wall_ms = ms_since(wall0);
@@ -779,8 +789,107 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     enqueued = true;
     joiner.armed = false;
     const auto enq0 = Clock::now();
+    // One field: once every codebook but the last round's is built (the pool's
+    // tasks), the pack pass codes those arrays while this thread builds the
+    // last codebook.  Their chunks come first in the stream's chunk order, so
+    // early_chunks chunks are packed here and the rest after the last codebook.
+    uint64_t early_chunks = 0;
+    auto early_pack = [&]() {
+        const Layout& lay = lays[0];
+        std::vector<HuffArrayDev> da;
+        uint64_t book_a = 0;
+        int len_a = 0;
+        for (uint64_t k = 0; k + 1 < narr; k++) {
+            const HuffArray& A = lay.arrays[k];
+            if (!A.count || A.single) continue;
+            HuffArrayDev D;
+            std::memset(&D, 0, sizeof(D));
+            D.src_off = k == 0 ? 0 : (k == 1 ? ntp : ntp + P.rounds[k - 2].qb_off);
+            D.count = A.count;
+            D.offset = A.offset;
+            D.range = A.range;
+            D.book_off = book_a;
+            D.chunk0 = early_chunks;
+            D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
+            D.lds_book = A.range <= kLdsBook ? 1 : 0;
+            early_chunks += D.nchunk;
+            book_a += A.range;
+            len_a = std::max(len_a, A.max_len);
+            da.push_back(D);
+        }
+        if (!early_chunks || len_a > 32) {
+            early_chunks = 0;
+            return;
+        }
+        // every later allocation of the chunk counts and the staging slots fits
+        // in these (the last array adds at most its count / kEncChunk chunks)
+        const uint64_t last = lay.arrays[narr - 1].count;
+        const uint64_t max_chunks = early_chunks + (last + kEncChunk - 1) / kEncChunk;
+        uint32_t* cbits = c->chunk_bits.get<uint32_t>(max_chunks + 1);
+        uint32_t* estage = c->enc_stage.get<uint32_t>(max_chunks * kEncChunk);
+        auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
+        const uint64_t o_code = 0, o_len = al8(book_a * 4), o_arr = al8(o_len + book_a),
+                       bytes = o_arr + da.size() * sizeof(HuffArrayDev);
+        uint8_t* hu = c->pin_early.get<uint8_t>(bytes);
+        {
+            size_t i = 0;
+            for (uint64_t k = 0; k + 1 < narr; k++) {
+                const HuffArray& A = lay.arrays[k];
+                if (!A.count || A.single) continue;
+                const uint64_t bo = da[i++].book_off;
+                std::memcpy(hu + o_code + bo * 4, A.code32.data(), A.range * 4);
+                std::memcpy(hu + o_len + bo, A.len.data(), A.range);
+            }
+        }
+        std::memcpy(hu + o_arr, da.data(), da.size() * sizeof(HuffArrayDev));
+        // the upload goes on the side stream, so that it overlaps the last round
+        uint8_t* du = c->hmeta_early.get<uint8_t>(bytes);
+        HIPCHECK(hipMemcpyAsync(du, hu, bytes, hipMemcpyHostToDevice, c->side));
+        HIPCHECK(hipEventRecord(c->side_ev[2], c->side));
+        HIPCHECK(hipStreamWaitEvent(st, c->side_ev[2], 0));
+        const int kq = kev_begin(c, KEV_ENCODE, 1);
+        launch_encode_pack(sym0, reinterpret_cast<const HuffArrayDev*>(du + o_arr), (uint32_t)da.size(), 0,
+                           early_chunks, reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, estage,
+                           st);
+        kev_end(c, kq);
+    };
+    // After the early pack, the last array's pack pass as soon as its codebook
+    // is built (its chunks follow the early ones), before the stream layout.
+    bool late_packed = false;
+    auto late_pack = [&]() {
+        const HuffArray& A = lays[0].arrays[narr - 1];
+        if (!A.count || A.single || A.max_len > 32) return;
+        HuffArrayDev D;
+        std::memset(&D, 0, sizeof(D));
+        D.src_off = narr == 1 ? 0 : (narr == 2 ? ntp : ntp + P.rounds[narr - 3].qb_off);
+        D.count = A.count;
+        D.offset = A.offset;
+        D.range = A.range;
+        D.chunk0 = early_chunks;
+        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
+        D.lds_book = A.range <= kLdsBook ? 1 : 0;
+        const uint64_t o_len = A.range * 4, o_arr = (o_len + A.range + 7) / 8 * 8, bytes = o_arr + sizeof(D);
+        uint8_t* hu = c->pin_last.get<uint8_t>(bytes);
+        std::memcpy(hu, A.code32.data(), A.range * 4);
+        std::memcpy(hu + o_len, A.len.data(), A.range);
+        std::memcpy(hu + o_arr, &D, sizeof(D));
+        uint8_t* du = c->hmeta_last.get<uint8_t>(bytes);
+        HIPCHECK(hipMemcpyAsync(du, hu, bytes, hipMemcpyHostToDevice, st));
+        const int kq = kev_begin(c, KEV_ENCODE, 2);
+        launch_encode_pack(sym0, reinterpret_cast<const HuffArrayDev*>(du + o_arr), 1, early_chunks,
+                           early_chunks + D.nchunk, reinterpret_cast<const uint32_t*>(du), du + o_len,
+                           c->chunk_bits.get<uint32_t>(early_chunks + D.nchunk + 1),
+                           c->enc_stage.get<uint32_t>((early_chunks + D.nchunk) * kEncChunk), st);
+        kev_end(c, kq);
+        late_packed = true;
+    };
     try {
+        if (nf == 1 && ntask < ntask_all && !c->wide_codes && stop_round < 0) {
+            pool.wait();
+            early_pack();
+        }
         for (size_t i = ntask; i < ntask_all; i++) task(i, true);
+        if (early_chunks) late_pack();
     } catch (...) {
         abort_all = true;
         try {
@@ -899,27 +1008,33 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.stream_bytes = lays[0].total_bytes;
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lays[0].arrays[k + 1].count;
     auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
-    // codebooks as three parallel tables (64-bit entries, 32-bit codes, byte lengths)
+    const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
+    // With the early pack, only the last array's codes go up (its descriptor
+    // last_dev pointing at them); otherwise every array's.  The 64-bit book
+    // only for the wide-code path.
+    // Every chunk already packed (early and late pack passes): only the
+    // descriptors and header bytes go up; otherwise every array's codes too
+    // (and the 64-bit book for the wide-code path).
+    const bool late_only = early_chunks && enc_len <= 32, need_book = enc_len > 32;
+    if (late_only && !late_packed && chunks > early_chunks) fail(ASZ_ERR_DATA, "encoder chunk plan mismatch");
+    const uint64_t tab_total = late_only ? 0 : book_total;
+    // codebooks as parallel tables (64-bit entries, 32-bit codes, byte lengths)
     const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
-                   o_code = al8(o_book + book_total * 8), o_len = al8(o_code + book_total * 4),
-                   o_arr = al8(o_len + book_total), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
+                   o_code = al8(o_book + (need_book ? tab_total * 8 : 0)), o_len = al8(o_code + tab_total * 4),
+                   o_arr = al8(o_len + tab_total), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
     uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
     std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
     std::memcpy(hu + o_blob, blob.data(), blob.size());
-    {
+    if (!late_only) {
         size_t i = 0;
         for (uint32_t f = 0; f < nf; f++)
             for (uint64_t k = 0; k < narr; k++) {
                 const HuffArray& A = lays[f].arrays[k];
                 if (!A.count || A.single) continue;
                 const uint64_t bo = darrs[i++].book_off;
-                std::memcpy(hu + o_book + bo * 8, A.book.data(), A.range * 8);
-                uint32_t* c32 = reinterpret_cast<uint32_t*>(hu + o_code) + bo;
-                uint8_t* l8 = hu + o_len + bo;
-                for (uint64_t e = 0; e < A.range; e++) {
-                    c32[e] = (uint32_t)A.book[e];
-                    l8[e] = (uint8_t)(A.book[e] >> 56);
-                }
+                if (need_book) std::memcpy(hu + o_book + bo * 8, A.book.data(), A.range * 8);
+                std::memcpy(hu + o_code + bo * 4, A.code32.data(), A.range * 4);
+                std::memcpy(hu + o_len + bo, A.len.data(), A.range);
             }
     }
     std::memcpy(hu + o_arr, darrs.data(), darrs.size() * sizeof(HuffArrayDev));
@@ -937,11 +1052,15 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     HIPCHECK(hipEventRecord(c->ev[4], st));
     int ke = kev_begin(c, KEV_ENCODE, 0);
     const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
-    const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
-    uint32_t* estage = enc_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(enc_len)) : nullptr;
-    launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
-                  reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, enc_len, estage,
-                  st);
+    uint32_t* estage = enc_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * kEncChunk) : nullptr;
+    if (late_only) {
+        // every chunk was packed above (the last array's too): place them all
+        launch_encode_place(darr, (uint32_t)darrs.size(), chunks, cbits, coffs, estage, dwords, st);
+    } else {
+        launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),
+                      reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, coffs, dwords, enc_len,
+                      estage, st);
+    }
     launch_scatter_bytes(reinterpret_cast<const ByteSeg*>(du + o_seg), (uint32_t)bsegs.size(), du + o_blob,
                          reinterpret_cast<uint8_t*>(dwords), st);
     kev_end(c, ke);
@@ -1295,6 +1414,8 @@ void asz_ctx_destroy(asz_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
     if (c->hres_host) (void)hipHostFree(c->hres_host);
+    for (hipEvent_t e : c->side_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->order_ev) (void)hipEventDestroy(c->order_ev);

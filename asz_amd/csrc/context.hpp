@@ -117,9 +117,14 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range;
-    hipStream_t side = nullptr;  // exact histograms of arrays wider than the global window
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, hmeta_last;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early, pin_last;
+    // exact histograms of arrays wider than the global window, and the t and
+    // pivot arrays' histograms, which no round waits for
+    hipStream_t side = nullptr;
+    // ordering: main stream -> side, side -> main stream, and the encoder's
+    // early table upload (side) -> its pack pass (main stream)
+    hipEvent_t side_ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t order_ev = nullptr;  // asz_stream_wait / batch workers: cross-stream ordering
     // host result slots (pinned, written by the GPU): kHostSlotWords per array
     uint32_t* hres_host = nullptr;
@@ -152,6 +157,15 @@ struct asz_ctx {
 };
 
 namespace asz {
+
+// The side stream and its ordering events, created on first use (so that a
+// context normally owns one HIP stream).  Not thread-safe: callers create it
+// before their worker threads can.
+inline void ensure_side(asz_ctx* c) {
+    if (!c->side) HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->side_ev)
+        if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
 
 inline TaskPool& host_pool(asz_ctx* c) {
     if (!c->pool) c->pool.reset(new TaskPool(TaskPool::default_workers()));

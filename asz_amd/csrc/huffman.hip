@@ -268,11 +268,12 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
     if (bits) words[(off + bits - 1) >> 5] = 0;
 }
 
-// Pack pass (every code at most 32 bits): workgroup g packs chunks g, g + G,
-// ... (persistent, the symbols of the next two chunks in flight while one is
-// packed), each into its slot of the staging buffer (slot c at stage +
-// c * slot_words, the chunk's first code at bit 0), and leaves the chunk's
-// bit count in chunk_bits[c].  A chunk is read as 1024 int4 groups of four
+// Pack pass (every code at most 32 bits) over chunks [c_begin, c_end):
+// workgroup g packs chunks c_begin + g, c_begin + g + G, ... (persistent, the
+// symbols of the next two chunks in flight while one is packed), each into
+// its slot of the staging buffer (slot c: kEncChunk words at stage + c *
+// kEncChunk, the chunk's first code at bit 0), and leaves the chunk's bit
+// count in chunk_bits[c].  A chunk is read as 1024 int4 groups of four
 // consecutive symbols, thread t owning groups t, t + 256, t + 512 and t + 768
 // (one in each quarter of the chunk), so every load is a coalesced 16-byte
 // load.  A thread joins its group's four codes into one 64-bit value (first
@@ -300,10 +301,10 @@ __device__ __forceinline__ void load_groups(const int32_t* __restrict__ sym, con
 
 __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__ sym,
                                                      const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                     uint64_t nchunks, const uint32_t* __restrict__ code32,
+                                                     uint64_t c_begin, uint64_t c_end,
+                                                     const uint32_t* __restrict__ code32,
                                                      const uint8_t* __restrict__ len8,
-                                                     uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage,
-                                                     uint64_t slot_words) {
+                                                     uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage) {
     __shared__ __align__(16) uint32_t lbuf[kEncChunk + 4];  // bit buffer: 32 bits per symbol at most
     __shared__ uint32_t lcode[kLdsBook];
     __shared__ uint8_t llen[kLdsBook];
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
         // the load of chunk c + 2G stays in flight while this one is packed
         // (the LDS-only barriers below do not wait for it)
         const uint64_t c2 = c + 2 * G;
-        if (c2 < nchunks) load_groups(sym, arrays[chunk_array(arrays, narr, c2, k)], c2, v);
+        if (c2 < c_end) load_groups(sym, arrays[chunk_array(arrays, narr, c2, k)], c2, v);
         // group positions: inclusive wave scans of the packed counts, then the wave totals
         const uint32_t p01 = gb[0] | (gb[1] << 16), p23 = gb[2] | (gb[3] << 16);
         uint32_t i01 = p01, i23 = p23;
@@ -434,31 +435,31 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
             }
         }
         lds_barrier();
-        uint4* dst = reinterpret_cast<uint4*>(stage + c * slot_words);
+        uint4* dst = reinterpret_cast<uint4*>(stage + c * kEncChunk);
         const uint4* src = reinterpret_cast<const uint4*>(lbuf);
         for (uint32_t w = t; w < nw4; w += blockDim.x) dst[w] = src[w];
         if (t == 0) chunk_bits[c] = total;
     };
 
-    uint64_t c = blockIdx.x;
-    if (c >= nchunks) return;
+    uint64_t c = c_begin + blockIdx.x;
+    if (c >= c_end) return;
     uint32_t k = chunk_array(arrays, narr, c, 0);
     int4 va[4], vb[4];
     load_groups(sym, arrays[k], c, va);
     uint32_t kb = k;
-    if (c + G < nchunks) {
+    if (c + G < c_end) {
         kb = chunk_array(arrays, narr, c + G, k);
         load_groups(sym, arrays[kb], c + G, vb);
     }
     // chunks alternate between the two register sets: va holds c, vb c + G
     while (true) {
         pack(va, c, k);
-        if (c + G >= nchunks) break;
+        if (c + G >= c_end) break;
         pack(vb, c + G, kb);
         c += 2 * G;
-        if (c >= nchunks) break;
+        if (c >= c_end) break;
         k = chunk_array(arrays, narr, c, kb);
-        if (c + G < nchunks) kb = chunk_array(arrays, narr, c + G, k);
+        if (c + G < c_end) kb = chunk_array(arrays, narr, c + G, k);
     }
 }
 
@@ -475,8 +476,7 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
 // in; words of a quad outside the chunk are left alone.
 __global__ __launch_bounds__(64) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
-                                                     const uint32_t* __restrict__ stage, uint64_t slot_words,
-                                                     uint32_t* words) {
+                                                     const uint32_t* __restrict__ stage, uint32_t* words) {
     const uint64_t c = blockIdx.x;
     if (c >= nchunks) return;
     const uint32_t bits = chunk_bits[c];
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(64) void k_encode_place(const uint32_t* __restrict_
     const uint64_t off = chunk_off[c];
     const uint32_t sh = (uint32_t)(off & 31);
     const int64_t nsw = (bits + 31) >> 5, ndw = (sh + bits + 31) >> 5;  // ndw <= nsw + 1
-    const uint4* src4 = reinterpret_cast<const uint4*>(stage + c * slot_words);  // 256-byte aligned slot
+    const uint4* src4 = reinterpret_cast<const uint4*>(stage + c * kEncChunk);  // 16 KB slots
     const uint64_t g0 = off >> 5;  // the chunk's first stream word
     const uint64_t q0 = g0 >> 2, q1 = (g0 + (uint64_t)ndw - 1) >> 2;
     // staged word i0 - 1 of every quad sits at position rem of staged quad m
@@ -637,9 +637,35 @@ void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64
     hipLaunchKernelGGL(dev::k_hist_tail, dim3(nfields), dim3(256), 0, st, acc, k, part, nparts, gw, hres, epoch,
                        part_stride, g_stride, h_stride);
 }
-uint64_t enc_slot_words(int max_len) {
-    const uint64_t w = ((uint64_t)kEncChunk * (uint64_t)std::max(max_len, 1) + 31) / 32;
-    return (w + 63) / 64 * 64;  // slots on 256-byte boundaries
+namespace {
+// Workgroups of a persistent pass: one resident wave of them (CUs x the
+// workgroups that fit on a CU), at most kEncGroups and at most the chunks.
+uint32_t enc_groups(const void* kernel, uint64_t nchunks) {
+    int dev_id = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev_id);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+    const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
+    return (uint32_t)std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
+}
+}  // namespace
+
+void launch_encode_pack(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t c_begin,
+                        uint64_t c_end, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
+                        uint32_t* stage, hipStream_t st) {
+    if (c_end <= c_begin || !narr) return;
+    hipLaunchKernelGGL(dev::k_encode_pack, dim3(enc_groups((const void*)dev::k_encode_pack, c_end - c_begin)),
+                       dim3(256), 0, st, sym, arrays, narr, c_begin, c_end, code32, len8, chunk_bits, stage);
+}
+
+void launch_encode_place(const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks, uint32_t* chunk_bits,
+                         uint64_t* chunk_off, const uint32_t* stage, uint32_t* stream_words, hipStream_t st) {
+    if (!nchunks || !narr) return;
+    if (nchunks > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "symbol arrays exceed the encoder's launch grid");
+    hipLaunchKernelGGL(dev::k_chunk_scan, dim3((uint32_t)((nchunks + 1023) / 1024)), dim3(1024), 0, st, arrays, narr,
+                       nchunks, chunk_bits, chunk_off, stream_words);
+    hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)nchunks), dim3(64), 0, st, chunk_bits, chunk_off, nchunks,
+                       stage, stream_words);
 }
 
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
@@ -647,36 +673,18 @@ void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr
                    uint64_t* chunk_off, uint32_t* stream_words, int max_len, uint32_t* stage, hipStream_t st) {
     if (!nchunks || !narr) return;
     if (nchunks > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "symbol arrays exceed the encoder's launch grid");
-    const bool packed = max_len <= 32;
-    if (packed && !stage) fail(ASZ_ERR_ARG, "encoder staging buffer missing");
-    // one resident wave of workgroups for each persistent pass: CUs x workgroups that fit on a CU
-    int dev_id = 0, cus = 0, per_cu = 0, per_cu_bits = 0;
-    (void)hipGetDevice(&dev_id);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-    auto groups = [&](int pc) {
-        const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(pc, 1);
-        return (uint32_t)std::min<uint64_t>(nchunks, std::min<uint64_t>(resident, kEncGroups));
-    };
-    const dim3 scan_grid((uint32_t)((nchunks + 1023) / 1024));
-    if (packed) {
-        const uint64_t slot = enc_slot_words(max_len);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_pack, 256, 0);
-        hipLaunchKernelGGL(dev::k_encode_pack, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks,
-                           code32, len8, chunk_bits, stage, slot);
-        hipLaunchKernelGGL(dev::k_chunk_scan, scan_grid, dim3(1024), 0, st, arrays, narr, nchunks, chunk_bits,
-                           chunk_off, stream_words);
-        hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)nchunks), dim3(64), 0, st, chunk_bits, chunk_off,
-                           nchunks, stage, slot, stream_words);
+    if (max_len <= 32) {
+        if (!stage) fail(ASZ_ERR_ARG, "encoder staging buffer missing");
+        launch_encode_pack(sym, arrays, narr, 0, nchunks, code32, len8, chunk_bits, stage, st);
+        launch_encode_place(arrays, narr, nchunks, chunk_bits, chunk_off, stage, stream_words, st);
         return;
     }
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, dev::k_chunk_bits, 256, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_encode_long, 256, 0);
-    hipLaunchKernelGGL(dev::k_chunk_bits, dim3(groups(per_cu_bits)), dim3(256), 0, st, sym, arrays, narr, nchunks,
-                       len8, chunk_bits);
-    hipLaunchKernelGGL(dev::k_chunk_scan, scan_grid, dim3(1024), 0, st, arrays, narr, nchunks, chunk_bits, chunk_off,
-                       stream_words);
-    hipLaunchKernelGGL(dev::k_encode_long, dim3(groups(per_cu)), dim3(256), 0, st, sym, arrays, narr, nchunks, book,
-                       chunk_off, stream_words);
+    hipLaunchKernelGGL(dev::k_chunk_bits, dim3(enc_groups((const void*)dev::k_chunk_bits, nchunks)), dim3(256), 0, st,
+                       sym, arrays, narr, nchunks, len8, chunk_bits);
+    hipLaunchKernelGGL(dev::k_chunk_scan, dim3((uint32_t)((nchunks + 1023) / 1024)), dim3(1024), 0, st, arrays, narr,
+                       nchunks, chunk_bits, chunk_off, stream_words);
+    hipLaunchKernelGGL(dev::k_encode_long, dim3(enc_groups((const void*)dev::k_encode_long, nchunks)), dim3(256), 0,
+                       st, sym, arrays, narr, nchunks, book, chunk_off, stream_words);
 }
 
 }  // namespace asz

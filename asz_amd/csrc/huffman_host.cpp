@@ -75,10 +75,12 @@ void type_fields(int64_t offset, uint64_t range, uint64_t count, bool& usemp, ui
 
 void pack_book(HuffArray& a) {
     a.book.resize(a.range);
+    a.code32.resize(a.range);
     int ml = 0;
     for (uint64_t s = 0; s < a.range; s++) {
         if (a.len[s] > 56) fail(ASZ_ERR_DATA, "Huffman code longer than 56 bits");
         a.book[s] = a.code[s] | ((uint64_t)a.len[s] << 56);
+        a.code32[s] = (uint32_t)a.code[s];
         ml = std::max<int>(ml, a.len[s]);
     }
     a.max_len = ml;

@@ -240,8 +240,10 @@ struct HuffArray {
     std::vector<uint8_t> len;    // per symbol index: length, 0 = absent
     uint64_t payload_bits = 0;
     bool single = false;         // maxval == 1 path (huffman.hpp:193-210, 342-347)
-    // encoder table (pack_book): code in bits 0..55, length in bits 56..63
+    // encoder tables (pack_book): code in bits 0..55, length in bits 56..63;
+    // and the code's low 32 bits (the whole code when max_len <= 32)
     std::vector<uint64_t> book;
+    std::vector<uint32_t> code32;
     int max_len = 0;
 };
 // Build the reference tree codebook from a dense histogram (huffman.hpp:263-320, 650-691).

@@ -186,20 +186,29 @@ void launch_hist_tail(const DevAcc* acc, uint32_t k, const Partial* part, uint64
 // Huffman encoder of the arrays' chunks into the stream buffer (header bytes
 // are scattered in after it has run, with the same stream).  Every code at
 // most 32 bits (max_len <= 32): a pack pass codes each chunk from bit 0 into
-// its slot of the staging buffer (enc_slot_words(max_len) words per chunk)
-// and counts its bits into chunk_bits, a scan turns the counts into
-// chunk_off (and zeroes the payload words that chunks share), and a place
-// pass copies each slot to its bit position.  Longer codes: a bit-count pass,
-// the scan, and an encode pass that ORs the codes straight into the stream
-// (stage unused).  The codebook of array k starts at its book_off in three
-// parallel tables: book (code in bits 0..55, length in bits 56..63), code32
-// (the code, when max_len <= 32) and len8 (the length).  chunk_bits /
-// chunk_off: nchunks entries.
+// its slot of the staging buffer (kEncChunk words per chunk: stage holds
+// nchunks * kEncChunk words) and counts its bits into chunk_bits, a scan
+// turns the counts into chunk_off (and zeroes the payload words that chunks
+// share), and a place pass copies each slot to its bit position.  Longer
+// codes: a bit-count pass, the scan, and an encode pass that ORs the codes
+// straight into the stream (stage unused).  The codebook of array k starts at
+// its book_off in three parallel tables: book (code in bits 0..55, length in
+// bits 56..63), code32 (the code, when max_len <= 32) and len8 (the length).
+// chunk_bits / chunk_off: nchunks entries.
 constexpr uint64_t kEncGroups = 2048;    // most workgroups of a persistent encoder pass
-uint64_t enc_slot_words(int max_len);
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,
                    const uint64_t* book, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
                    uint64_t* chunk_off, uint32_t* stream_words, int max_len, uint32_t* stage, hipStream_t st);
+// The encoder's two halves for codes of at most 32 bits, so that arrays can be
+// packed as their codebooks arrive: the pack pass over chunks [c_begin,
+// c_end) of the listed arrays (any subset whose chunk numbers cover that
+// range), then, once every chunk is packed, the scan and place passes over all
+// nchunks chunks of the full array list.
+void launch_encode_pack(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t c_begin,
+                        uint64_t c_end, const uint32_t* code32, const uint8_t* len8, uint32_t* chunk_bits,
+                        uint32_t* stage, hipStream_t st);
+void launch_encode_place(const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks, uint32_t* chunk_bits,
+                         uint64_t* chunk_off, const uint32_t* stage, uint32_t* stream_words, hipStream_t st);
 
 // Decoder (self-synchronising parallel decode of one bit stream per array).
 struct DecArrayDev {

@@ -340,7 +340,7 @@ void generate_plan(asz_ctx* c, const float* d_field, const uint64_t dims[3], con
             if (chunks) {
                 const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
                 const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
-                uint32_t* estage = enc_len <= 32 ? c->enc_stage.get<uint32_t>(chunks * enc_slot_words(enc_len)) : nullptr;
+                uint32_t* estage = enc_len <= 32 ? c->enc_stage.get<uint32_t>(chunks * kEncChunk) : nullptr;
                 launch_encode(dq, darr, ncand, chunks, reinterpret_cast<const uint64_t*>(du + o_book),
                               reinterpret_cast<const uint32_t*>(du + o_code), du + o_len,
                               c->chunk_bits.get<uint32_t>(chunks + 1), c->chunk_off.get<uint64_t>(chunks + 1), dwords,
