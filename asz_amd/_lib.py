@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("round_ms", ctypes.c_double * MAX_LAYERS), ("hist_ms", ctypes.c_double),
                 ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double),
                 ("codebook_wait_ms", ctypes.c_double), ("codebook_max_ms", ctypes.c_double),
-                ("layout_ms", ctypes.c_double), ("plan_cache_hit", ctypes.c_uint64)]
+                ("layout_ms", ctypes.c_double), ("plan_cache_hit", ctypes.c_uint64),
+                ("coarse_rounds", ctypes.c_uint64), ("coarse_ms", ctypes.c_double)]
 
 class BatchField(ctypes.Structure):
     _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("default_slope", ctypes.c_float),
@@ -85,6 +86,7 @@ ASZ_OPT_STREAM_ON_DEVICE = 2
 ASZ_OPT_TIMED_ROUND = 3
 ASZ_OPT_BATCH_FIELDS = 4
 ASZ_OPT_WIDE_CODES = 5
+ASZ_OPT_COARSE_ROUNDS = 6
 
 
 # exported entry points (must match include/asz_gpu.h)

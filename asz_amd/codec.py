@@ -123,6 +123,11 @@ class Context:
         """Route the Huffman encoder through its path for codes longer than 32 bits."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_WIDE_CODES, 1 if on else 0))
 
+    def set_coarse_rounds(self, n: int = -1):
+        """The most leading rounds that run as block rounds (t selection and
+        targets in one launch; -1: every round that qualifies, 0: none)."""
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_COARSE_ROUNDS, int(n)))
+
     def set_stream_on_device(self, on: bool = True):
         """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))

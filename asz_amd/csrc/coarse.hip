@@ -1,0 +1,152 @@
+// Block rounds: a coarse round (at most kCoarseTargets targets per block)
+// with its per-block t selection in one launch, one workgroup per block.
+//
+// The reference runs block after block, each through every round
+// (compress.tpp:179-208); a round's targets read only coarse points that
+// earlier rounds (or the pivots) finalised, so all blocks of a round can run
+// at once after the previous round (the level-synchronous schedule of the
+// other round kernels).  For the coarse rounds the per-round work is small
+// and two launches per round (selection, then prediction) cost more than the
+// work: here one workgroup selects its block's t from the block's samples
+// (tsel.hpp, exactly as k_tsel) and then predicts, quantizes and
+// reconstructs the block's targets of the round, leaving the block's partial
+// and histogram window for the fold like any round kernel's workgroup.
+#include "tsel.hpp"
+#include "layout.hpp"
+
+namespace asz {
+namespace dev {
+
+static_assert(kHistWinMax <= 2 * kMaxKeep, "the round's LDS histogram reuses the selection's key buffers");
+
+constexpr int kBlockBatch = 4;  // targets per thread whose loads are issued together
+
+// Grid: x = block, y = field.  tsel: 0 takes each block's t from tblk, 1
+// selects it with the five-candidate scan, 2 with the gradient-tuned form.
+template <bool QUIN, bool GRAD>
+__global__ __launch_bounds__(256) void k_round_block(TselDesc T, uint32_t tsel, float* work, int32_t* sym,
+                                                     int32_t* qb, const uint32_t* __restrict__ tabs,
+                                                     const uint32_t* __restrict__ ss, float* tblk, Partial* part,
+                                                     uint32_t* phist, uint32_t* gwin, DevAcc* acc, FieldSet fs) {
+    __shared__ TselLds<GRAD> L;
+    const uint32_t b = blockIdx.x, f = blockIdx.y;
+    const RoundDesc& R = T.R;
+    // the field's layer bound (its range gives it; the plan's is field 0's)
+    double inv_half_eb = R.inv_half_eb, eb_d = R.eb_d, neg_two_eb = R.neg_two_eb;
+    float* wk = work;
+    if (fs.fd) {  // field f of a multi-field (or device-range) call
+        const FieldDesc& F = fs.fd[f];
+        const FieldRound& fr = F.r[R.layer];
+        inv_half_eb = fr.inv_half_eb;
+        eb_d = fr.eb_d;
+        neg_two_eb = fr.neg_two_eb;
+        wk = F.work;
+        qb += (uint64_t)f * fs.sym;
+        part += (uint64_t)f * fs.part;
+        phist += (uint64_t)f * fs.phist;
+        gwin += (uint64_t)f * fs.ghist;
+    }
+    // the block's t: selected here (every sample it reads is final) or given
+    float t;
+    if (tsel) {
+        tsel_block<GRAD>(T, work, ss, tblk, sym, acc, fs, b, f, L);
+        t = L.best;
+    } else {
+        t = tblk[(uint64_t)f * fs.tblk + b];
+    }
+    // the round's LDS histogram window in the selection's key buffers
+    uint32_t* lh = reinterpret_cast<uint32_t*>(L.sh.af);
+    const uint32_t win = R.lds_win;
+    for (uint32_t i = threadIdx.x; i < win; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+
+    // the block's targets: along d from bB + h in steps of 2h, along another
+    // axis from 0 (block 0) or bB + s in steps of s (the plan's per-axis
+    // counts and prefixes give the block's shape and QB base)
+    const uint32_t d = R.d;
+    const uint32_t bb[3] = {b / (R.nb[1] * R.nb[2]), (b / R.nb[2]) % R.nb[1], b % R.nb[2]};
+    uint32_t cnt[3], pre[3], first[3], step[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        cnt[a] = tabs[R.tab_off[a] + bb[a]];
+        pre[a] = tabs[R.tab_off[a] + R.nb[a] + bb[a]];
+        if ((uint32_t)a == d) {
+            first[a] = bb[a] * R.B[a] + R.h;
+            step[a] = 2 * R.h;
+        } else {
+            first[a] = bb[a] == 0 ? 0 : bb[a] * R.B[a] + R.s[a];
+            step[a] = R.s[a];
+        }
+    }
+    const uint32_t right = (uint32_t)min<uint64_t>((uint64_t)(bb[d] + 1) * R.B[d] + 1, R.D[d]);
+    const uint64_t base = R.qb_off + block_base(R, cnt, pre);
+    const uint32_t c12 = cnt[1] * cnt[2];
+    const uint32_t ntar = cnt[0] * c12;
+    const int64_t dist = (int64_t)R.dist, hh = R.h, Dd = R.D[d];
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    for (uint32_t k0 = threadIdx.x; k0 < ntar; k0 += kBlockBatch * blockDim.x) {
+        // every load of the batch first (the stores below touch only targets,
+        // which no load of the round reads but their own)
+        float w[kBlockBatch][6], x[kBlockBatch];
+        uint64_t pos[kBlockBatch];
+        uint32_t id[kBlockBatch], jj[kBlockBatch][3];
+#pragma unroll
+        for (int u = 0; u < kBlockBatch; u++) {
+            const uint32_t k = min(k0 + u * blockDim.x, ntar - 1);
+            const uint32_t j0 = k / c12, rem = k - j0 * c12, j1 = rem / cnt[2], j2 = rem - j1 * cnt[2];
+            jj[u][0] = j0;
+            jj[u][1] = j1;
+            jj[u][2] = j2;
+            const uint32_t i0 = first[0] + j0 * step[0], i1 = first[1] + j1 * step[1], i2 = first[2] + j2 * step[2];
+            id[u] = d == 0 ? i0 : (d == 1 ? i1 : i2);
+            pos[u] = ((uint64_t)i0 * R.D[1] + i1) * R.D[2] + i2;
+            // coarse window w[m] at id + (2m - 5) h (inside the field; the
+            // stencil reads only entries inside the block)
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                w[u][m] = 0.0f;
+                if (!QUIN && (m == 0 || m == 5)) continue;
+                const int64_t c = (int64_t)id[u] + (2 * m - 5) * hh;
+                if (c >= 0 && c < Dd) w[u][m] = wk[(int64_t)pos[u] + (2 * m - 5) * dist];
+            }
+            x[u] = wk[pos[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kBlockBatch; u++) {
+            if (k0 + u * blockDim.x >= ntar) break;
+            const float p = predict_w<QUIN>(id[u], R.h, right, w[u], t);
+            const int32_t q = d == 0 ? quantize_eb<0>(p, x[u], inv_half_eb, eb_d)
+                                     : quantize_eb<1>(p, x[u], inv_half_eb, eb_d);
+            const float xr = reconstruct(p, q, neg_two_eb);
+            qb[base + local_slot(R.order, jj[u], cnt)] = q;
+            wk[pos[u]] = xr;
+            const double e = (double)daz(x[u]) - (double)xr;
+            err = fmax(err, fabs(e));
+            sq += e * e;
+            qmin = min(qmin, q);
+            qmax = max(qmax, q);
+            hist_add(lh, win, gwin, q);
+        }
+    }
+    if (R.nwg) finish_wg(err, sq, qmin, qmax, R, b, part, lh, phist, gwin);
+}
+
+}  // namespace dev
+
+void launch_round_block(const TselDesc& T, uint32_t tsel, bool quintic, float* work, int32_t* sym, int32_t* qb,
+                        const uint32_t* tabs, const uint32_t* ss, float* tblk, Partial* part, uint32_t* phist,
+                        uint32_t* gwin, DevAcc* acc, hipStream_t st, const FieldSet& fs, uint32_t nfields) {
+    const uint64_t nbl = (uint64_t)T.R.nb[0] * T.R.nb[1] * T.R.nb[2];
+    if (!nbl || !nfields) return;
+    if (nbl > 0x7FFFFFFFull || nfields > 65535) throw Error(ASZ_ERR_ARG, "too many blocks for one launch");
+    const bool grad = tsel == 2;
+    using K = void (*)(TselDesc, uint32_t, float*, int32_t*, int32_t*, const uint32_t*, const uint32_t*, float*,
+                       Partial*, uint32_t*, uint32_t*, DevAcc*, FieldSet);
+    const K k = quintic ? (grad ? dev::k_round_block<true, true> : dev::k_round_block<true, false>)
+                        : (grad ? dev::k_round_block<false, true> : dev::k_round_block<false, false>);
+    hipLaunchKernelGGL(k, dim3((uint32_t)nbl, nfields), dim3(256), 0, st, T, tsel, work, sym, qb, tabs, ss, tblk, part,
+                       phist, gwin, acc, fs);
+}
+
+}  // namespace asz

@@ -51,7 +51,7 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE };
+enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE, KEV_COARSE };
 
 int kev_begin(asz_ctx* c, int kind, int r) {
     if (!c->kev_on || c->kev_n * 2 + 2 > (int)c->kev.size()) return -1;
@@ -77,6 +77,10 @@ void kev_collect(asz_ctx* c) {
             case KEV_ROUND: S.round_ms[r] += ms; break;
             case KEV_HIST: S.hist_ms += ms; break;
             case KEV_ENCODE: S.encode_ms += ms; break;
+            case KEV_COARSE:
+                S.coarse_ms += ms;
+                S.round_ms[r] += ms;
+                break;
             default: S.decode_ms += ms; break;
         }
     }
@@ -372,6 +376,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     key.plan = canonical_plan(plan);
     key.bpr = p.block_pivot_ratio;
     key.h1 = p.h1_block_sampling_num;
+    // leading rounds that may run as block rounds (t selection and targets in
+    // one launch per round)
+    key.coarse = c->coarse_max < 0 ? (uint32_t)ASZ_MAX_LAYERS : (uint32_t)c->coarse_max;
     PlanCache* pslot = nullptr;
     for (PlanCache& e : c->pcs)
         if (e.valid && std::memcmp(&key, &e.key, sizeof(key)) == 0) pslot = &e;
@@ -387,7 +394,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         pc.valid = false;
         Geometry g0 = make_geometry(dims, plan, p.block_pivot_ratio);
         pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, fin[0].default_slope,
-                         p.h1_block_sampling_num);
+                         p.h1_block_sampling_num, key.coarse);
         const Plan& P0 = pc.P;
         const PlanUpload u = plan_upload_layout(P0);
         uint8_t* hm = c->pin_meta.get<uint8_t>(u.bytes);
@@ -527,7 +534,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // Rounds [0, batch_end) are folded together after the last of them (one
     // launch for the small coarse rounds, whose codebooks are then built while
     // the six finest rounds run); every later round is folded on its own.
-    const uint64_t batch_end = L >= 7 ? L - 6 : 0;
+    // The block rounds [0, ncoarse) are always folded together.
+    const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
+    const uint64_t ncoarse = std::min<uint64_t>(P.coarse, Lrun);
+    const uint64_t batch_end = std::max<uint64_t>(L >= 7 ? L - 6 : 0, ncoarse);
     // Arrays in the order the GPU finishes them: pivots, the batch, the later
     // rounds, with the t array (final after the last t selection) before the
     // last round.
@@ -637,7 +647,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // concurrently with the launches below and with the later rounds on the GPU.
     // With one field, this thread builds the last round's codebook itself once
     // its launches are done (no hand-over to a sleeping worker at the end).
-    const uint64_t Lrun = stop_round >= 0 ? std::min<uint64_t>(L, (uint64_t)stop_round) : L;
     const uint64_t ntask_all = order.size() * nf, ntask = ntask_all - (nf == 1 ? 1 : 0);
     host_ms_w.assign(ntask_all, 0.0);
     auto task = [&](size_t i, bool spin) {
@@ -683,11 +692,40 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         for (uint64_t q = r0; q < r0 + nr; q++) mg = std::max(mg, pc.folds[q].groups);
         return mg;
     };
+    // every t is final: the t array's range, histogram and tail, on the side
+    // stream beside the last round
+    auto t_array_side = [&]() {
+        std::lock_guard<std::mutex> lk(side_mu);
+        HIPCHECK(hipEventRecord(c->side_ev[0], st));
+        HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
+        launch_minmax_i32(sym, nt, (int32_t*)&acc[0].arr[0], c->side, nf, fs.sym, sizeof(DevAcc) / 4);
+        launch_hist_small(sym, nt, nullptr, 0, gwin(0, 0), nullptr, c->side, nf, fs.sym, fs.ghist);
+        launch_hist_tail(acc, 0, nullptr, 0, gwin(0, 0), c->hres_dev, epoch, c->side, nf, 0, fs.ghist, fs.hres);
+        HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
+    };
+    // after round r: its fold (with the next round's t selection when there
+    // is one), or nothing while the coarse batch is still running
+    auto after_round = [&](uint64_t r) {
+        if (r + 1 < batch_end) return;
+        const uint64_t f0 = r + 1 == batch_end ? 0 : r;
+        if (gpu_tsel && r + 1 < Lrun) {
+            fold_r0 = f0;  // with the next round's t selection
+            fold_nr = r + 1 - f0;
+        } else {
+            int kh = kev_begin(c, KEV_HIST, (int)r);
+            launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), fold_groups(f0, r + 1 - f0), part, phist,
+                               fpart, acc, ghist, c->hres_dev, epoch, st, fs, nf);
+            kev_end(c, kh);
+        }
+    };
+    S.coarse_rounds = ncoarse;
     for (uint64_t r = 0; r < Lrun; r++) {
         const RoundDesc& R = P.rounds[r];
-        check_grid(R, nf);
+        // a block round (r < ncoarse) selects its t inside its own launch
+        const bool block_round = r < ncoarse;
+        if (!block_round) check_grid(R, nf);
         float* tb = tblk + r * nbl;
-        if (gpu_tsel) {
+        if (gpu_tsel && !block_round) {
             TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
                        p.t_min, p.t_max};
             int kt = kev_begin(c, KEV_TSEL, (int)r);
@@ -735,18 +773,22 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             if (r + 1 == Lrun) HIPCHECK(hipMemcpyAsync(sym, t_host.data(), nt * 4, hipMemcpyHostToDevice, st));
             tsel_host_ms += ms_since(h0);
         }
-        if (r + 1 == L) {
-            // every t is final: the t array's range, histogram and tail, on the
-            // side stream beside the last round
-            std::lock_guard<std::mutex> lk(side_mu);
-            HIPCHECK(hipEventRecord(c->side_ev[0], st));
-            HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
-            launch_minmax_i32(sym, nt, (int32_t*)&acc[0].arr[0], c->side, nf, fs.sym, sizeof(DevAcc) / 4);
-            launch_hist_small(sym, nt, nullptr, 0, gwin(0, 0), nullptr, c->side, nf, fs.sym, fs.ghist);
-            launch_hist_tail(acc, 0, nullptr, 0, gwin(0, 0), c->hres_dev, epoch, c->side, nf, 0, fs.ghist,
-                             fs.hres);
-            HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
+        if (block_round) {
+            // one launch: each block's t (selected here with the GPU modes, from
+            // tblk otherwise) and its targets
+            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                       p.t_min, p.t_max};
+            const uint32_t tsel = !gpu_tsel ? 0u : (tsel_mode == ASZ_TSEL_GRAD ? 2u : 1u);
+            int kc = kev_begin(c, KEV_COARSE, (int)r);
+            launch_round_block(T, tsel, p.quintic != 0, d_work, sym, qb, tabs, ss + r * nbl, tb, part, phist,
+                               gwin(0, 2 + r), acc, st, fs, nf);
+            kev_end(c, kc);
+            // the last round's t values are written by its own launch
+            if (r + 1 == L) t_array_side();
+            after_round(r);
+            continue;
         }
+        if (r + 1 == L) t_array_side();
         int kr = kev_begin(c, KEV_ROUND, (int)r);
         if (round_needs_staging(R)) {
             RoundDesc Rs = R;
@@ -764,18 +806,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                                   fs, nf);
         }
         kev_end(c, kr);
-        if (r + 1 >= batch_end) {
-            const uint64_t f0 = r + 1 == batch_end ? 0 : r;
-            if (gpu_tsel && r + 1 < Lrun) {
-                fold_r0 = f0;  // with the next round's t selection
-                fold_nr = r + 1 - f0;
-            } else {
-                int kh = kev_begin(c, KEV_HIST, (int)r);
-                launch_fold_rounds(dfolds, (uint32_t)f0, (uint32_t)(r + 1 - f0), fold_groups(f0, r + 1 - f0), part,
-                                   phist, fpart, acc, ghist, c->hres_dev, epoch, st, fs, nf);
-                kev_end(c, kh);
-            }
-        }
+        after_round(r);
     }
     HIPCHECK(hipGetLastError());
     // the main stream resumes after the side stream's histograms (its last work)
@@ -1435,6 +1466,11 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
             c->wide_codes = value != 0;
             return;
         }
+        if (option == ASZ_OPT_COARSE_ROUNDS) {
+            if (value < -1 || value > ASZ_MAX_LAYERS) fail(ASZ_ERR_ARG, "coarse rounds must be in [-1, 63]");
+            c->coarse_max = (int)value;
+            return;
+        }
         if (option == ASZ_OPT_BATCH_FIELDS) {
             if (value < 0 || value > 65535) fail(ASZ_ERR_ARG, "fields per chunk must be in [0, 65535]");
             c->batch_fields = (uint32_t)value;
@@ -1694,6 +1730,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
             c->kids.push_back(k);
         }
         const bool on_device = c->stream_on_device, wide = c->wide_codes;
+        const int coarse_max = c->coarse_max;
         // the workers' streams start after everything enqueued on this context's
         // stream (asz_stream_wait orders that after the caller's)
         if (!c->order_ev) HIPCHECK(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
@@ -1729,6 +1766,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
                     for (uint32_t j = 0; j < nf; j++) fin[j] = asz::FieldIn{d_fields[ch[j]], mn[j], mx[j], sr[j].slope};
                     k->stream_on_device = on_device;
                     k->wide_codes = wide;
+                    k->coarse_max = coarse_max;
                     asz::compress_fields(k, nf, fin.data(), d, plans[ch[0]], *p, tsel_mode, nullptr, nullptr,
                                          nullptr);
                     for (uint32_t j = 0; j < nf; j++) {

@@ -93,6 +93,8 @@ struct PlanKey {
     uint64_t dims[3];
     asz_plan plan;
     uint64_t bpr, h1;
+    uint32_t coarse;  // most leading rounds run as block rounds (kRoundBlock)
+    uint32_t pad;
 };
 
 struct PlanCache {
@@ -140,6 +142,7 @@ struct asz_ctx {
     bool stream_on_device = false;
     uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
     bool wide_codes = false;  // ASZ_OPT_WIDE_CODES
+    int coarse_max = -1;      // ASZ_OPT_COARSE_ROUNDS (-1: every leading round that qualifies)
     float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;

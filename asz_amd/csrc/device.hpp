@@ -84,6 +84,13 @@ __device__ __forceinline__ int32_t quantize(float p, float x, const RoundDesc& R
     if (AX == 0) return lround_q(R.inv_half_eb * diff);
     return lround_q((diff * 0.5) / R.eb_d);
 }
+// The same with the bound's two forms given (a field's own bounds).
+template <int AX>
+__device__ __forceinline__ int32_t quantize_eb(float p, float x, double inv_half_eb, double eb_d) {
+    double diff = (double)p - (double)daz(x);
+    if (AX == 0) return lround_q(inv_half_eb * diff);
+    return lround_q((diff * 0.5) / eb_d);
+}
 __device__ __forceinline__ float reconstruct(float p, int32_t q, double neg_two_eb) {
     return (float)__fma_rn(neg_two_eb, (double)q, (double)p);
 }
@@ -133,6 +140,43 @@ __device__ __forceinline__ void hist_add(uint32_t* lh, uint32_t win, uint32_t* g
     } else {
         const uint32_t g = (uint32_t)q + (uint32_t)kHistCentre;
         if (g < kHistGlobal) atomicAdd(&gwin[g], 1u);
+    }
+}
+
+// Per-workgroup results of a compress round kernel, stored at the
+// workgroup's linear index wgl within the round: one Partial (max |x - x'|,
+// sum (x - x')^2, QB range) and the LDS histogram window.  Called by every
+// thread after its last QB; the barrier inside also completes the LDS
+// histogram.
+__device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, int32_t qmax, const RoundDesc& R,
+                                          uint64_t wgl, Partial* part, const uint32_t* lh, uint32_t* phist,
+                                          uint32_t* gwin) {
+    __shared__ double s_err[4], s_sq[4];
+    __shared__ int32_t s_min[4], s_max[4];
+    err = wave_max(err);
+    sq = wave_sum(sq);
+    qmin = wave_min(qmin);
+    qmax = wave_max(qmax);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) { s_err[w] = err; s_sq[w] = sq; s_min[w] = qmin; s_max[w] = qmax; }
+    __syncthreads();
+    const uint64_t wg = R.part_off + wgl;
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            err = fmax(err, s_err[k]);
+            sq += s_sq[k];
+            qmin = min(qmin, s_min[k]);
+            qmax = max(qmax, s_max[k]);
+        }
+        part[wg] = Partial{err, sq, qmin, qmax};
+    }
+    // central bins to the partial, the others (nonzero) to the global window
+    uint32_t* out = phist + R.hist_off + (wg - R.part_off) * R.hist_win;
+    const uint32_t c0 = (R.lds_win - R.hist_win) / 2;
+    for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) {
+        const uint32_t v = lh[i];
+        if (i - c0 < R.hist_win) out[i - c0] = v;
+        else if (v) atomicAdd(&gwin[(uint32_t)kHistCentre - R.lds_win / 2 + i], v);
     }
 }
 

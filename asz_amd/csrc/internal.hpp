@@ -117,6 +117,11 @@ struct FieldSet {
 
 constexpr uint32_t kRoundRow = 0;     // round axis 2: waves walk lattice rows along DIM2
 constexpr uint32_t kRoundSweep = 1;   // register-window sweep along the round axis (axis 0 / 1)
+constexpr uint32_t kRoundBlock = 2;   // block round: a workgroup per block selects its t and predicts its targets
+// A leading round runs as a block round when it has at most kCoarseTargets
+// targets per block (one 256-thread workgroup per block does its selection
+// and its targets in one launch).
+constexpr uint64_t kCoarseTargets = 16384;
 constexpr uint32_t kSweepSeg = 24;    // most targets per sweep segment (register window length)
 constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave processes together
 
@@ -191,11 +196,14 @@ struct Plan {
     uint64_t total_hist_words = 0; // per-workgroup histogram windows over all rounds
     uint64_t pivot_count;
     uint64_t qb_total;             // QB buffer elements: pivots and rounds, each array aligned (kSymAlign)
+    uint32_t coarse = 0;           // leading rounds of kind kRoundBlock
     uint32_t pn[3];                // pivot lattice counts per axis
     float pivot_eb;
 };
+// coarse_max: the most leading rounds that may run as block rounds (those
+// with at most kCoarseTargets targets per block).
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num);
+               float default_slope, uint64_t h1_num, uint32_t coarse_max = 0);
 // The per-call scalars of a plan (pivot and layer error bounds, layer slopes);
 // the tables do not depend on them.
 void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,

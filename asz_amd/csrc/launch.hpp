@@ -114,6 +114,16 @@ void launch_fold_tsel(const TselDesc& T, const float* work, const uint32_t* ss, 
                       uint32_t* ghist, uint32_t* hres, uint32_t epoch, hipStream_t st, const FieldSet& fs = FieldSet{},
                       uint32_t nfields = 1);
 
+// Block round (coarse.hip): round T.R of kind kRoundBlock with its t
+// selection, one workgroup per (block, field).  tsel: 0 takes each block's t
+// from tblk (the round's slice), 1 selects it with the five-candidate scan,
+// 2 with the gradient-tuned form (the t array sym and tblk receive it).
+// gwin: the round array's global window; the pointers are field 0's (with
+// fs.fd set, field f's slices at the FieldSet strides).
+void launch_round_block(const TselDesc& T, uint32_t tsel, bool quintic, float* work, int32_t* sym, int32_t* qb,
+                        const uint32_t* tabs, const uint32_t* ss, float* tblk, Partial* part, uint32_t* phist,
+                        uint32_t* gwin, DevAcc* acc, hipStream_t st, const FieldSet& fs, uint32_t nfields);
+
 // Residual energy of round R per block (forward: E[b], FP64) or its adjoint
 // (g[b * np + j] = gE[b] * dE_b/dtheta_j, FP32), np = 1 (cubic t) or 2 (quintic a, b).
 void launch_energy(const RoundDesc& R, uint64_t nblocks, bool quintic, bool adjoint, const float* work,

@@ -50,7 +50,7 @@ void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio
 }
 
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num) {
+               float default_slope, uint64_t h1_num, uint32_t coarse_max) {
     Plan P;
     P.g = g;
     P.pivot_count = 1;
@@ -145,6 +145,13 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             uint64_t rpw = std::min<uint64_t>(std::max<uint64_t>(4, std::min(want, cap)), 1024);
             R.rpw = (uint32_t)((rpw + g - 1) / g * g);
             R.nwg = (rows + R.rpw - 1) / R.rpw;
+        }
+        // a leading round with few targets per block: a block round, one
+        // workgroup per block (a partial and a window each)
+        if ((uint32_t)r == P.coarse && P.coarse < coarse_max && R.count <= kCoarseTargets * nbl) {
+            R.kind = kRoundBlock;
+            R.nwg = nbl;
+            P.coarse++;
         }
         if (R.count == 0) R.nwg = 0;
         R.part_off = P.total_partials;

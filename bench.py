@@ -83,6 +83,7 @@ def parse():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--tsel", default="gpu", choices=["gpu", "exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the main workload (no other-workload keys)")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -335,7 +336,7 @@ def main():
     }
     # the gradient-tuned form against the discrete t scan at REL 1e-3 on the
     # half-smooth / half-rough 512^3 field (compression ratio, max error, t-selection time)
-    if a.workload == "c2":
+    if a.workload == "c2" and not a.no_extras:
         del works, out, recon, err
         torch.cuda.empty_cache()
         from asz_amd import tune

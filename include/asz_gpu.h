@@ -91,6 +91,11 @@ typedef struct asz_stats {
     /* 1 when the call found its plan tables among the context's cached plans
      * (the last four (dims, plan, block pivot ratio, sampling count) keys) */
     uint64_t plan_cache_hit;
+    /* the leading rounds that ran as block rounds (t selection and targets in
+     * one launch each) and those launches' device time (ASZ_OPT_KERNEL_EVENTS;
+     * also in round_ms) */
+    uint64_t coarse_rounds;
+    double coarse_ms;
 } asz_stats;
 
 /* Options (asz_set_option).  ASZ_OPT_KERNEL_EVENTS: record a HIP event pair on
@@ -116,6 +121,13 @@ typedef struct asz_stats {
  * into the stream) whatever the longest code is.  The stream is the same;
  * the option exists to exercise that path. */
 #define ASZ_OPT_WIDE_CODES 5
+/* ASZ_OPT_COARSE_ROUNDS: the most leading rounds that run as block rounds:
+ * one launch per round in which a workgroup selects its block's t and then
+ * predicts the block's targets (rounds with at most 16384 targets per block
+ * qualify), instead of a t-selection launch and a round launch.  -1
+ * (default): every leading round that qualifies; 0: none.  The stream is the
+ * same either way. */
+#define ASZ_OPT_COARSE_ROUNDS 6
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 /* Orders the context's later work after everything already enqueued on
  * `stream` (a hipStream_t of the same device; NULL: the null stream), without
