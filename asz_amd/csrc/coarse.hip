@@ -11,6 +11,14 @@
 // (tsel.hpp, exactly as k_tsel) and then predicts, quantizes and
 // reconstructs the block's targets of the round, leaving the block's partial
 // and histogram window for the fold like any round kernel's workgroup.
+#ifdef ASZ_COARSE_TRACE
+#include <hip/hip_runtime.h>
+// timing builds: per (round, block) phase marks of the block-round kernel (s_memrealtime)
+__device__ unsigned long long g_tsel_trace[16 * 4096 * 8];
+#define TSEL_MARK(k)                                          \
+    if (threadIdx.x == 0 && blockIdx.x < 4096 && T.r < 16)    \
+    g_tsel_trace[((unsigned long long)T.r * 4096 + blockIdx.x) * 8 + (k)] = __builtin_amdgcn_s_memrealtime()
+#endif
 #include "tsel.hpp"
 #include "layout.hpp"
 
@@ -19,7 +27,7 @@ namespace dev {
 
 static_assert(kHistWinMax <= 2 * kMaxKeep, "the round's LDS histogram reuses the selection's key buffers");
 
-constexpr int kBlockBatch = 4;  // targets per thread whose loads are issued together
+constexpr int kBlockBatch = 8;  // targets per thread whose loads are issued together
 
 // Grid: x = block, y = field.  tsel: 0 takes each block's t from tblk, 1
 // selects it with the five-candidate scan, 2 with the gradient-tuned form.
@@ -51,6 +59,7 @@ __global__ __launch_bounds__(256) void k_round_block(TselDesc T, uint32_t tsel, 
     if (tsel) {
         tsel_block<GRAD>(T, work, ss, tblk, sym, acc, fs, b, f, L);
         t = L.best;
+        TSEL_MARK(6);
     } else {
         t = tblk[(uint64_t)f * fs.tblk + b];
     }
@@ -85,42 +94,49 @@ __global__ __launch_bounds__(256) void k_round_block(TselDesc T, uint32_t tsel, 
     const int64_t dist = (int64_t)R.dist, hh = R.h, Dd = R.D[d];
     double err = 0, sq = 0;
     int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    // lattice coordinates of local target k (j: the block-local index)
+    auto coords = [&](uint32_t k, uint32_t j[3], uint32_t i[3]) {
+        j[0] = k / c12;
+        const uint32_t rem = k - j[0] * c12;
+        j[1] = rem / cnt[2];
+        j[2] = rem - j[1] * cnt[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) i[a] = first[a] + j[a] * step[a];
+    };
     for (uint32_t k0 = threadIdx.x; k0 < ntar; k0 += kBlockBatch * blockDim.x) {
         // every load of the batch first (the stores below touch only targets,
         // which no load of the round reads but their own)
         float w[kBlockBatch][6], x[kBlockBatch];
-        uint64_t pos[kBlockBatch];
-        uint32_t id[kBlockBatch], jj[kBlockBatch][3];
 #pragma unroll
         for (int u = 0; u < kBlockBatch; u++) {
-            const uint32_t k = min(k0 + u * blockDim.x, ntar - 1);
-            const uint32_t j0 = k / c12, rem = k - j0 * c12, j1 = rem / cnt[2], j2 = rem - j1 * cnt[2];
-            jj[u][0] = j0;
-            jj[u][1] = j1;
-            jj[u][2] = j2;
-            const uint32_t i0 = first[0] + j0 * step[0], i1 = first[1] + j1 * step[1], i2 = first[2] + j2 * step[2];
-            id[u] = d == 0 ? i0 : (d == 1 ? i1 : i2);
-            pos[u] = ((uint64_t)i0 * R.D[1] + i1) * R.D[2] + i2;
+            uint32_t j[3], i[3];
+            coords(min(k0 + u * blockDim.x, ntar - 1), j, i);
+            const uint32_t id = i[d];
+            const uint64_t pos = ((uint64_t)i[0] * R.D[1] + i[1]) * R.D[2] + i[2];
             // coarse window w[m] at id + (2m - 5) h (inside the field; the
             // stencil reads only entries inside the block)
 #pragma unroll
             for (int m = 0; m < 6; m++) {
                 w[u][m] = 0.0f;
                 if (!QUIN && (m == 0 || m == 5)) continue;
-                const int64_t c = (int64_t)id[u] + (2 * m - 5) * hh;
-                if (c >= 0 && c < Dd) w[u][m] = wk[(int64_t)pos[u] + (2 * m - 5) * dist];
+                const int64_t c = (int64_t)id + (2 * m - 5) * hh;
+                if (c >= 0 && c < Dd) w[u][m] = wk[(int64_t)pos + (2 * m - 5) * dist];
             }
-            x[u] = wk[pos[u]];
+            x[u] = wk[pos];
         }
 #pragma unroll
         for (int u = 0; u < kBlockBatch; u++) {
-            if (k0 + u * blockDim.x >= ntar) break;
-            const float p = predict_w<QUIN>(id[u], R.h, right, w[u], t);
+            const uint32_t k = k0 + u * blockDim.x;
+            if (k >= ntar) break;
+            uint32_t j[3], i[3];
+            coords(k, j, i);
+            const uint64_t pos = ((uint64_t)i[0] * R.D[1] + i[1]) * R.D[2] + i[2];
+            const float p = predict_w<QUIN>(i[d], R.h, right, w[u], t);
             const int32_t q = d == 0 ? quantize_eb<0>(p, x[u], inv_half_eb, eb_d)
                                      : quantize_eb<1>(p, x[u], inv_half_eb, eb_d);
             const float xr = reconstruct(p, q, neg_two_eb);
-            qb[base + local_slot(R.order, jj[u], cnt)] = q;
-            wk[pos[u]] = xr;
+            qb[base + local_slot(R.order, j, cnt)] = q;
+            wk[pos] = xr;
             const double e = (double)daz(x[u]) - (double)xr;
             err = fmax(err, fabs(e));
             sq += e * e;
@@ -130,9 +146,17 @@ __global__ __launch_bounds__(256) void k_round_block(TselDesc T, uint32_t tsel, 
         }
     }
     if (R.nwg) finish_wg(err, sq, qmin, qmax, R, b, part, lh, phist, gwin);
+    TSEL_MARK(7);
 }
 
 }  // namespace dev
+
+#ifdef ASZ_COARSE_TRACE
+extern "C" int asz_coarse_trace(unsigned long long* out, uint64_t n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_tsel_trace), std::min<uint64_t>(n, 16 * 4096 * 8) * 8, 0,
+                               hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+}
+#endif
 
 void launch_round_block(const TselDesc& T, uint32_t tsel, bool quintic, float* work, int32_t* sym, int32_t* qb,
                         const uint32_t* tabs, const uint32_t* ss, float* tblk, Partial* part, uint32_t* phist,

@@ -51,18 +51,19 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE, KEV_COARSE };
+enum KevKind { KEV_PIVOT, KEV_TSEL, KEV_ROUND, KEV_HIST, KEV_ENCODE, KEV_DECODE, KEV_COARSE, KEV_RANGE };
 
-int kev_begin(asz_ctx* c, int kind, int r) {
+// s: the stream the kernel runs on (nullptr: the context's stream)
+int kev_begin(asz_ctx* c, int kind, int r, hipStream_t s = nullptr) {
     if (!c->kev_on || c->kev_n * 2 + 2 > (int)c->kev.size()) return -1;
-    if (c->kev_round >= 0 && !(kind == KEV_ROUND && r == c->kev_round)) return -1;
+    if (c->kev_round >= 0 && !((kind == KEV_ROUND || kind == KEV_COARSE) && r == c->kev_round)) return -1;
     int i = c->kev_n++;
     c->kev_tag[i] = {kind, r};
-    (void)hipEventRecord(c->kev[2 * i], c->stream);
+    (void)hipEventRecord(c->kev[2 * i], s ? s : c->stream);
     return i;
 }
-void kev_end(asz_ctx* c, int i) {
-    if (i >= 0) (void)hipEventRecord(c->kev[2 * i + 1], c->stream);
+void kev_end(asz_ctx* c, int i, hipStream_t s = nullptr) {
+    if (i >= 0) (void)hipEventRecord(c->kev[2 * i + 1], s ? s : c->stream);
 }
 // After the stream is synchronised: fold the recorded pairs into the stats.
 void kev_collect(asz_ctx* c) {
@@ -77,6 +78,7 @@ void kev_collect(asz_ctx* c) {
             case KEV_ROUND: S.round_ms[r] += ms; break;
             case KEV_HIST: S.hist_ms += ms; break;
             case KEV_ENCODE: S.encode_ms += ms; break;
+            case KEV_RANGE: S.range_ms += ms; break;
             case KEV_COARSE:
                 S.coarse_ms += ms;
                 S.round_ms[r] += ms;
@@ -438,6 +440,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
     asz_stats& S = c->stats;
     S = asz_stats{};
+    c->kev_n = 0;
     S.layers = L;
     S.t_count = nt;
     S.plan_cache_hit = hit ? 1 : 0;
@@ -457,6 +460,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     fs.fpart = al64(npp + pc.fpart_total + 1);
     fs.ghist = narr * kHistStride;  // global histogram windows, one kHistStride slot per array
     fs.hres = narr * kHostSlotWords;
+    fs.aligned16 = 1;
+    for (uint32_t f = 0; f < nf; f++)
+        if (reinterpret_cast<uintptr_t>(fin[f].work) & 15u) fs.aligned16 = 0;
     int32_t* sym0 = c->sym.get<int32_t>(fs.sym * nf);
     float* tblk = c->tblk.get<float>(fs.tblk * nf);
     Partial* part = c->partials.get<Partial>(fs.part * nf);
@@ -475,7 +481,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         uint32_t* rng = c->misc.get<uint32_t>(kRangeWords + 4);
         const std::vector<uint32_t> init = range_init(1);
         HIPCHECK(hipMemcpyAsync(rng, init.data(), kRangeWords * 4, hipMemcpyHostToDevice, st));
+        const int kg = kev_begin(c, KEV_RANGE, 0);
         launch_range(d_work, N, rng, st);
+        kev_end(c, kg);
         RangeSeed seed{};
         seed.work = d_work;
         seed.rel_eb = p.rel_eb;
@@ -629,7 +637,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         build_codebook(A, hexact.data());
         if (!A.single) pack_book(A);
     };
-    c->kev_n = 0;
     HIPCHECK(hipEventRecord(c->ev[0], st));
     PivotDesc PD = pivot_desc(P);
     int kp = kev_begin(c, KEV_PIVOT, 0);
@@ -873,7 +880,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             }
         }
         std::memcpy(hu + o_arr, da.data(), da.size() * sizeof(HuffArrayDev));
-        // the upload goes on the side stream, so that it overlaps the last round
+        // The upload goes on the side stream, so that it overlaps the last
+        // round; the pack pass follows the last round on the main stream (run
+        // beside it, the two only share the HBM bandwidth).
+        std::lock_guard<std::mutex> lk(side_mu);
         uint8_t* du = c->hmeta_early.get<uint8_t>(bytes);
         HIPCHECK(hipMemcpyAsync(du, hu, bytes, hipMemcpyHostToDevice, c->side));
         HIPCHECK(hipEventRecord(c->side_ev[2], c->side));
@@ -884,43 +894,12 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                            st);
         kev_end(c, kq);
     };
-    // After the early pack, the last array's pack pass as soon as its codebook
-    // is built (its chunks follow the early ones), before the stream layout.
-    bool late_packed = false;
-    auto late_pack = [&]() {
-        const HuffArray& A = lays[0].arrays[narr - 1];
-        if (!A.count || A.single || A.max_len > 32) return;
-        HuffArrayDev D;
-        std::memset(&D, 0, sizeof(D));
-        D.src_off = narr == 1 ? 0 : (narr == 2 ? ntp : ntp + P.rounds[narr - 3].qb_off);
-        D.count = A.count;
-        D.offset = A.offset;
-        D.range = A.range;
-        D.chunk0 = early_chunks;
-        D.nchunk = (A.count + kEncChunk - 1) / kEncChunk;
-        D.lds_book = A.range <= kLdsBook ? 1 : 0;
-        const uint64_t o_len = A.range * 4, o_arr = (o_len + A.range + 7) / 8 * 8, bytes = o_arr + sizeof(D);
-        uint8_t* hu = c->pin_last.get<uint8_t>(bytes);
-        std::memcpy(hu, A.code32.data(), A.range * 4);
-        std::memcpy(hu + o_len, A.len.data(), A.range);
-        std::memcpy(hu + o_arr, &D, sizeof(D));
-        uint8_t* du = c->hmeta_last.get<uint8_t>(bytes);
-        HIPCHECK(hipMemcpyAsync(du, hu, bytes, hipMemcpyHostToDevice, st));
-        const int kq = kev_begin(c, KEV_ENCODE, 2);
-        launch_encode_pack(sym0, reinterpret_cast<const HuffArrayDev*>(du + o_arr), 1, early_chunks,
-                           early_chunks + D.nchunk, reinterpret_cast<const uint32_t*>(du), du + o_len,
-                           c->chunk_bits.get<uint32_t>(early_chunks + D.nchunk + 1),
-                           c->enc_stage.get<uint32_t>((early_chunks + D.nchunk) * kEncChunk), st);
-        kev_end(c, kq);
-        late_packed = true;
-    };
     try {
         if (nf == 1 && ntask < ntask_all && !c->wide_codes && stop_round < 0) {
             pool.wait();
             early_pack();
         }
         for (size_t i = ntask; i < ntask_all; i++) task(i, true);
-        if (early_chunks) late_pack();
     } catch (...) {
         abort_all = true;
         try {
@@ -1040,20 +1019,30 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     for (uint64_t k = 0; k <= L; k++) S.qb_counts[k] = lays[0].arrays[k + 1].count;
     auto al8 = [](uint64_t v) { return (v + 7) / 8 * 8; };
     const int enc_len = c->wide_codes ? std::max(max_len, 33) : max_len;  // > 32: the wide-code path
-    // With the early pack, only the last array's codes go up (its descriptor
-    // last_dev pointing at them); otherwise every array's.  The 64-bit book
-    // only for the wide-code path.
-    // Every chunk already packed (early and late pack passes): only the
-    // descriptors and header bytes go up; otherwise every array's codes too
-    // (and the 64-bit book for the wide-code path).
+    // With the early pack (every array but the last already packed), only the
+    // last array's codes go up, with a descriptor of its own for its pack
+    // pass; otherwise every array's (and the 64-bit book for the wide-code
+    // path).  One upload carries them with the header segments and bytes and
+    // the array descriptors.
     const bool late_only = early_chunks && enc_len <= 32, need_book = enc_len > 32;
-    if (late_only && !late_packed && chunks > early_chunks) fail(ASZ_ERR_DATA, "encoder chunk plan mismatch");
-    const uint64_t tab_total = late_only ? 0 : book_total;
+    const HuffArray& AL = lays[0].arrays[narr - 1];
+    const bool late = late_only && AL.count && !AL.single;
+    const uint64_t late_chunks = late ? (AL.count + kEncChunk - 1) / kEncChunk : 0;
+    if (late_only && chunks != early_chunks + late_chunks) fail(ASZ_ERR_DATA, "encoder chunk plan mismatch");
+    const uint64_t tab_total = late_only ? (late ? AL.range : 0) : book_total;
     // codebooks as parallel tables (64-bit entries, 32-bit codes, byte lengths)
     const uint64_t o_seg = 0, o_blob = al8(bsegs.size() * sizeof(ByteSeg)), o_book = al8(o_blob + blob.size()),
                    o_code = al8(o_book + (need_book ? tab_total * 8 : 0)), o_len = al8(o_code + tab_total * 4),
-                   o_arr = al8(o_len + tab_total), up_bytes = o_arr + darrs.size() * sizeof(HuffArrayDev) + 8;
+                   o_arr = al8(o_len + tab_total), o_late = o_arr + darrs.size() * sizeof(HuffArrayDev),
+                   up_bytes = o_late + sizeof(HuffArrayDev) + 8;
     uint8_t* hu = c->pin_huff.get<uint8_t>(up_bytes);
+    if (late) {
+        std::memcpy(hu + o_code, AL.code32.data(), AL.range * 4);
+        std::memcpy(hu + o_len, AL.len.data(), AL.range);
+        HuffArrayDev D = darrs.back();  // the last array: its chunks follow the early ones
+        D.book_off = 0;
+        std::memcpy(hu + o_late, &D, sizeof(D));
+    }
     std::memcpy(hu + o_seg, bsegs.data(), bsegs.size() * sizeof(ByteSeg));
     std::memcpy(hu + o_blob, blob.data(), blob.size());
     if (!late_only) {
@@ -1085,7 +1074,13 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     const HuffArrayDev* darr = reinterpret_cast<const HuffArrayDev*>(du + o_arr);
     uint32_t* estage = enc_len <= 32 && chunks ? c->enc_stage.get<uint32_t>(chunks * kEncChunk) : nullptr;
     if (late_only) {
-        // every chunk was packed above (the last array's too): place them all
+        // the last array's pack pass, then the scan and place passes over every chunk
+        if (late) {
+            const int kq = kev_begin(c, KEV_ENCODE, 2);
+            launch_encode_pack(sym0, reinterpret_cast<const HuffArrayDev*>(du + o_late), 1, early_chunks, chunks,
+                               reinterpret_cast<const uint32_t*>(du + o_code), du + o_len, cbits, estage, st);
+            kev_end(c, kq);
+        }
         launch_encode_place(darr, (uint32_t)darrs.size(), chunks, cbits, coffs, estage, dwords, st);
     } else {
         launch_encode(sym0, darr, (uint32_t)darrs.size(), chunks, reinterpret_cast<const uint64_t*>(du + o_book),

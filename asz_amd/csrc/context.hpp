@@ -119,8 +119,8 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, hmeta_last;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early, pin_last;
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early;
     // exact histograms of arrays wider than the global window, and the t and
     // pivot arrays' histograms, which no round waits for
     hipStream_t side = nullptr;

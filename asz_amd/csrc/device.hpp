@@ -103,23 +103,58 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 
 // ---------------------------------------------------------------- reductions
 
+// Wave-wide reductions on DPP row operations (VALU data movement) instead of
+// LDS-path lane shuffles: an all-reduce inside each quad (quad_perm
+// [1,0,3,2], [2,3,0,1]) and each row of 16 lanes (row_ror 4, 8), then the
+// four row results read with readlane.  Every lane gets the result; the wave
+// must be fully active.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp32<CTRL>(__builtin_bit_cast(uint32_t, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = dpp32<CTRL>((uint32_t)u), hi = dpp32<CTRL>((uint32_t)(u >> 32));
+        return __builtin_bit_cast(T, lo | (hi << 32));
+    }
+}
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint32_t, v), lane));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+        return __builtin_bit_cast(T, lo | (hi << 32));
+    }
+}
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+    v = op(v, dpp_move<kDppXor1>(v));
+    v = op(v, dpp_move<kDppXor2>(v));
+    v = op(v, dpp_move<kDppRor4>(v));
+    v = op(v, dpp_move<kDppRor8>(v));
+    return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
+}
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return wave_reduce(v, [](T a, T b) { return a + b; });
 }
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, kWave); v = w > v ? w : v; }
-    return v;
+    return wave_reduce(v, [](T a, T b) { return b > a ? b : a; });
 }
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, kWave); v = w < v ? w : v; }
-    return v;
+    return wave_reduce(v, [](T a, T b) { return b < a ? b : a; });
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double* p, double v) {

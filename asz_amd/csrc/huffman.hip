@@ -463,10 +463,11 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
     }
 }
 
-// Place pass: the staged bits of chunk c (workgroup c, one wave) at bit
-// chunk_off[c] of the stream.  The wave writes the chunk's aligned 16-byte
-// quads of stream words, lane l taking quads q0 + l, q0 + l + 64, ... four at
-// a time (their loads all issued before the stores).  Stream word i of the
+// Place pass: the staged bits of chunk c (one wave; a workgroup's four waves
+// take four consecutive chunks) at bit chunk_off[c] of the stream.  The wave
+// writes the chunk's aligned 16-byte quads of stream words, lane l taking
+// quads q0 + l, q0 + l + 64, ... four at a time (their loads all issued
+// before the stores).  Stream word i of the
 // chunk (i = 0 at word chunk_off[c] / 32) is (s[i] << sh) | (s[i-1] >> (32 -
 // sh)) over the staged words s (zero outside the chunk's nsw words), so a
 // quad needs the five staged words from i0 - 1 on, which lie in two aligned
@@ -474,11 +475,12 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
 // with one 16-byte store; the chunk's first and last words (zeroed by
 // k_chunk_scan, shared with the neighbouring chunks or header bytes) are ORed
 // in; words of a quad outside the chunk are left alone.
-__global__ __launch_bounds__(64) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
-                                                     const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
-                                                     const uint32_t* __restrict__ stage, uint32_t* words) {
-    const uint64_t c = blockIdx.x;
-    if (c >= nchunks) return;
+__global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict__ chunk_bits,
+                                                      const uint64_t* __restrict__ chunk_off, uint64_t nchunks,
+                                                      const uint32_t* __restrict__ stage, uint32_t* words) {
+    const uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (c >= nchunks) return;  // no barrier below: each wave works alone
     const uint32_t bits = chunk_bits[c];
     if (!bits) return;
     const uint64_t off = chunk_off[c];
@@ -504,14 +506,14 @@ __global__ __launch_bounds__(64) void k_encode_place(const uint32_t* __restrict_
         uint4 A[4], B[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const uint64_t q = qb + threadIdx.x + 64 * u;
+            const uint64_t q = qb + lane + 64 * u;
             const int64_t m = (a0 + 4 * (int64_t)(q - q0) - (int64_t)rem) / 4;  // exact: the numerator is 4 m
             A[u] = q <= q1 ? quad(m) : make_uint4(0, 0, 0, 0);
             B[u] = q <= q1 ? quad(m + 1) : make_uint4(0, 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const uint64_t q = qb + threadIdx.x + 64 * u;
+            const uint64_t q = qb + lane + 64 * u;
             if (q > q1) break;
             // w[k]: staged word i0 - 1 + k
             const uint32_t x[8] = {A[u].x, A[u].y, A[u].z, A[u].w, B[u].x, B[u].y, B[u].z, B[u].w};
@@ -664,8 +666,8 @@ void launch_encode_place(const HuffArrayDev* arrays, uint32_t narr, uint64_t nch
     if (nchunks > 0x7FFFFFFFull) fail(ASZ_ERR_ARG, "symbol arrays exceed the encoder's launch grid");
     hipLaunchKernelGGL(dev::k_chunk_scan, dim3((uint32_t)((nchunks + 1023) / 1024)), dim3(1024), 0, st, arrays, narr,
                        nchunks, chunk_bits, chunk_off, stream_words);
-    hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)nchunks), dim3(64), 0, st, chunk_bits, chunk_off, nchunks,
-                       stage, stream_words);
+    hipLaunchKernelGGL(dev::k_encode_place, dim3((uint32_t)((nchunks + 3) / 4)), dim3(256), 0, st, chunk_bits,
+                       chunk_off, nchunks, stage, stream_words);
 }
 
 void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr, uint64_t nchunks,

@@ -113,6 +113,8 @@ struct FieldSet {
     uint64_t fpart;  // pivot partials then fold partials
     uint64_t ghist;  // global histogram windows (words)
     uint64_t hres;   // host result slots (words)
+    uint32_t aligned16;  // with fd set: every field's array starts on a 16-byte boundary
+    uint32_t pad;
 };
 
 constexpr uint32_t kRoundRow = 0;     // round axis 2: waves walk lattice rows along DIM2

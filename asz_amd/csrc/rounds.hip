@@ -17,7 +17,8 @@
 //    staged in LDS and written as contiguous runs when it runs along the round
 //    axis ("120" on axis 0, "021" on axis 1), and are scattered otherwise.
 //  * k_round_row (round axis 2, every order): waves walk whole lattice rows
-//    along DIM2, one target per lane per step.
+//    along DIM2, one target per lane per step; k_round_quad for h = 1, two
+//    targets per lane in one 16-byte load and store.
 // Error statistics and QB ranges are reduced per workgroup into partials and
 // folded right after each round (k_fold_round); each compress workgroup also
 // leaves an LDS histogram window of its QBs for the Huffman coder.
@@ -56,8 +57,7 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
     const float4* x4 = reinterpret_cast<const float4*>(x);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const float4 v = x4[i];
+    auto scan4 = [&](const float4 v, uint64_t i) {
         const float e[4] = {v.x, v.y, v.z, v.w};
         uint32_t key[4];
         bool z = false;
@@ -79,7 +79,17 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
                 }
             }
         }
+    };
+    // four 16-byte loads in flight per thread, then their keys
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = x4[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++) scan4(v[u], i + u * stride);
     }
+    for (; i < n4; i += stride) scan4(x4[i], i);
     for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         bad |= !isfinite(x[i]);
         const uint32_t key = order_key(x[i]);
@@ -370,6 +380,159 @@ __global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restric
                 }
                 if (pair) *reinterpret_cast<float2*>(rowp[u] + i2 - 1) = make_float2(w[u][2], xr);
                 else rowp[u][i2] = xr;
+            }
+        }
+    }
+    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist, gwin);
+}
+
+// Axis-2 rounds with h = 1 on rows of a multiple of four elements (16-byte
+// aligned field): targets are the odd elements of every lattice row, the
+// coarse values the even ones.  The workgroup layout is k_round_row's (waves
+// walk R.rpw rows per workgroup, kRowsPerWave at a time), but a lane reads
+// and writes four consecutive elements (two targets) with one 16-byte access
+// each, instead of five loads and one store per target.
+template <bool QUIN, bool DEC, int ORDER>
+__global__ __launch_bounds__(256) void k_round_quad(RoundDesc R, float* __restrict__ work, int32_t* __restrict__ qb,
+                                                    const uint32_t* __restrict__ tabs, const float* __restrict__ tblk,
+                                                    Partial* part, uint32_t* __restrict__ phist,
+                                                    uint32_t* __restrict__ gwin, FieldSet fs) {
+    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.lds_win bins
+    if (!DEC && fs.fd) {  // field blockIdx.y of a multi-field call
+        const uint32_t f = blockIdx.y;
+        const FieldDesc& F = fs.fd[f];
+        work = F.work;
+        qb += f * fs.sym;
+        tblk += f * fs.tblk;
+        part += f * fs.part;
+        phist += f * fs.phist;
+        gwin += f * fs.ghist;
+        field_round(F, R);
+    }
+    constexpr int RU = (int)kRowsPerWave;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t win = R.lds_win;
+    if (!DEC) {
+        for (uint32_t i = threadIdx.x; i < win; i += blockDim.x) lh[i] = 0;
+        __syncthreads();
+    }
+    const uint32_t n1 = R.n[1];
+    const int64_t D2 = R.D[2];
+    const uint64_t rows = (uint64_t)R.n[0] * n1;
+    double err = 0, sq = 0;
+    int32_t qmin = INT32_MAX, qmax = INT32_MIN;
+    for (uint32_t rr0 = wave * RU; rr0 < R.rpw; rr0 += 4 * RU) {
+        // Lane l of a wave holds elements 4l .. 4l+3 of a 256-element segment
+        // (coarse 4l, 4l+2; targets 4l+1, 4l+3) in one float4; the other
+        // coarse neighbours come from lanes l-1, l+1 (l+2 quintic) by
+        // shuffles, and across the segment's ends from the halo float4s
+        // x[seg-4 .. seg-1] (lane 0) and x[seg+256 .. seg+263] (lane 63).
+        // Each lane stores its four elements back whole (the coarse values
+        // unchanged), so rows are rewritten in full lines.
+        const uint64_t row0 = (uint64_t)blockIdx.x * R.rpw + rr0;
+        if (row0 >= rows) break;
+        float* rowp[RU];
+        const float* trow[RU];
+        uint32_t j0[RU], j1[RU], c0[RU], c1[RU], p0[RU], p1[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const uint64_t row = row0 + u < rows ? row0 + u : row0;
+            const uint32_t k0 = (uint32_t)(row / n1), k1 = (uint32_t)(row - (uint64_t)k0 * n1);
+            uint32_t b0, b1;
+            side_loc(R.s[0] * k0, k0, R.B[0], R.s[0], b0, j0[u]);
+            side_loc(R.s[1] * k1, k1, R.B[1], R.s[1], b1, j1[u]);
+            c0[u] = tabs[R.tab_off[0] + b0];
+            p0[u] = tabs[R.tab_off[0] + R.nb[0] + b0];
+            c1[u] = tabs[R.tab_off[1] + b1];
+            p1[u] = tabs[R.tab_off[1] + R.nb[1] + b1];
+            rowp[u] = work + ((uint64_t)(R.s[0] * k0) * R.D[1] + R.s[1] * k1) * R.D[2];
+            trow[u] = tblk + ((uint64_t)b0 * R.nb[1] + b1) * R.nb[2];
+        }
+        for (uint32_t seg = 0; seg < (uint32_t)D2; seg += 256) {
+            const uint32_t e0 = seg + 4 * lane;  // this lane's first element
+            const bool live = e0 < (uint32_t)D2;
+            // the lane's two targets i2 = e0 + 1 + 2e: block along DIM2, local
+            // index, block end and the block's table entries
+            uint32_t b2[2], j2[2], right[2], c2[2], p2[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const uint32_t k2 = live ? (e0 + 2 * e) / 2 : 0;
+                b2[e] = fdiv(k2, R.tpb);
+                j2[e] = k2 - b2[e] * R.tpb.d;
+                right[e] = (uint32_t)min<uint64_t>((uint64_t)(b2[e] + 1) * R.B[2] + 1, R.D[2]);
+                c2[e] = tabs[R.tab_off[2] + b2[e]];
+                p2[e] = tabs[R.tab_off[2] + R.nb[2] + b2[e]];
+            }
+            // halo: lane 0 the coarse x[seg-4], x[seg-2]; lane 63 x[seg+256],
+            // x[seg+258] and (quintic) x[seg+260]
+            const bool hl = lane == 0 && seg > 0, hr = lane == 63 && seg + 256 < (uint32_t)D2;
+            const bool hr2 = QUIN && lane == 63 && seg + 260 < (uint32_t)D2;
+            const uint32_t ha = hl ? seg - 4 : (hr ? seg + 256 : e0), hb = ha + 2, hc = hr2 ? seg + 260 : e0;
+            float4 v[RU];
+            float h0[RU], h1[RU], h2[RU], tv[RU][2];
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+                const float4* r4 = reinterpret_cast<const float4*>(rowp[u]);
+                v[u] = live ? r4[e0 / 4] : make_float4(0.f, 0.f, 0.f, 0.f);
+                h0[u] = hl || hr ? rowp[u][ha] : 0.0f;
+                h1[u] = hl || hr ? rowp[u][hb] : 0.0f;
+                h2[u] = hr2 ? rowp[u][hc] : 0.0f;
+                tv[u][0] = trow[u][b2[0]];
+                tv[u][1] = trow[u][b2[1]];
+            }
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+                // coarse neighbours: the previous lane's .x / .z, the next lanes' .x / .z
+                float px = __shfl_up(v[u].x, 1, 64), pz = __shfl_up(v[u].z, 1, 64);
+                float nx = __shfl_down(v[u].x, 1, 64), nz = __shfl_down(v[u].z, 1, 64);
+                float n2x = __shfl_down(v[u].x, 2, 64);
+                const float hx1 = __shfl(h0[u], 63, 64);
+                if (lane == 0) {
+                    px = h0[u];
+                    pz = h1[u];
+                }
+                if (lane == 62) n2x = hx1;
+                if (lane == 63) {
+                    nx = h0[u];
+                    nz = h1[u];
+                    n2x = h2[u];
+                }
+                if (row0 + u >= rows || !live) continue;
+                float4 out = v[u];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const uint32_t i2 = e0 + 1 + 2 * e;  // target 4l+1, then 4l+3
+                    if (i2 >= (uint32_t)D2) break;
+                    // window at i2 - 5 .. i2 + 5 over the coarse values
+                    float w[6];
+                    if (e == 0) {
+                        w[0] = px; w[1] = pz; w[2] = v[u].x; w[3] = v[u].z; w[4] = nx; w[5] = nz;
+                    } else {
+                        w[0] = pz; w[1] = v[u].x; w[2] = v[u].z; w[3] = nx; w[4] = nz; w[5] = n2x;
+                    }
+                    const uint32_t cnt[3] = {c0[u], c1[u], c2[e]}, pre[3] = {p0[u], p1[u], p2[e]};
+                    const uint32_t jj[3] = {j0[u], j1[u], j2[e]};
+                    const uint64_t slot = R.qb_off + block_base(R, cnt, pre) + local_slot(ORDER, jj, cnt);
+                    const float p = predict_w<QUIN>(i2, 1, right[e], w, tv[u][e]);
+                    float xr;
+                    if (DEC) {
+                        xr = reconstruct(p, qb[slot], R.neg_two_eb);
+                    } else {
+                        const float x = e == 0 ? v[u].y : v[u].w;
+                        const int32_t q = quantize<2>(p, x, R);
+                        xr = reconstruct(p, q, R.neg_two_eb);
+                        qb[slot] = q;
+                        const double er = (double)daz(x) - (double)xr;
+                        err = fmax(err, fabs(er));
+                        sq += er * er;
+                        qmin = min(qmin, q);
+                        qmax = max(qmax, q);
+                        hist_add(lh, win, gwin, q);
+                    }
+                    if (e == 0) out.y = xr;
+                    else out.w = xr;
+                }
+                reinterpret_cast<float4*>(rowp[u])[e0 / 4] = out;
             }
         }
     }
@@ -674,7 +837,8 @@ struct RoundArgs {
     uint32_t* phist;
     uint32_t* gwin;
     FieldSet fs;
-    uint32_t nf;  // fields (grid dimension)
+    uint32_t nf;      // fields (grid dimension)
+    bool aligned16;   // every field's array starts on a 16-byte boundary
 };
 
 template <bool Q, bool DEC, int ORDER>
@@ -683,8 +847,21 @@ void row(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
                        DEC ? 0 : R.lds_win * 4, st, R, a.w, a.qb, a.tabs, a.tblk, a.part, a.phist, a.gwin, a.fs);
 }
 
+template <bool Q, bool DEC, int ORDER>
+void quad(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((dev::k_round_quad<Q, DEC, ORDER>), dim3((uint32_t)R.nwg, a.nf), dim3(256),
+                       DEC ? 0 : R.lds_win * 4, st, R, a.w, a.qb, a.tabs, a.tblk, a.part, a.phist, a.gwin, a.fs);
+}
+
 template <bool Q, bool DEC>
 void row_order(const RoundDesc& R, const RoundArgs& a, hipStream_t st) {
+    // h = 1 on rows of a multiple of four elements, every field 16-byte aligned: the quad kernel
+    if (R.h == 1 && (R.D[2] & 3u) == 0 && a.aligned16) {
+        if (R.order == 0) quad<Q, DEC, 0>(R, a, st);
+        else if (R.order == 1) quad<Q, DEC, 1>(R, a, st);
+        else quad<Q, DEC, 2>(R, a, st);
+        return;
+    }
     if (R.order == 0) row<Q, DEC, 0>(R, a, st);
     else if (R.order == 1) row<Q, DEC, 1>(R, a, st);
     else row<Q, DEC, 2>(R, a, st);
@@ -731,7 +908,8 @@ void launch_round_compress(const RoundDesc& R, bool quintic, float* work, int32_
                            const float* tblk, const SweepSeg* segs, Partial* part, uint32_t* phist, uint32_t* gwin,
                            hipStream_t st, const FieldSet& fs, uint32_t nfields) {
     if (R.count == 0) return;
-    dispatch<false>(R, quintic, RoundArgs{work, qb, tabs, tblk, segs, part, phist, gwin, fs, nfields}, st);
+    const bool a16 = fs.fd ? fs.aligned16 != 0 : (reinterpret_cast<uintptr_t>(work) & 15u) == 0;
+    dispatch<false>(R, quintic, RoundArgs{work, qb, tabs, tblk, segs, part, phist, gwin, fs, nfields, a16}, st);
 }
 
 bool round_needs_staging(const RoundDesc& R) {
@@ -780,7 +958,8 @@ void launch_round_decompress(const RoundDesc& R, bool quintic, float* out, const
                              const float* tblk, const SweepSeg* segs, hipStream_t st) {
     if (R.count == 0) return;
     dispatch<true>(R, quintic,
-                   RoundArgs{out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, nullptr, nullptr, FieldSet{}, 1},
+                   RoundArgs{out, const_cast<int32_t*>(qb), tabs, tblk, segs, nullptr, nullptr, nullptr, FieldSet{}, 1,
+                             (reinterpret_cast<uintptr_t>(out) & 15u) == 0},
                    st);
 }
 

@@ -19,6 +19,11 @@
 #include "device.hpp"
 #include "launch.hpp"
 
+// Phase marks for timing builds (the includer defines TSEL_MARK; none here).
+#ifndef TSEL_MARK
+#define TSEL_MARK(k)
+#endif
+
 namespace asz {
 namespace dev {
 
@@ -31,7 +36,7 @@ constexpr int kMaxKeep = 2048;     // samples whose keys stay in LDS; more are r
 // NC: candidates scored per block (5 for the discrete scan, 3 for the
 // gradient-tuned form, whose smaller footprint lets more workgroups share a CU).
 template <int NC>
-struct TselShared {
+struct alignas(16) TselShared {
     float af[kMaxKeep], bf[kMaxKeep];
     uint32_t bins[NC][kDirect];
     long long hkey[NC][kHashSlots];
@@ -130,6 +135,7 @@ struct TselLds {
     static constexpr int NC = GRAD ? 3 : kNCand;  // GRAD: 1/4 and the two grid points around the minimiser
     TselShared<NC> sh;
     float s_cand[NC];
+    float h1[NC], psnr[NC];
     int s_nc;
     float best;
 };
@@ -160,6 +166,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
     const uint32_t b2 = bid % R.nb[2], b1 = (bid / R.nb[2]) % R.nb[1], b0 = bid / (R.nb[2] * R.nb[1]);
     const uint32_t bb[3] = {b0, b1, b2};
     const uint32_t ss = ss_tab[bid];
+    TSEL_MARK(0);
     float best_t = 0.25f;  // insufficient_sample_t (asz_param.hpp:167)
     const bool fallback = ss == 0;
     if (!fallback) {
@@ -182,12 +189,18 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         const float neg2 = 0.5f / stat_eb;
         const float neg_2_eb = 0.5f / eb;
         const float two_eb = eb * 2.0f;
-        for (int i = threadIdx.x; i < NC * kDirect; i += kTselThreads) (&sh.bins[0][0])[i] = 0;
-        for (int i = threadIdx.x; i < NC * kHashSlots; i += kTselThreads) {
-            (&sh.hkey[0][0])[i] = LLONG_MIN;
-            (&sh.hcnt[0][0])[i] = 0;
+        // 16-byte LDS stores: the direct bins and hash counts to 0, the hash keys to LLONG_MIN
+        {
+            uint4* b4 = reinterpret_cast<uint4*>(&sh.bins[0][0]);
+            for (int i = threadIdx.x; i < NC * kDirect / 4; i += kTselThreads) b4[i] = make_uint4(0, 0, 0, 0);
+            uint4* k4 = reinterpret_cast<uint4*>(&sh.hkey[0][0]);
+            for (int i = threadIdx.x; i < NC * kHashSlots / 2; i += kTselThreads)
+                k4[i] = make_uint4(0, 0x80000000u, 0, 0x80000000u);
+            uint4* c4 = reinterpret_cast<uint4*>(&sh.hcnt[0][0]);
+            for (int i = threadIdx.x; i < NC * kHashSlots / 4; i += kTselThreads) c4[i] = make_uint4(0, 0, 0, 0);
         }
         if (threadIdx.x < NC) sh.ovf[threadIdx.x] = 0;
+        TSEL_MARK(1);
         double saa = 0, sab = 0;  // GRAD: sum A^2, sum A B over the samples
         // four samples per thread at a time: all twenty loads in flight before any use
         const uint32_t ns_pass = GRAD ? ns : min(ns, (uint32_t)kMaxKeep);
@@ -208,6 +221,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
                 if (GRAD) { saa += (double)A * (double)A; sab += (double)A * (double)B; }
             }
         }
+        TSEL_MARK(2);
         if (GRAD) {
             const int w0 = threadIdx.x / kWave;
             saa = wave_sum(saa);
@@ -259,15 +273,20 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
             }
         }
         __syncthreads();
+        TSEL_MARK(3);
         const float dn = (float)ns;
         const float rec = 1.0f / dn;
+        // each thread: four direct bins and two hash slots per candidate (16- and 8-byte LDS reads)
+        static_assert(kDirect == 4 * kTselThreads && kHashSlots == 2 * kTselThreads, "entropy read shape");
         double h[NC] = {};
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            for (int i = threadIdx.x; i < kDirect; i += kTselThreads)
-                if (sh.bins[k][i]) h[k] -= entropy_term(sh.bins[k][i], rec);
-            for (int i = threadIdx.x; i < kHashSlots; i += kTselThreads)
-                if (sh.hcnt[k][i]) h[k] -= entropy_term(sh.hcnt[k][i], rec);
+            const uint4 d4 = reinterpret_cast<const uint4*>(&sh.bins[k][0])[threadIdx.x];
+            const uint2 c2 = reinterpret_cast<const uint2*>(&sh.hcnt[k][0])[threadIdx.x];
+            const uint32_t cs[6] = {d4.x, d4.y, d4.z, d4.w, c2.x, c2.y};
+#pragma unroll
+            for (int e = 0; e < 6; e++)
+                if (cs[e]) h[k] -= entropy_term(cs[e], rec);
             if (threadIdx.x == 0 && sh.ovf[k]) h[k] -= (double)sh.ovf[k] * entropy_term(1, rec);
         }
         const int w = threadIdx.x / kWave;
@@ -277,14 +296,21 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
             if ((threadIdx.x & (kWave - 1)) == 0) { sh.red[w][k] = hs; sh.red[w][NC + k] = ms; }
         }
         __syncthreads();
+        TSEL_MARK(4);
+        // lane k of the first wave: candidate k's H1 and PSNR
+        if (threadIdx.x < (uint32_t)nc) {
+            const int k = threadIdx.x;
+            double hs = 0, ms = 0;
+            for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][NC + k]; }
+            L.h1[k] = (float)hs;
+            L.psnr[k] = log10f((range * range / (float)ms) * dn) * 10.0f;
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
             float h1[NC], psnr[NC];
-            const float r2 = range * range;
             for (int k = 0; k < nc; k++) {
-                double hs = 0, ms = 0;
-                for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][NC + k]; }
-                h1[k] = (float)hs;
-                psnr[k] = log10f((r2 / (float)ms) * dn) * 10.0f;
+                h1[k] = L.h1[k];
+                psnr[k] = L.psnr[k];
             }
             // start from 1/4, then the candidates in order (compress.tpp:127-142)
             int best = GRAD ? 0 : 1;
@@ -303,6 +329,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         L.best = best_t;
     }
     __syncthreads();
+    TSEL_MARK(5);
 }
 
 }  // namespace dev

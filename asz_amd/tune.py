@@ -127,8 +127,11 @@ def compare_tuned_vs_plain(field: torch.Tensor, rel: float, plan, block_pivot_ra
                            device=0):
     """Compress `field` (a CUDA tensor) with the discrete t scan and with the
     gradient-tuned form; returns CR, max |error| / eb and the t-selection
-    kernel time of each, the tuned/plain ratios, and the agreement of t."""
+    kernel time of each, the tuned/plain ratios, and the agreement of t.
+    Block rounds are off here, so that every round's t selection is a launch
+    of its own and its kernel time is measured."""
     ctx = codec.Context(device)
+    ctx.set_coarse_rounds(0)
     try:
         shape = tuple(field.shape)
         n = field.numel()
