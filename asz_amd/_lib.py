@@ -48,7 +48,8 @@ class Stats(ctypes.Structure):
                 ("chunk_ms", ctypes.c_double), ("encode_ms", ctypes.c_double), ("decode_ms", ctypes.c_double),
                 ("codebook_wait_ms", ctypes.c_double), ("codebook_max_ms", ctypes.c_double),
                 ("layout_ms", ctypes.c_double), ("plan_cache_hit", ctypes.c_uint64),
-                ("coarse_rounds", ctypes.c_uint64), ("coarse_ms", ctypes.c_double)]
+                ("coarse_rounds", ctypes.c_uint64), ("coarse_ms", ctypes.c_double),
+                ("graph_launched", ctypes.c_uint64), ("graph_captured", ctypes.c_uint64)]
 
 class BatchField(ctypes.Structure):
     _fields_ = [("min_val", ctypes.c_float), ("max_val", ctypes.c_float), ("default_slope", ctypes.c_float),
@@ -87,6 +88,7 @@ ASZ_OPT_TIMED_ROUND = 3
 ASZ_OPT_BATCH_FIELDS = 4
 ASZ_OPT_WIDE_CODES = 5
 ASZ_OPT_COARSE_ROUNDS = 6
+ASZ_OPT_GRAPHS = 7
 
 
 # exported entry points (must match include/asz_gpu.h)

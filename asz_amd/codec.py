@@ -128,6 +128,10 @@ class Context:
         targets in one launch; -1: every round that qualifies, 0: none)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_COARSE_ROUNDS, int(n)))
 
+    def set_graphs(self, on: bool = True):
+        """Run multi-field compressions' device sequence as captured HIP graphs."""
+        self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_GRAPHS, 1 if on else 0))
+
     def set_stream_on_device(self, on: bool = True):
         """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))

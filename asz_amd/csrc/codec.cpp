@@ -327,6 +327,47 @@ static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
     c->epoch = 0;
 }
 
+// Runs a device sequence as a HIP graph: captured from `st` (with the side
+// stream it forks to) the first time a signature is seen, instantiated, kept
+// among the context's last kGraphSlots graphs, then launched on `st`.
+template <typename F>
+static void graph_run(asz_ctx* c, hipStream_t st, const std::vector<uint64_t>& sig, F&& fn) {
+    GraphEntry* e = nullptr;
+    for (GraphEntry& g : c->graph_cache)
+        if (g.sig == sig) e = &g;
+    if (!e) {
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            fn();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(st, &g));
+        hipGraphExec_t x = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            fail(ASZ_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        if (c->graph_cache.size() >= asz_ctx::kGraphSlots) {
+            auto lru = c->graph_cache.begin();
+            for (auto it = c->graph_cache.begin(); it != c->graph_cache.end(); ++it)
+                if (it->used < lru->used) lru = it;
+            lru->release();
+            c->graph_cache.erase(lru);
+        }
+        c->graph_cache.push_back(GraphEntry{sig, g, x, 0});
+        e = &c->graph_cache.back();
+        c->stats.graph_captured = 1;
+    }
+    e->used = ++c->pc_tick;
+    HIPCHECK(hipGraphLaunch(e->x, st));
+    c->stats.graph_launched = 1;
+}
+
 // One field of a compress call: its array (rewritten with the
 // reconstruction), value range and default slope.
 struct FieldIn {
@@ -471,10 +512,10 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     uint32_t* phist = c->phist.get<uint32_t>(fs.phist * nf);
     uint32_t* ghist = c->ghist.get<uint32_t>(fs.ghist * nf);
     DevAcc* acc = c->xfer.get<DevAcc>(nf);
+    FieldDesc* dfd_multi = nullptr;  // multi-field: the device copy of hfd (uploaded with the device sequence)
     if (multi) {
-        FieldDesc* dfd = c->fdesc.get<FieldDesc>(nf);
-        HIPCHECK(hipMemcpyAsync(dfd, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
-        fs.fd = dfd;
+        dfd_multi = c->fdesc.get<FieldDesc>(nf);
+        fs.fd = dfd_multi;
     }
     uint32_t* range_res = nullptr;  // dev_range: min bits, max bits, status (pinned)
     if (dev_range) {
@@ -521,10 +562,26 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     auto sym_of = [&](uint32_t f) { return sym0 + f * fs.sym; };
     auto gwin = [&](uint32_t f, uint64_t k) { return ghist + f * fs.ghist + k * kHistStride; };
     ensure_host_slots(c, narr * nf);
-    const uint32_t epoch = ++c->epoch ? c->epoch : ++c->epoch;  // never 0 (the slots start zeroed)
-    launch_init_acc(acc, st, nf);
-    launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * fs.ghist, 0, st);
-
+    // Multi-field calls with the GPU t selection run their device sequence
+    // (field descriptors, pivots, every round, folds) as a HIP graph, captured
+    // once per (plan, field count, buffers) and replayed (ASZ_OPT_GRAPHS).
+    const bool use_graph = c->graphs && multi && (tsel_mode == ASZ_TSEL_GPU || tsel_mode == ASZ_TSEL_GRAD) &&
+                           stop_round < 0 && !c->kev_on;
+    // The epoch the kernels publish with: a fresh one per call (never 0, the
+    // slots start zeroed; never kGraphEpoch), or kGraphEpoch for a graph, whose
+    // kernels keep the value they were captured with (the flags are cleared
+    // before every graph launch instead).
+    uint32_t epoch = kGraphEpoch;
+    if (!use_graph) {
+        do {
+            epoch = ++c->epoch;
+        } while (epoch == 0 || epoch == kGraphEpoch);
+    } else {
+        for (uint64_t i = 0; i < narr * nf; i++)
+            __atomic_store_n(c->hres_host + i * kHostSlotWords, 0u, __ATOMIC_RELAXED);
+    }
+    std::vector<float> tf_inj;
+    std::vector<int32_t> ti_inj;
     if (tsel_mode == ASZ_TSEL_INJECT) {
         if (!t_inject) fail(ASZ_ERR_ARG, "t_inject is required with ASZ_TSEL_INJECT");
         std::vector<float> tf(nt);
@@ -535,8 +592,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                 tf[r * nbl + b] = (float)v / (float)p.t_precision;
                 ti[b * L + r] = (int32_t)v;
             }
-        HIPCHECK(hipMemcpyAsync(tblk, tf.data(), nt * 4, hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(sym, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
+        tf_inj.swap(tf);
+        ti_inj.swap(ti);
     }
 
     // Rounds [0, batch_end) are folded together after the last of them (one
@@ -637,18 +694,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         build_codebook(A, hexact.data());
         if (!A.single) pack_book(A);
     };
-    HIPCHECK(hipEventRecord(c->ev[0], st));
-    PivotDesc PD = pivot_desc(P);
-    int kp = kev_begin(c, KEV_PIVOT, 0);
-    launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
-    kev_end(c, kp);
-    // the pivot array's histogram on the side stream: no round waits for it
-    ensure_side(c);
-    HIPCHECK(hipEventRecord(c->side_ev[0], st));
-    HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
-    launch_hist_small(nullptr, 0, sym + ntp, P.pivot_count, nullptr, gwin(0, 1), c->side, nf, fs.sym, fs.ghist);
-    launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, c->side, nf, fs.fpart, fs.ghist, fs.hres);
-    HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
     // Codebook workers: the host pool's threads take the arrays in `order`
     // (every field's copy of an array in turn) as their results arrive,
     // concurrently with the launches below and with the later rounds on the GPU.
@@ -666,7 +711,12 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         host_ms_w[i] = ms_since(h0);
     };
     TaskPool& pool = host_pool(c);
-    if (stop_round < 0) pool.start(ntask, [&](size_t i) { task(i, false); });
+    bool workers_started = false;
+    auto start_workers = [&]() {
+        if (stop_round >= 0 || workers_started) return;
+        pool.start(ntask, [&](size_t i) { task(i, false); });
+        workers_started = true;
+    };
     // stops and joins the workers on every exit path below
     struct Joiner {
         TaskPool& pool;
@@ -726,98 +776,152 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         }
     };
     S.coarse_rounds = ncoarse;
-    for (uint64_t r = 0; r < Lrun; r++) {
-        const RoundDesc& R = P.rounds[r];
-        // a block round (r < ncoarse) selects its t inside its own launch
-        const bool block_round = r < ncoarse;
-        if (!block_round) check_grid(R, nf);
-        float* tb = tblk + r * nbl;
-        if (gpu_tsel && !block_round) {
-            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
-                       p.t_min, p.t_max};
-            int kt = kev_begin(c, KEV_TSEL, (int)r);
-            if (fold_nr) {
-                launch_fold_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, dfolds,
-                                 (uint32_t)fold_r0, (uint32_t)fold_nr, fold_groups(fold_r0, fold_nr), part, phist,
-                                 fpart, ghist, c->hres_dev, epoch, st, fs, nf);
-                fold_nr = 0;
-            } else {
-                launch_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, st, fs, nf);
-            }
-            kev_end(c, kt);
-        } else if (tsel_mode == ASZ_TSEL_EXACT) {
-            auto h0 = Clock::now();
-            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
-                       p.t_min, p.t_max};
-            koff.assign(nbl + 1, 0);
-            for (uint64_t b = 0; b < nbl; b++) koff[b + 1] = koff[b] + P.ns[r * nbl + b];
-            uint64_t* dko = c->key_off.get<uint64_t>(nbl + 1);
-            int64_t* dk = c->keys.get<int64_t>(2 * koff[nbl] + 2);
-            HIPCHECK(hipMemcpyAsync(dko, koff.data(), (nbl + 1) * 8, hipMemcpyHostToDevice, st));
-            launch_tsel_keys(T, d_work, ss + r * nbl, dko, dk, nbl, st);
-            hkeys.resize(2 * koff[nbl] + 2);
-            HIPCHECK(hipMemcpyAsync(hkeys.data(), dk, 2 * koff[nbl] * 8, hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
-            std::vector<float> tf(nbl);
-            std::vector<int32_t> ti(nbl);
-            const float stat_eb = p.stat_eb_ratio * R.eb;
-            for (uint64_t b = 0; b < nbl; b++) {
-                uint64_t n = koff[b + 1] - koff[b];
-                float t = 0.25f;
-                if (P.ss[r * nbl + b] == 0 || n == 0) S.t_fallbacks++;
-                else t = select_t_exact(hkeys.data() + 2 * koff[b], n, R.eb, stat_eb, R.layer_slope, mn, mx);
-                tf[b] = t;
-                float prod = t * (float)p.t_precision;
-                ti[b] = (int32_t)(prod + std::copysign(0.49999997f, prod));
-            }
-            // this round's t for the round kernel; the block-major t array
-            // (compress.tpp:201) is kept on the host and goes up once, after
-            // the last round's selection (the only host wait of a round is the
-            // key download above)
-            std::memcpy(tf_all.data() + r * nbl, tf.data(), nbl * 4);
-            HIPCHECK(hipMemcpyAsync(tb, tf_all.data() + r * nbl, nbl * 4, hipMemcpyHostToDevice, st));
-            for (uint64_t b = 0; b < nbl; b++) t_host[b * L + r] = ti[b];
-            if (r + 1 == Lrun) HIPCHECK(hipMemcpyAsync(sym, t_host.data(), nt * 4, hipMemcpyHostToDevice, st));
-            tsel_host_ms += ms_since(h0);
+    // The device sequence of the call: field descriptors, accumulators, the
+    // injected t, pivots (their histogram on the side stream), every round
+    // with its t selection and folds, and the join of the side stream.
+    auto device_sequence = [&]() {
+        if (multi) HIPCHECK(hipMemcpyAsync(dfd_multi, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
+        launch_init_acc(acc, st, nf);
+        launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * fs.ghist, 0, st);
+        if (tsel_mode == ASZ_TSEL_INJECT) {
+            HIPCHECK(hipMemcpyAsync(tblk, tf_inj.data(), nt * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(sym, ti_inj.data(), nt * 4, hipMemcpyHostToDevice, st));
         }
-        if (block_round) {
-            // one launch: each block's t (selected here with the GPU modes, from
-            // tblk otherwise) and its targets
-            TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
-                       p.t_min, p.t_max};
-            const uint32_t tsel = !gpu_tsel ? 0u : (tsel_mode == ASZ_TSEL_GRAD ? 2u : 1u);
-            int kc = kev_begin(c, KEV_COARSE, (int)r);
-            launch_round_block(T, tsel, p.quintic != 0, d_work, sym, qb, tabs, ss + r * nbl, tb, part, phist,
-                               gwin(0, 2 + r), acc, st, fs, nf);
-            kev_end(c, kc);
-            // the last round's t values are written by its own launch
+        PivotDesc PD = pivot_desc(P);
+        int kp = kev_begin(c, KEV_PIVOT, 0);
+        launch_pivots(true, d_work, qb, PD, ppart, st, fs, nf);
+        kev_end(c, kp);
+        // the pivot array's histogram on the side stream: no round waits for it
+        HIPCHECK(hipEventRecord(c->side_ev[0], st));
+        HIPCHECK(hipStreamWaitEvent(c->side, c->side_ev[0], 0));
+        launch_hist_small(nullptr, 0, sym + ntp, P.pivot_count, nullptr, gwin(0, 1), c->side, nf, fs.sym, fs.ghist);
+        launch_hist_tail(acc, 1, ppart, npp, gwin(0, 1), c->hres_dev, epoch, c->side, nf, fs.fpart, fs.ghist,
+                         fs.hres);
+        HIPCHECK(hipEventRecord(c->side_ev[1], c->side));
+        // codebook workers from here on (a graph starts them after its launch)
+        if (!use_graph) start_workers();
+        for (uint64_t r = 0; r < Lrun; r++) {
+            const RoundDesc& R = P.rounds[r];
+            // a block round (r < ncoarse) selects its t inside its own launch
+            const bool block_round = r < ncoarse;
+            if (!block_round) check_grid(R, nf);
+            float* tb = tblk + r * nbl;
+            if (gpu_tsel && !block_round) {
+                TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                           p.t_min, p.t_max};
+                int kt = kev_begin(c, KEV_TSEL, (int)r);
+                if (fold_nr) {
+                    launch_fold_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, dfolds,
+                                     (uint32_t)fold_r0, (uint32_t)fold_nr, fold_groups(fold_r0, fold_nr), part, phist,
+                                     fpart, ghist, c->hres_dev, epoch, st, fs, nf);
+                    fold_nr = 0;
+                } else {
+                    launch_tsel(T, d_work, ss + r * nbl, tb, sym, acc, nbl, tsel_mode == ASZ_TSEL_GRAD, st, fs, nf);
+                }
+                kev_end(c, kt);
+            } else if (tsel_mode == ASZ_TSEL_EXACT) {
+                auto h0 = Clock::now();
+                TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                           p.t_min, p.t_max};
+                koff.assign(nbl + 1, 0);
+                for (uint64_t b = 0; b < nbl; b++) koff[b + 1] = koff[b] + P.ns[r * nbl + b];
+                uint64_t* dko = c->key_off.get<uint64_t>(nbl + 1);
+                int64_t* dk = c->keys.get<int64_t>(2 * koff[nbl] + 2);
+                HIPCHECK(hipMemcpyAsync(dko, koff.data(), (nbl + 1) * 8, hipMemcpyHostToDevice, st));
+                launch_tsel_keys(T, d_work, ss + r * nbl, dko, dk, nbl, st);
+                hkeys.resize(2 * koff[nbl] + 2);
+                HIPCHECK(hipMemcpyAsync(hkeys.data(), dk, 2 * koff[nbl] * 8, hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipStreamSynchronize(st));
+                std::vector<float> tf(nbl);
+                std::vector<int32_t> ti(nbl);
+                const float stat_eb = p.stat_eb_ratio * R.eb;
+                for (uint64_t b = 0; b < nbl; b++) {
+                    uint64_t n = koff[b + 1] - koff[b];
+                    float t = 0.25f;
+                    if (P.ss[r * nbl + b] == 0 || n == 0) S.t_fallbacks++;
+                    else t = select_t_exact(hkeys.data() + 2 * koff[b], n, R.eb, stat_eb, R.layer_slope, mn, mx);
+                    tf[b] = t;
+                    float prod = t * (float)p.t_precision;
+                    ti[b] = (int32_t)(prod + std::copysign(0.49999997f, prod));
+                }
+                // this round's t for the round kernel; the block-major t array
+                // (compress.tpp:201) is kept on the host and goes up once, after
+                // the last round's selection (the only host wait of a round is the
+                // key download above)
+                std::memcpy(tf_all.data() + r * nbl, tf.data(), nbl * 4);
+                HIPCHECK(hipMemcpyAsync(tb, tf_all.data() + r * nbl, nbl * 4, hipMemcpyHostToDevice, st));
+                for (uint64_t b = 0; b < nbl; b++) t_host[b * L + r] = ti[b];
+                if (r + 1 == Lrun) HIPCHECK(hipMemcpyAsync(sym, t_host.data(), nt * 4, hipMemcpyHostToDevice, st));
+                tsel_host_ms += ms_since(h0);
+            }
+            if (block_round) {
+                // one launch: each block's t (selected here with the GPU modes, from
+                // tblk otherwise) and its targets
+                TselDesc T{R, (uint32_t)L, (uint32_t)r, p.stat_eb_ratio, daz(mx) - daz(mn), (float)p.t_precision,
+                           p.t_min, p.t_max};
+                const uint32_t tsel = !gpu_tsel ? 0u : (tsel_mode == ASZ_TSEL_GRAD ? 2u : 1u);
+                int kc = kev_begin(c, KEV_COARSE, (int)r);
+                launch_round_block(T, tsel, p.quintic != 0, d_work, sym, qb, tabs, ss + r * nbl, tb, part, phist,
+                                   gwin(0, 2 + r), acc, st, fs, nf);
+                kev_end(c, kc);
+                // the last round's t values are written by its own launch
+                if (r + 1 == L) t_array_side();
+                after_round(r);
+                continue;
+            }
             if (r + 1 == L) t_array_side();
+            int kr = kev_begin(c, KEV_ROUND, (int)r);
+            if (round_needs_staging(R)) {
+                RoundDesc Rs = R;
+                Rs.order = 2;
+                Rs.qb_off = 0;
+                FieldSet fss = fs;
+                fss.sym = stage_stride;
+                launch_round_compress(Rs, p.quintic != 0, d_work, stage, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
+                                      fss, nf);
+                uint32_t cmax[3];
+                round_cmax(P, R, cmax);
+                launch_qb_order(R, tabs, cmax, stage, qb + R.qb_off, stage_stride, fs.sym, nf, true, st);
+            } else {
+                launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
+                                      fs, nf);
+            }
+            kev_end(c, kr);
             after_round(r);
-            continue;
         }
-        if (r + 1 == L) t_array_side();
-        int kr = kev_begin(c, KEV_ROUND, (int)r);
-        if (round_needs_staging(R)) {
-            RoundDesc Rs = R;
-            Rs.order = 2;
-            Rs.qb_off = 0;
-            FieldSet fss = fs;
-            fss.sym = stage_stride;
-            launch_round_compress(Rs, p.quintic != 0, d_work, stage, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
-                                  fss, nf);
-            uint32_t cmax[3];
-            round_cmax(P, R, cmax);
-            launch_qb_order(R, tabs, cmax, stage, qb + R.qb_off, stage_stride, fs.sym, nf, true, st);
-        } else {
-            launch_round_compress(R, p.quintic != 0, d_work, qb, tabs, tb, segs, part, phist, gwin(0, 2 + r), st,
-                                  fs, nf);
-        }
-        kev_end(c, kr);
-        after_round(r);
+        HIPCHECK(hipGetLastError());
+        // the main stream resumes after the side stream's histograms (its last work)
+        HIPCHECK(hipStreamWaitEvent(st, c->side_ev[1], 0));
+    };
+    // what the captured launches depend on besides the field descriptors:
+    // the plan, the mode, every buffer address and stride, the parameters
+    // in their arguments (a multi-field call's kernels take each field's
+    // array and layer bounds from its descriptor, never from their own
+    // arguments, so field 0's pointer and bounds baked into them do not count)
+    auto graph_signature = [&]() {
+        std::vector<uint64_t> sig;
+        const uint64_t* kw = reinterpret_cast<const uint64_t*>(&pc.key);
+        sig.assign(kw, kw + sizeof(PlanKey) / 8);
+        auto bits = [](float v) { uint32_t u; std::memcpy(&u, &v, 4); return (uint64_t)u; };
+        const uint64_t vals[] = {nf, (uint64_t)tsel_mode, (uint64_t)p.quintic, ncoarse, Lrun, batch_end, epoch,
+                                 bits(p.stat_eb_ratio), (uint64_t)p.t_precision, bits(p.t_min), bits(p.t_max),
+                                 (uint64_t)(uintptr_t)dfd_multi, (uint64_t)(uintptr_t)hfd,
+                                 (uint64_t)(uintptr_t)sym0, (uint64_t)(uintptr_t)tblk, (uint64_t)(uintptr_t)part,
+                                 (uint64_t)(uintptr_t)ppart, (uint64_t)(uintptr_t)phist, (uint64_t)(uintptr_t)ghist,
+                                 (uint64_t)(uintptr_t)acc, (uint64_t)(uintptr_t)stage, stage_stride,
+                                 (uint64_t)(uintptr_t)pc.meta.p, (uint64_t)(uintptr_t)c->hres_dev, fs.sym, fs.tblk,
+                                 fs.part, fs.phist, fs.fpart, fs.ghist, fs.hres, fs.aligned16};
+        sig.insert(sig.end(), std::begin(vals), std::end(vals));
+        return sig;
+    };
+    ensure_side(c);
+    HIPCHECK(hipEventRecord(c->ev[0], st));
+    if (use_graph) {
+        graph_run(c, st, graph_signature(), device_sequence);
+        start_workers();
+    } else {
+        device_sequence();
     }
-    HIPCHECK(hipGetLastError());
-    // the main stream resumes after the side stream's histograms (its last work)
-    HIPCHECK(hipStreamWaitEvent(st, c->side_ev[1], 0));
     if (stop_round >= 0) {
         HIPCHECK(hipStreamSynchronize(st));
         S.wall_ms = ms_since(wall0);
@@ -1444,6 +1548,8 @@ void asz_ctx_destroy(asz_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
+    for (auto& g : c->graph_cache) g.release();
+    c->graph_cache.clear();
     if (c->order_ev) (void)hipEventDestroy(c->order_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1459,6 +1565,10 @@ int asz_set_option(asz_ctx* c, int option, int64_t value) {
         }
         if (option == ASZ_OPT_WIDE_CODES) {
             c->wide_codes = value != 0;
+            return;
+        }
+        if (option == ASZ_OPT_GRAPHS) {
+            c->graphs = value != 0;
             return;
         }
         if (option == ASZ_OPT_COARSE_ROUNDS) {
@@ -1726,12 +1836,14 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         }
         const bool on_device = c->stream_on_device, wide = c->wide_codes;
         const int coarse_max = c->coarse_max;
+        const bool graphs = c->graphs;
         // the workers' streams start after everything enqueued on this context's
         // stream (asz_stream_wait orders that after the caller's)
         if (!c->order_ev) HIPCHECK(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming));
         HIPCHECK(hipEventRecord(c->order_ev, c->stream));
         for (uint32_t t = 0; t < W; t++) HIPCHECK(hipStreamWaitEvent(c->kids[t]->stream, c->order_ev, 0));
         std::atomic<uint32_t> next{0};
+        std::atomic<uint64_t> graph_launches{0}, graph_captures{0};
         std::mutex err_mu, pre_mu;
         std::string first_err;
         int first_code = ASZ_OK;
@@ -1762,8 +1874,11 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
                     k->stream_on_device = on_device;
                     k->wide_codes = wide;
                     k->coarse_max = coarse_max;
+                    k->graphs = graphs;
                     asz::compress_fields(k, nf, fin.data(), d, plans[ch[0]], *p, tsel_mode, nullptr, nullptr,
                                          nullptr);
+                    graph_launches += k->stats.graph_launched;
+                    graph_captures += k->stats.graph_captured;
                     for (uint32_t j = 0; j < nf; j++) {
                         const uint32_t i = ch[j];
                         const auto& o = k->fouts[j];
@@ -1815,6 +1930,9 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
         std::vector<std::thread> ts;
         for (uint32_t t = 0; t < W; t++) ts.emplace_back(worker, c->kids[t]);
         for (auto& t : ts) t.join();
+        // chunks whose device sequence ran as a graph, and how many were captured
+        c->stats.graph_launched = graph_launches;
+        c->stats.graph_captured = graph_captures;
         if (first_code != ASZ_OK) {
             for (uint32_t i = 0; i < nfields; i++) {
                 if (on_device) (void)hipFree(streams[i]);

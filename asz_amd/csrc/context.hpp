@@ -110,6 +110,22 @@ struct PlanCache {
     } up{};
 };
 
+// A captured device sequence (multi-field compression) and its instance.
+struct GraphEntry {
+    std::vector<uint64_t> sig;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    uint64_t used = 0;
+    void release() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
+    }
+};
+// The epoch the kernels of a graph publish with (calls without a graph never use it).
+constexpr uint32_t kGraphEpoch = 0xFFFFFFFFu;
+
 }  // namespace asz
 
 struct asz_ctx {
@@ -143,6 +159,9 @@ struct asz_ctx {
     uint32_t batch_fields = 0;  // ASZ_OPT_BATCH_FIELDS
     bool wide_codes = false;  // ASZ_OPT_WIDE_CODES
     int coarse_max = -1;      // ASZ_OPT_COARSE_ROUNDS (-1: every leading round that qualifies)
+    bool graphs = true;       // ASZ_OPT_GRAPHS
+    static constexpr size_t kGraphSlots = 8;
+    std::vector<asz::GraphEntry> graph_cache;  // captured multi-field device sequences, least recently used first out
     float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field
     uint8_t* out_stream = nullptr;  // last compressed stream (pin_stream, or device memory)
     uint64_t out_bytes = 0;

@@ -80,16 +80,18 @@ __global__ __launch_bounds__(256) void k_range(const float* __restrict__ x, uint
             }
         }
     };
-    // four 16-byte loads in flight per thread, then their keys
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
+    // a workgroup takes 4 x 256 consecutive float4 (16 KB) at a time: four
+    // coalesced 16-byte loads in flight per thread, then their keys
+    const uint64_t span = 4ull * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * span + threadIdx.x;
+    for (; i + 3 * blockDim.x < n4; i += span * gridDim.x) {
         float4 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = x4[i + u * stride];
+        for (int u = 0; u < 4; u++) v[u] = x4[i + u * blockDim.x];
 #pragma unroll
-        for (int u = 0; u < 4; u++) scan4(v[u], i + u * stride);
+        for (int u = 0; u < 4; u++) scan4(v[u], i + u * blockDim.x);
     }
-    for (; i < n4; i += stride) scan4(x4[i], i);
+    for (; i < n4; i += blockDim.x) scan4(x4[i], i);
     for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         bad |= !isfinite(x[i]);
         const uint32_t key = order_key(x[i]);

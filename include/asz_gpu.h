@@ -96,6 +96,10 @@ typedef struct asz_stats {
      * also in round_ms) */
     uint64_t coarse_rounds;
     double coarse_ms;
+    /* multi-field calls: the device sequence ran as a HIP graph (launched),
+     * captured by this call (captured); after asz_compress_batch, the number
+     * of chunks launched and captured that way */
+    uint64_t graph_launched, graph_captured;
 } asz_stats;
 
 /* Options (asz_set_option).  ASZ_OPT_KERNEL_EVENTS: record a HIP event pair on
@@ -128,6 +132,13 @@ typedef struct asz_stats {
  * (default): every leading round that qualifies; 0: none.  The stream is the
  * same either way. */
 #define ASZ_OPT_COARSE_ROUNDS 6
+/* ASZ_OPT_GRAPHS: when nonzero (default), a multi-field compression (the
+ * batched form's chunks, GPU t selection) runs its device sequence (field
+ * descriptors, pivots, every round with its t selection and folds) as a HIP
+ * graph, captured the first time a (plan, field count, buffers) combination
+ * is seen and replayed after that; 0: the launches are issued one by one.
+ * The streams are the same either way. */
+#define ASZ_OPT_GRAPHS 7
 int asz_set_option(asz_ctx *ctx, int option, int64_t value);
 /* Orders the context's later work after everything already enqueued on
  * `stream` (a hipStream_t of the same device; NULL: the null stream), without
