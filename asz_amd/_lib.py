@@ -91,6 +91,15 @@ ASZ_OPT_COARSE_ROUNDS = 6
 ASZ_OPT_GRAPHS = 7
 
 
+class RoundTune(ctypes.Structure):
+    _fields_ = [("seg", ctypes.c_uint8), ("xsh", ctypes.c_uint8), ("rpw", ctypes.c_uint16)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("rounds_tuned", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("default_ms", ctypes.c_double),
+                ("tuned_ms", ctypes.c_double)]
+
+
 # exported entry points (must match include/asz_gpu.h)
 EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_default", "asz_get_stats",
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
@@ -99,7 +108,8 @@ EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_de
            "asz_free",
            "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_compress_field", "asz_prepare_round", "asz_round_energy",
            "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
-           "asz_select_family", "asz_range_f32_noalloc")
+           "asz_select_family", "asz_range_f32_noalloc", "asz_autotune", "asz_tune_lookup", "asz_tune_save",
+           "asz_tune_load")
 
 _lib = None
 
@@ -158,6 +168,11 @@ def lib():
     L.asz_generate_plan.argtypes = [_P, _P, _P, ctypes.POINTER(Params), ctypes.c_float, ctypes.c_float,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Plan),
                                     ctypes.POINTER(PlanReport)]
+    L.asz_autotune.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_uint32,
+                               ctypes.POINTER(TuneReport)]
+    L.asz_tune_lookup.argtypes = [_P, _P, ctypes.POINTER(Plan), _U64, _U64, ctypes.POINTER(RoundTune)]
+    L.asz_tune_save.argtypes = [_P, ctypes.c_char_p]
+    L.asz_tune_load.argtypes = [_P, ctypes.c_char_p]
     L.asz_select_family.argtypes = [_P, _P, _P, ctypes.POINTER(Plan), ctypes.POINTER(Params), ctypes.c_float,
                                     ctypes.c_float, ctypes.POINTER(FamilyOpts), ctypes.POINTER(FamilyReport)]
     L.asz_free.argtypes = [_P]

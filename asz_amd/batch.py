@@ -47,11 +47,40 @@ def c4_fields(steps=48, seed=4, device="cuda"):
     return out
 
 
+def autotune(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, family="cubic", cache=None, reps=3):
+    """Tile shapes for every distinct (shape, plan) of the batch: read from the
+    cache file when it has them, otherwise autotuned on the first field of
+    that shape (a copy; the fields are not modified) and written back.
+    Returns the number of (shape, plan) keys tuned now."""
+    import os
+    params = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=block_pivot_ratio, family=family)
+    if cache and os.path.exists(cache):
+        ctx.load_tunes(cache)
+    done, tuned = set(), 0
+    for f, pl in zip(flds, plans):
+        key = (tuple(f.shape), bytes(pl))
+        if key in done:
+            continue
+        done.add(key)
+        if ctx.tune_lookup(tuple(f.shape), pl, params) is not None:
+            continue
+        mn, mx = ctx.range(f.data_ptr(), f.numel())
+        slope, *_ = ctx.default_slope(f.data_ptr(), tuple(f.shape), params, mn, mx)
+        params.default_slope = slope
+        ctx.autotune(f, pl, params, reps=reps)
+        tuned += 1
+    if cache and tuned:
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        ctx.save_tunes(cache)
+    return tuned
+
+
 def compress_fields(ctx: codec.Context, flds, plans, rel=1e-3, block_pivot_ratio=3, tsel="grad", concurrency=6,
                     on_device=False, fields_per_chunk=0, family="cubic"):
     """Batched form; returns (streams, per-field info, wall seconds).  With
     on_device the streams stay in HBM as (pointer, bytes) pairs;
-    fields_per_chunk = 1 compresses one field at a time."""
+    fields_per_chunk = 1 compresses one field at a time.  Tile shapes
+    autotuned on ctx (autotune()) apply to the matching fields."""
     params = codec.make_params(rel_eb=float(np.float32(rel)), block_pivot_ratio=block_pivot_ratio, family=family)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -82,22 +111,29 @@ def c4_plans(flds):
             for f in flds]
 
 
-def config4_comparison(concurrency=6, reps=3):
+def config4_comparison(concurrency=6, reps=3, tune_cache=None):
     """Batched (gradient-tuned, concurrent) vs plain per-field GPU form on the
     60 config-4 fields at REL 1e-3: wall time, total compression ratio (host
     Zstd after the timed region) and max error over eb.  The plain form is
     the same C entry point with one worker (one field after another on one
-    stream) and the discrete t scan, so the comparison carries no Python
-    overhead on either side; inputs and streams stay in HBM for both.
-    batched_per_field runs the batched configuration with one field per
-    chunk (concurrency alone, no multi-field kernels)."""
+    stream), the discrete t scan and the default tile shapes, so the
+    comparison carries no Python overhead on either side; inputs and streams
+    stay in HBM for both.  The batched forms run with the tile shapes the
+    autotuner picked for each (shape, plan) (read from tune_cache when it has
+    them; the tuning time is reported apart).  batched_per_field runs the
+    batched configuration with one field per chunk (concurrency alone, no
+    multi-field kernels)."""
     flds = c4_fields()
     plans = c4_plans(flds)
     n_bytes = sum(f.numel() * 4 for f in flds)
     ctx = codec.Context(0)
+    plain_ctx = codec.Context(0)
     try:
         res = {}
-        for name, fn in (("plain", lambda w: compress_fields(ctx, w, plans, tsel="gpu", concurrency=1,
+        t0 = time.perf_counter()
+        tuned_keys = autotune(ctx, flds, plans, cache=tune_cache)
+        res["autotune"] = {"keys_tuned": tuned_keys, "seconds": time.perf_counter() - t0, "cache": tune_cache}
+        for name, fn in (("plain", lambda w: compress_fields(plain_ctx, w, plans, tsel="gpu", concurrency=1,
                                                              on_device=True, fields_per_chunk=1)),
                          ("batched_tuned", lambda w: compress_fields(ctx, w, plans, tsel="grad",
                                                                      concurrency=concurrency, on_device=True)),
@@ -139,6 +175,7 @@ def config4_comparison(concurrency=6, reps=3):
         return res
     finally:
         ctx.close()
+        plain_ctx.close()
 
 
 if __name__ == "__main__":

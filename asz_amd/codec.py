@@ -132,6 +132,31 @@ class Context:
         """Run multi-field compressions' device sequence as captured HIP graphs."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_GRAPHS, 1 if on else 0))
 
+    def autotune(self, field, plan: Plan, params: Params, reps: int = 3):
+        """Time every candidate tile shape of each large round of `plan` on the
+        CUDA tensor `field` (left untouched) and keep the fastest for this
+        (shape, plan, block pivot ratio); returns the TuneReport."""
+        rep = _lib.TuneReport()
+        d = np.asarray(field.shape, np.uint64)
+        self.after_torch()
+        self._check(self._L.asz_autotune(self._c, ctypes.c_void_p(field.data_ptr()), _ptr(d), ctypes.byref(plan),
+                                         ctypes.byref(params), int(reps), ctypes.byref(rep)))
+        return rep
+
+    def tune_lookup(self, shape, plan: Plan, params: Params):
+        """The autotuned tile shapes for (shape, plan, block pivot ratio), or None."""
+        t = (_lib.RoundTune * _lib.MAX_LAYERS)()
+        d = np.asarray(shape, np.uint64)
+        found = self._L.asz_tune_lookup(self._c, _ptr(d), ctypes.byref(plan), params.block_pivot_ratio,
+                                        params.h1_block_sampling_num, t)
+        return [(x.seg, x.xsh, x.rpw) for x in t[:plan.layers]] if found else None
+
+    def save_tunes(self, path: str):
+        self._check(self._L.asz_tune_save(self._c, path.encode()))
+
+    def load_tunes(self, path: str):
+        self._check(self._L.asz_tune_load(self._c, path.encode()))
+
     def set_stream_on_device(self, on: bool = True):
         """Leave compress_device's stream in HBM (stream_view() then copies it out)."""
         self._check(self._L.asz_set_option(self._c, _lib.ASZ_OPT_STREAM_ON_DEVICE, 1 if on else 0))

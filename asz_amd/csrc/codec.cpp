@@ -327,6 +327,16 @@ static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
     c->epoch = 0;
 }
 
+// The autotuned tile shapes for (dims, plan, block pivot ratio, sampling count), if any.
+const TuneEntry* find_tune(const asz_ctx* c, const uint64_t dims[3], const asz_plan& plan, uint64_t bpr, uint64_t h1) {
+    const asz_plan cp = canonical_plan(plan);
+    for (const TuneEntry& e : c->tunes)
+        if (std::memcmp(e.dims, dims, sizeof(e.dims)) == 0 && e.bpr == bpr && e.h1 == h1 &&
+            std::memcmp(&e.plan, &cp, sizeof(cp)) == 0)
+            return &e;
+    return nullptr;
+}
+
 // Runs a device sequence as a HIP graph: captured from `st` (with the side
 // stream it forks to) the first time a signature is seen, instantiated, kept
 // among the context's last kGraphSlots graphs, then launched on `st`.
@@ -422,6 +432,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // leading rounds that may run as block rounds (t selection and targets in
     // one launch per round)
     key.coarse = c->coarse_max < 0 ? (uint32_t)ASZ_MAX_LAYERS : (uint32_t)c->coarse_max;
+    if (const TuneEntry* te = find_tune(c, dims, plan, p.block_pivot_ratio, p.h1_block_sampling_num))
+        std::memcpy(key.tune, te->tune, sizeof(key.tune));
     PlanCache* pslot = nullptr;
     for (PlanCache& e : c->pcs)
         if (e.valid && std::memcmp(&key, &e.key, sizeof(key)) == 0) pslot = &e;
@@ -437,7 +449,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         pc.valid = false;
         Geometry g0 = make_geometry(dims, plan, p.block_pivot_ratio);
         pc.P = make_plan(g0, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, fin[0].default_slope,
-                         p.h1_block_sampling_num, key.coarse);
+                         p.h1_block_sampling_num, key.coarse, key.tune);
         const Plan& P0 = pc.P;
         const PlanUpload u = plan_upload_layout(P0);
         uint8_t* hm = c->pin_meta.get<uint8_t>(u.bytes);
@@ -1235,6 +1247,36 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     S.wall_ms = ms_since(wall0);
 }
 
+// One compression of a device field for the autotuner: GPU t selection, the
+// stream left in device memory, an event pair around round `timed_round`'s
+// kernel only; returns that kernel's device time (ms).
+double compress_timed_round(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
+                            const asz_params& p, float mn, float mx, int timed_round) {
+    const bool on_dev = c->stream_on_device, kev = c->kev_on;
+    const int kr = c->kev_round;
+    c->stream_on_device = true;
+    c->kev_on = true;
+    c->kev_round = timed_round;
+    if (c->kev.empty()) {
+        c->kev.resize(2 * 256);
+        c->kev_tag.resize(256);
+        for (auto& e : c->kev) HIPCHECK(hipEventCreate(&e));
+    }
+    struct Restore {
+        asz_ctx* c;
+        bool on_dev, kev;
+        int kr;
+        ~Restore() {
+            c->stream_on_device = on_dev;
+            c->kev_on = kev;
+            c->kev_round = kr;
+        }
+    } restore{c, on_dev, kev, kr};
+    const FieldIn f{d_work, mn, mx, p.default_slope};
+    compress_fields(c, 1, &f, dims, plan, p, ASZ_TSEL_GPU, nullptr, nullptr, nullptr);
+    return c->stats.round_ms[timed_round];
+}
+
 // One field (the plain drop-in).
 static void compress_impl(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
                           const asz_params& p, float mn, float mx, int tsel_mode, const int64_t* t_inject,
@@ -1875,6 +1917,7 @@ int asz_compress_batch(asz_ctx* c, uint32_t nfields, float* const* d_fields, con
                     k->wide_codes = wide;
                     k->coarse_max = coarse_max;
                     k->graphs = graphs;
+                    k->tunes = c->tunes;
                     asz::compress_fields(k, nf, fin.data(), d, plans[ch[0]], *p, tsel_mode, nullptr, nullptr,
                                          nullptr);
                     graph_launches += k->stats.graph_launched;

@@ -95,6 +95,16 @@ struct PlanKey {
     uint64_t bpr, h1;
     uint32_t coarse;  // most leading rounds run as block rounds (kRoundBlock)
     uint32_t pad;
+    RoundTune tune[ASZ_MAX_LAYERS];  // the autotuner's tile shapes (all zero: the defaults)
+};
+
+// The autotuner's choices for one (dims, plan, block pivot ratio, sampling
+// count): a tile shape per round.
+struct TuneEntry {
+    uint64_t dims[3];
+    asz_plan plan;  // canonical
+    uint64_t bpr, h1;
+    RoundTune tune[ASZ_MAX_LAYERS];
 };
 
 struct PlanCache {
@@ -135,7 +145,7 @@ struct asz_ctx {
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
         sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early;
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, tune_work;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early;
     // exact histograms of arrays wider than the global window, and the t and
     // pivot arrays' histograms, which no round waits for
@@ -160,6 +170,7 @@ struct asz_ctx {
     bool wide_codes = false;  // ASZ_OPT_WIDE_CODES
     int coarse_max = -1;      // ASZ_OPT_COARSE_ROUNDS (-1: every leading round that qualifies)
     bool graphs = true;       // ASZ_OPT_GRAPHS
+    std::vector<asz::TuneEntry> tunes;  // autotuned tile shapes (asz_autotune, asz_tune_load)
     static constexpr size_t kGraphSlots = 8;
     std::vector<asz::GraphEntry> graph_cache;  // captured multi-field device sequences, least recently used first out
     float dev_mn = 0, dev_mx = 0;  // range of the last asz_compress_field
@@ -196,6 +207,11 @@ inline TaskPool& host_pool(asz_ctx* c) {
 
 // The range of one field from its launch_range result words (codec.cpp).
 void range_result(asz_ctx* c, const uint32_t* w, const float* d_field, float* mn, float* mx, const std::string& who);
+// The autotuned tile shapes for (dims, plan, block pivot ratio, sampling count), if any (codec.cpp).
+const TuneEntry* find_tune(const asz_ctx* c, const uint64_t dims[3], const asz_plan& plan, uint64_t bpr, uint64_t h1);
+// One compression timing one round's kernel (codec.cpp; the autotuner's trial).
+double compress_timed_round(asz_ctx* c, float* d_work, const uint64_t dims[3], const asz_plan& plan,
+                            const asz_params& p, float mn, float mx, int timed_round);
 // Resolves, once per device (current), the address of the allocation-free
 // range scan's scratch array (noalloc.hip).
 void noalloc_setup(int device);

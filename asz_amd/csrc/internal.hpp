@@ -202,10 +202,21 @@ struct Plan {
     uint32_t pn[3];                // pivot lattice counts per axis
     float pivot_eb;
 };
+// Tile shape of one round's kernel, chosen by the autotuner (0: the default
+// rule): sweep rounds' segment length (targets a lane walks along the round
+// axis, at most kSweepSeg) and lanes along DIM2 (1 << xsh, 6 <= xsh <= 8);
+// row rounds' lattice rows per workgroup (rounded up to a multiple of
+// 4 kRowsPerWave).  The QB layout and every value are the same for any tile
+// shape.
+struct RoundTune {
+    uint8_t seg, xsh;
+    uint16_t rpw;
+};
 // coarse_max: the most leading rounds that may run as block rounds (those
-// with at most kCoarseTargets targets per block).
+// with at most kCoarseTargets targets per block).  tune: one entry per round
+// or nullptr.
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num, uint32_t coarse_max = 0);
+               float default_slope, uint64_t h1_num, uint32_t coarse_max = 0, const RoundTune* tune = nullptr);
 // The per-call scalars of a plan (pivot and layer error bounds, layer slopes);
 // the tables do not depend on them.
 void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,

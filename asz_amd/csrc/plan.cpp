@@ -50,7 +50,7 @@ void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio
 }
 
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num, uint32_t coarse_max) {
+               float default_slope, uint64_t h1_num, uint32_t coarse_max, const RoundTune* tune) {
     Plan P;
     P.g = g;
     P.pivot_count = 1;
@@ -110,16 +110,18 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.kind = d == 2 ? kRoundRow : kRoundSweep;
         // targets per block along d; any value above the target count gives block 0
         R.tpb = make_fastdiv((uint32_t)std::min<uint64_t>(g.block_shape[d] / s[d], 1ull << 30));
+        const RoundTune T = tune ? tune[r] : RoundTune{0, 0, 0};
         if (R.kind == kRoundSweep) {
+            const uint32_t seg_len = T.seg ? std::min<uint32_t>(T.seg, kSweepSeg) : kSweepSeg;
             R.seg_off = (uint32_t)P.segs.size();
             const uint32_t per_block = (uint32_t)(g.block_shape[d] / s[d]);  // targets per full block along d
             const uint32_t* cntd = P.tables.data() + R.tab_off[d];
             for (uint32_t b = 0; b < R.nb[d]; b++) {
                 const uint32_t k0 = b * per_block;
-                // equal segments of at most kSweepSeg targets: a block run that fits one
+                // equal segments of at most seg_len targets: a block run that fits one
                 // segment is walked whole, so its QB run (orders along the round axis)
                 // is written by one lane and a wave's runs form one contiguous span
-                const uint32_t nseg = (cntd[b] + kSweepSeg - 1) / kSweepSeg;
+                const uint32_t nseg = (cntd[b] + seg_len - 1) / seg_len;
                 for (uint32_t q = 0; q < nseg; q++) {
                     const uint32_t c0 = (uint32_t)((uint64_t)cntd[b] * q / nseg);
                     const uint32_t c1 = (uint32_t)((uint64_t)cntd[b] * (q + 1) / nseg);
@@ -133,6 +135,7 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             // lanes) and as many lattice rows of the other axis as fill its 256 lanes
             R.xsh = 6;
             while (R.xsh < 8 && (1u << R.xsh) < R.n[2]) R.xsh++;
+            if (T.xsh) R.xsh = std::min<uint32_t>(std::max<uint32_t>(T.xsh, 6), 8);
             const uint32_t rows = 256u >> R.xsh;
             R.nwg = (uint64_t)((R.n[2] + (1u << R.xsh) - 1) >> R.xsh) * ((R.n[1 - d] + rows - 1) / rows) * R.nseg;
         } else {
@@ -143,6 +146,7 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             const uint64_t cap = up4((rows + 2047) / 2048);
             const uint64_t g = 4 * kRowsPerWave;  // rows of one pass of the four waves
             uint64_t rpw = std::min<uint64_t>(std::max<uint64_t>(4, std::min(want, cap)), 1024);
+            if (T.rpw) rpw = std::min<uint64_t>(T.rpw, 1024);
             R.rpw = (uint32_t)((rpw + g - 1) / g * g);
             R.nwg = (rows + R.rpw - 1) / R.rpw;
         }

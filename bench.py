@@ -388,9 +388,11 @@ def c4_line():
     """The 60-field batch: batched gradient-tuned form vs plain per-field GPU form."""
     import torch
     from asz_amd import batch
-    r = batch.config4_comparison(reps=3)
+    # tile shapes autotuned once per (shape, plan) into a cache file, then reused
+    r = batch.config4_comparison(reps=3, tune_cache=os.path.join(tempfile.gettempdir(), "asz_amd_tunes.txt"))
     torch.cuda.empty_cache()
-    keep = {"plain": r["plain"], "batched_tuned": r["batched_tuned"], "batched_plain_t": r["batched_plain_t"]}
+    keep = {"plain": r["plain"], "batched_tuned": r["batched_tuned"], "batched_plain_t": r["batched_plain_t"],
+            "batched_per_field": r["batched_per_field"], "autotune": r["autotune"]}
     return {"workload": "48 time steps of 100x500x500 (one saved plan) + 12 mixed shapes, REL 1e-3, "
                         f"{r['fields']} fields, {r['bytes'] / 1e9:.2f} GB; best of 3 runs each",
             **keep,

@@ -365,6 +365,39 @@ int asz_round_energy_adjoint(asz_ctx *ctx, const float *d_work, const uint64_t d
                              const asz_params *p, int round, int quintic, const float *d_theta, const float *d_gE,
                              float *d_gtheta);
 
+/* ------------------------------------------------------------------ autotuner
+ * (no reference counterpart; for the batched form)  Tile shape of one round's
+ * kernel: a sweep round's segment length (targets a lane walks along the
+ * round axis, at most 24) and lanes along DIM2 (1 << xsh, 6 <= xsh <= 8); a
+ * DIM2 round's lattice rows per workgroup.  0 keeps the default rule.  The
+ * stream and the reconstruction are the same for every tile shape.
+ *
+ * asz_autotune: for each round of at least 1/64 of the field's points that is
+ * not a block round, times its kernel (HIP events, the median of `reps`
+ * compressions of a copy of d_field, which is left untouched) under every
+ * candidate tile shape and keeps the fastest; the context then uses the
+ * choices for every compression of this (dims, plan, block pivot ratio,
+ * sampling count).  p->default_slope is used as given.  report (optional):
+ * rounds tuned, trials, and the summed kernel time of those rounds under the
+ * default and the chosen shapes.
+ * asz_tune_lookup: the context's choices for a key (1 found, 0 none).
+ * asz_tune_save / asz_tune_load: the context's choices as a text file, one
+ * line per key (load merges, replacing entries of the same key). */
+typedef struct asz_round_tune {
+    uint8_t seg, xsh;
+    uint16_t rpw;
+} asz_round_tune;
+typedef struct asz_tune_report {
+    uint32_t rounds_tuned, trials;
+    double default_ms, tuned_ms;
+} asz_tune_report;
+int asz_autotune(asz_ctx *ctx, const float *d_field, const uint64_t dims[3], const asz_plan *plan,
+                 const asz_params *p, uint32_t reps, asz_tune_report *report);
+int asz_tune_lookup(asz_ctx *ctx, const uint64_t dims[3], const asz_plan *plan, uint64_t block_pivot_ratio,
+                    uint64_t h1_block_sampling_num, asz_round_tune tune[ASZ_MAX_LAYERS]);
+int asz_tune_save(asz_ctx *ctx, const char *path);
+int asz_tune_load(asz_ctx *ctx, const char *path);
+
 /* ------------------------------------------------------------------ Huffman (host side)
  * The codebook and header of one array as huffman_encoder_class<T>::preprocess_encode
  * + save build them (huffman.hpp:263-320, 650-691), from a dense histogram of
