@@ -178,7 +178,35 @@ def config4_comparison(concurrency=6, reps=3, tune_cache=None):
         plain_ctx.close()
 
 
+def run_batched(concurrency=6, reps=1, tsel="grad", tune_cache=None):
+    """The batched form alone on the 60 config-4 fields (for profiling):
+    returns the wall seconds of each repetition."""
+    flds = c4_fields()
+    plans = c4_plans(flds)
+    with codec.Context(0) as ctx:
+        autotune(ctx, flds, plans, cache=tune_cache)
+        walls = []
+        for _ in range(reps):
+            work = [f.clone() for f in flds]
+            streams, _, wall = compress_fields(ctx, work, plans, tsel=tsel, concurrency=concurrency, on_device=True)
+            for ptr, _ in streams:
+                ctx.free_device(ptr)
+            walls.append(wall)
+            del work
+        return walls
+
+
 if __name__ == "__main__":
+    import argparse
     import json
-    import sys
-    print(json.dumps(config4_comparison(concurrency=int(sys.argv[1]) if len(sys.argv) > 1 else 6), indent=1))
+    ap = argparse.ArgumentParser(description="the 60-field batch: batched form vs plain per-field form")
+    ap.add_argument("--concurrency", type=int, default=6)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--tune-cache", default=None)
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched gradient-tuned form")
+    a = ap.parse_args()
+    if a.batched_only:
+        print(json.dumps({"wall_s": run_batched(a.concurrency, a.reps, tune_cache=a.tune_cache)}))
+    else:
+        print(json.dumps(config4_comparison(concurrency=a.concurrency, reps=a.reps, tune_cache=a.tune_cache),
+                         indent=1))
