@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_round_block(TselDesc T, uint32_t tsel, 
             hist_add(lh, win, gwin, q);
         }
     }
-    if (R.nwg) finish_wg(err, sq, qmin, qmax, R, b, part, lh, phist, gwin);
+    if (R.nwg) finish_wg(err, sq, qmin, qmax, R, b, part, lh, phist);
     TSEL_MARK(7);
 }
 

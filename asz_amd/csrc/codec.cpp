@@ -521,8 +521,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     Partial* part = c->partials.get<Partial>(fs.part * nf);
     Partial* ppart = c->fparts.get<Partial>(fs.fpart * nf);
     Partial* fpart = ppart + npp;
-    uint32_t* phist = c->phist.get<uint32_t>(fs.phist * nf);
-    uint32_t* ghist = c->ghist.get<uint32_t>(fs.ghist * nf);
+    // the global windows, then the rounds' histogram copies: one fill clears both
+    uint32_t* ghist = c->ghist.get<uint32_t>((fs.ghist + fs.phist) * nf);
+    uint32_t* phist = ghist + fs.ghist * nf;
     DevAcc* acc = c->xfer.get<DevAcc>(nf);
     FieldDesc* dfd_multi = nullptr;  // multi-field: the device copy of hfd (uploaded with the device sequence)
     if (multi) {
@@ -794,7 +795,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     auto device_sequence = [&]() {
         if (multi) HIPCHECK(hipMemcpyAsync(dfd_multi, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
         launch_init_acc(acc, st, nf);
-        launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * fs.ghist, 0, st);
+        launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * (fs.ghist + fs.phist), 0, st);
         if (tsel_mode == ASZ_TSEL_INJECT) {
             HIPCHECK(hipMemcpyAsync(tblk, tf_inj.data(), nt * 4, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(sym, ti_inj.data(), nt * 4, hipMemcpyHostToDevice, st));

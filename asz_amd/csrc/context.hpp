@@ -144,7 +144,7 @@ struct asz_ctx {
     std::string err;
     asz_stats stats{};
     asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
-        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, phist, ghist, xfer, hmeta, scratch,
+        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, ghist, xfer, hmeta, scratch,
         chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, tune_work;
     asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early;
     // exact histograms of arrays wider than the global window, and the t and

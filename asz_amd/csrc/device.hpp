@@ -178,16 +178,17 @@ __device__ __forceinline__ void hist_add(uint32_t* lh, uint32_t win, uint32_t* g
     }
 }
 
-// Per-workgroup results of a compress round kernel, stored at the
+// Per-workgroup results of a compress round kernel (NW waves), at the
 // workgroup's linear index wgl within the round: one Partial (max |x - x'|,
-// sum (x - x')^2, QB range) and the LDS histogram window.  Called by every
-// thread after its last QB; the barrier inside also completes the LDS
+// sum (x - x')^2, QB range) stored, and the LDS histogram window's nonzero
+// bins added into copy wgl % hist_copies of the round's window.  Called by
+// every thread after its last QB; the barrier inside also completes the LDS
 // histogram.
+template <int NW = 4>
 __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, int32_t qmax, const RoundDesc& R,
-                                          uint64_t wgl, Partial* part, const uint32_t* lh, uint32_t* phist,
-                                          uint32_t* gwin) {
-    __shared__ double s_err[4], s_sq[4];
-    __shared__ int32_t s_min[4], s_max[4];
+                                          uint64_t wgl, Partial* part, const uint32_t* lh, uint32_t* phist) {
+    __shared__ double s_err[NW], s_sq[NW];
+    __shared__ int32_t s_min[NW], s_max[NW];
     err = wave_max(err);
     sq = wave_sum(sq);
     qmin = wave_min(qmin);
@@ -197,7 +198,7 @@ __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, i
     __syncthreads();
     const uint64_t wg = R.part_off + wgl;
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; k++) {
+        for (int k = 1; k < NW; k++) {
             err = fmax(err, s_err[k]);
             sq += s_sq[k];
             qmin = min(qmin, s_min[k]);
@@ -205,13 +206,10 @@ __device__ __forceinline__ void finish_wg(double err, double sq, int32_t qmin, i
         }
         part[wg] = Partial{err, sq, qmin, qmax};
     }
-    // central bins to the partial, the others (nonzero) to the global window
-    uint32_t* out = phist + R.hist_off + (wg - R.part_off) * R.hist_win;
-    const uint32_t c0 = (R.lds_win - R.hist_win) / 2;
+    uint32_t* cw = phist + R.hist_off + (uint64_t)((uint32_t)wgl % R.hist_copies) * R.lds_win;
     for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) {
         const uint32_t v = lh[i];
-        if (i - c0 < R.hist_win) out[i - c0] = v;
-        else if (v) atomicAdd(&gwin[(uint32_t)kHistCentre - R.lds_win / 2 + i], v);
+        if (v) atomicAdd(&cw[i], v);
     }
 }
 

@@ -69,54 +69,39 @@ __device__ inline Partial fold_partials(const Partial* __restrict__ part, uint64
     return r;
 }
 
-// One fold workgroup: workgroup grp = pg F.slices + slice of round r (one
-// launch after a round kernel covers one round, or a batch of finished
-// rounds; the pointers are the field's).  It owns round r's per-workgroup
-// results [pg F.pper, (pg+1) F.pper) over one slice of at most 256 bins: it
-// sums their histogram windows (4096 words per workgroup, coalesced along the
-// bins) and adds the nonzero sums to the array's global window; the
-// first-slice workgroup folds their Partials into fpart[F.fpart_off + pg].
-// The last workgroup of the round to finish (counted in acc->fold_count[r],
-// which it resets) folds those into acc->arr[2 + r] and the window's tail.
+// One fold workgroup grp of round r (one launch after a round kernel covers
+// one round, or a batch of finished rounds; the pointers are the field's).
+// grp < F.slices: bins [256 grp, 256 grp + 256) of the LDS window, summed
+// over the round's F.copies histogram copies into the array's global window
+// (coalesced along the bins).  Otherwise partial group pg = grp - F.slices:
+// the Partials of round workgroups [pg kFoldPartials, (pg + 1) kFoldPartials)
+// folded into fpart[F.fpart_off + pg].  The last workgroup of the round to
+// finish (counted in acc->fold_count[r], which it resets) folds those into
+// acc->arr[2 + r] and the window's tail.
 __device__ inline void fold_group(const FoldDesc* __restrict__ descs, uint32_t r, uint32_t grp,
                                   const Partial* __restrict__ part, const uint32_t* __restrict__ phist,
                                   Partial* fpart, DevAcc* acc, uint32_t* ghist, uint32_t* hres, uint32_t epoch) {
-    __shared__ uint32_t lsum[256];
     __shared__ int s_last;
     const FoldDesc F = descs[r];
     if (grp >= F.groups) return;
     uint32_t* gw = ghist + (uint64_t)(2 + r) * kHistStride;
-    // fold workgroup = (partial group pg, bin slice)
-    const uint32_t slice = grp % F.slices, pg = grp / F.slices;
-    const uint64_t p0 = (uint64_t)pg * F.pper, p1 = min<uint64_t>(F.nwg, p0 + F.pper);
-    const uint32_t win = F.win;
-    const uint32_t* src = phist + F.hist_off;
-    uint32_t* g = gw + kHistCentre - win / 2;
-    if (win <= 256) {
-        // a thread keeps one bin; 256 / win windows per step
-        const uint32_t bin = threadIdx.x & (win - 1), sub = threadIdx.x / win, step = 256 / win;
-        uint32_t s = 0;
-#pragma unroll 8
-        for (uint64_t p = p0 + sub; p < p1; p += step) s += src[p * win + bin];
-        lsum[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x < win) {
-            for (uint32_t k = 1; k < step; k++) s += lsum[threadIdx.x + k * win];
-            if (s) atomicAdd(&g[bin], s);
-        }
-    } else {
-        // one bin of this workgroup's 256-bin slice per thread
-        const uint32_t bin = slice * 256 + threadIdx.x;
-        uint32_t s = 0;
-#pragma unroll 8
-        for (uint64_t p = p0; p < p1; p++) s += src[p * win + bin];
-        if (s) atomicAdd(&g[bin], s);
-    }
     Partial* fp = fpart + F.fpart_off;
-    if (slice == 0) {  // uniform per workgroup
+    if (grp < F.slices) {
+        // bin slice grp: one bin per thread, summed over the copies; the
+        // global window's bins inside the LDS window receive nothing else
+        const uint32_t bin = grp * 256 + threadIdx.x;
+        const uint32_t* src = phist + F.hist_off + bin;
+        uint32_t s = 0;
+#pragma unroll 8
+        for (uint32_t k = 0; k < F.copies; k++) s += src[(uint64_t)k * F.win];
+        gw[kHistCentre - F.win / 2 + bin] = s;
+    } else {  // uniform per workgroup
+        const uint32_t pg = grp - F.slices;
+        const uint64_t p0 = (uint64_t)pg * kFoldPartials, p1 = min<uint64_t>(F.nwg, p0 + kFoldPartials);
         const Partial mine = fold_partials(part + F.part_off, p0, p1);
         if (threadIdx.x == 0) fp[pg] = mine;
     }
+    __syncthreads();  // every wave's stores done before the release below
     if (threadIdx.x == 0) {
         __threadfence();
         s_last = atomicAdd(&acc->fold_count[r], 1u) == F.groups - 1;
@@ -124,7 +109,7 @@ __device__ inline void fold_group(const FoldDesc* __restrict__ descs, uint32_t r
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    const Partial all = fold_partials(fp, 0, F.groups / F.slices);
+    const Partial all = fold_partials(fp, 0, F.pgroups);
     if (threadIdx.x == 0) {
         acc->rerr[r] = ErrAcc{all.max_err, all.sq};
         acc->arr[2 + r].qmin = all.qmin;

@@ -78,9 +78,9 @@ struct RoundDesc {
     uint32_t rpw;       // row kernel: lattice rows per workgroup (multiple of kRowsPerWave * 4)
     uint64_t part_off;  // first per-workgroup partial of this round
     uint64_t nwg;       // workgroups of this round's kernel
-    uint64_t hist_off;  // first word of this round's per-workgroup histogram windows
-    uint32_t hist_win;  // bins of each workgroup's stored (partial) window (power of 2)
-    uint32_t lds_win;   // bins of each workgroup's LDS histogram (power of 2, >= hist_win)
+    uint64_t hist_off;    // first word of this round's histogram copies (hist_copies x lds_win words)
+    uint32_t hist_copies; // copies of the LDS window the workgroups add into (workgroup w: copy w % hist_copies)
+    uint32_t lds_win;     // bins of each workgroup's LDS histogram (power of 2, >= kLdsWinMin)
     uint32_t layer;     // index of this round in the plan
     uint32_t xsh;       // sweep: log2 of a workgroup's lanes along DIM2 (6..8); 256 >> xsh lattice rows each
     FastDiv tpb;        // targets per full block along the round axis, B[d] / (2h)
@@ -128,24 +128,24 @@ constexpr uint32_t kSweepSeg = 24;    // most targets per sweep segment (registe
 constexpr uint32_t kRowsPerWave = 4;  // row kernel: lattice rows a wave processes together
 
 // QB histograms.  Every round kernel bins its QBs into a per-workgroup LDS
-// histogram of lds_win symbols around 0.  At the end of the workgroup the
-// central hist_win bins are stored as its partial histogram (folded later
-// without atomics) and the other nonzero bins are added to the array's global
-// window of kHistGlobal bins (centre kHistCentre) with atomics; the rare QB
-// outside the LDS histogram goes there directly.  No global atomic sits in a
-// round kernel's main loop: gfx9 counts them in vmcnt with the loads, so a
-// slow one would stall every later load.  Rounds with few workgroups (the
-// coarse ones, whose QBs spread widest) get the widest partial windows, so the
-// partials stay near a round's budget: a quarter word per target, between
-// kHistPartialMin and kHistPartialWords words (the partials are written once
-// and folded once, so a larger budget costs HBM traffic on small rounds,
-// while every QB outside the stored window is one more atomic at most).
-// Arrays whose QB range leaves the global window are histogrammed again by a
-// dedicated pass.
+// histogram of lds_win symbols around 0.  At the end of the workgroup its
+// nonzero bins are added with atomics into one of the round's hist_copies
+// copies of that window (workgroup w into copy w % hist_copies, so that at
+// most kWgPerCopy-odd workgroups share an address); the rare QB outside the
+// LDS histogram goes to the array's global window of kHistGlobal bins
+// (centre kHistCentre) directly.  The fold then sums the copies, a few KB
+// per round whatever its workgroup count.  No global atomic sits in a round
+// kernel's main loop: gfx9 counts them in vmcnt with the loads, so a slow one
+// would stall every later load.  Rounds with few workgroups (the coarse
+// ones, whose QBs spread widest) get the widest LDS windows: the window
+// doubles while 2 x window x workgroups stays within a quarter word per
+// target, clamped to [kHistWidthMin, kHistWidthWords].  Arrays whose QB
+// range leaves the global window are histogrammed again by a dedicated pass.
 constexpr int kHistWin = 128;                   // window of the t / pivot arrays
 constexpr uint32_t kHistWinMin = 128, kHistWinMax = 4096;
 constexpr uint32_t kLdsWinMin = 1024;  // 4 KB: the axis-0/1 sweep then fits five workgroups per CU
-constexpr uint64_t kHistPartialWords = 1 << 20, kHistPartialMin = 1 << 15;
+constexpr uint64_t kHistWidthWords = 1 << 20, kHistWidthMin = 1 << 15;
+constexpr uint32_t kHistCopies = 16, kWgPerCopy = 32;
 constexpr uint32_t kHistGlobal = 65536;
 constexpr int32_t kHistCentre = 32768;
 
@@ -195,7 +195,7 @@ struct Plan {
     std::vector<uint32_t> ns;      // t-selection sample count per (round, block), 0 for fallback blocks
     std::vector<SweepSeg> segs;    // sweep segments of every sweep round
     uint64_t total_partials = 0;   // per-workgroup partials over all rounds
-    uint64_t total_hist_words = 0; // per-workgroup histogram windows over all rounds
+    uint64_t total_hist_words = 0; // the rounds' histogram copies (words)
     uint64_t pivot_count;
     uint64_t qb_total;             // QB buffer elements: pivots and rounds, each array aligned (kSymAlign)
     uint32_t coarse = 0;           // leading rounds of kind kRoundBlock

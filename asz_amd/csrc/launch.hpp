@@ -162,23 +162,23 @@ void launch_hist_small(const int32_t* s0, uint64_t n0, const int32_t* s1, uint64
 // round, built once per plan (make_fold_descs) and kept in device memory.
 struct FoldDesc {
     uint64_t part_off, nwg, hist_off, fpart_off;
-    uint32_t win, pper, groups, slices;
+    uint32_t win, copies, slices, pgroups, groups, pad;
 };
-// A fold workgroup sums the windows of pper consecutive round workgroups over
-// one slice of min(win, 256) bins (every thread 16 windows of one bin) and adds
-// the sums to the global window; a round has slices x ceil(nwg / pper) fold
-// workgroups, of which the first-slice ones also fold the Partials.
-inline uint32_t fold_slice(uint32_t win) { return std::min<uint32_t>(win, 256); }
-inline uint32_t fold_per_group(uint32_t win) { return 16 * (256 / fold_slice(win)); }
+// A round's fold workgroups: win / 256 bin slices, each summing its 256 bins
+// over the round's histogram copies into the global window, then
+// ceil(nwg / kFoldPartials) groups folding the workgroups' Partials.
+constexpr uint32_t kFoldPartials = 1024;
 inline void make_fold_descs(const std::vector<RoundDesc>& rounds, std::vector<FoldDesc>& out, uint64_t& fpart_total) {
     out.clear();
     fpart_total = 0;
     for (const RoundDesc& R : rounds) {
-        FoldDesc F{R.part_off, R.nwg, R.hist_off, fpart_total, R.hist_win, fold_per_group(R.hist_win), 0,
-                   R.hist_win / fold_slice(R.hist_win)};
-        const uint64_t npg = (R.nwg + F.pper - 1) / F.pper;
-        F.groups = (uint32_t)(npg * F.slices);
-        fpart_total += npg;
+        FoldDesc F{R.part_off, R.nwg, R.hist_off, fpart_total, R.lds_win, R.hist_copies, 0, 0, 0, 0};
+        if (R.nwg) {
+            F.slices = R.lds_win / 256;
+            F.pgroups = (uint32_t)((R.nwg + kFoldPartials - 1) / kFoldPartials);
+            F.groups = F.slices + F.pgroups;
+        }
+        fpart_total += F.pgroups;
         out.push_back(F);
     }
 }

@@ -161,12 +161,12 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.part_off = P.total_partials;
         P.total_partials += R.nwg;
         uint32_t win = kHistWinMin;
-        const uint64_t budget = std::min(kHistPartialWords, std::max(kHistPartialMin, R.count / 4));
+        const uint64_t budget = std::min(kHistWidthWords, std::max(kHistWidthMin, R.count / 4));
         while (win < kHistWinMax && (uint64_t)win * 2 * std::max<uint64_t>(R.nwg, 1) <= budget) win *= 2;
-        R.hist_win = win;
         R.lds_win = std::max(win, kLdsWinMin);
+        R.hist_copies = (uint32_t)std::min<uint64_t>(kHistCopies, std::max<uint64_t>(1, R.nwg / kWgPerCopy));
         R.hist_off = P.total_hist_words;
-        P.total_hist_words += R.nwg * win;
+        P.total_hist_words += R.nwg ? (uint64_t)R.hist_copies * R.lds_win : 0;
         // t-selection sample strides (compress.tpp:96-116, get_AB_map.tpp:9-12); 0 marks a fallback block
         for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
             for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)

@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restric
                                                    const uint32_t* __restrict__ tabs, const float* __restrict__ tblk,
                                                    Partial* part, uint32_t* __restrict__ phist,
                                                    uint32_t* __restrict__ gwin, FieldSet fs) {
-    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
+    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.lds_win bins
     if (!DEC && fs.fd) {  // field blockIdx.y of a multi-field call
         const uint32_t f = blockIdx.y;
         const FieldDesc& F = fs.fd[f];
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_round_row(RoundDesc R, float* __restric
             }
         }
     }
-    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist, gwin);
+    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist);
 }
 
 // Axis-2 rounds with h = 1 on rows of a multiple of four elements (16-byte
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void k_round_quad(RoundDesc R, float* __restri
             }
         }
     }
-    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist, gwin);
+    if (!DEC) finish_wg(err, sq, qmin, qmax, R, blockIdx.x, part, lh, phist);
 }
 
 // ------------------------------------------------------------------ sweep rounds (axis 0 / 1)
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
     }
     __shared__ int32_t tile[WMODE == 1 ? 4 : 1][WMODE == 1 ? kSweepSeg : 1][65];
     __shared__ uint64_t lbase[WMODE == 1 ? 4 : 1][64];
-    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.hist_win bins
+    extern __shared__ uint32_t lh[];  // compress: LDS histogram window of R.lds_win bins
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (!DEC) {
         for (uint32_t i = threadIdx.x; i < R.lds_win; i += blockDim.x) lh[i] = 0;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(256) void k_round_sweep(RoundDesc R, float* __restr
             }
         }
         finish_wg(err, sq, qmin, qmax, R, blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * sz),
-                  part, lh, phist, gwin);
+                  part, lh, phist);
     }
 }
 
