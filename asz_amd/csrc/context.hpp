@@ -193,11 +193,12 @@ namespace asz {
 
 // The side stream and its ordering events, created on first use (so that a
 // context normally owns one HIP stream).  Not thread-safe: callers create it
-// before their worker threads can.
+// before their worker threads can.  The events only order the two streams of
+// one device (no host waits on them), so they skip the system-scope fence.
 inline void ensure_side(asz_ctx* c) {
     if (!c->side) HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     for (hipEvent_t& e : c->side_ev)
-        if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
 }
 
 inline TaskPool& host_pool(asz_ctx* c) {

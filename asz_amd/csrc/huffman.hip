@@ -1,12 +1,13 @@
 // GPU side of the Huffman coder for the .asz stream (huffman.hpp).
 //
-// Histogram.  The round kernels leave one LDS window histogram per workgroup
-// (symbols [-win/2, win/2), win chosen per round) and add the rest of their
-// QBs to the array's global window (kHistGlobal bins around 0) with atomics.
-// k_fold_round, launched right after each round, folds that round's windows
-// into the global window (each workgroup of the fold owns a slice of the
-// windows, so the global atomics are few); k_hist_small bins the two short
-// arrays (t values and pivots) the same way.  The workgroup that finishes an
+// Histogram.  Every round workgroup bins its QBs in an LDS window (symbols
+// [-win/2, win/2), win chosen per round) and adds the window into one of a
+// few interleaved copies per round with atomics; the rare QB outside the
+// window goes to the array's global window (kHistGlobal bins around 0).
+// k_fold_rounds (or the fold workgroups beside the next round's t selection)
+// sums the copies into the global window and folds the workgroups' errors
+// and QB ranges; k_hist_small bins the two short arrays (t values and
+// pivots) into their global windows.  The workgroup that finishes an
 // array writes its QB range, errors and used bins straight into pinned host
 // memory and raises a flag, so the host builds that codebook while later
 // rounds run, without a copy or an API call in between.  An array whose range
@@ -283,8 +284,8 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const HuffArrayDev* __restr
 // LDS atomics.  A group longer than 64 bits (four codes averaging more than
 // 16 bits) is ORed code by code from its symbols, read again.  The codes come
 // from code32 / len8 (a 32-bit code and a byte length per symbol of every
-// array, at its book offset), in LDS when the array's alphabet is at most
-// kLdsBook symbols.
+// array, at its book offset), in LDS as (code, length) pairs when the array's
+// alphabet is at most kLdsBook symbols.
 __device__ __forceinline__ void load_groups(const int32_t* __restrict__ sym, const HuffArrayDev& A, uint64_t c,
                                             int4 v[4]) {
     const uint64_t first = (c - A.chunk0) * kEncChunk;
@@ -306,8 +307,7 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
                                                      const uint8_t* __restrict__ len8,
                                                      uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage) {
     __shared__ __align__(16) uint32_t lbuf[kEncChunk + 4];  // bit buffer: 32 bits per symbol at most
-    __shared__ uint32_t lcode[kLdsBook];
-    __shared__ uint8_t llen[kLdsBook];
+    __shared__ uint2 lbook[kLdsBook];                        // (code, length)
     __shared__ uint32_t s_tot[4][2];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint64_t G = gridDim.x;
@@ -323,42 +323,56 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
         const uint8_t* L = len8 + A.book_off;
         const bool lds = A.lds_book != 0;
         if (lds && book_k != (int)k) {
-            for (uint32_t i = t; i < A.range; i += blockDim.x) {
-                lcode[i] = C[i];
-                llen[i] = L[i];
+            // a fixed number of loads (no loop of unknown trip count), so the
+            // waits for the next chunk's symbols stay exact on every path
+            static_assert(kLdsBook == 4 * 256, "four codebook entries per thread");
+            uint32_t cb[4], lb[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++) {
+                const uint32_t e = min(t + 256 * i, (uint32_t)A.range - 1);
+                cb[i] = C[e];
+                lb[i] = L[e];
             }
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++)
+                if (t + 256 * i < A.range) lbook[t + 256 * i] = make_uint2(cb[i], lb[i]);
             book_k = (int)k;
         }
         lds_barrier();  // LDS codebook complete; the previous chunk's buffer written out
         uint64_t acc[4];
         uint32_t gb[4];
+        // symbol - min in 32 bits (the index is below the alphabet size); the
+        // padding past n (read as 0) is clamped into the table and given no bits
+        const uint32_t off32 = (uint32_t)A.offset, top = (uint32_t)A.range - 1;
         // the table lookups, LDS or global, chosen at compile time: a run-time
-        // choice would merge the two into flat (generic) loads
+        // choice would merge the two into flat (generic) loads.  No branch per
+        // symbol: the four codes of a group are joined with shifts below 64
+        // (a group of more than 64 bits is redone code by code below)
         auto join = [&](auto from_lds) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const uint32_t i0 = 1024 * j + 4 * t;
                 const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-                uint64_t a = 0;
-                uint32_t sh = 0;
+                uint32_t cd[4], ln[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    if (i0 + e < n) {
-                        const uint32_t idx = (uint32_t)((int64_t)s[e] - A.offset);
-                        uint32_t cd, ln;
-                        if constexpr (decltype(from_lds)::value) {
-                            cd = lcode[idx];
-                            ln = llen[idx];
-                        } else {
-                            cd = C[idx];
-                            ln = L[idx];
-                        }
-                        if (sh < 64) a |= (uint64_t)cd << sh;
-                        sh += ln;
+                    const uint32_t idx = min((uint32_t)s[e] - off32, top);
+                    if constexpr (decltype(from_lds)::value) {
+                        const uint2 b = lbook[idx];
+                        cd[e] = b.x;
+                        ln[e] = b.y;
+                    } else {
+                        cd[e] = C[idx];
+                        ln[e] = L[idx];
                     }
+                    const bool live = i0 + e < n;
+                    cd[e] = live ? cd[e] : 0u;
+                    ln[e] = live ? ln[e] : 0u;
                 }
-                acc[j] = a;
-                gb[j] = sh;
+                const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
+                acc[j] = (uint64_t)cd[0] | ((uint64_t)cd[1] << (p1 & 63)) | ((uint64_t)cd[2] << (p2 & 63)) |
+                         ((uint64_t)cd[3] << (p3 & 63));
+                gb[j] = p3 + ln[3];
             }
         };
         if (lds) join(std::true_type{});
@@ -399,8 +413,11 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
         const uint32_t pos[4] = {e01 & 0xFFFFu, T0 + (e01 >> 16), T0 + T1 + (e23 & 0xFFFFu),
                                  T0 + T1 + T2 + (e23 >> 16)};
         const uint32_t total = T0 + T1 + T2 + T3;
-        const uint32_t nw4 = (total + 127) / 128;  // 16-byte units of the buffer
-        for (uint32_t w = t; w < 4 * nw4; w += blockDim.x) lbuf[w] = 0;
+        const uint32_t nw4 = (total + 127) / 128;  // 16-byte units of the buffer (at most kEncChunk / 4)
+        static_assert(kEncChunk / 4 == 4 * 256, "four buffer quads per thread at most");
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+            if (t + 256 * i < nw4) reinterpret_cast<uint4*>(lbuf)[t + 256 * i] = make_uint4(0, 0, 0, 0);
         lds_barrier();
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -421,8 +438,8 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
                     const uint32_t idx = (uint32_t)((int64_t)s[i0 + e] - A.offset);
                     uint32_t cd, ln;
                     if (lds) {
-                        cd = lcode[idx];
-                        ln = llen[idx];
+                        cd = lbook[idx].x;
+                        ln = lbook[idx].y;
                     } else {
                         cd = C[idx];
                         ln = L[idx];
@@ -435,9 +452,13 @@ __global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__
             }
         }
         lds_barrier();
+        // a fixed number of (predicated) stores: the wait for the next
+        // chunk's symbols then still knows how many memory operations follow them
         uint4* dst = reinterpret_cast<uint4*>(stage + c * kEncChunk);
         const uint4* src = reinterpret_cast<const uint4*>(lbuf);
-        for (uint32_t w = t; w < nw4; w += blockDim.x) dst[w] = src[w];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+            if (t + 256 * i < nw4) dst[t + 256 * i] = src[t + 256 * i];
         if (t == 0) chunk_bits[c] = total;
     };
 
@@ -482,8 +503,9 @@ __global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict
     const uint32_t lane = threadIdx.x & 63;
     if (c >= nchunks) return;  // no barrier below: each wave works alone
     const uint32_t bits = chunk_bits[c];
-    if (!bits) return;
     const uint64_t off = chunk_off[c];
+    asm volatile("" ::"v"(bits), "v"(off));  // both loads in flight before the branch needs either
+    if (!bits) return;
     const uint32_t sh = (uint32_t)(off & 31);
     const int64_t nsw = (bits + 31) >> 5, ndw = (sh + bits + 31) >> 5;  // ndw <= nsw + 1
     const uint4* src4 = reinterpret_cast<const uint4*>(stage + c * kEncChunk);  // 16 KB slots
@@ -493,23 +515,41 @@ __global__ __launch_bounds__(256) void k_encode_place(const uint32_t* __restrict
     // (i0 - 1 = 4 m + rem; rem is the same for every quad of the chunk)
     const int64_t a0 = (int64_t)(4 * q0) - (int64_t)g0 - 1;  // >= -4
     const uint32_t rem = (uint32_t)(a0 & 3);
-    auto quad = [&](int64_t m) -> uint4 {  // staged quad m, words past nsw (or before 0) as 0
-        if (m < 0 || 4 * m >= nsw) return make_uint4(0, 0, 0, 0);
-        uint4 v = src4[m];
+    // staged quad m: loaded unconditionally from a clamped index (so that
+    // every load of a group is in flight at once), then words past nsw (or
+    // before 0) cleared
+    const int64_t mmax = (nsw - 1) >> 2;
+    auto load_quad = [&](int64_t m) { return src4[min(max(m, (int64_t)0), mmax)]; };
+    auto clip_quad = [&](uint4 v, int64_t m, bool use) -> uint4 {
         const int64_t j = 4 * m;
-        if (j + 1 >= nsw) v.y = 0;
-        if (j + 2 >= nsw) v.z = 0;
-        if (j + 3 >= nsw) v.w = 0;
+        const bool in = use && m >= 0 && j < nsw;
+        v.x = in ? v.x : 0u;
+        v.y = in && j + 1 < nsw ? v.y : 0u;
+        v.z = in && j + 2 < nsw ? v.z : 0u;
+        v.w = in && j + 3 < nsw ? v.w : 0u;
         return v;
     };
     for (uint64_t qb = q0; qb <= q1; qb += 4 * 64) {
+        // output quad q needs staged quads m and m + 1, and quad q + 1 (the
+        // next lane) needs m + 1 and m + 2: each lane loads its quad m (lanes
+        // up to q1 + 1) and takes m + 1 from the next lane; lane 63 takes it
+        // from lane 0 of the next group, or loads it after the last group
+        const int64_t m0 = (a0 + 4 * (int64_t)(qb + lane - q0) - (int64_t)rem) / 4;  // exact: the numerator is 4 m
         uint4 A[4], B[4];
 #pragma unroll
+        for (int u = 0; u < 4; u++) A[u] = load_quad(m0 + 64 * u);
+        uint4 tail = load_quad(m0 + 193);
+#pragma unroll
+        for (int u = 0; u < 4; u++) A[u] = clip_quad(A[u], m0 + 64 * u, qb + lane + 64 * u <= q1 + 1);
+        tail = clip_quad(tail, m0 + 193, lane == 63 && qb + 255 <= q1);
+#pragma unroll
         for (int u = 0; u < 4; u++) {
-            const uint64_t q = qb + lane + 64 * u;
-            const int64_t m = (a0 + 4 * (int64_t)(q - q0) - (int64_t)rem) / 4;  // exact: the numerator is 4 m
-            A[u] = q <= q1 ? quad(m) : make_uint4(0, 0, 0, 0);
-            B[u] = q <= q1 ? quad(m + 1) : make_uint4(0, 0, 0, 0);
+            B[u] = make_uint4(__shfl_down(A[u].x, 1, 64), __shfl_down(A[u].y, 1, 64), __shfl_down(A[u].z, 1, 64),
+                              __shfl_down(A[u].w, 1, 64));
+            const uint4 n0 = u < 3 ? make_uint4(__shfl(A[u + 1].x, 0, 64), __shfl(A[u + 1].y, 0, 64),
+                                                __shfl(A[u + 1].z, 0, 64), __shfl(A[u + 1].w, 0, 64))
+                                   : tail;
+            if (lane == 63) B[u] = n0;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {

@@ -151,8 +151,17 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
             R.nwg = (rows + R.rpw - 1) / R.rpw;
         }
         // a leading round with few targets per block: a block round, one
-        // workgroup per block (a partial and a window each)
-        if ((uint32_t)r == P.coarse && P.coarse < coarse_max && R.count <= kCoarseTargets * nbl) {
+        // workgroup per block (a partial and a window each); its kernel
+        // divides block-local target indices in float, exact below 2^24
+        uint64_t most = 1;
+        for (int a = 0; a < 3; a++) {
+            const uint32_t* cnt = P.tables.data() + R.tab_off[a];
+            most *= *std::max_element(cnt, cnt + R.nb[a]);
+        }
+        // and addresses a block's points by 32-bit offsets from its first target
+        const uint64_t slab = (uint64_t)std::min<uint64_t>(g.block_shape[0] + 1, g.D[0]) * g.D[1] * g.D[2];
+        if ((uint32_t)r == P.coarse && P.coarse < coarse_max && R.count <= kCoarseTargets * nbl &&
+            most < (1ull << 24) && slab < (1ull << 31)) {
             R.kind = kRoundBlock;
             R.nwg = nbl;
             P.coarse++;

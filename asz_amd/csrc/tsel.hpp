@@ -1,10 +1,11 @@
 // Device code of the per-block t selection (compress.tpp:90-144), shared by
-// the per-round selection kernels (tsel.hip) and the persistent coarse-round
-// kernel (coarse.hip).
+// the per-round selection kernels (tsel.hip) and the block-round kernel
+// (coarse.hip).
 //
-// One 256-thread workgroup scores one block.  The per-sample arithmetic is the
-// reference's, operation for operation (get_AB_map.tpp:18-21 and
-// get_h_psnr.tpp:12-20 as GCC lowers them): A = (a+b)-(c+e),
+// One workgroup (256 threads, or 512 in a block round) scores one block.
+// The per-sample arithmetic is the reference's, operation for operation
+// (get_AB_map.tpp:18-21 and get_h_psnr.tpp:12-20 as GCC lowers them):
+// A = (a+b)-(c+e),
 // B = (float)fma(S, 0.25, -x0) with S = (b+e)+(a+c), keys = lroundf(A or B / 2 sigma),
 // err = fmaf(Bk, 2, Ak*2t)*sigma, q = lroundf(err/2eb), de = err - q*2eb.
 // The reference then sums over distinct keys in unordered_map order in float;
@@ -27,7 +28,8 @@
 namespace asz {
 namespace dev {
 
-constexpr int kTselThreads = 256;
+constexpr int kTselThreads = 256;  // the selection kernels' workgroup
+constexpr int kTselMaxWaves = 8;   // the widest workgroup that runs a selection: 512 threads
 constexpr int kNCand = 5;          // block candidates {1/16, 1/4, 9/32, 5/16, 11/32} (asz_param.hpp:203)
 constexpr int kDirect = 1024;      // direct LDS bins per candidate: QB in [-512, 512)
 constexpr int kHashSlots = 512;    // per candidate, for QBs outside the direct bins
@@ -42,7 +44,7 @@ struct alignas(16) TselShared {
     long long hkey[NC][kHashSlots];
     uint32_t hcnt[NC][kHashSlots];
     uint32_t ovf[NC];              // QBs that found no hash slot (counted as distinct symbols)
-    double red[kTselThreads / kWave][2 * NC];
+    double red[kTselMaxWaves][2 * NC];
 };
 
 // The five values sample idx reads: a = x[-h], c = x[-3h], b = x[+h], e = x[+3h], x0.
@@ -140,12 +142,13 @@ struct TselLds {
     float best;
 };
 
-template <bool GRAD>
+template <bool GRAD, int NT = kTselThreads>
 __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __restrict__ work,
                                            const uint32_t* __restrict__ ss_tab, float* tblk, int32_t* t_int,
                                            DevAcc* acc, const FieldSet& fs, uint32_t bid, uint32_t f,
                                            TselLds<GRAD>& L) {
     constexpr int NC = TselLds<GRAD>::NC;
+    static_assert(NT % kWave == 0 && NT / kWave <= kTselMaxWaves, "selection workgroup shape");
     TselShared<NC>& sh = L.sh;
     float* s_cand = L.s_cand;
     int& s_nc = L.s_nc;
@@ -192,28 +195,27 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         // 16-byte LDS stores: the direct bins and hash counts to 0, the hash keys to LLONG_MIN
         {
             uint4* b4 = reinterpret_cast<uint4*>(&sh.bins[0][0]);
-            for (int i = threadIdx.x; i < NC * kDirect / 4; i += kTselThreads) b4[i] = make_uint4(0, 0, 0, 0);
+            for (int i = threadIdx.x; i < NC * kDirect / 4; i += NT) b4[i] = make_uint4(0, 0, 0, 0);
             uint4* k4 = reinterpret_cast<uint4*>(&sh.hkey[0][0]);
-            for (int i = threadIdx.x; i < NC * kHashSlots / 2; i += kTselThreads)
-                k4[i] = make_uint4(0, 0x80000000u, 0, 0x80000000u);
+            for (int i = threadIdx.x; i < NC * kHashSlots / 2; i += NT) k4[i] = make_uint4(0, 0x80000000u, 0, 0x80000000u);
             uint4* c4 = reinterpret_cast<uint4*>(&sh.hcnt[0][0]);
-            for (int i = threadIdx.x; i < NC * kHashSlots / 4; i += kTselThreads) c4[i] = make_uint4(0, 0, 0, 0);
+            for (int i = threadIdx.x; i < NC * kHashSlots / 4; i += NT) c4[i] = make_uint4(0, 0, 0, 0);
         }
         if (threadIdx.x < NC) sh.ovf[threadIdx.x] = 0;
         TSEL_MARK(1);
         double saa = 0, sab = 0;  // GRAD: sum A^2, sum A B over the samples
         // four samples per thread at a time: all twenty loads in flight before any use
         const uint32_t ns_pass = GRAD ? ns : min(ns, (uint32_t)kMaxKeep);
-        for (uint32_t idx0 = threadIdx.x; idx0 < ns_pass; idx0 += 4 * kTselThreads) {
+        for (uint32_t idx0 = threadIdx.x; idx0 < ns_pass; idx0 += 4 * NT) {
             float v[4][5];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint32_t idx = idx0 + u * kTselThreads;
+                const uint32_t idx = idx0 + u * NT;
                 if (idx < ns_pass) sample_load(T, work, lo, step, cnt, idx, v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint32_t idx = idx0 + u * kTselThreads;
+                const uint32_t idx = idx0 + u * NT;
                 if (idx >= ns_pass) break;
                 float af, bf, A, B;
                 sample_from(v[u], neg2, af, bf, A, B);
@@ -230,7 +232,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
             __syncthreads();
             if (threadIdx.x == 0) {
                 double a2 = 0, ab = 0;
-                for (int v = 0; v < kTselThreads / kWave; v++) { a2 += sh.red[v][0]; ab += sh.red[v][1]; }
+                for (int v = 0; v < NT / kWave; v++) { a2 += sh.red[v][0]; ab += sh.red[v][1]; }
                 // projected Newton on [t_min, t_max] from t = 1/4
                 double t = 0.25;
                 for (int it = 0; it < 3 && a2 > 0; it++) {
@@ -258,7 +260,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
 #pragma unroll
         for (int k = 0; k < NC; k++) cand[k] = k < nc ? s_cand[k] : 0.0f;
         double mse[NC] = {};
-        for (uint32_t idx = threadIdx.x; idx < ns; idx += kTselThreads) {
+        for (uint32_t idx = threadIdx.x; idx < ns; idx += NT) {
             float af, bf, A, B;
             if (idx < kMaxKeep) { af = sh.af[idx]; bf = sh.bf[idx]; }
             else sample_keys(T, work, lo, step, cnt, idx, neg2, af, bf, A, B);
@@ -276,18 +278,21 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         TSEL_MARK(3);
         const float dn = (float)ns;
         const float rec = 1.0f / dn;
-        // each thread: four direct bins and two hash slots per candidate (16- and 8-byte LDS reads)
-        static_assert(kDirect == 4 * kTselThreads && kHashSlots == 2 * kTselThreads, "entropy read shape");
+        // the first 256 threads: four direct bins and two hash slots per
+        // candidate each (16- and 8-byte LDS reads)
+        static_assert(kDirect == 4 * 256 && kHashSlots == 2 * 256 && NT >= 256, "entropy read shape");
         double h[NC] = {};
+        if (threadIdx.x < 256) {
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            const uint4 d4 = reinterpret_cast<const uint4*>(&sh.bins[k][0])[threadIdx.x];
-            const uint2 c2 = reinterpret_cast<const uint2*>(&sh.hcnt[k][0])[threadIdx.x];
-            const uint32_t cs[6] = {d4.x, d4.y, d4.z, d4.w, c2.x, c2.y};
+            for (int k = 0; k < NC; k++) {
+                const uint4 d4 = reinterpret_cast<const uint4*>(&sh.bins[k][0])[threadIdx.x];
+                const uint2 c2 = reinterpret_cast<const uint2*>(&sh.hcnt[k][0])[threadIdx.x];
+                const uint32_t cs[6] = {d4.x, d4.y, d4.z, d4.w, c2.x, c2.y};
 #pragma unroll
-            for (int e = 0; e < 6; e++)
-                if (cs[e]) h[k] -= entropy_term(cs[e], rec);
-            if (threadIdx.x == 0 && sh.ovf[k]) h[k] -= (double)sh.ovf[k] * entropy_term(1, rec);
+                for (int e = 0; e < 6; e++)
+                    if (cs[e]) h[k] -= entropy_term(cs[e], rec);
+                if (threadIdx.x == 0 && sh.ovf[k]) h[k] -= (double)sh.ovf[k] * entropy_term(1, rec);
+            }
         }
         const int w = threadIdx.x / kWave;
 #pragma unroll
@@ -301,7 +306,7 @@ __device__ __forceinline__ void tsel_block(const TselDesc& T, const float* __res
         if (threadIdx.x < (uint32_t)nc) {
             const int k = threadIdx.x;
             double hs = 0, ms = 0;
-            for (int v = 0; v < kTselThreads / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][NC + k]; }
+            for (int v = 0; v < NT / kWave; v++) { hs += sh.red[v][k]; ms += sh.red[v][NC + k]; }
             L.h1[k] = (float)hs;
             L.psnr[k] = log10f((range * range / (float)ms) * dn) * 10.0f;
         }
