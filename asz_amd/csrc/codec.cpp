@@ -307,22 +307,28 @@ PlanUpload plan_upload_layout(const Plan& P) {
 }  // namespace
 
 // Pinned host slots the GPU writes each array's histogram results into
-// (coherent, mapped: written by kernels while they run, read by the host).
+// (coherent, mapped: written by kernels while they run, read by the host),
+// followed by the device-range verdict words.
 static void ensure_host_slots(asz_ctx* c, uint64_t narr) {
     if (c->hres_arrays >= narr) return;
     if (c->hres_host) HIPCHECK(hipHostFree(c->hres_host));
     c->hres_host = nullptr;
     c->hres_dev = nullptr;
+    c->hrange_host = nullptr;
+    c->hrange_dev = nullptr;
     c->hres_arrays = 0;
     const uint64_t n = std::max<uint64_t>(narr, 16);
+    const uint64_t bytes = n * kHostSlotWords * 4 + 64;
     void* h = nullptr;
-    if (hipHostMalloc(&h, n * kHostSlotWords * 4, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    if (hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         fail(ASZ_ERR_NOMEM, "pinned host allocation of the histogram slots failed");
-    std::memset(h, 0, n * kHostSlotWords * 4);
+    std::memset(h, 0, bytes);
     void* d = nullptr;
     HIPCHECK(hipHostGetDevicePointer(&d, h, 0));
     c->hres_host = static_cast<uint32_t*>(h);
     c->hres_dev = static_cast<uint32_t*>(d);
+    c->hrange_host = c->hres_host + n * kHostSlotWords;
+    c->hrange_dev = c->hres_dev + n * kHostSlotWords;
     c->hres_arrays = n;
     c->epoch = 0;
 }
@@ -530,6 +536,8 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
         dfd_multi = c->fdesc.get<FieldDesc>(nf);
         fs.fd = dfd_multi;
     }
+    // before anything is launched: a reallocation moves the pinned slots
+    ensure_host_slots(c, narr * nf);
     uint32_t* range_res = nullptr;  // dev_range: min bits, max bits, status (pinned)
     if (dev_range) {
         uint32_t* rng = c->misc.get<uint32_t>(kRangeWords + 4);
@@ -550,17 +558,17 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
                 (float)((double)fin[0].default_slope * std::pow((double)p.slope_ratio, (double)(L - r - 1)));
         }
         FieldDesc* dfd = c->fdesc.get<FieldDesc>(1);
-        launch_field_desc(rng, d_work, seed, dfd, rng + kRangeWords, st);
-        range_res = c->pin_range.get<uint32_t>(4);
-        HIPCHECK(hipMemcpyAsync(range_res, rng + kRangeWords, 16, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipEventRecord(c->ev[2], st));
+        // the verdict goes straight into the pinned words (no copy, no event):
+        // the host reads it only after a later kernel's published histogram
+        // (or the call's final synchronisation), so it is there by then
+        range_res = c->hrange_host;
+        launch_field_desc(rng, d_work, seed, dfd, c->hrange_dev, st);
         fs.fd = dfd;
     }
-    // dev_range: the range's verdict (from any thread; the copy is long done
-    // by the time a codebook is built)
+    // dev_range: the range's verdict (from any thread: it is read after an
+    // array's histogram arrived, and k_field_desc ran before that array's round)
     auto check_range = [&] {
         if (!dev_range) return;
-        HIPCHECK(hipEventSynchronize(c->ev[2]));
         if (range_res[2] == 1) fail(ASZ_ERR_DATA, "input contains a nonfinite value");
         if (range_res[2] == 2) fail(ASZ_ERR_DATA, "max_val==min_val");
     };
@@ -574,7 +582,6 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     int32_t* stage = stage_max ? c->stage.get<int32_t>(stage_stride * nf) : nullptr;
     auto sym_of = [&](uint32_t f) { return sym0 + f * fs.sym; };
     auto gwin = [&](uint32_t f, uint64_t k) { return ghist + f * fs.ghist + k * kHistStride; };
-    ensure_host_slots(c, narr * nf);
     // Multi-field calls with the GPU t selection run their device sequence
     // (field descriptors, pivots, every round, folds) as a HIP graph, captured
     // once per (plan, field count, buffers) and replayed (ASZ_OPT_GRAPHS).

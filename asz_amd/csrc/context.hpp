@@ -158,6 +158,10 @@ struct asz_ctx {
     uint32_t* hres_host = nullptr;
     uint32_t* hres_dev = nullptr;
     uint64_t hres_arrays = 0;
+    // after the slots: the device-range verdict (min bits, max bits, status)
+    // that k_field_desc writes for asz_compress_field
+    uint32_t* hrange_host = nullptr;
+    uint32_t* hrange_dev = nullptr;
     uint32_t epoch = 0;
     std::vector<asz_ctx*> kids;  // batch workers: one context (and HIP stream) each
     // plans of the last kPlanSlots (dims, plan, pivot ratio, sampling count)

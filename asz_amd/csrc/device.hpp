@@ -144,6 +144,24 @@ __device__ __forceinline__ T wave_reduce(T v, Op op) {
     v = op(v, dpp_move<kDppRor8>(v));
     return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
 }
+// Inclusive prefix sum of 32-bit lanes over the wave on DPP moves: shifts
+// right by 1, 2, 4, 8 inside each row of 16 (lanes without a source add 0),
+// then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), and
+// the total of rows 0-1 into rows 2 and 3 (row_bcast:31).  The wave must be
+// fully active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_rows(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += dpp_rows<0x111, 0xF>(x);  // row_shr:1
+    x += dpp_rows<0x112, 0xF>(x);  // row_shr:2
+    x += dpp_rows<0x114, 0xF>(x);  // row_shr:4
+    x += dpp_rows<0x118, 0xF>(x);  // row_shr:8
+    x += dpp_rows<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_rows<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
     return wave_reduce(v, [](T a, T b) { return a + b; });

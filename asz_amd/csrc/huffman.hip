@@ -300,170 +300,201 @@ __device__ __forceinline__ void load_groups(const int32_t* __restrict__ sym, con
     }
 }
 
-__global__ __launch_bounds__(256) void k_encode_pack(const int32_t* __restrict__ sym,
-                                                     const HuffArrayDev* __restrict__ arrays, uint32_t narr,
-                                                     uint64_t c_begin, uint64_t c_end,
-                                                     const uint32_t* __restrict__ code32,
-                                                     const uint8_t* __restrict__ len8,
-                                                     uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage) {
-    __shared__ __align__(16) uint32_t lbuf[kEncChunk + 4];  // bit buffer: 32 bits per symbol at most
-    __shared__ uint2 lbook[kLdsBook];                        // (code, length)
-    __shared__ uint32_t s_tot[4][2];
+// LDS of an encoder workgroup: the chunk's bit buffer (32 bits per symbol at
+// most, and the words the last group may spill into), the array's codebook as
+// (code, length) pairs, and the four waves' group totals.
+struct alignas(16) EncLds {
+    uint32_t lbuf[kEncChunk + 4];
+    uint2 lbook[kLdsBook];
+    uint32_t s_tot[4][2];
+};
+
+// Array k's codebook into LDS unless it is there already (book_k: the array
+// whose codebook LDS holds).  A fixed number of loads (no loop of unknown trip
+// count), so that the waits for symbols loaded before it stay exact.  The
+// caller's barrier completes it.
+__device__ __forceinline__ void enc_book(EncLds& E, const HuffArrayDev& A, const uint32_t* C, const uint8_t* L,
+                                         uint32_t k, int& book_k) {
+    if (!A.lds_book || book_k == (int)k) return;
+    static_assert(kLdsBook == 4 * 256, "four codebook entries per thread");
+    const uint32_t t = threadIdx.x;
+    uint32_t cb[4], lb[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t e = min(t + 256 * i, (uint32_t)A.range - 1);
+        cb[i] = C[e];
+        lb[i] = L[e];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        if (t + 256 * i < A.range) E.lbook[t + 256 * i] = make_uint2(cb[i], lb[i]);
+    book_k = (int)k;
+}
+
+// The thread's four groups (group j: symbols 1024 j + 4 t .. + 3 of the chunk,
+// the first n live) joined: acc[j] their codes from bit 0, gb[j] their bit
+// count.  Symbol - min in 32 bits (the index is below the alphabet size); the
+// padding past n (read as 0) is clamped into the table and given no bits.
+// LDS or global tables chosen at compile time (a run-time choice would merge
+// the two into flat loads).  No branch per symbol: the codes are joined with
+// shifts below 64 (a group of more than 64 bits is redone code by code).
+template <bool LDS>
+__device__ __forceinline__ void enc_join(const EncLds& E, const HuffArrayDev& A, const uint32_t* C, const uint8_t* L,
+                                         const int4 v[4], uint32_t n, uint64_t acc[4], uint32_t gb[4]) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t off32 = (uint32_t)A.offset, top = (uint32_t)A.range - 1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t i0 = 1024 * j + 4 * t;
+        const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+        uint32_t cd[4], ln[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const uint32_t idx = min((uint32_t)s[e] - off32, top);
+            if constexpr (LDS) {
+                const uint2 b = E.lbook[idx];
+                cd[e] = b.x;
+                ln[e] = b.y;
+            } else {
+                cd[e] = C[idx];
+                ln[e] = L[idx];
+            }
+            const bool live = i0 + e < n;
+            cd[e] = live ? cd[e] : 0u;
+            ln[e] = live ? ln[e] : 0u;
+        }
+        const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
+        acc[j] = (uint64_t)cd[0] | ((uint64_t)cd[1] << (p1 & 63)) | ((uint64_t)cd[2] << (p2 & 63)) |
+                 ((uint64_t)cd[3] << (p3 & 63));
+        gb[j] = p3 + ln[3];
+    }
+}
+
+// The joined groups into the (zero) bit buffer from bit 0: positions from
+// inclusive wave scans of the group counts (DPP; two counts per 32-bit word,
+// a quarter holds at most 32768 bits) and the four wave totals; groups of at
+// most 64 bits ORed in with one to three LDS atomics and no wait between,
+// longer ones code by code from their symbols (read again).  Returns the
+// chunk's bit count; the buffer is complete on return.
+__device__ __forceinline__ uint32_t enc_place_lds(EncLds& E, const HuffArrayDev& A, const uint32_t* C,
+                                                  const uint8_t* L, const int32_t* sym, uint64_t first, uint32_t n,
+                                                  const uint64_t acc[4], const uint32_t gb[4]) {
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint32_t p01 = gb[0] | (gb[1] << 16), p23 = gb[2] | (gb[3] << 16);
+    const uint32_t i01 = wave_incl_scan(p01), i23 = wave_incl_scan(p23);
+    if (lane == 63) {
+        E.s_tot[wave][0] = i01;
+        E.s_tot[wave][1] = i23;
+    }
+    lds_barrier();
+    uint32_t a01 = 0, a23 = 0, w01 = 0, w23 = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) {
+        const uint32_t x01 = E.s_tot[w][0], x23 = E.s_tot[w][1];
+        a01 += x01;
+        a23 += x23;
+        if (w < wave) {
+            w01 += x01;
+            w23 += x23;
+        }
+    }
+    const uint32_t T0 = a01 & 0xFFFFu, T1 = a01 >> 16, T2 = a23 & 0xFFFFu, T3 = a23 >> 16;
+    const uint32_t e01 = w01 + i01 - p01, e23 = w23 + i23 - p23;
+    const uint32_t pos[4] = {e01 & 0xFFFFu, T0 + (e01 >> 16), T0 + T1 + (e23 & 0xFFFFu), T0 + T1 + T2 + (e23 >> 16)};
+    uint32_t* lbuf = E.lbuf;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t g = gb[j], p = pos[j];
+        if (g && g <= 64) {
+            const uint32_t w = p >> 5, s2 = p & 31;
+            const uint64_t lo = acc[j] << s2;
+            atomicOr(&lbuf[w], (uint32_t)lo);
+            if (s2 + g > 32) atomicOr(&lbuf[w + 1], (uint32_t)(lo >> 32));
+            if (s2 + g > 64) atomicOr(&lbuf[w + 2], (uint32_t)(acc[j] >> (64 - s2)));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint32_t p = pos[j];
+        if (gb[j] > 64) {
+            const int32_t* s = sym + A.src_off + first;
+            const uint32_t i0 = 1024 * j + 4 * t;
+            for (int e = 0; e < 4; e++) {
+                if (i0 + e >= n) break;
+                const uint32_t idx = (uint32_t)((int64_t)s[i0 + e] - A.offset);
+                uint32_t cd, ln;
+                if (A.lds_book) {
+                    cd = E.lbook[idx].x;
+                    ln = E.lbook[idx].y;
+                } else {
+                    cd = C[idx];
+                    ln = L[idx];
+                }
+                const uint32_t w = p >> 5, s2 = p & 31;
+                if (ln) atomicOr(&lbuf[w], cd << s2);
+                if (s2 + ln > 32) atomicOr(&lbuf[w + 1], cd >> (32 - s2));
+                p += ln;
+            }
+        }
+    }
+    lds_barrier();
+    return T0 + T1 + T2 + T3;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_encode_pack(
+    const int32_t* __restrict__ sym, const HuffArrayDev* __restrict__ arrays, uint32_t narr, uint64_t c_begin,
+    uint64_t c_end, const uint32_t* __restrict__ code32, const uint8_t* __restrict__ len8,
+    uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ stage) {
+    __shared__ EncLds E;
+    const uint32_t t = threadIdx.x;
     const uint64_t G = gridDim.x;
     int book_k = -1;
 
     // Packs chunk c of array k from the symbols in v, then starts the load of
     // chunk c + 2G into v (two chunks in flight while this one is packed).
+    // Workgroup g takes chunks g, g + G, ...: the few chunks of the arrays
+    // whose codebook is too large for LDS (the slow ones) spread over many
+    // workgroups.
     auto pack = [&](int4 (&v)[4], uint64_t c, uint32_t k) {
         const HuffArrayDev A = arrays[k];
         const uint64_t first = (c - A.chunk0) * kEncChunk;
         const uint32_t n = (uint32_t)min<uint64_t>(kEncChunk, A.count - first);
         const uint32_t* C = code32 + A.book_off;
         const uint8_t* L = len8 + A.book_off;
-        const bool lds = A.lds_book != 0;
-        if (lds && book_k != (int)k) {
-            // a fixed number of loads (no loop of unknown trip count), so the
-            // waits for the next chunk's symbols stay exact on every path
-            static_assert(kLdsBook == 4 * 256, "four codebook entries per thread");
-            uint32_t cb[4], lb[4];
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++) {
-                const uint32_t e = min(t + 256 * i, (uint32_t)A.range - 1);
-                cb[i] = C[e];
-                lb[i] = L[e];
-            }
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++)
-                if (t + 256 * i < A.range) lbook[t + 256 * i] = make_uint2(cb[i], lb[i]);
-            book_k = (int)k;
-        }
+        enc_book(E, A, C, L, k, book_k);
         lds_barrier();  // LDS codebook complete; the previous chunk's buffer written out
         uint64_t acc[4];
         uint32_t gb[4];
-        // symbol - min in 32 bits (the index is below the alphabet size); the
-        // padding past n (read as 0) is clamped into the table and given no bits
-        const uint32_t off32 = (uint32_t)A.offset, top = (uint32_t)A.range - 1;
-        // the table lookups, LDS or global, chosen at compile time: a run-time
-        // choice would merge the two into flat (generic) loads.  No branch per
-        // symbol: the four codes of a group are joined with shifts below 64
-        // (a group of more than 64 bits is redone code by code below)
-        auto join = [&](auto from_lds) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t i0 = 1024 * j + 4 * t;
-                const int32_t s[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-                uint32_t cd[4], ln[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const uint32_t idx = min((uint32_t)s[e] - off32, top);
-                    if constexpr (decltype(from_lds)::value) {
-                        const uint2 b = lbook[idx];
-                        cd[e] = b.x;
-                        ln[e] = b.y;
-                    } else {
-                        cd[e] = C[idx];
-                        ln[e] = L[idx];
-                    }
-                    const bool live = i0 + e < n;
-                    cd[e] = live ? cd[e] : 0u;
-                    ln[e] = live ? ln[e] : 0u;
-                }
-                const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
-                acc[j] = (uint64_t)cd[0] | ((uint64_t)cd[1] << (p1 & 63)) | ((uint64_t)cd[2] << (p2 & 63)) |
-                         ((uint64_t)cd[3] << (p3 & 63));
-                gb[j] = p3 + ln[3];
-            }
-        };
-        if (lds) join(std::true_type{});
-        else join(std::false_type{});
+        if (A.lds_book) enc_join<true>(E, A, C, L, v, n, acc, gb);
+        else enc_join<false>(E, A, C, L, v, n, acc, gb);
         // the load of chunk c + 2G stays in flight while this one is packed
         // (the LDS-only barriers below do not wait for it)
         const uint64_t c2 = c + 2 * G;
         if (c2 < c_end) load_groups(sym, arrays[chunk_array(arrays, narr, c2, k)], c2, v);
-        // group positions: inclusive wave scans of the packed counts, then the wave totals
-        const uint32_t p01 = gb[0] | (gb[1] << 16), p23 = gb[2] | (gb[3] << 16);
-        uint32_t i01 = p01, i23 = p23;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u01 = __shfl_up(i01, o, 64), u23 = __shfl_up(i23, o, 64);
-            if (lane >= (uint32_t)o) {
-                i01 += u01;
-                i23 += u23;
-            }
-        }
-        if (lane == 63) {
-            s_tot[wave][0] = i01;
-            s_tot[wave][1] = i23;
-        }
-        lds_barrier();
-        uint32_t a01 = 0, a23 = 0, w01 = 0, w23 = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < 4; w++) {
-            const uint32_t x01 = s_tot[w][0], x23 = s_tot[w][1];
-            a01 += x01;
-            a23 += x23;
-            if (w < wave) {
-                w01 += x01;
-                w23 += x23;
-            }
-        }
-        const uint32_t T0 = a01 & 0xFFFFu, T1 = a01 >> 16, T2 = a23 & 0xFFFFu, T3 = a23 >> 16;
-        const uint32_t e01 = w01 + i01 - p01, e23 = w23 + i23 - p23;
-        const uint32_t pos[4] = {e01 & 0xFFFFu, T0 + (e01 >> 16), T0 + T1 + (e23 & 0xFFFFu),
-                                 T0 + T1 + T2 + (e23 >> 16)};
-        const uint32_t total = T0 + T1 + T2 + T3;
+        const uint32_t total = enc_place_lds(E, A, C, L, sym, first, n, acc, gb);
         const uint32_t nw4 = (total + 127) / 128;  // 16-byte units of the buffer (at most kEncChunk / 4)
         static_assert(kEncChunk / 4 == 4 * 256, "four buffer quads per thread at most");
+        // a fixed number of (predicated) stores: the wait for the next
+        // chunk's symbols then still knows how many memory operations follow
+        // them; each quad is cleared by the thread that copied it out
+        uint4* dst = reinterpret_cast<uint4*>(stage + c * kEncChunk);
+        uint4* src = reinterpret_cast<uint4*>(E.lbuf);
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++)
-            if (t + 256 * i < nw4) reinterpret_cast<uint4*>(lbuf)[t + 256 * i] = make_uint4(0, 0, 0, 0);
-        lds_barrier();
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t g = gb[j];
-            if (!g) continue;
-            uint32_t p = pos[j];
-            if (g <= 64) {
-                const uint32_t w = p >> 5, s2 = p & 31;
-                const uint64_t lo = acc[j] << s2;
-                atomicOr(&lbuf[w], (uint32_t)lo);
-                if (s2 + g > 32) atomicOr(&lbuf[w + 1], (uint32_t)(lo >> 32));
-                if (s2 + g > 64) atomicOr(&lbuf[w + 2], (uint32_t)(acc[j] >> (64 - s2)));
-            } else {
-                const int32_t* s = sym + A.src_off + first;
-                const uint32_t i0 = 1024 * j + 4 * t;
-                for (int e = 0; e < 4; e++) {
-                    if (i0 + e >= n) break;
-                    const uint32_t idx = (uint32_t)((int64_t)s[i0 + e] - A.offset);
-                    uint32_t cd, ln;
-                    if (lds) {
-                        cd = lbook[idx].x;
-                        ln = lbook[idx].y;
-                    } else {
-                        cd = C[idx];
-                        ln = L[idx];
-                    }
-                    const uint32_t w = p >> 5, s2 = p & 31;
-                    if (ln) atomicOr(&lbuf[w], cd << s2);
-                    if (s2 + ln > 32) atomicOr(&lbuf[w + 1], cd >> (32 - s2));
-                    p += ln;
-                }
+        for (uint32_t i = 0; i < 4; i++) {
+            if (t + 256 * i < nw4) {
+                dst[t + 256 * i] = src[t + 256 * i];
+                src[t + 256 * i] = make_uint4(0, 0, 0, 0);
             }
         }
-        lds_barrier();
-        // a fixed number of (predicated) stores: the wait for the next
-        // chunk's symbols then still knows how many memory operations follow them
-        uint4* dst = reinterpret_cast<uint4*>(stage + c * kEncChunk);
-        const uint4* src = reinterpret_cast<const uint4*>(lbuf);
-#pragma unroll
-        for (uint32_t i = 0; i < 4; i++)
-            if (t + 256 * i < nw4) dst[t + 256 * i] = src[t + 256 * i];
         if (t == 0) chunk_bits[c] = total;
     };
 
     uint64_t c = c_begin + blockIdx.x;
     if (c >= c_end) return;
+    // the bit buffer starts zero (the first chunk's barrier orders this)
+    static_assert((kEncChunk + 4) % 4 == 0, "buffer of whole quads");
+    for (uint32_t i = t; i < (kEncChunk + 4) / 4; i += blockDim.x)
+        reinterpret_cast<uint4*>(E.lbuf)[i] = make_uint4(0, 0, 0, 0);
     uint32_t k = chunk_array(arrays, narr, c, 0);
     int4 va[4], vb[4];
     load_groups(sym, arrays[k], c, va);
@@ -730,3 +761,4 @@ void launch_encode(const int32_t* sym, const HuffArrayDev* arrays, uint32_t narr
 }
 
 }  // namespace asz
+

@@ -180,6 +180,7 @@ __global__ __launch_bounds__(64) void k_field_desc(const uint32_t* __restrict__ 
         res[1] = __float_as_uint(vmax);
         res[2] = rng[2] ? 1u : (daz(vmin) == daz(vmax) ? 2u : 0u);
         res[3] = 0;
+        __threadfence_system();  // res is pinned host memory, read after later kernels' results
     }
 }
 
