@@ -801,8 +801,7 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
     // with its t selection and folds, and the join of the side stream.
     auto device_sequence = [&]() {
         if (multi) HIPCHECK(hipMemcpyAsync(dfd_multi, hfd, nf * sizeof(FieldDesc), hipMemcpyHostToDevice, st));
-        launch_init_acc(acc, st, nf);
-        launch_fill_i32(reinterpret_cast<int32_t*>(ghist), nf * (fs.ghist + fs.phist), 0, st);
+        launch_init_call(acc, nf, ghist, nf * (fs.ghist + fs.phist), st);
         if (tsel_mode == ASZ_TSEL_INJECT) {
             HIPCHECK(hipMemcpyAsync(tblk, tf_inj.data(), nt * 4, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(sym, ti_inj.data(), nt * 4, hipMemcpyHostToDevice, st));

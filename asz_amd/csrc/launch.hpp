@@ -95,6 +95,9 @@ void launch_qb_order(const RoundDesc& R, const uint32_t* tabs, const uint32_t cm
                      bool to_order, hipStream_t st);
 // Resets the accumulators: errors and fallbacks to 0, every QB range to empty.
 void launch_init_acc(DevAcc* acc, hipStream_t st, uint32_t nfields = 1);
+// launch_init_acc and the clearing of n words (a multiple of 4, 16-byte
+// aligned) in one launch.
+void launch_init_call(DevAcc* acc, uint32_t nfields, uint32_t* words, uint64_t n, hipStream_t st);
 
 // grad: gradient-tuned candidates (projected Newton on the sampled residual
 // energy, snapped to the t grid) instead of the five block candidates.

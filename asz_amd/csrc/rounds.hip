@@ -187,8 +187,7 @@ __global__ __launch_bounds__(64) void k_field_desc(const uint32_t* __restrict__ 
 // ------------------------------------------------------------------ partial reductions
 
 
-__global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
-    acc += blockIdx.x;  // one workgroup per field
+__device__ __forceinline__ void init_acc(DevAcc* acc) {
     const uint32_t i = threadIdx.x;
     if (i == 0) {
         acc->err = ErrAcc{0.0, 0.0};
@@ -199,6 +198,22 @@ __global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
         acc->rerr[i] = ErrAcc{0.0, 0.0};
         acc->fold_count[i] = 0;
     }
+}
+
+__global__ __launch_bounds__(128) void k_init_acc(DevAcc* acc) {
+    init_acc(acc + blockIdx.x);  // one workgroup per field
+}
+
+// A compression call's start in one launch: workgroups [0, nfields) set the
+// fields' accumulators (as k_init_acc), the others clear n4 16-byte words.
+__global__ __launch_bounds__(256) void k_init_call(DevAcc* acc, uint32_t nfields, uint4* v, uint64_t n4) {
+    if (blockIdx.x < nfields) {
+        init_acc(acc + blockIdx.x);
+        return;
+    }
+    const uint64_t stride = (uint64_t)(gridDim.x - nfields) * blockDim.x;
+    for (uint64_t i = (uint64_t)(blockIdx.x - nfields) * blockDim.x + threadIdx.x; i < n4; i += stride)
+        v[i] = make_uint4(0, 0, 0, 0);
 }
 
 // ------------------------------------------------------------------ pivots (compress.tpp:161-175)
@@ -826,6 +841,13 @@ void launch_pivots(bool compress, float* work, int32_t* qb, const PivotDesc& P, 
 
 void launch_init_acc(DevAcc* acc, hipStream_t st, uint32_t nfields) {
     hipLaunchKernelGGL(dev::k_init_acc, dim3(nfields), dim3(128), 0, st, acc);
+}
+
+void launch_init_call(DevAcc* acc, uint32_t nfields, uint32_t* words, uint64_t n, hipStream_t st) {
+    if (n % 4 || reinterpret_cast<uintptr_t>(words) % 16) throw Error(ASZ_ERR_ARG, "unaligned call-start words");
+    const uint64_t n4 = n / 4, blocks = std::min<uint64_t>((n4 + 255) / 256, 4096);
+    hipLaunchKernelGGL(dev::k_init_call, dim3((uint32_t)(nfields + blocks)), dim3(256), 0, st, acc, nfields,
+                       reinterpret_cast<uint4*>(words), n4);
 }
 
 namespace {
