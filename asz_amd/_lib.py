@@ -104,7 +104,7 @@ class TuneReport(ctypes.Structure):
 EXPORTS = ("asz_ctx_create", "asz_ctx_destroy", "asz_last_error", "asz_params_default", "asz_get_stats",
            "asz_device_alloc", "asz_device_free", "asz_copy_to_device", "asz_copy_to_host", "asz_geometry",
            "asz_range_f32", "asz_default_slope", "asz_compress", "asz_compress_host", "asz_decompress",
-           "asz_decompress_host", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_compress_mt", "asz_zstd_decompress",
+           "asz_decompress_host", "asz_decompress_device", "asz_huffman_codebook", "asz_zstd_compress", "asz_zstd_compress_mt", "asz_zstd_decompress",
            "asz_free",
            "asz_round_layout", "asz_set_option", "asz_stream_wait", "asz_compress_field", "asz_prepare_round", "asz_round_energy",
            "asz_round_energy_adjoint", "asz_compress_batch", "asz_generate_plan",
@@ -151,6 +151,7 @@ def lib():
                                     ctypes.POINTER(_P), _U64P, _P, _P, _P]
     L.asz_decompress.argtypes = [_P, _P, _U64, _P, ctypes.POINTER(Params), _P, _P, _P]
     L.asz_decompress_host.argtypes = [_P, _P, _U64, _P, ctypes.POINTER(Params), _P]
+    L.asz_decompress_device.argtypes = [_P, _P, _U64, _P, ctypes.POINTER(Params), _P]
     L.asz_huffman_codebook.argtypes = [_P, _U64, ctypes.c_int64, _U64, ctypes.c_int, _P, _U64, _U64P, _P, _P,
                                        _U64P]
     L.asz_zstd_compress.argtypes = [_P, _U64, ctypes.POINTER(_P), _U64P]

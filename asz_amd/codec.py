@@ -277,6 +277,13 @@ class Context:
         return CompressResult(stream, mn.value, mx.value, st, None, None, None,
                               [int(st.qb_counts[k]) for k in range(plan.layers + 1)])
 
+    def stream_device(self):
+        """(device pointer, bytes) of the last compressed stream, left in this
+        context's stream buffer by set_stream_on_device(True)."""
+        if not getattr(self, "_stream_dev", False):
+            raise AszError("the last stream is not in device memory (set_stream_on_device(True))")
+        return self._last_stream
+
     def stream_view(self) -> np.ndarray:
         """The last compressed stream: the context's pinned buffer (no copy), or
         a host copy when the stream was left in device memory."""
@@ -371,6 +378,16 @@ class Context:
                                            ctypes.c_void_p(dev.data_ptr()), _ptr(qb), _ptr(t)))
         out[...] = dev.cpu().numpy().reshape(out.shape)
         return out, qb, t
+
+    def decompress_device(self, d_stream: int, nbytes: int, dims, params: Params, d_out: int):
+        """Decompresses a stream in device memory (d_stream, nbytes) into the
+        device float32 buffer d_out.  A stream in this context's stream buffer
+        (stream_view() after a compression with set_stream_on_device(True))
+        is decoded where it lies."""
+        d = np.asarray(dims, np.uint64)
+        self.after_torch()
+        self._check(self._L.asz_decompress_device(self._c, ctypes.c_void_p(d_stream), nbytes, _ptr(d),
+                                                  ctypes.byref(params), ctypes.c_void_p(d_out)))
 
 
 def family_line(rep) -> str:

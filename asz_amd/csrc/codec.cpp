@@ -1133,7 +1133,9 @@ static void compress_fields(asz_ctx* c, uint32_t nf, const FieldIn* fin_in, cons
             darrs.push_back(D);
         }
         lay.total_bytes = spos;
-        words_total += ((lay.total_bytes + 3) / 4 + 4 + 63) / 64 * 64;
+        // 128 spare words: the decoder's padding, so that a stream left here
+        // is decompressed in place
+        words_total += ((lay.total_bytes + 3) / 4 + 128 + 63) / 64 * 64;
         max_len = std::max(max_len, lay.max_len);
         c->fstats[f].stream_bytes = lay.total_bytes;
         for (uint64_t k = 0; k <= L; k++) c->fstats[f].qb_counts[k] = lay.arrays[k + 1].count;
@@ -1341,8 +1343,49 @@ static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p) {
     return P;
 }
 
-static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint64_t dims[3], const asz_params& p,
-                            float* d_out, int32_t* qb_out, int64_t* t_out) {
+// A stream in device memory: its header bytes (the ranges k_stream_regions
+// lists) copied to the host, at their stream offsets in image, the only bytes
+// the host parser reads.  A stream the walk finds malformed comes back whole,
+// for the parser to report.
+static const uint8_t* stream_headers(asz_ctx* c, const uint8_t* s, uint64_t n, hipStream_t st,
+                                     std::unique_ptr<uint8_t[]>& image) {
+    const uint64_t mwords = 3 + 2 * kStreamRegions;
+    uint64_t cap = 1ull << 20;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        uint8_t* dout = c->hdr_bytes.get<uint8_t>(cap);
+        uint64_t* dmeta = c->hdr_meta.get<uint64_t>(mwords);
+        uint64_t* hmeta = c->pin_hdr.get<uint64_t>(mwords);
+        launch_stream_regions(s, n, dout, cap, dmeta, st);
+        HIPCHECK(hipMemcpyAsync(hmeta, dmeta, mwords * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        const uint64_t status = hmeta[0], need = hmeta[1], nreg = hmeta[2];
+        if (status == 2) {
+            cap = need;
+            continue;
+        }
+        image.reset(new uint8_t[n]);
+        if (status != 0 || nreg > kStreamRegions) {
+            HIPCHECK(hipMemcpy(image.get(), s, n, hipMemcpyDeviceToHost));
+            return image.get();
+        }
+        uint8_t* hb = c->pin_hdr_bytes.get<uint8_t>(need + 1);
+        HIPCHECK(hipMemcpyAsync(hb, dout, need, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        uint64_t o = 0;
+        for (uint64_t r = 0; r < nreg; r++) {
+            std::memcpy(image.get() + hmeta[3 + 2 * r], hb + o, hmeta[4 + 2 * r]);
+            o += hmeta[4 + 2 * r];
+        }
+        return image.get();
+    }
+    fail(ASZ_ERR_DATA, "compressed stream headers could not be gathered");
+}
+
+// dev_stream: s is in device memory (decompressed where it lies when it is
+// the context's own stream buffer, copied into it otherwise).
+static void decompress_impl(asz_ctx* c, const uint8_t* s_in, uint64_t n, const uint64_t dims[3], const asz_params& p,
+                            float* d_out, int32_t* qb_out, int64_t* t_out, bool dev_stream = false) {
+    const uint8_t* s = s_in;
     auto wall0 = Clock::now();
     validate_params(p);
     hipStream_t st = c->stream;
@@ -1353,10 +1396,40 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     // zero words after the stream: a write-pass thread parses up to 31
     // codewords of at most 64 bits past its own symbols
     const uint64_t nwords = (n + 3) / 4 + 128;
+    // a device stream in the context's own stream buffer is decoded in place
+    // (the buffer grown first, keeping the stream, when it lacks the padding)
+    bool in_place = false;
+    if (dev_stream) {
+        const uint8_t* b = static_cast<const uint8_t*>(c->words.p);
+        if (b && s < b + c->words.cap && b < s + n) {
+            if (s != b) fail(ASZ_ERR_ARG, "a stream in the context's stream buffer must start it");
+            if (c->words.cap < nwords * 4) {
+                asz::DevBuf grown;
+                uint32_t* nw = grown.get<uint32_t>(nwords);
+                HIPCHECK(hipMemcpyAsync(nw, s, n, hipMemcpyDeviceToDevice, st));
+                HIPCHECK(hipStreamSynchronize(st));
+                std::swap(c->words.p, grown.p);
+                std::swap(c->words.cap, grown.cap);
+                s = static_cast<const uint8_t*>(c->words.p);
+            }
+            in_place = true;
+        }
+    }
     uint32_t* dwords = c->words.get<uint32_t>(nwords);
-    launch_fill_i32(reinterpret_cast<int32_t*>(dwords + n / 4), nwords - n / 4, 0, st);
-    HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
-    Parsed Q = parse_stream(s, n, p);
+    // zero words past the stream (in place: past its last word, whose bytes
+    // after the stream the encoder left zero; otherwise from its last partial
+    // word on, which the copy below then fills)
+    const uint64_t z0 = in_place ? (n + 3) / 4 : n / 4;
+    launch_fill_i32(reinterpret_cast<int32_t*>(dwords + z0), nwords - z0, 0, st);
+    std::unique_ptr<uint8_t[]> image;
+    const uint8_t* hs = s;
+    if (dev_stream) {
+        if (!in_place) HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyDeviceToDevice, st));
+        hs = stream_headers(c, s, n, st, image);
+    } else {
+        HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
+    }
+    Parsed Q = parse_stream(hs, n, p);
     Geometry g = make_geometry(dims, Q.plan, p.block_pivot_ratio);
     const float aeb = abs_eb_of(p.rel_eb, Q.mn, Q.mx);
     Plan P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, 1.0f, p.h1_block_sampling_num);
@@ -1410,12 +1483,26 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
         da.push_back(D);
     }
     const uint64_t nwg = wg_arr.size(), nsub = nwg * 256;
-    uint32_t* dtab = c->dec_tab.get<uint32_t>(tables.size() + 1);
-    HIPCHECK(hipMemcpyAsync(dtab, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, st));
-    DecArrayDev* dda = c->dec_arrays.get<DecArrayDev>(da.size() + 1);
-    HIPCHECK(hipMemcpyAsync(dda, da.data(), da.size() * sizeof(DecArrayDev), hipMemcpyHostToDevice, st));
-    uint32_t* dwg = c->sub_arr.get<uint32_t>(nwg + 1);
-    HIPCHECK(hipMemcpyAsync(dwg, wg_arr.data(), nwg * 4, hipMemcpyHostToDevice, st));
+    // decode tables, array descriptors, workgroup map, the plan's tables and
+    // sweep segments: one upload from pinned memory (a pageable copy waits
+    // on the host for its staging)
+    auto al = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+    const uint64_t o_da = al(tables.size() * 4), o_wg = al(o_da + da.size() * sizeof(DecArrayDev)),
+                   o_tabs = al(o_wg + nwg * 4), o_segs = al(o_tabs + P.tables.size() * 4),
+                   up_bytes = al(o_segs + P.segs.size() * sizeof(SweepSeg));
+    uint8_t* hup = c->pin_dec.get<uint8_t>(up_bytes);
+    std::memcpy(hup, tables.data(), tables.size() * 4);
+    std::memcpy(hup + o_da, da.data(), da.size() * sizeof(DecArrayDev));
+    std::memcpy(hup + o_wg, wg_arr.data(), nwg * 4);
+    std::memcpy(hup + o_tabs, P.tables.data(), P.tables.size() * 4);
+    if (!P.segs.empty()) std::memcpy(hup + o_segs, P.segs.data(), P.segs.size() * sizeof(SweepSeg));
+    uint8_t* dup = c->dec_up.get<uint8_t>(up_bytes);
+    HIPCHECK(hipMemcpyAsync(dup, hup, up_bytes, hipMemcpyHostToDevice, st));
+    const uint32_t* dtab = reinterpret_cast<const uint32_t*>(dup);
+    const DecArrayDev* dda = reinterpret_cast<const DecArrayDev*>(dup + o_da);
+    const uint32_t* dwg = reinterpret_cast<const uint32_t*>(dup + o_wg);
+    const uint32_t* tabs = reinterpret_cast<const uint32_t*>(dup + o_tabs);
+    const SweepSeg* dsegs = reinterpret_cast<const SweepSeg*>(dup + o_segs);
     uint64_t* dstart = c->start.get<uint64_t>(nsub + 1);
     uint64_t* dex0 = c->exit0.get<uint64_t>(nsub + 1);
     uint64_t* dex1 = c->exit1.get<uint64_t>(nsub + 1);
@@ -1424,13 +1511,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s, uint64_t n, const uint
     // flags[0]: error bits of the write pass; flags[8 + i]: "changed" of fix pass i of a group
     constexpr uint32_t kFixGroup = 6;
     unsigned int* flags = c->misc.get<unsigned int>(8 + kFixGroup);
-    // reconstruction inputs (independent of the decoded symbols)
-    uint32_t* tabs = c->tabs.get<uint32_t>(P.tables.size() + 1);
-    HIPCHECK(hipMemcpyAsync(tabs, P.tables.data(), P.tables.size() * 4, hipMemcpyHostToDevice, st));
     float* tblk = c->tblk.get<float>(nt + 1);
-    SweepSeg* dsegs = c->segs.get<SweepSeg>(P.segs.size() + 1);
-    if (!P.segs.empty())
-        HIPCHECK(hipMemcpyAsync(dsegs, P.segs.data(), P.segs.size() * sizeof(SweepSeg), hipMemcpyHostToDevice, st));
     uint64_t stage_max = 0;
     for (uint64_t r = 0; r < L; r++)
         if (round_needs_staging(P.rounds[r])) stage_max = std::max(stage_max, P.rounds[r].count);
@@ -2001,6 +2082,16 @@ int asz_decompress(asz_ctx* c, const uint8_t* stream, uint64_t n, const uint64_t
         asz::decompress_impl(c, stream, n, dims, *p, d_out, qb_out, t_out);
     });
     // an error can leave the stream's upload running: it reads the caller's buffer
+    if (rc != ASZ_OK) (void)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+int asz_decompress_device(asz_ctx* c, const uint8_t* d_stream, uint64_t n, const uint64_t dims[3],
+                          const asz_params* p, float* d_out) {
+    const int rc = guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        asz::decompress_impl(c, d_stream, n, dims, *p, d_out, nullptr, nullptr, true);
+    });
     if (rc != ASZ_OK) (void)hipStreamSynchronize(c->stream);
     return rc;
 }

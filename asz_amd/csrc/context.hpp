@@ -143,10 +143,10 @@ struct asz_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     asz_stats stats{};
-    asz::DevBuf work, sym, tabs, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_tab, dec_arrays,
-        sub_arr, start, exit0, exit1, count, out_pos, segs, partials, fparts, ghist, xfer, hmeta, scratch,
-        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, tune_work;
-    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early;
+    asz::DevBuf work, sym, tblk, hist, arrays, chunk_arr, words, keys, key_off, misc, dec_up, start, exit0, exit1,
+        count, out_pos, partials, fparts, ghist, xfer, hmeta, scratch,
+        chunk_bits, chunk_off, pg_meta, pg_qb, pg_err, fdesc, stage, enc_stage, hmeta_early, tune_work, hdr_bytes, hdr_meta;
+    asz::PinnedBuf pin_meta, pin_xfer, pin_huff, pin_stream, pin_fdesc, pin_range, pin_early, pin_hdr, pin_hdr_bytes, pin_dec;
     // exact histograms of arrays wider than the global window, and the t and
     // pivot arrays' histograms, which no round waits for
     hipStream_t side = nullptr;

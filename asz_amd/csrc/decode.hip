@@ -17,6 +17,8 @@
 //   3. k_dec_wgsum / k_dec_wgscan: output offsets (two-level scan).
 //   4. k_dec_write: decode again from the exact starts, each thread storing
 //      whole 128-byte lines of the output straight from registers.
+// Every pass reads the workgroup's stretch of the stream from a window in LDS
+// (arrays whose codes fit 32 bits).
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -26,55 +28,97 @@ namespace dev {
 constexpr int kRootMax = 1 << kRootBits;
 constexpr uint64_t kSyncLead = 96;  // bits the first parse of a subsequence starts early
 
-struct BitReader {
-    const uint32_t* __restrict__ w;
-    uint64_t idx;   // next word to load
-    uint64_t buf;   // unread bits, LSB first
-    int nb;         // valid bits in buf
-    uint64_t pos;   // absolute bit position of buf bit 0
-
-    __device__ __forceinline__ void init(const uint32_t* words, uint64_t p) {
-        w = words;
-        idx = p >> 5;
-        const uint32_t sh = (uint32_t)(p & 31);
-        buf = ((uint64_t)w[idx] | ((uint64_t)w[idx + 1] << 32)) >> sh;
-        nb = 64 - (int)sh;
-        idx += 2;
-        pos = p;
-    }
-    __device__ __forceinline__ void refill() {
-        if (nb <= 32) {
-            buf |= (uint64_t)w[idx++] << nb;
-            nb += 32;
-        }
-    }
-    __device__ __forceinline__ void skip(uint32_t n) {
-        buf >>= n;
-        nb -= (int)n;
-        pos += n;
-    }
+// Where a parse reads: the stream words, the workgroup's window of them in
+// LDS (arrays with codes of at most 32 bits), the root table in LDS and every
+// table in global memory.
+struct ParseSrc {
+    const uint32_t* __restrict__ words;
+    const uint32_t* win;  // window rows (kWinRowWords words each)
+    uint64_t w0;          // stream word of window word 0
+    uint32_t nwin;        // window words loaded
+    const uint32_t* lds;  // root table
+    const uint32_t* __restrict__ gtab;
+    uint32_t rb;          // root bits
 };
 
-// Decode one codeword (codes <= 32 bits: buf holds >= 33 bits after refill).
-__device__ __forceinline__ uint32_t decode_short(BitReader& br, const uint32_t* __restrict__ lds_tab,
-                                                 const uint32_t* __restrict__ gtab, uint32_t root_bits,
-                                                 uint32_t& sym) {
-    br.refill();
-    uint64_t b = br.buf;
-    uint32_t e = lds_tab[b & ((1u << root_bits) - 1)];
-    uint32_t consumed = 0, bits = root_bits;
-    while (e & 0x80000000u) {
-        consumed += bits;
-        b >>= bits;
-        const uint32_t* t = gtab + (e & 0xFFFFFF);
-        bits = (e >> 24) & 0x1F;
-        e = t[b & ((1u << bits) - 1)];
+// The window: the stream words a workgroup's 256 subsequences read, from the
+// sync pass's lead before the first one to kWinTail words past the last one
+// (a write-pass thread parses up to kLineSyms - 1 codewords of at most 32
+// bits past its own).  Row r holds words 16r .. 16r + 16 (the next row's
+// first word repeated), so the two words under any bit position lie in one
+// row and one ds_read2 fetches them; neighbouring lanes, 16 words apart in
+// the stream, land 17 banks apart.
+constexpr uint32_t kLineSyms = 32;
+constexpr uint32_t kWinTail = 40;
+constexpr uint32_t kWinMaxWords = 256 * (kSubBits / 32) + 64;
+constexpr uint32_t kWinRowWords = 17;
+constexpr uint32_t kWinLds = (kWinMaxWords / 16) * kWinRowWords;
+static_assert(kWinTail * 32 >= (kLineSyms + 2) * 32, "window tail shorter than a write-pass overrun");
+
+__device__ __forceinline__ void load_window(uint32_t* win, const uint32_t* __restrict__ words, const DecArrayDev& A,
+                                            uint64_t& w0, uint32_t& nwin) {
+    const uint64_t u0 = (uint64_t)blockIdx.x * blockDim.x - A.sub0;
+    const uint64_t b0 = A.payload_bit + u0 * kSubBits;
+    w0 = (b0 - min<uint64_t>(kSyncLead, u0 * kSubBits)) >> 5;
+    const uint64_t eb = A.payload_bit + min((u0 + blockDim.x) * kSubBits, A.bits);
+    nwin = (uint32_t)min<uint64_t>((eb + 31) / 32 + kWinTail - w0, kWinMaxWords);
+    for (uint32_t k = threadIdx.x; k < nwin; k += blockDim.x) {
+        const uint32_t v = words[w0 + k];
+        const uint32_t r = k >> 4, c = k & 15;
+        win[r * kWinRowWords + c] = v;
+        if (c == 0 && r) win[(r - 1) * kWinRowWords + 16] = v;
     }
-    sym = e & 0xFFFFFF;
-    const uint32_t len = consumed + ((e >> 24) & 0x7F);
-    br.skip(len);
-    return len;
 }
+
+// Parsers over one array's bit stream.  Positions are 32-bit, counted from a
+// base bit the caller picks (a thread parses a few thousand bits at most).
+//
+// WinParser (codes of at most 32 bits): the 32 bits at the position come from
+// the window (one ds_read2 and a funnel shift), then one lookup in the root
+// table in LDS; longer codes continue through the subtables in global memory.
+// There is no bit buffer: every step reads at its position, so lanes never
+// branch to refill one.  A position past the window (only in a malformed
+// stream) reads the window's last words instead.
+struct WinParser {
+    const uint32_t* win;
+    const uint32_t* lds;
+    const uint32_t* __restrict__ gtab;
+    uint32_t o;     // window bit of the base
+    uint32_t rel;   // position, bits past the base
+    uint32_t imax;  // last window word a read may start at
+    uint32_t rb;
+
+    __device__ __forceinline__ void init(const ParseSrc& S, uint64_t p, uint64_t base) {
+        win = S.win;
+        lds = S.lds;
+        gtab = S.gtab;
+        rb = S.rb;
+        o = (uint32_t)(base - S.w0 * 32);
+        rel = (uint32_t)(p - base);
+        imax = S.nwin - 2;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t q = o + rel;
+        const uint32_t i = min(q >> 5, imax);
+        const uint32_t a = (i >> 4) * kWinRowWords + (i & 15);
+        const uint32_t b = __builtin_amdgcn_alignbit(win[a + 1], win[a], q & 31);
+        uint32_t e = lds[b & ((1u << rb) - 1)];
+        uint32_t len = e >> 24;  // a leaf: bit 31 clear, the length above the symbol
+        if (e & 0x80000000u) {
+            uint32_t consumed = rb, bb = b >> rb;
+            for (;;) {
+                const uint32_t bits = (e >> 24) & 0x1F;
+                e = gtab[(e & 0xFFFFFF) + (bb & ((1u << bits) - 1))];
+                if (!(e & 0x80000000u)) break;
+                consumed += bits;
+                bb >>= bits;
+            }
+            len = consumed + (e >> 24);
+        }
+        rel += len;
+        return e & 0xFFFFFF;
+    }
+};
 
 __device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ w, uint64_t p) {
     const uint64_t i = p >> 5;
@@ -84,59 +128,78 @@ __device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ w, uin
     return (lo >> sh) | ((uint64_t)w[i + 2] << (64 - sh));
 }
 
-// Codes up to 64 bits: a fresh 64-bit window per codeword.
-__device__ __forceinline__ uint32_t decode_long(const uint32_t* __restrict__ words, uint64_t p,
-                                                const uint32_t* __restrict__ gtab, uint32_t root_bits,
-                                                uint32_t& sym) {
-    uint64_t b = window64(words, p);
-    uint32_t e = gtab[b & ((1u << root_bits) - 1)];
-    uint32_t consumed = 0, bits = root_bits;
-    while (e & 0x80000000u) {
-        consumed += bits;
-        b >>= bits;
-        const uint32_t* t = gtab + (e & 0xFFFFFF);
-        bits = (e >> 24) & 0x1F;
-        e = t[b & ((1u << bits) - 1)];
-    }
-    sym = e & 0xFFFFFF;
-    return consumed + ((e >> 24) & 0x7F);
-}
-
-// Parser over one array: short codes through the LDS table and a bit buffer,
-// long codes through global windows.
-struct Parser {
-    BitReader br;
-    uint64_t p;
-    bool lng;
-    const uint32_t* lds;
-    const uint32_t* gtab;
+// LongParser (codes up to 64 bits): a fresh 64-bit window per codeword from
+// the stream words and every table in global memory.
+struct LongParser {
+    const uint32_t* __restrict__ w;
+    const uint32_t* __restrict__ gtab;
+    uint64_t base;
+    uint32_t rel;
     uint32_t rb;
-    __device__ __forceinline__ void init(const uint32_t* words, uint64_t start, bool long_codes, const uint32_t* l,
-                                         const uint32_t* g, uint32_t root_bits) {
-        lng = long_codes;
-        lds = l;
-        gtab = g;
-        rb = root_bits;
-        p = start;
-        if (!lng) br.init(words, start);
-        else br.w = words;
+
+    __device__ __forceinline__ void init(const ParseSrc& S, uint64_t p, uint64_t b) {
+        w = S.words;
+        gtab = S.gtab;
+        rb = S.rb;
+        base = b;
+        rel = (uint32_t)(p - b);
     }
     __device__ __forceinline__ uint32_t next() {
-        uint32_t sym;
-        if (!lng) {
-            decode_short(br, lds, gtab, rb, sym);
-            p = br.pos;
-        } else {
-            p += decode_long(br.w, p, gtab, rb, sym);
+        uint64_t b = window64(w, base + rel);
+        uint32_t e = gtab[b & ((1u << rb) - 1)];
+        uint32_t consumed = 0, bits = rb;
+        while (e & 0x80000000u) {
+            consumed += bits;
+            b >>= bits;
+            bits = (e >> 24) & 0x1F;
+            e = gtab[(e & 0xFFFFFF) + ((uint32_t)b & ((1u << bits) - 1))];
         }
-        return sym;
+        rel += consumed + ((e >> 24) & 0x7F);
+        return e & 0xFFFFFF;
     }
 };
 
-__device__ __forceinline__ void load_root(uint32_t* lds, const DecArrayDev& A, const uint32_t* __restrict__ tables) {
+// The array's root table and (codes of at most 32 bits) the workgroup's
+// window into LDS, then one barrier.
+__device__ __forceinline__ ParseSrc load_src(uint32_t* lds, uint32_t* win, const uint32_t* __restrict__ words,
+                                             const DecArrayDev& A, const uint32_t* __restrict__ tables) {
+    ParseSrc S;
+    S.words = words;
+    S.win = win;
+    S.w0 = 0;
+    S.nwin = 2;
+    S.lds = lds;
+    S.gtab = tables + A.table_off;
+    S.rb = A.root_bits;
     const uint32_t n = 1u << A.root_bits;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = tables[A.table_off + i];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = S.gtab[i];
+    if (A.max_len <= 32) load_window(win, words, A, S.w0, S.nwin);
     __syncthreads();
+    return S;
+}
+
+// The parse starts kSyncLead bits early, inside the predecessor: by the
+// subsequence's first bit it has almost always fallen into step with the true
+// codeword boundaries, and the first boundary at or past that bit is the
+// start (the fix passes correct the rest).
+template <class PR>
+__device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t j,
+                                         uint64_t* start, uint64_t* exit_pos, uint32_t* count) {
+    const uint32_t lead = (uint32_t)min<uint64_t>(kSyncLead, u * kSubBits);
+    const uint64_t base = A.payload_bit + u * kSubBits - lead;
+    const uint32_t lim = lead + (uint32_t)min<uint64_t>(kSubBits, A.bits - u * kSubBits);
+    PR P;
+    P.init(S, base, base);
+    while (P.rel < lead) P.next();
+    const uint32_t s = P.rel;
+    uint32_t n = 0;
+    while (P.rel < lim) {
+        P.next();
+        n++;
+    }
+    start[j] = base + s;
+    exit_pos[j] = base + P.rel;
+    count[j] = n;
 }
 
 __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ words,
@@ -145,8 +208,9 @@ __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ w
                                                   const uint32_t* __restrict__ tables, uint64_t* start,
                                                   uint64_t* exit_pos, uint32_t* count) {
     __shared__ uint32_t lds[kRootMax];
+    __shared__ uint32_t win[kWinLds];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    load_root(lds, A, tables);
+    const ParseSrc S = load_src(lds, win, words, A, tables);
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t u = j - A.sub0;
     if (u >= A.nsub) {
@@ -154,69 +218,40 @@ __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ w
         count[j] = 0;
         return;
     }
-    const uint64_t lim = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    const uint64_t s0 = A.payload_bit + u * kSubBits;
-    // the parse starts kSyncLead bits early, inside the predecessor: by the
-    // subsequence's first bit it has almost always fallen into step with the
-    // true codeword boundaries, and the first boundary at or past that bit is
-    // the start (the fix passes correct the rest)
-    const uint64_t lead = min<uint64_t>(kSyncLead, u * kSubBits);
-    Parser P;
-    P.init(words, s0 - lead, A.max_len > 32, lds, tables + A.table_off, A.root_bits);
-    while (P.p < s0) P.next();
-    const uint64_t s = P.p;
-    uint32_t n = 0;
-    while (P.p < lim) {
-        P.next();
-        n++;
-    }
-    start[j] = s;
-    exit_pos[j] = P.p;
-    count[j] = n;
+    if (A.max_len > 32) sync_run<LongParser>(S, A, u, j, start, exit_pos, count);
+    else sync_run<WinParser>(S, A, u, j, start, exit_pos, count);
 }
 
-__global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ words,
-                                                 const DecArrayDev* __restrict__ arrays,
-                                                 const uint32_t* __restrict__ wg_arr,
-                                                 const uint32_t* __restrict__ tables, uint64_t* start,
-                                                 const uint64_t* __restrict__ exit_in, uint64_t* exit_out,
-                                                 uint32_t* count, unsigned int* changed) {
-    __shared__ uint32_t lds[kRootMax];
-    const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t u = j - A.sub0;
-    const bool settled = u >= A.nsub || u == 0 || start[j] == exit_in[j - 1];
-    // the table only when some lane re-parses (after convergence: none)
-    if (!__syncthreads_or(!settled)) {
-        exit_out[j] = exit_in[j];
-        return;
-    }
-    load_root(lds, A, tables);
-    if (settled) {
-        exit_out[j] = exit_in[j];
-        return;
-    }
-    const uint64_t lim = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    const uint64_t ns = exit_in[j - 1];
-    const bool lng = A.max_len > 32;
-    Parser pn, po;
-    pn.init(words, ns, lng, lds, tables + A.table_off, A.root_bits);
-    po.init(words, start[j], lng, lds, tables + A.table_off, A.root_bits);
+// Re-parse of a subsequence from its predecessor's exit ns, in lockstep with
+// its old parse from start[j]; both positions count from the lower of the two
+// (a predecessor's exit and every start lie at or past the subsequence's
+// first bit).
+template <class PR>
+__device__ __forceinline__ void fix_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t j,
+                                        uint64_t* start, const uint64_t* __restrict__ exit_in, uint64_t* exit_out,
+                                        uint32_t* count, unsigned int* changed) {
+    const uint64_t ns = exit_in[j - 1], os = start[j];
+    const uint64_t base = min(ns, os);
+    const uint64_t lim_abs = A.payload_bit + min((u + 1) * kSubBits, A.bits);
+    const uint32_t lim = lim_abs > base ? (uint32_t)(lim_abs - base) : 0u;
+    PR pn, po;
+    pn.init(S, ns, base);
+    po.init(S, os, base);
     uint32_t dn = 0, dold = 0;
     uint64_t new_exit;
     uint32_t new_count;
     for (;;) {
-        if (pn.p >= lim) {  // the new parse left the subsequence without meeting the old one
-            new_exit = pn.p;
+        if (pn.rel >= lim) {  // the new parse left the subsequence without meeting the old one
+            new_exit = base + pn.rel;
             new_count = dn;
             break;
         }
-        if (pn.p == po.p) {  // merged: the rest is the old parse
+        if (pn.rel == po.rel) {  // merged: the rest is the old parse
             new_exit = exit_in[j];
             new_count = dn + (count[j] - dold);
             break;
         }
-        if (po.p < pn.p && po.p < lim) {
+        if (po.rel < pn.rel && po.rel < lim) {
             po.next();
             dold++;
         } else {
@@ -228,6 +263,32 @@ __global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ wo
     count[j] = new_count;
     exit_out[j] = new_exit;
     if (new_exit != exit_in[j]) atomicOr(changed, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ words,
+                                                 const DecArrayDev* __restrict__ arrays,
+                                                 const uint32_t* __restrict__ wg_arr,
+                                                 const uint32_t* __restrict__ tables, uint64_t* start,
+                                                 const uint64_t* __restrict__ exit_in, uint64_t* exit_out,
+                                                 uint32_t* count, unsigned int* changed) {
+    __shared__ uint32_t lds[kRootMax];
+    __shared__ uint32_t win[kWinLds];
+    const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t u = j - A.sub0;
+    const bool settled = u >= A.nsub || u == 0 || start[j] == exit_in[j - 1];
+    // the tables only when some lane re-parses (after convergence: none)
+    if (!__syncthreads_or(!settled)) {
+        exit_out[j] = exit_in[j];
+        return;
+    }
+    const ParseSrc S = load_src(lds, win, words, A, tables);
+    if (settled) {
+        exit_out[j] = exit_in[j];
+        return;
+    }
+    if (A.max_len > 32) fix_run<LongParser>(S, A, u, j, start, exit_in, exit_out, count, changed);
+    else fix_run<WinParser>(S, A, u, j, start, exit_in, exit_out, count, changed);
 }
 
 // Symbols per workgroup.
@@ -295,12 +356,42 @@ __global__ __launch_bounds__(1024) void k_dec_wgscan(const DecArrayDev* __restri
 // boundary (the previous owner stores them) and, past its own symbols, parses
 // on into its successors' first symbols up to the next boundary (the bit
 // stream is continuous and the starts are exact), at most kLineSyms - 1 more
-// codewords.  A line is decoded into the thread's own LDS row and then stored
-// with eight 16-byte stores back to back (a line written piecemeal over a
-// long decode leaves half-written lines in L2, which reach HBM as partial
-// writes).  Errors: a run past its array's end (2), the parse of an
-// array's last subsequence not ending at the payload's end (4).
-constexpr uint32_t kLineSyms = 32;
+// codewords.  A line is decoded into registers and then stored with eight
+// 16-byte stores back to back (a line written piecemeal over a long decode
+// leaves half-written lines in L2, which reach HBM as partial writes).
+// Errors: a run past its array's end (2), the parse of an array's last
+// subsequence not ending at the payload's end (4).
+template <class PR>
+__device__ __forceinline__ void write_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t st,
+                                          uint64_t pos, uint32_t n, uint64_t a, uint64_t b, int32_t* out,
+                                          unsigned int* err) {
+    PR P;
+    P.init(S, st, st);
+    const uint64_t end = A.payload_bit + A.bits;
+    if (b <= a) {  // no line starts among this thread's slots: nothing to store
+        if (u + 1 == A.nsub) {
+            for (uint32_t k = 0; k < n; k++) P.next();
+            if (st + P.rel != end) atomicOr(err, 4u);
+        }
+        return;
+    }
+    // symbols as int32: the offset added modulo 2^32
+    const uint32_t off = (uint32_t)A.offset;
+    for (uint64_t k = pos; k < a; k++) P.next();
+    uint64_t k = a;
+    for (; k + kLineSyms <= b; k += kLineSyms) {
+        int32_t v[kLineSyms];
+#pragma unroll
+        for (uint32_t i = 0; i < kLineSyms; i++) v[i] = (int32_t)(off + P.next());
+        int4* d = reinterpret_cast<int4*>(out + k);
+#pragma unroll
+        for (uint32_t i = 0; i < kLineSyms / 4; i++) d[i] = make_int4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+    for (; k < b; k++) out[k] = (int32_t)(off + P.next());  // the partial line at an array's end
+    // the array's last run ends at the payload's end (b = oend = pos + n)
+    if (u + 1 == A.nsub && st + P.rel != end) atomicOr(err, 4u);
+}
+
 __global__ __launch_bounds__(256) void k_dec_write(const uint32_t* __restrict__ words,
                                                    const DecArrayDev* __restrict__ arrays,
                                                    const uint32_t* __restrict__ wg_arr,
@@ -310,10 +401,9 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t* __restrict__ 
                                                    const uint64_t* __restrict__ wg_off, int32_t* out,
                                                    unsigned int* err) {
     __shared__ uint32_t lds[kRootMax];
+    __shared__ uint32_t win[kWinLds];
     __shared__ uint32_t wsum[4];
-    __shared__ int32_t lrow[256][kLineSyms + 1];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    load_root(lds, A, tables);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t u = j - A.sub0;
@@ -327,7 +417,7 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t* __restrict__ 
         if (lane >= (uint32_t)o) incl += t;
     }
     if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
+    const ParseSrc S = load_src(lds, win, words, A, tables);  // its barrier also publishes wsum
     uint64_t pos = wg_off[blockIdx.x] + incl - n;
     for (uint32_t w = 0; w < wave; w++) pos += wsum[w];
     if (!live) return;
@@ -338,28 +428,8 @@ __global__ __launch_bounds__(256) void k_dec_write(const uint32_t* __restrict__ 
     }
     auto up = [](uint64_t x) { return (x + kLineSyms - 1) / kLineSyms * kLineSyms; };
     const uint64_t a = min(up(pos), oend), b = min(up(pos + n), oend);  // this thread's slots [a, b)
-    Parser P;
-    P.init(words, start[j], A.max_len > 32, lds, tables + A.table_off, A.root_bits);
-    if (b <= a) {  // no line starts among this thread's slots: nothing to store
-        if (u + 1 == A.nsub) {
-            for (uint32_t k = 0; k < n; k++) P.next();
-            if (P.p != A.payload_bit + A.bits) atomicOr(err, 4u);
-        }
-        return;
-    }
-    const int64_t off = A.offset;
-    for (uint64_t k = pos; k < a; k++) P.next();
-    int32_t* row = &lrow[threadIdx.x][0];  // this thread's line (padded rows: no bank conflicts)
-    uint64_t k = a;
-    for (; k + kLineSyms <= b; k += kLineSyms) {
-        for (uint32_t i = 0; i < kLineSyms; i++) row[i] = (int32_t)(off + (int64_t)P.next());
-        int4* d = reinterpret_cast<int4*>(out + k);
-#pragma unroll
-        for (uint32_t i = 0; i < kLineSyms / 4; i++) d[i] = make_int4(row[4 * i], row[4 * i + 1], row[4 * i + 2], row[4 * i + 3]);
-    }
-    for (; k < b; k++) out[k] = (int32_t)(off + (int64_t)P.next());  // the partial line at an array's end
-    // the array's last run ends at the payload's end (b = oend = pos + n)
-    if (u + 1 == A.nsub && P.p != A.payload_bit + A.bits) atomicOr(err, 4u);
+    if (A.max_len > 32) write_run<LongParser>(S, A, u, start[j], pos, n, a, b, out, err);
+    else write_run<WinParser>(S, A, u, start[j], pos, n, a, b, out, err);
 }
 
 }  // namespace dev

@@ -200,39 +200,44 @@ void build_codebook(HuffArray& a, const uint32_t* hist) {
 
 namespace {
 
-struct TNode { int64_t l = -1, r = -1; uint64_t c = 0; };
+// One leaf of a serialized tree: symbol index, code (LSB-first: the branch
+// taken at depth k is bit k, 1 = right) and length.
+struct Leaf {
+    uint64_t code;
+    uint32_t sym;
+    uint32_t len;
+};
 
-int depth_below(const std::vector<TNode>& t, int64_t u, int cap) {
-    // maximum leaf depth below u, capped
-    struct It { int64_t u; int d; };
-    std::vector<It> st{{u, 0}};
-    int best = 0;
-    while (!st.empty()) {
-        It it = st.back();
-        st.pop_back();
-        if (t[it.u].l < 0) { best = std::max(best, it.d); continue; }
-        if (it.d >= cap) { best = cap; continue; }
-        st.push_back({t[it.u].l, it.d + 1});
-        st.push_back({t[it.u].r, it.d + 1});
-    }
-    return best;
-}
-
-void fill_table(const std::vector<TNode>& t, int64_t sub_root, int bits, std::vector<uint32_t>& tab, uint64_t base) {
-    for (uint64_t p = 0; p < (1ull << bits); p++) {
-        int64_t u = sub_root;
-        int k = 0;
-        while (k < bits && t[u].l >= 0) u = ((p >> k++) & 1) ? t[u].r : t[u].l;
-        if (t[u].l < 0) {
-            tab[base + p] = (uint32_t)(t[u].c & 0xFFFFFF) | ((uint32_t)k << 24);
-        } else {
-            int need = depth_below(t, u, 64);
-            int sb = std::min(need, kRootBits);
-            uint64_t off = tab.size();
-            tab.resize(off + (1ull << sb));
-            tab[base + p] = 0x80000000u | (uint32_t)off | ((uint32_t)sb << 24);
-            fill_table(t, u, sb, tab, off);
+// Fill the table of 2^bits entries at tab[base] for leaves lv[a, b), all of
+// which share their first `used` code bits: a leaf with at most used + bits
+// bits fills every entry whose low (len - used) bits are its remaining code;
+// longer leaves, contiguous per (used + bits)-bit prefix because the leaves
+// are in preorder, get a subtable of min(max length - used - bits,
+// kSubTableBits) bits reached through an entry with the top bit set.
+void fill_level(const std::vector<Leaf>& lv, uint64_t a, uint64_t b, int used, int bits, std::vector<uint32_t>& tab,
+                uint64_t base) {
+    const uint64_t mask = (1ull << bits) - 1;
+    for (uint64_t i = a; i < b;) {
+        const Leaf& L = lv[i];
+        const int rem = (int)L.len - used;
+        const uint64_t code = L.code >> used;
+        if (rem <= bits) {
+            const uint32_t e = L.sym | ((uint32_t)rem << 24);
+            for (uint64_t q = code & mask; q <= mask; q += 1ull << rem) tab[base + q] = e;
+            i++;
+            continue;
         }
+        uint64_t j = i + 1;
+        uint32_t mx = L.len;
+        while (j < b && (int)lv[j].len - used > bits && ((lv[j].code >> used) & mask) == (code & mask))
+            mx = std::max(mx, lv[j++].len);
+        const int sb = std::min((int)mx - used - bits, kSubTableBits);
+        const uint64_t off = tab.size();
+        if (off + (1ull << sb) > 0xFFFFFFull) fail(ASZ_ERR_DATA, "Huffman decode table too large");
+        tab.resize(off + (1ull << sb), 0);
+        tab[base + (code & mask)] = 0x80000000u | (uint32_t)off | ((uint32_t)sb << 24);
+        fill_level(lv, i, j, used + bits, sb, tab, off);
+        i = j;
     }
 }
 
@@ -258,39 +263,61 @@ DecodeInfo parse_huffman(const uint8_t* s, uint64_t n, uint64_t off, uint64_t co
         return di;
     }
     if (mbft > 31) fail(ASZ_ERR_DATA, "Huffman symbol width too large");
-    // rebuild the tree (huffman.hpp:741-774); bit 0 is the root marker
-    std::vector<TNode> t;
-    t.reserve(2 * di.nleaf);
-    t.push_back({});
-    std::vector<int64_t> st{0};
-    uint64_t i = 1;
+    // walk the DFS preorder tree (huffman.hpp:741-774) without building it:
+    // bit 0 is the root marker (the root is internal), then per node 0 =
+    // internal, 1 = leaf followed by its mbft-bit symbol.  The position is
+    // the path of d branches, kept MSB-first in m and LSB-first (the code) in
+    // c: an internal node descends left (m <<= 1, c unchanged); after a leaf
+    // the walk climbs the t trailing right branches and steps to the right
+    // sibling (m = (m >> t) + 1; c keeps its low d - t - 1 bits and sets bit
+    // d - t - 1), and ends when it climbs past the root.
     const uint64_t avail_bits = (n - pos) * 8;
-    auto bit = [&](uint64_t k) -> int {
-        if (k >= avail_bits) fail(ASZ_ERR_DATA, "compressed stream is truncated in a Huffman tree");
-        return (s[pos + (k >> 3)] >> (k & 7)) & 1;
+    auto need = [&](uint64_t k) {
+        if (k > avail_bits) fail(ASZ_ERR_DATA, "compressed stream is truncated in a Huffman tree");
     };
-    uint64_t leaves = 0;
-    while (!st.empty()) {
-        int64_t u = st.back();
-        if (t.size() > 2 * di.nleaf) fail(ASZ_ERR_DATA, "malformed Huffman tree");
-        if (bit(i++) == 0) {
-            t.push_back({});
-            int64_t v = (int64_t)t.size() - 1;
-            if (t[u].l < 0) t[u].l = v; else t[u].r = v;
-            st.push_back(v);
-        } else {
-            uint64_t c = 0;
-            for (int j = 0; j < mbft; j++) c |= (uint64_t)bit(i++) << j;
-            if (c >= (1ull << 24)) fail(ASZ_ERR_DATA, "Huffman alphabet too wide for the GPU decoder");
-            TNode leaf;
-            leaf.c = c;
-            t.push_back(leaf);
-            int64_t v = (int64_t)t.size() - 1;
-            if (t[u].l < 0) t[u].l = v; else t[u].r = v;
-            leaves++;
-            while (!st.empty() && t[st.back()].r >= 0) st.pop_back();
+    // m <= 32 bits from bit k on, LSB-first: up to five bytes in one word
+    auto bits = [&](uint64_t k, int m) -> uint64_t {
+        need(k + m);
+        const uint64_t b0 = pos + (k >> 3), nb = ((k & 7) + m + 7) >> 3;
+        uint64_t w = 0;
+        for (uint64_t j = 0; j < nb; j++) w |= (uint64_t)s[b0 + j] << (8 * j);
+        return (w >> (k & 7)) & ((1ull << m) - 1);
+    };
+    std::vector<Leaf> lv;
+    lv.reserve(di.nleaf);
+    uint64_t i = 1, m = 0, c = 0;
+    int d = 1;
+    uint32_t max_len = 0;
+    for (;;) {
+        // a run of internal nodes, then a leaf
+        const int chunk = (int)std::min<uint64_t>(32, avail_bits > i ? avail_bits - i : 0);
+        const uint64_t w = chunk ? bits(i, chunk) : 0;
+        if (w == 0) {
+            need(i + chunk + 1);  // no leaf in the bits left: truncated
+            d += chunk;
+            m <<= std::min(chunk, 63);
+            i += chunk;
+            if (d > 64) fail(ASZ_ERR_DATA, "Huffman code longer than 64 bits");
+            continue;
         }
+        const int z = __builtin_ctzll(w);
+        d += z;
+        if (d > 64) fail(ASZ_ERR_DATA, "Huffman code longer than 64 bits");
+        m = z ? m << z : m;
+        i += z + 1;
+        const uint64_t sym = mbft ? bits(i, mbft) : 0;
+        i += mbft;
+        if (sym >= (1ull << 24)) fail(ASZ_ERR_DATA, "Huffman alphabet too wide for the GPU decoder");
+        if (lv.size() == di.nleaf) fail(ASZ_ERR_DATA, "malformed Huffman tree");
+        lv.push_back({c, (uint32_t)sym, (uint32_t)d});
+        max_len = std::max(max_len, (uint32_t)d);
+        const int t = (~m == 0) ? 64 : __builtin_ctzll(~m);
+        if (t >= d) break;
+        m = (m >> t) + 1;
+        d -= t;
+        c = (c & ((1ull << (d - 1)) - 1)) | (1ull << (d - 1));
     }
+    const uint64_t leaves = lv.size();
     if (leaves != di.nleaf) fail(ASZ_ERR_DATA, "Huffman tree leaf count mismatch");
     pos += (i + 7) >> 3;
     if (pos + 8 > n) fail(ASZ_ERR_DATA, "compressed stream is truncated in a Huffman payload");
@@ -299,10 +326,10 @@ DecodeInfo parse_huffman(const uint8_t* s, uint64_t n, uint64_t off, uint64_t co
     di.payload_off = pos;
     di.end_off = pos + ((di.payload_bits + 7) >> 3);
     if (di.end_off > n) fail(ASZ_ERR_DATA, "compressed stream is truncated in a Huffman payload");
-    di.max_len = depth_below(t, 0, 64);
-    int rb = std::min(di.max_len, kRootBits);
+    di.max_len = (int)max_len;
+    const int rb = std::min(di.max_len, kRootBits);
     di.table.assign(1ull << rb, 0);
-    fill_table(t, 0, rb, di.table, 0);
+    fill_level(lv, 0, leaves, 0, rb, di.table, 0);
     return di;
 }
 

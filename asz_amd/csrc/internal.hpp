@@ -288,6 +288,7 @@ struct DecodeInfo {
 DecodeInfo parse_huffman(const uint8_t* s, uint64_t n, uint64_t off, uint64_t count);
 
 // Lookup-table geometry for the GPU decoder.
-constexpr int kRootBits = 12;
+constexpr int kRootBits = 12;     // root table (LDS)
+constexpr int kSubTableBits = 8;  // largest subtable below it (global memory)
 
 }  // namespace asz

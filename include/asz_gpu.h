@@ -247,6 +247,14 @@ int asz_decompress(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, c
                    const asz_params *p, float *d_out, int32_t *qb_out, int64_t *t_out);
 int asz_decompress_host(asz_ctx *ctx, const uint8_t *stream, uint64_t stream_bytes, const uint64_t dims[3],
                         const asz_params *p, float *out);
+/* asz_decompress for a stream already in device memory (no reference
+ * counterpart; for callers that keep streams on the GPU, e.g. the one
+ * asz_compress_field leaves there under ASZ_OPT_STREAM_ON_DEVICE).  Only the
+ * header bytes travel to the host (for the codebooks); the payload is read
+ * where it lies.  A stream in the context's own stream buffer must start it
+ * (it is then decoded in place); any other device buffer is copied first. */
+int asz_decompress_device(asz_ctx *ctx, const uint8_t *d_stream, uint64_t stream_bytes, const uint64_t dims[3],
+                          const asz_params *p, float *d_out);
 
 /* ------------------------------------------------------------------ batches
  * Many fields in one call (no reference counterpart: the reference compresses
