@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <atomic>
 #include <immintrin.h>
 #include <memory>
@@ -94,6 +95,13 @@ uint64_t get_le64(const uint8_t* p) {
     for (int k = 0; k < 8; k++) v |= (uint64_t)p[k] << (8 * k);
     return v;
 }
+uint64_t get_be64(const uint8_t* p) {
+    uint64_t v = 0;
+    for (int k = 0; k < 8; k++) v = (v << 8) | p[k];
+    return v;
+}
+// the Huffman payload's bit length is stored xor this (huffman.hpp)
+constexpr uint64_t kPayloadLenMask = 0x1234abcdull;
 
 void validate_params(const asz_params& p) {
     if (!(p.rel_eb > 0 && std::isfinite(p.rel_eb))) fail(ASZ_ERR_ARG, "relative error bound must be positive and finite");
@@ -1305,7 +1313,11 @@ struct Parsed {
     std::vector<uint64_t> counts;     // symbols per array
 };
 
-static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p) {
+// beside: host work that needs only the preamble (min, max, plan), run on
+// this thread while the trees are parsed on the others; its errors are
+// reported after the parse's.
+static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p,
+                           const std::function<void(const Parsed&)>& beside) {
     Parsed P;
     if (n < 9) fail(ASZ_ERR_DATA, "compressed stream is truncated");
     std::memcpy(&P.mn, s, 4);
@@ -1329,6 +1341,73 @@ static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p) {
         P.plan.order[r] = (uint8_t)(o - '0');
         off += 2;
     }
+    // Every array's header offset follows from the fixed-size fields (a tree
+    // of nleaf leaves takes 2 nleaf - 1 + nleaf * width bits, whole bytes;
+    // k_stream_regions walks the same way), so the trees of a stream with
+    // many leaves are parsed and tabled on several threads.  A stream whose
+    // fields do not add up takes the sequential parse, which reports the
+    // first error in stream order.
+    std::vector<uint64_t> hoff, cnts;
+    uint64_t leaves = 0;
+    bool walk_ok = true;
+    {
+        uint64_t o = off;
+        for (uint64_t k = 0; k < L + 2 && walk_ok; k++) {
+            if (o + 8 > n) { walk_ok = false; break; }
+            const uint64_t cnt = get_le64(s + o);
+            o += 8;
+            cnts.push_back(cnt);
+            hoff.push_back(o);
+            if (cnt == 0 && k > 0) continue;
+            if (o + 25 > n) { walk_ok = false; break; }
+            const uint64_t width = s[o] & 0x3F, nleaf = get_be64(s + o + 9), maxval = get_be64(s + o + 17);
+            if (nleaf == 0 || nleaf > n * 8) { walk_ok = false; break; }
+            if (maxval == 1 || nleaf == 1) { o += 33; continue; }
+            const uint64_t tree = (2 * nleaf - 1 + nleaf * width + 7) / 8;
+            if (width > 31 || tree > n || o + 33 + tree > n) { walk_ok = false; break; }
+            const uint64_t bits = get_be64(s + o + 25 + tree) ^ kPayloadLenMask;
+            if (bits > n * 8 || (bits + 7) / 8 > n - (o + 33 + tree)) { walk_ok = false; break; }
+            o += 33 + tree + (bits + 7) / 8;
+            leaves += nleaf;
+        }
+    }
+    const uint64_t na = L + 2;
+    if (walk_ok && leaves >= 4096) {
+        P.counts = cnts;
+        P.nt = cnts[0];
+        P.arrays.assign(na, DecodeInfo{});
+        std::vector<std::exception_ptr> errs(na);
+        std::atomic<uint64_t> next{0};
+        auto work = [&] {
+            for (uint64_t k; (k = next.fetch_add(1)) < na;) {
+                if (cnts[k] == 0 && k > 0) continue;
+                try {
+                    P.arrays[k] = parse_huffman(s, n, hoff[k], cnts[k]);
+                } catch (...) {
+                    errs[k] = std::current_exception();
+                }
+            }
+        };
+        const unsigned nth = std::min<unsigned>(4, std::max(1u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nth; t++) th.emplace_back(work);
+        std::exception_ptr berr;
+        try {
+            beside(P);
+        } catch (...) {
+            berr = std::current_exception();
+        }
+        work();
+        for (auto& t : th) t.join();
+        for (uint64_t k = 0; k < na; k++) {
+            if (errs[k]) std::rethrow_exception(errs[k]);
+            const bool empty = cnts[k] == 0 && k > 0;
+            const uint64_t end = empty ? hoff[k] : P.arrays[k].end_off;
+            if (k + 1 < na && end + 8 != hoff[k + 1]) fail(ASZ_ERR_DATA, "malformed Huffman tree");
+        }
+        if (berr) std::rethrow_exception(berr);
+        return P;
+    }
     for (uint64_t k = 0; k < L + 2; k++) {
         if (off + 8 > n) fail(ASZ_ERR_DATA, "compressed stream is truncated");
         uint64_t cnt = get_le64(s + off);
@@ -1340,6 +1419,7 @@ static Parsed parse_stream(const uint8_t* s, uint64_t n, const asz_params& p) {
         off = di.end_off;
         P.arrays.push_back(std::move(di));
     }
+    beside(P);
     return P;
 }
 
@@ -1424,15 +1504,21 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s_in, uint64_t n, const u
     std::unique_ptr<uint8_t[]> image;
     const uint8_t* hs = s;
     if (dev_stream) {
-        if (!in_place) HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyDeviceToDevice, st));
+        // the headers first (the walk reads the caller's buffer), then the
+        // copy, which runs while the host parses them
         hs = stream_headers(c, s, n, st, image);
+        if (!in_place) HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyDeviceToDevice, st));
     } else {
         HIPCHECK(hipMemcpyAsync(dwords, s, n, hipMemcpyHostToDevice, st));
     }
-    Parsed Q = parse_stream(hs, n, p);
-    Geometry g = make_geometry(dims, Q.plan, p.block_pivot_ratio);
-    const float aeb = abs_eb_of(p.rel_eb, Q.mn, Q.mx);
-    Plan P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, 1.0f, p.h1_block_sampling_num);
+    Geometry g;
+    Plan P;
+    const Parsed Q = parse_stream(hs, n, p, [&](const Parsed& head) {
+        g = make_geometry(dims, head.plan, p.block_pivot_ratio);
+        const float aeb = abs_eb_of(p.rel_eb, head.mn, head.mx);
+        P = make_plan(g, aeb, p.pivot_eb_ratio, p.eb_ratio, p.slope_ratio, 1.0f, p.h1_block_sampling_num, 0,
+                      nullptr, false);
+    });
     const uint64_t N = g.N(), nbl = g.nblocks(), L = (uint64_t)g.L, nt = nbl * L;
     if (Q.nt < nt) fail(ASZ_ERR_DATA, "compressed stream contains too few t values");
     if (Q.counts[1] != P.pivot_count) fail(ASZ_ERR_DATA, "compressed stream has a wrong pivot count");
@@ -1474,6 +1560,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s_in, uint64_t n, const u
         D.offset = di.offset;
         D.root_bits = (uint32_t)std::min(di.max_len, kRootBits);
         D.max_len = (uint32_t)di.max_len;
+        D.lead = sync_lead(di.payload_bits, Q.counts[k]);
         D.nsub = (di.payload_bits + kSubBits - 1) / kSubBits;
         D.wg0 = wg_arr.size();
         D.nwg = (D.nsub + 255) / 256;

@@ -26,7 +26,6 @@ namespace asz {
 namespace dev {
 
 constexpr int kRootMax = 1 << kRootBits;
-constexpr uint64_t kSyncLead = 96;  // bits the first parse of a subsequence starts early
 
 // Where a parse reads: the stream words, the workgroup's window of them in
 // LDS (arrays with codes of at most 32 bits), the root table in LDS and every
@@ -50,7 +49,7 @@ struct ParseSrc {
 // the stream, land 17 banks apart.
 constexpr uint32_t kLineSyms = 32;
 constexpr uint32_t kWinTail = 40;
-constexpr uint32_t kWinMaxWords = 256 * (kSubBits / 32) + 64;
+constexpr uint32_t kWinMaxWords = 256 * (kSubBits / 32) + kMaxSyncLead / 32 + kWinTail + 8;
 constexpr uint32_t kWinRowWords = 17;
 constexpr uint32_t kWinLds = (kWinMaxWords / 16) * kWinRowWords;
 static_assert(kWinTail * 32 >= (kLineSyms + 2) * 32, "window tail shorter than a write-pass overrun");
@@ -59,7 +58,7 @@ __device__ __forceinline__ void load_window(uint32_t* win, const uint32_t* __res
                                             uint64_t& w0, uint32_t& nwin) {
     const uint64_t u0 = (uint64_t)blockIdx.x * blockDim.x - A.sub0;
     const uint64_t b0 = A.payload_bit + u0 * kSubBits;
-    w0 = (b0 - min<uint64_t>(kSyncLead, u0 * kSubBits)) >> 5;
+    w0 = (b0 - min<uint64_t>(A.lead, u0 * kSubBits)) >> 5;
     const uint64_t eb = A.payload_bit + min((u0 + blockDim.x) * kSubBits, A.bits);
     nwin = (uint32_t)min<uint64_t>((eb + 31) / 32 + kWinTail - w0, kWinMaxWords);
     for (uint32_t k = threadIdx.x; k < nwin; k += blockDim.x) {
@@ -178,14 +177,14 @@ __device__ __forceinline__ ParseSrc load_src(uint32_t* lds, uint32_t* win, const
     return S;
 }
 
-// The parse starts kSyncLead bits early, inside the predecessor: by the
+// The parse starts A.lead bits early, inside the predecessor: by the
 // subsequence's first bit it has almost always fallen into step with the true
 // codeword boundaries, and the first boundary at or past that bit is the
 // start (the fix passes correct the rest).
 template <class PR>
 __device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t j,
                                          uint64_t* start, uint64_t* exit_pos, uint32_t* count) {
-    const uint32_t lead = (uint32_t)min<uint64_t>(kSyncLead, u * kSubBits);
+    const uint32_t lead = (uint32_t)min<uint64_t>(A.lead, u * kSubBits);
     const uint64_t base = A.payload_bit + u * kSubBits - lead;
     const uint32_t lim = lead + (uint32_t)min<uint64_t>(kSubBits, A.bits - u * kSubBits);
     PR P;

@@ -214,9 +214,11 @@ struct RoundTune {
 };
 // coarse_max: the most leading rounds that may run as block rounds (those
 // with at most kCoarseTargets targets per block).  tune: one entry per round
-// or nullptr.
+// or nullptr.  tsel_strides: fill ss / ns, the t selection's per-block sample
+// strides (compression only; decompression leaves them empty).
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num, uint32_t coarse_max = 0, const RoundTune* tune = nullptr);
+               float default_slope, uint64_t h1_num, uint32_t coarse_max = 0, const RoundTune* tune = nullptr,
+               bool tsel_strides = true);
 // The per-call scalars of a plan (pivot and layer error bounds, layer slopes);
 // the tables do not depend on them.
 void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,

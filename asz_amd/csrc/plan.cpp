@@ -50,7 +50,8 @@ void set_plan_bounds(Plan& P, float abs_eb, float pivot_eb_ratio, float eb_ratio
 }
 
 Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_ratio, float slope_ratio,
-               float default_slope, uint64_t h1_num, uint32_t coarse_max, const RoundTune* tune) {
+               float default_slope, uint64_t h1_num, uint32_t coarse_max, const RoundTune* tune,
+               bool tsel_strides) {
     Plan P;
     P.g = g;
     P.pivot_count = 1;
@@ -59,8 +60,10 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         P.pivot_count *= P.pn[a];
     }
     const uint64_t nbl = g.nblocks();
-    P.ss.assign((uint64_t)g.L * nbl, 0);
-    P.ns.assign((uint64_t)g.L * nbl, 0);
+    if (tsel_strides) {
+        P.ss.assign((uint64_t)g.L * nbl, 0);
+        P.ns.assign((uint64_t)g.L * nbl, 0);
+    }
     uint64_t s[3] = {g.pivot_stride[0], g.pivot_stride[1], g.pivot_stride[2]};
     // every array starts on a 256-byte boundary of the symbol buffer (whole
     // cache lines for the kernels that write and read it)
@@ -177,7 +180,7 @@ Plan make_plan(const Geometry& g, float abs_eb, float pivot_eb_ratio, float eb_r
         R.hist_off = P.total_hist_words;
         P.total_hist_words += R.nwg ? (uint64_t)R.hist_copies * R.lds_win : 0;
         // t-selection sample strides (compress.tpp:96-116, get_AB_map.tpp:9-12); 0 marks a fallback block
-        for (uint64_t b0 = 0; b0 < g.block_num[0]; b0++)
+        for (uint64_t b0 = 0; tsel_strides && b0 < g.block_num[0]; b0++)
             for (uint64_t b1 = 0; b1 < g.block_num[1]; b1++)
                 for (uint64_t b2 = 0; b2 < g.block_num[2]; b2++) {
                     uint64_t bb[3] = {b0, b1, b2};
