@@ -1561,11 +1561,26 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s_in, uint64_t n, const u
         D.root_bits = (uint32_t)std::min(di.max_len, kRootBits);
         D.max_len = (uint32_t)di.max_len;
         D.lead = sync_lead(di.payload_bits, Q.counts[k]);
+        D.pair_off = kNoPairTable;
         D.nsub = (di.payload_bits + kSubBits - 1) / kSubBits;
         D.wg0 = wg_arr.size();
         D.nwg = (D.nsub + 255) / 256;
         D.sub0 = D.wg0 * 256;
         tables.insert(tables.end(), di.table.begin(), di.table.end());
+        if (di.max_len <= 32) {
+            // the sync pass's root table: two codes per lookup where both fit
+            const uint32_t rb = D.root_bits, mask = (1u << rb) - 1;
+            D.pair_off = (uint32_t)tables.size();
+            for (uint32_t q = 0; q <= mask; q++) {
+                const uint32_t e1 = di.table[q];
+                uint32_t e = e1;
+                if (!(e1 & 0x80000000u)) {
+                    const uint32_t l1 = e1 >> 24, e2 = di.table[(q >> l1) & mask];
+                    if (!(e2 & 0x80000000u) && (e2 >> 24) <= rb - l1) e = kPairFlag | l1 << 8 | (l1 + (e2 >> 24));
+                }
+                tables.push_back(e);
+            }
+        }
         for (uint64_t i = 0; i < D.nwg; i++) wg_arr.push_back((uint32_t)da.size());
         da.push_back(D);
     }

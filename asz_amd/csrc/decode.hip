@@ -4,10 +4,11 @@
 // payload is cut into subsequences of kSubBits bits; a subsequence owns the
 // codewords that start inside it.  Workgroups hold 256 consecutive
 // subsequences of one array, with that array's root lookup table in LDS.
-//   1. k_dec_sync: decode every subsequence from a little before its first
-//      bit, taking the first codeword boundary at or past that bit as its
-//      start (a guess, nearly always right), recording the start, the exit
-//      (first codeword boundary past its end) and the codeword count.
+//   1. k_dec_sync: parse every subsequence from a little before its first
+//      bit (two codewords per lookup where the root bits hold both), taking
+//      the first codeword boundary at or past that bit as its start (a
+//      guess, nearly always right), recording the start, the exit (first
+//      codeword boundary past its end) and the codeword count.
 //   2. k_dec_fix (repeated until no exit changes): a subsequence whose start
 //      differs from its predecessor's exit re-parses from that exit, walking
 //      its old parse in lockstep; Huffman parses from different starts merge
@@ -19,6 +20,8 @@
 //      whole 128-byte lines of the output straight from registers.
 // Every pass reads the workgroup's stretch of the stream from a window in LDS
 // (arrays whose codes fit 32 bits).
+#include <type_traits>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -117,6 +120,36 @@ struct WinParser {
         rel += len;
         return e & 0xFFFFFF;
     }
+    // With the sync pass's root table in lds: one codeword, or two when the
+    // root bits hold two whole codes and the second starts before bound.
+    // Returns the codewords taken.
+    __device__ __forceinline__ uint32_t skip(uint32_t bound) {
+        const uint32_t q = o + rel;
+        const uint32_t i = min(q >> 5, imax);
+        const uint32_t a = (i >> 4) * kWinRowWords + (i & 15);
+        const uint32_t b = __builtin_amdgcn_alignbit(win[a + 1], win[a], q & 31);
+        uint32_t e = lds[b & ((1u << rb) - 1)];
+        if (e & kPairFlag) {
+            const uint32_t l1 = (e >> 8) & 0xF;
+            const bool both = rel + l1 < bound;
+            rel += both ? (e & 0xFF) : l1;
+            return both ? 2u : 1u;
+        }
+        uint32_t len = e >> 24;
+        if (e & 0x80000000u) {
+            uint32_t consumed = rb, bb = b >> rb;
+            for (;;) {
+                const uint32_t bits = (e >> 24) & 0x1F;
+                e = gtab[(e & 0xFFFFFF) + (bb & ((1u << bits) - 1))];
+                if (!(e & 0x80000000u)) break;
+                consumed += bits;
+                bb >>= bits;
+            }
+            len = consumed + (e >> 24);
+        }
+        rel += len;
+        return 1u;
+    }
 };
 
 __device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ w, uint64_t p) {
@@ -158,10 +191,12 @@ struct LongParser {
     }
 };
 
-// The array's root table and (codes of at most 32 bits) the workgroup's
-// window into LDS, then one barrier.
+// The array's root table (sync: its two-code form, codes of at most 32 bits)
+// and (codes of at most 32 bits) the workgroup's window into LDS, then one
+// barrier.
 __device__ __forceinline__ ParseSrc load_src(uint32_t* lds, uint32_t* win, const uint32_t* __restrict__ words,
-                                             const DecArrayDev& A, const uint32_t* __restrict__ tables) {
+                                             const DecArrayDev& A, const uint32_t* __restrict__ tables,
+                                             bool sync = false) {
     ParseSrc S;
     S.words = words;
     S.win = win;
@@ -171,7 +206,8 @@ __device__ __forceinline__ ParseSrc load_src(uint32_t* lds, uint32_t* win, const
     S.gtab = tables + A.table_off;
     S.rb = A.root_bits;
     const uint32_t n = 1u << A.root_bits;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = S.gtab[i];
+    const uint32_t* root = sync && A.max_len <= 32 ? tables + A.pair_off : S.gtab;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = root[i];
     if (A.max_len <= 32) load_window(win, words, A, S.w0, S.nwin);
     __syncthreads();
     return S;
@@ -189,12 +225,20 @@ __device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A
     const uint32_t lim = lead + (uint32_t)min<uint64_t>(kSubBits, A.bits - u * kSubBits);
     PR P;
     P.init(S, base, base);
-    while (P.rel < lead) P.next();
-    const uint32_t s = P.rel;
     uint32_t n = 0;
-    while (P.rel < lim) {
-        P.next();
-        n++;
+    if constexpr (std::is_same<PR, WinParser>::value) {
+        while (P.rel < lead) P.skip(lead);
+    } else {
+        while (P.rel < lead) P.next();
+    }
+    const uint32_t s = P.rel;
+    if constexpr (std::is_same<PR, WinParser>::value) {
+        while (P.rel < lim) n += P.skip(lim);
+    } else {
+        while (P.rel < lim) {
+            P.next();
+            n++;
+        }
     }
     start[j] = base + s;
     exit_pos[j] = base + P.rel;
@@ -209,7 +253,7 @@ __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ w
     __shared__ uint32_t lds[kRootMax];
     __shared__ uint32_t win[kWinLds];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    const ParseSrc S = load_src(lds, win, words, A, tables);
+    const ParseSrc S = load_src(lds, win, words, A, tables, true);
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t u = j - A.sub0;
     if (u >= A.nsub) {

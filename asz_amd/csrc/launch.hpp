@@ -237,8 +237,12 @@ struct DecArrayDev {
     uint32_t root_bits;
     uint32_t max_len;
     uint32_t lead;         // bits the sync pass starts a subsequence's parse early (<= kMaxSyncLead)
-    uint32_t pad;
+    uint32_t pair_off;     // the sync pass's root table (decode table words), kNoPairTable: none
 };
+// The sync pass's root table: the root table with every entry whose root-bit
+// pattern holds two whole codes replaced by kPairFlag | len1 << 8 | len1 +
+// len2 (the sync pass counts codewords and needs no symbols).
+constexpr uint32_t kPairFlag = 0x40000000u, kNoPairTable = 0xFFFFFFFFu;
 constexpr uint32_t kSubBits = 512;  // bits per subsequence
 // The sync lead: about 48 codewords of the array's mean length, by which a
 // parse from an arbitrary bit has almost always fallen into step with the
