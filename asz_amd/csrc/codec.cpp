@@ -1611,7 +1611,7 @@ static void decompress_impl(asz_ctx* c, const uint8_t* s_in, uint64_t n, const u
     uint32_t* dcnt = c->count.get<uint32_t>(nsub + 1);
     uint64_t* dwgs = c->out_pos.get<uint64_t>(2 * nwg + 2);
     // flags[0]: error bits of the write pass; flags[8 + i]: "changed" of fix pass i of a group
-    constexpr uint32_t kFixGroup = 6;
+    constexpr uint32_t kFixGroup = 3;  // a pass settles chains inside workgroups; the next, their boundaries
     unsigned int* flags = c->misc.get<unsigned int>(8 + kFixGroup);
     float* tblk = c->tblk.get<float>(nt + 1);
     uint64_t stage_max = 0;

@@ -9,7 +9,8 @@
 //      the first codeword boundary at or past that bit as its start (a
 //      guess, nearly always right), recording the start, the exit (first
 //      codeword boundary past its end) and the codeword count.
-//   2. k_dec_fix (repeated until no exit changes): a subsequence whose start
+//   2. k_dec_fix (repeated until no exit changes; inside a workgroup each
+//      pass repeats until its subsequences agree): a subsequence whose start
 //      differs from its predecessor's exit re-parses from that exit, walking
 //      its old parse in lockstep; Huffman parses from different starts merge
 //      after a few codewords, after which count and exit follow from the old
@@ -266,14 +267,13 @@ __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ w
 }
 
 // Re-parse of a subsequence from its predecessor's exit ns, in lockstep with
-// its old parse from start[j]; both positions count from the lower of the two
-// (a predecessor's exit and every start lie at or past the subsequence's
-// first bit).
+// its current parse from os (count oc codewords, exit oe); both positions
+// count from the lower of the two (a predecessor's exit and every start lie
+// at or past the subsequence's first bit).  Returns the new exit; nc the new
+// count.
 template <class PR>
-__device__ __forceinline__ void fix_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t j,
-                                        uint64_t* start, const uint64_t* __restrict__ exit_in, uint64_t* exit_out,
-                                        uint32_t* count, unsigned int* changed) {
-    const uint64_t ns = exit_in[j - 1], os = start[j];
+__device__ __forceinline__ uint64_t fix_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t ns,
+                                            uint64_t os, uint32_t oc, uint64_t oe, uint32_t& nc) {
     const uint64_t base = min(ns, os);
     const uint64_t lim_abs = A.payload_bit + min((u + 1) * kSubBits, A.bits);
     const uint32_t lim = lim_abs > base ? (uint32_t)(lim_abs - base) : 0u;
@@ -281,18 +281,14 @@ __device__ __forceinline__ void fix_run(const ParseSrc& S, const DecArrayDev& A,
     pn.init(S, ns, base);
     po.init(S, os, base);
     uint32_t dn = 0, dold = 0;
-    uint64_t new_exit;
-    uint32_t new_count;
     for (;;) {
         if (pn.rel >= lim) {  // the new parse left the subsequence without meeting the old one
-            new_exit = base + pn.rel;
-            new_count = dn;
-            break;
+            nc = dn;
+            return base + pn.rel;
         }
         if (pn.rel == po.rel) {  // merged: the rest is the old parse
-            new_exit = exit_in[j];
-            new_count = dn + (count[j] - dold);
-            break;
+            nc = dn + (oc - dold);
+            return oe;
         }
         if (po.rel < pn.rel && po.rel < lim) {
             po.next();
@@ -302,12 +298,13 @@ __device__ __forceinline__ void fix_run(const ParseSrc& S, const DecArrayDev& A,
             dn++;
         }
     }
-    start[j] = ns;
-    count[j] = new_count;
-    exit_out[j] = new_exit;
-    if (new_exit != exit_in[j]) atomicOr(changed, 1u);
 }
 
+// One fix pass.  Inside a workgroup the pass repeats until no subsequence's
+// start differs from its predecessor's exit (the exits in LDS, a barrier per
+// step), so a chain of changed exits inside the workgroup settles in one
+// launch; only a workgroup's first subsequence waits for the next pass to
+// see its predecessor's new exit.  changed: some exit differs from exit_in.
 __global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ words,
                                                  const DecArrayDev* __restrict__ arrays,
                                                  const uint32_t* __restrict__ wg_arr,
@@ -316,22 +313,44 @@ __global__ __launch_bounds__(256) void k_dec_fix(const uint32_t* __restrict__ wo
                                                  uint32_t* count, unsigned int* changed) {
     __shared__ uint32_t lds[kRootMax];
     __shared__ uint32_t win[kWinLds];
+    __shared__ uint64_t lex[256];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + t;
     const uint64_t u = j - A.sub0;
-    const bool settled = u >= A.nsub || u == 0 || start[j] == exit_in[j - 1];
+    const bool live = u < A.nsub && u != 0;
+    const uint64_t ex0 = exit_in[j];
+    const uint64_t prev0 = live ? exit_in[j - 1] : 0;
+    uint64_t st = start[j];
     // the tables only when some lane re-parses (after convergence: none)
-    if (!__syncthreads_or(!settled)) {
-        exit_out[j] = exit_in[j];
+    if (!__syncthreads_or(live && st != prev0)) {
+        exit_out[j] = ex0;
         return;
     }
     const ParseSrc S = load_src(lds, win, words, A, tables);
-    if (settled) {
-        exit_out[j] = exit_in[j];
-        return;
+    lex[t] = ex0;
+    uint32_t cnt = count[j];
+    __syncthreads();
+    for (uint32_t it = 0; it < blockDim.x; it++) {
+        // the predecessor's exit: from the previous workgroup as this pass
+        // found it, or from this workgroup's current exits
+        const uint64_t prev = t == 0 ? prev0 : lex[t - 1];
+        const bool need = live && st != prev;
+        __syncthreads();  // every lane has read lex before any writes it
+        if (need) {
+            uint32_t nc;
+            const uint64_t ne = A.max_len > 32 ? fix_run<LongParser>(S, A, u, prev, st, cnt, lex[t], nc)
+                                               : fix_run<WinParser>(S, A, u, prev, st, cnt, lex[t], nc);
+            st = prev;
+            cnt = nc;
+            lex[t] = ne;
+        }
+        if (!__syncthreads_or(need)) break;
     }
-    if (A.max_len > 32) fix_run<LongParser>(S, A, u, j, start, exit_in, exit_out, count, changed);
-    else fix_run<WinParser>(S, A, u, j, start, exit_in, exit_out, count, changed);
+    start[j] = st;
+    count[j] = cnt;
+    exit_out[j] = lex[t];
+    if (lex[t] != ex0) atomicOr(changed, 1u);
 }
 
 // Symbols per workgroup.
