@@ -214,13 +214,49 @@ __device__ __forceinline__ ParseSrc load_src(uint32_t* lds, uint32_t* win, const
     return S;
 }
 
+// Re-parse of a subsequence from its predecessor's exit ns, in lockstep with
+// its current parse from os (count oc codewords, exit oe); both positions
+// count from the lower of the two (a predecessor's exit and every start lie
+// at or past the subsequence's first bit).  Returns the new exit; nc the new
+// count.  PAIRS: the two-code root table in LDS (one codeword per step).
+template <class PR, bool PAIRS = false>
+__device__ __forceinline__ uint64_t fix_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t ns,
+                                            uint64_t os, uint32_t oc, uint64_t oe, uint32_t& nc) {
+    const uint64_t base = min(ns, os);
+    const uint64_t lim_abs = A.payload_bit + min((u + 1) * kSubBits, A.bits);
+    const uint32_t lim = lim_abs > base ? (uint32_t)(lim_abs - base) : 0u;
+    PR pn, po;
+    pn.init(S, ns, base);
+    po.init(S, os, base);
+    uint32_t dn = 0, dold = 0;
+    for (;;) {
+        if (pn.rel >= lim) {  // the new parse left the subsequence without meeting the old one
+            nc = dn;
+            return base + pn.rel;
+        }
+        if (pn.rel == po.rel) {  // merged: the rest is the old parse
+            nc = dn + (oc - dold);
+            return oe;
+        }
+        if (po.rel < pn.rel && po.rel < lim) {
+            if constexpr (PAIRS) po.skip(0u);
+            else po.next();
+            dold++;
+        } else {
+            if constexpr (PAIRS) pn.skip(0u);
+            else pn.next();
+            dn++;
+        }
+    }
+}
+
 // The parse starts A.lead bits early, inside the predecessor: by the
 // subsequence's first bit it has almost always fallen into step with the true
 // codeword boundaries, and the first boundary at or past that bit is the
 // start (the fix passes correct the rest).
 template <class PR>
-__device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t j,
-                                         uint64_t* start, uint64_t* exit_pos, uint32_t* count) {
+__device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t& st,
+                                         uint64_t& ex, uint32_t& cnt) {
     const uint32_t lead = (uint32_t)min<uint64_t>(A.lead, u * kSubBits);
     const uint64_t base = A.payload_bit + u * kSubBits - lead;
     const uint32_t lim = lead + (uint32_t)min<uint64_t>(kSubBits, A.bits - u * kSubBits);
@@ -241,9 +277,9 @@ __device__ __forceinline__ void sync_run(const ParseSrc& S, const DecArrayDev& A
             n++;
         }
     }
-    start[j] = base + s;
-    exit_pos[j] = base + P.rel;
-    count[j] = n;
+    st = base + s;
+    ex = base + P.rel;
+    cnt = n;
 }
 
 __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ words,
@@ -253,51 +289,42 @@ __global__ __launch_bounds__(256) void k_dec_sync(const uint32_t* __restrict__ w
                                                   uint64_t* exit_pos, uint32_t* count) {
     __shared__ uint32_t lds[kRootMax];
     __shared__ uint32_t win[kWinLds];
+    __shared__ uint64_t lex[256];
     const DecArrayDev A = arrays[wg_arr[blockIdx.x]];
     const ParseSrc S = load_src(lds, win, words, A, tables, true);
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + t;
     const uint64_t u = j - A.sub0;
-    if (u >= A.nsub) {
-        start[j] = exit_pos[j] = 0;
-        count[j] = 0;
-        return;
+    const bool in = u < A.nsub;
+    uint64_t st = 0, ex = 0;
+    uint32_t cnt = 0;
+    if (in) {
+        if (A.max_len > 32) sync_run<LongParser>(S, A, u, st, ex, cnt);
+        else sync_run<WinParser>(S, A, u, st, ex, cnt);
     }
-    if (A.max_len > 32) sync_run<LongParser>(S, A, u, j, start, exit_pos, count);
-    else sync_run<WinParser>(S, A, u, j, start, exit_pos, count);
-}
-
-// Re-parse of a subsequence from its predecessor's exit ns, in lockstep with
-// its current parse from os (count oc codewords, exit oe); both positions
-// count from the lower of the two (a predecessor's exit and every start lie
-// at or past the subsequence's first bit).  Returns the new exit; nc the new
-// count.
-template <class PR>
-__device__ __forceinline__ uint64_t fix_run(const ParseSrc& S, const DecArrayDev& A, uint64_t u, uint64_t ns,
-                                            uint64_t os, uint32_t oc, uint64_t oe, uint32_t& nc) {
-    const uint64_t base = min(ns, os);
-    const uint64_t lim_abs = A.payload_bit + min((u + 1) * kSubBits, A.bits);
-    const uint32_t lim = lim_abs > base ? (uint32_t)(lim_abs - base) : 0u;
-    PR pn, po;
-    pn.init(S, ns, base);
-    po.init(S, os, base);
-    uint32_t dn = 0, dold = 0;
-    for (;;) {
-        if (pn.rel >= lim) {  // the new parse left the subsequence without meeting the old one
-            nc = dn;
-            return base + pn.rel;
+    // The workgroup's own chains settle here, while its tables and window are
+    // in LDS (as in a fix pass); a workgroup's first subsequence, whose
+    // predecessor another workgroup parses, is left to the fix passes.
+    lex[t] = ex;
+    __syncthreads();
+    const bool live = in && t != 0;
+    for (uint32_t it = 0; it < blockDim.x; it++) {
+        const uint64_t prev = t ? lex[t - 1] : 0;
+        const bool need = live && st != prev;
+        __syncthreads();  // every lane has read lex before any writes it
+        if (need) {
+            uint32_t nc;
+            const uint64_t ne = A.max_len > 32 ? fix_run<LongParser>(S, A, u, prev, st, cnt, lex[t], nc)
+                                               : fix_run<WinParser, true>(S, A, u, prev, st, cnt, lex[t], nc);
+            st = prev;
+            cnt = nc;
+            lex[t] = ne;
         }
-        if (pn.rel == po.rel) {  // merged: the rest is the old parse
-            nc = dn + (oc - dold);
-            return oe;
-        }
-        if (po.rel < pn.rel && po.rel < lim) {
-            po.next();
-            dold++;
-        } else {
-            pn.next();
-            dn++;
-        }
+        if (!__syncthreads_or(need)) break;
     }
+    start[j] = st;
+    exit_pos[j] = lex[t];
+    count[j] = cnt;
 }
 
 // One fix pass.  Inside a workgroup the pass repeats until no subsequence's
